@@ -1,0 +1,134 @@
+"""Seeded case generators: write ``mesh.su2`` and ``input.dat`` in the grammar
+the reference reads.
+
+SU2 grammar (reference mesh.c:10-42 tokenizer, mesh.c:115-220 reader): tokens
+split on single spaces, ``\\r`` or newlines; elements ``type p0 p1 p2 [p3] idx``
+(5 = triangle, 9 = quad); points ``x y idx``; boundary lines ``3 p0 p1 `` with
+the trailing space that mesh.c:102-107 needs to consume the line end.
+
+input.dat grammar (reference input.c:7-95): ``name, value`` lines.
+
+All geometry is produced with ``numpy.random.default_rng(seed)`` so every
+machine with the same numpy writes byte-identical files.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+BC_NAMES = {0: "symmetry", 1: "inlet", 2: "outlet", 3: "wall", 4: "wallT"}
+
+
+@dataclass
+class Mesh2D:
+    nodes: np.ndarray                     # [Np, 2] float64
+    cells: list                           # list of vertex tuples (3 or 4 ids)
+    markers: list = field(default_factory=list)   # [(name, [(p0, p1), ...])]
+
+    @property
+    def ncell(self) -> int:
+        return len(self.cells)
+
+
+def _fmt(x: float) -> str:
+    # repr() round-trips a float64 exactly through strtod
+    return repr(float(x))
+
+
+def write_su2(path: str, mesh: Mesh2D) -> None:
+    out = ["NDIME= 2\n", f"NELEM= {mesh.ncell}\n"]
+    for i, c in enumerate(mesh.cells):
+        t = 5 if len(c) == 3 else 9
+        out.append(f"{t} " + " ".join(str(int(p)) for p in c) + f" {i}\n")
+    out.append(f"NPOIN= {len(mesh.nodes)}\n")
+    for i, (x, y) in enumerate(mesh.nodes):
+        out.append(f"{_fmt(x)} {_fmt(y)} {i}\n")
+    out.append(f"NMARK= {len(mesh.markers)}\n")
+    for name, edges in mesh.markers:
+        out.append(f"MARKER_TAG= {name}\n")
+        out.append(f"MARKER_ELEMS= {len(edges)}\n")
+        for p0, p1 in edges:
+            out.append(f"3 {int(p0)} {int(p1)} \n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def write_input(path: str, entries: dict) -> None:
+    with open(path, "w") as f:
+        for k, v in entries.items():
+            f.write(f"{k}, {v}\n")
+
+
+# --------------------------------------------------------------------------
+# Supersonic compression ramp (Euler): domain [0,1]x[0,0.5], lower wall
+# deflected by `angle` degrees for x > x_ramp, structured (nx x ny) quads
+# split into two triangles along a fixed diagonal, interior nodes jittered
+# by a seeded uniform +-jitter of the local spacing.
+# --------------------------------------------------------------------------
+def ramp_mesh(nx: int = 200, ny: int = 100, angle: float = 15.0, x_ramp: float = 0.3,
+              height: float = 0.5, jitter: float = 0.1, seed: int = 0,
+              split: bool = True) -> Mesh2D:
+    rng = np.random.default_rng(seed)
+    tan_a = math.tan(math.radians(angle))
+    xi = np.linspace(0.0, 1.0, nx + 1)
+    X = np.empty((nx + 1, ny + 1))
+    Y = np.empty((nx + 1, ny + 1))
+    for i, x in enumerate(xi):
+        yw = max(0.0, x - x_ramp) * tan_a
+        X[i, :] = x
+        Y[i, :] = yw + (height - yw) * np.linspace(0.0, 1.0, ny + 1)
+    dx = 1.0 / nx
+    jx = rng.uniform(-jitter, jitter, size=(nx + 1, ny + 1))
+    jy = rng.uniform(-jitter, jitter, size=(nx + 1, ny + 1))
+    for i in range(1, nx):
+        for j in range(1, ny):
+            dy = (Y[i, j + 1] - Y[i, j - 1]) * 0.5
+            X[i, j] += jx[i, j] * dx
+            Y[i, j] += jy[i, j] * dy
+    nid = lambda i, j: i * (ny + 1) + j          # noqa: E731
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+    cells = []
+    for i in range(nx):
+        for j in range(ny):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            if split:
+                cells.append((a, b, c))
+                cells.append((a, c, d))
+            else:
+                cells.append((a, b, c, d))
+    inlet = [(nid(0, j + 1), nid(0, j)) for j in range(ny)]
+    outlet = [(nid(nx, j), nid(nx, j + 1)) for j in range(ny)]
+    floor = [(nid(i, 0), nid(i + 1, 0)) for i in range(nx)]
+    top = [(nid(i + 1, ny), nid(i, ny)) for i in range(nx)]
+    markers = [("inlet", inlet), ("outlet", outlet), ("ramp", floor), ("top", top)]
+    return Mesh2D(nodes=nodes, cells=cells, markers=markers)
+
+
+def ramp_input(threads: int = 8, nmax: int = 2000, **over) -> dict:
+    d = {
+        "pressure": "1e5", "temperature": "300.", "mach": "2.5", "nx": "1.", "ny": "0.",
+        "BC:inlet": "inlet", "BC:outlet": "outlet", "BC:ramp": "wall", "BC:top": "symmetry",
+        "Nmax": str(nmax), "order": "2", "stages": "5", "threads": str(threads),
+        "flux": "ROE", "tube": "0", "axisymmetric": "0", "CFL": "0.5",
+        "limiter": "0", "limK": "5", "dtLocal": "1", "timeScheme": "RK",
+    }
+    d.update({k: str(v) for k, v in over.items()})
+    return d
+
+
+def write_case(dirpath: str, mesh: Mesh2D, inputs: dict) -> str:
+    os.makedirs(dirpath, exist_ok=True)
+    if not dirpath.endswith("/"):
+        dirpath += "/"
+    write_su2(dirpath + "mesh.su2", mesh)
+    write_input(dirpath + "input.dat", inputs)
+    return dirpath
+
+
+def ramp_case(dirpath: str, nx: int = 200, ny: int = 100, seed: int = 0, threads: int = 8,
+              nmax: int = 2000, split: bool = True, **inputs) -> str:
+    return write_case(dirpath, ramp_mesh(nx, ny, seed=seed, split=split),
+                      ramp_input(threads=threads, nmax=nmax, **inputs))

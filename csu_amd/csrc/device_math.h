@@ -1,0 +1,490 @@
+// device_math.h -- per-point arithmetic of the residual path (gfx950).
+//
+// Every function restates a reference routine with the same operation order,
+// so that FP64 results round as the reference's (gcc x86-64, no FMA
+// contraction; this code is compiled with -ffp-contract=off).  Templated on
+// the arithmetic type T: double for the parity path, float for the mixed
+// path's reconstruction and flux.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace csu {
+
+// ---------------------------------------------------------------------------
+// Gas model: reference gasprop.c:14-244 (calorically perfect when tp == 0,
+// thermally perfect 5th-order e(T) polynomial otherwise).
+// ---------------------------------------------------------------------------
+struct GasD {
+  double gamma, R, Cp, Cv;
+  double cc[6];
+  int tp;
+};
+
+template <typename T>
+__device__ __forceinline__ T gas_T2e(const GasD& g, T Tk) {
+  if (g.tp) {
+    T P = 1, aux = (T)g.cc[0];
+    for (int i = 1; i < 6; i++) { P *= Tk; aux += (T)g.cc[i] * P; }
+    return aux;
+  }
+  return (T)g.Cv * Tk;
+}
+
+template <typename T>
+__device__ __forceinline__ T gas_T2Cv(const GasD& g, T Tk) {
+  if (g.tp) {
+    T P = 1, aux = (T)g.cc[1];
+    for (int i = 2; i < 6; i++) { P *= Tk; aux += (T)i * (T)g.cc[i] * P; }
+    return aux;
+  }
+  return (T)g.Cv;
+}
+
+template <typename T>
+__device__ __forceinline__ void gas_T2eCv(const GasD& g, T Tk, T& e, T& Cv) {
+  // gasprop.c:126-142
+  T P = 1;
+  e = (T)g.cc[0];
+  Cv = (T)g.cc[1];
+  for (int i = 1; i < 6; i++) {
+    P *= Tk;
+    e += (T)g.cc[i] * P;
+    if (i + 1 < 6) Cv += (T)(i + 1) * (T)g.cc[i + 1] * P;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T gas_e2T(const GasD& g, T e) {
+  // gasprop.c:144-166: Newton on e(T), at most 8 steps, |dT| > 1e-14
+  T Tk = e / (T)g.Cv;
+  if (g.tp) {
+    T err = 1, e1, Cv;
+    int i = 0;
+    while ((fabs(err) > (T)1e-14) & (i < 8)) {
+      gas_T2eCv<T>(g, Tk, e1, Cv);
+      err = (e - e1) / Cv;
+      Tk += err;
+      i++;
+    }
+  }
+  return Tk;
+}
+
+template <typename T>
+__device__ __forceinline__ T gas_T2gamma(const GasD& g, T Tk) {
+  T Cv = gas_T2Cv<T>(g, Tk);
+  return (Cv + (T)g.R) / Cv;
+}
+
+template <typename T>
+__device__ __forceinline__ T gas_T2c(const GasD& g, T Tk) {
+  return sqrt(gas_T2gamma<T>(g, Tk) * (T)g.R * Tk);
+}
+
+template <typename T>
+__device__ __forceinline__ T gas_T2Cp(const GasD& g, T Tk) {
+  return gas_T2Cv<T>(g, Tk) + (T)g.R;
+}
+
+template <typename T>
+__device__ __forceinline__ T gas_critic_H2c(const GasD& g, T H) {
+  // gasprop.c:242-274: critical sound speed from total enthalpy
+  T c2 = (T)2 * ((T)g.gamma - (T)1) * H / ((T)g.gamma + (T)1);
+  if (g.tp) {
+    T Tk = c2 / ((T)g.gamma * (T)g.R);
+    T e, Cv;
+    gas_T2eCv<T>(g, Tk, e, Cv);
+    T RT = (T)g.R * Tk;
+    T gam = (Cv + (T)g.R) / Cv;
+    T H0 = e + RT + gam * RT * (T)0.5;
+    T dH = Cv + (T)g.R + gam * (T)g.R * (T)0.5;
+    T err = (H - H0) / dH;
+    Tk += err;
+    int i = 0;
+    while ((fabs(err) > (T)1.0e-12) & (i < 3)) {
+      gas_T2eCv<T>(g, Tk, e, Cv);
+      RT = (T)g.R * Tk;
+      gam = (Cv + (T)g.R) / Cv;
+      H0 = e + RT + gam * RT * (T)0.5;
+      dH = Cv + (T)g.R + gam * (T)g.R * (T)0.5;
+      err = (H - H0) / dH;
+      Tk += err;
+      i++;
+    }
+    return gas_T2c<T>(g, Tk);
+  }
+  return sqrt(c2);
+}
+
+__device__ __forceinline__ double sutherland(double Tk) {
+  // reference solver.c:1222-1225
+  return 1.458e-6 * Tk * sqrt(Tk) / (Tk + 110.4);
+}
+
+// ---------------------------------------------------------------------------
+// Double-double helpers.  The reference accumulates its least-squares sums in
+// x87 long double (solver.c:1041); the device carries them as unevaluated
+// double pairs (Dekker/Knuth two-sum and an FMA two-product), which is at
+// least as precise, and rounds to double only at the end.
+// ---------------------------------------------------------------------------
+struct dd { double hi, lo; };
+
+__device__ __forceinline__ dd two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+__device__ __forceinline__ dd quick_two_sum(double a, double b) {
+  double s = a + b;
+  return {s, b - (s - a)};
+}
+__device__ __forceinline__ dd two_prod(double a, double b) {
+  double p = a * b;
+  return {p, __fma_rn(a, b, -p)};
+}
+__device__ __forceinline__ dd dd_add_d(dd a, double b) {
+  dd s = two_sum(a.hi, b);
+  s.lo += a.lo;
+  return quick_two_sum(s.hi, s.lo);
+}
+__device__ __forceinline__ dd dd_add(dd a, dd b) {
+  dd s = two_sum(a.hi, b.hi);
+  dd t = two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = quick_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return quick_two_sum(s.hi, s.lo);
+}
+__device__ __forceinline__ dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
+__device__ __forceinline__ dd dd_sub(dd a, dd b) { return dd_add(a, dd_neg(b)); }
+__device__ __forceinline__ dd dd_mul(dd a, dd b) {
+  dd p = two_prod(a.hi, b.hi);
+  p.lo = __fma_rn(a.hi, b.lo, p.lo);
+  p.lo = __fma_rn(a.lo, b.hi, p.lo);
+  return quick_two_sum(p.hi, p.lo);
+}
+__device__ __forceinline__ dd dd_mul_d(dd a, double b) {
+  dd p = two_prod(a.hi, b);
+  p.lo = __fma_rn(a.lo, b, p.lo);
+  return quick_two_sum(p.hi, p.lo);
+}
+__device__ __forceinline__ dd dd_div(dd a, dd b) {
+  double q1 = a.hi / b.hi;
+  dd r = dd_sub(a, dd_mul_d(b, q1));
+  double q2 = r.hi / b.hi;
+  r = dd_sub(r, dd_mul_d(b, q2));
+  double q3 = r.hi / b.hi;
+  dd q = quick_two_sum(q1, q2);
+  return dd_add_d(q, q3);
+}
+
+// ---------------------------------------------------------------------------
+// Face-frame rotation, reference solver.c:441-448.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void rotate(T& u, T& v, T dSx, T dSy, T dS) {
+  T aux = (u * dSx + v * dSy) / dS;
+  v = (-u * dSy + v * dSx) / dS;
+  u = aux;
+}
+
+// ---------------------------------------------------------------------------
+// Convective fluxes in the face frame (reference flux.c).  Primitive inputs
+// (r, u normal, v tangential, p, passive scalars s[]); outputs f[0..3+NS].
+// NS = number of passive scalars transported with the mass flux (SA: 1).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void entropy_fix(T efix, T& l) {
+  // flux.c:48-57, Harten-Hyman with an absolute threshold
+  if ((l < efix) & (l > -efix)) l = (T)0.5 * (l * l / efix + efix);
+}
+
+template <typename T>
+__device__ __forceinline__ void flux_roe(const GasD& g, T efix, T rL, T uL, T vL, T pL,
+                                         T rR, T uR, T vR, T pR, T* f) {
+  // flux.c:59-129 (Roe 1981); ideal-gas gamma only, as the reference
+  const T gm1 = (T)g.gamma - (T)1;
+  T U0L = rL, U1L = rL * uL, U2L = rL * vL;
+  T U3L = pL / gm1 + (uL * uL + vL * vL) * rL / (T)2;
+  T HL = (U3L + pL) / rL;
+  T U0R = rR, U1R = rR * uR, U2R = rR * vR;
+  T U3R = pR / gm1 + (uR * uR + vR * vR) * rR / (T)2;
+  T HR = (U3R + pR) / rR;
+  T rqL = sqrt(rL), rqR = sqrt(rR);
+  T ub = (rqL * uL + rqR * uR) / (rqL + rqR);
+  T vb = (rqL * vL + rqR * vR) / (rqL + rqR);
+  T Hb = (rqL * HL + rqR * HR) / (rqL + rqR);
+  T ab = sqrt(gm1 * (Hb - (ub * ub + vb * vb) / (T)2));
+  T l1 = ub - ab, l2 = ub, l4 = ub, l5 = ub + ab;
+  T e1[4] = {(T)1, ub - ab, vb, Hb - ub * ab};
+  T e2v[4] = {(T)0, (T)0, (T)1, vb};
+  T e4[4] = {(T)1, ub, vb, (T)0.5 * (ub * ub + vb * vb)};
+  T e5[4] = {(T)1, ub + ab, vb, Hb + ub * ab};
+  T d1 = U0R - U0L, d2 = U1R - U1L, d3 = U2R - U2L, d5 = U3R - U3L;
+  T a4 = (Hb - (ub * ub + vb * vb)) * d1 + ub * d2 + vb * d3 - d5;
+  a4 /= (ab * ab) / gm1;
+  T a2v = d3 - d1 * vb;
+  T a5 = ((d1 - a4) + (d2 - ub * d1) / ab) * (T)0.5;
+  T a1 = ((d1 - a4) - (d2 - ub * d1) / ab) * (T)0.5;
+  T fL[4] = {U1L, U1L * uL + pL, U1L * vL, uL * (U3L + pL)};
+  T fR[4] = {U1R, U1R * uR + pR, U1R * vR, uR * (U3R + pR)};
+  entropy_fix<T>(efix, l1);
+  entropy_fix<T>(efix, l2);
+  entropy_fix<T>(efix, l4);
+  entropy_fix<T>(efix, l5);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    f[i] = (T)0.5 * (fR[i] + fL[i] - a1 * fabs(l1) * e1[i] - a2v * fabs(l2) * e2v[i] -
+                     a4 * fabs(l4) * e4[i] - a5 * fabs(l5) * e5[i]);
+}
+
+// Total enthalpy and sound speed from (r, u, v, p[, k]) via the gas model
+template <typename T>
+__device__ __forceinline__ void state_HTc(const GasD& g, T r, T u, T v, T p, T k, T& H, T& Tk) {
+  Tk = p / ((T)g.R * r);
+  T U3 = gas_T2e<T>(g, Tk) * r + (u * u + v * v) * r / (T)2 + k * r;
+  H = (U3 + p) / r;
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ void flux_ausm(const GasD& g, T rL, T uL, T vL, T pL, const T* sL,
+                                          T rR, T uR, T vR, T pR, const T* sR, T kL, T kR, T* f) {
+  // flux.c:132-203 (Liou-Steffen 1993); _sa: 512-585; _sst: 901-
+  T TL, HL, TR, HR;
+  state_HTc<T>(g, rL, uL, vL, pL, kL, HL, TL);
+  state_HTc<T>(g, rR, uR, vR, pR, kR, HR, TR);
+  T cL = gas_T2c<T>(g, TL), cR = gas_T2c<T>(g, TR);
+  T ML = uL / cL, MR = uR / cR;
+  T Mplus, Mminus, Pplus, Pminus;
+  if (fabs(ML) >= (T)1) { Mplus = (T)0.5 * (ML + fabs(ML)); Pplus = Mplus / ML; }
+  else { T M2p = (T)0.25 * (ML + (T)1) * (ML + (T)1); Mplus = M2p; Pplus = M2p * ((T)2 - ML); }
+  if (fabs(MR) >= (T)1) { Mminus = (T)0.5 * (MR - fabs(MR)); Pminus = Mminus / MR; }
+  else { T M2m = (T)-0.25 * (MR - (T)1) * (MR - (T)1); Mminus = M2m; Pminus = -M2m * ((T)2 + MR); }
+  T Mm = Mplus + Mminus;
+  T pm = Pplus * pL + Pminus * pR;
+  if (Mm > (T)0) {
+    f[0] = rL * cL * Mm; f[1] = f[0] * uL + pm; f[2] = f[0] * vL; f[3] = f[0] * HL;
+#pragma unroll
+    for (int s = 0; s < NS; s++) f[4 + s] = f[0] * sL[s];
+  } else {
+    f[0] = rR * cR * Mm; f[1] = f[0] * uR + pm; f[2] = f[0] * vR; f[3] = f[0] * HR;
+#pragma unroll
+    for (int s = 0; s < NS; s++) f[4 + s] = f[0] * sR[s];
+  }
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ void flux_ausmdv(const GasD& g, T rL, T uL, T vL, T pL, const T* sL,
+                                            T rR, T uR, T vR, T pR, const T* sR, T kL, T kR, T* f) {
+  // flux.c:206-287 (Wada-Liou 1994); _sa: 588-670
+  T TL, HL, TR, HR, aux;
+  state_HTc<T>(g, rL, uL, vL, pL, kL, HL, TL);
+  state_HTc<T>(g, rR, uR, vR, pR, kR, HR, TR);
+  T cL = gas_T2c<T>(g, TL), cR = gas_T2c<T>(g, TR);
+  T cm = fmax(cL, cR);
+  T alphaL = ((T)2.0 * pL / rL) / (pL / rL + pR / rR);
+  T alphaR = ((T)2.0 * pR / rR) / (pL / rL + pR / rR);
+  T uLPlus, pLPlus;
+  aux = (T)0.5 * (uL + fabs(uL));
+  if (fabs(uL) < cm) {
+    uLPlus = alphaL * ((T)0.25 * (uL + cm) * (uL + cm) / cm - aux) + aux;
+    pLPlus = (T)0.25 * pL * (uL + cm) * (uL + cm) * ((T)2.0 - uL / cm) / (cm * cm);
+  } else {
+    uLPlus = aux;
+    pLPlus = pL * aux / uL;
+  }
+  T uRMinus, pRMinus;
+  aux = (T)0.5 * (uR - fabs(uR));
+  if (fabs(uR) < cm) {
+    uRMinus = alphaR * ((T)-0.25 * (uR - cm) * (uR - cm) / cm - aux) + aux;
+    pRMinus = (T)0.25 * pR * (uR - cm) * (uR - cm) * ((T)2.0 + uR / cm) / (cm * cm);
+  } else {
+    uRMinus = aux;
+    pRMinus = pR * aux / uR;
+  }
+  T rU = uLPlus * rL + uRMinus * rR;
+  f[0] = rU;
+  f[1] = (pLPlus + pRMinus);
+  f[2] = (T)0.5 * (rU * (vR + vL) - fabs(rU) * (vR - vL));
+  f[3] = (T)0.5 * (rU * (HR + HL) - fabs(rU) * (HR - HL));
+#pragma unroll
+  for (int s = 0; s < NS; s++) f[4 + s] = (T)0.5 * (rU * (sR[s] + sL[s]) - fabs(rU) * (sR[s] - sL[s]));
+  T f1AUSMD = (T)0.5 * (rU * (uR + uL) - fabs(rU) * (uR - uL));
+  T f1AUSMV = uLPlus * rL * uL + uRMinus * rR * uR;
+  T s = (T)0.5 * fmin((T)1, (T)10 * fabs(pR - pL) / fmin(pL, pR));
+  f[1] += ((T)0.5 + s) * f1AUSMV + ((T)0.5 - s) * f1AUSMD;
+  int caseA = (uL - cL < (T)0.0) & (uR - cR > (T)0.0);
+  int caseB = (uL + cL < (T)0.0) & (uR + cR > (T)0.0);
+  T psiL[4 + NS] = {(T)1.0, uL, vL, HL};
+  T psiR[4 + NS] = {(T)1.0, uR, vR, HR};
+#pragma unroll
+  for (int q = 0; q < NS; q++) { psiL[4 + q] = sL[q]; psiR[4 + q] = sR[q]; }
+  if (caseA & ~caseB) {
+    aux = (T)0.125 * ((uR - cR) - (uL - cL));
+#pragma unroll
+    for (int k = 0; k < 4 + NS; k++) f[k] -= aux * (rR * psiR[k] - rL * psiL[k]);
+  } else if (~caseA & caseB) {
+    aux = (T)0.125 * ((uR + cR) - (uL + cL));
+#pragma unroll
+    for (int k = 0; k < 4 + NS; k++) f[k] -= aux * (rR * psiR[k] - rL * psiL[k]);
+  }
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ void flux_ausmpup(const GasD& g, T minf, int variant,
+                                             T rL, T uL, T vL, T pL, const T* sL,
+                                             T rR, T uR, T vR, T pR, const T* sR, T kL, T kR,
+                                             T* f) {
+  // flux.c:290-398 (AUSM+up, Liou 2006) when variant == 0;
+  // flux.c:401-508 (pressure flux of the "2" variant) when variant == 1
+  T TL, HL, TR, HR;
+  state_HTc<T>(g, rL, uL, vL, pL, kL, HL, TL);
+  state_HTc<T>(g, rR, uR, vR, pR, kR, HR, TR);
+  T astar = gas_critic_H2c<T>(g, HL);
+  T ahL = astar * astar / fmax(astar, fabs(uL));
+  astar = gas_critic_H2c<T>(g, HR);
+  T ahR = astar * astar / fmax(astar, fabs(uR));
+  T am = fmin(ahL, ahR);
+  const T Kp = (T)0.25, Ku = (T)0.75, sig = (T)1.0, beta = (T)1.0 / (T)8.0;
+  T ML = uL / am, MR = uR / am;
+  T Mbar = sqrt((uL * uL + uR * uR) / ((T)2 * am * am));
+  T M0 = sqrt(fmin((T)1, fmax(Mbar * Mbar, minf * minf)));
+  T fa = M0 * ((T)2 - M0);
+  if (fa < (T)1e-3) fa = (T)1e-3;
+  T alpha = ((T)3.0 / (T)16.0) * ((T)-4 + (T)5 * fa * fa);
+  T rhom = (T)0.5 * (rL + rR);
+  T Mplus, Mminus, Pplus, Pminus, M2p, M2m;
+  if (fabs(ML) >= (T)1) { Mplus = (T)0.5 * (ML + fabs(ML)); Pplus = Mplus / ML; }
+  else {
+    M2p = (T)0.25 * (ML + (T)1) * (ML + (T)1);
+    M2m = (T)-0.25 * (ML - (T)1) * (ML - (T)1);
+    Mplus = M2p * ((T)1 - (T)16 * beta * M2m);
+    Pplus = M2p * (((T)2 - ML) - (T)16 * alpha * ML * M2m);
+  }
+  if (fabs(MR) >= (T)1) { Mminus = (T)0.5 * (MR - fabs(MR)); Pminus = Mminus / MR; }
+  else {
+    M2p = (T)0.25 * (MR + (T)1) * (MR + (T)1);
+    M2m = (T)-0.25 * (MR - (T)1) * (MR - (T)1);
+    Mminus = M2m * ((T)1 + (T)16 * beta * M2p);
+    Pminus = M2m * (((T)-2 - MR) + (T)16 * alpha * MR * M2p);
+  }
+  T Mm = Mplus + Mminus - (Kp / fa) * fmax((T)1 - sig * Mbar * Mbar, (T)0) * (pR - pL) / (rhom * am * am);
+  T pm;
+  if (variant == 0) {
+    pm = Pplus * pL + Pminus * pR - Ku * Pplus * Pminus * (rL + rR) * (fa * am) * (uR - uL);
+  } else {
+    pm = (T)0.5 * (pL + pR) + (T)0.5 * (Pplus - Pminus) * (pL - pR) +
+         sqrt((T)0.5 * (uL * uL + vL * vL + uR * uR + vR * vR)) * (Pplus + Pminus - (T)1) * (T)0.5 * (pL + pR) / am;
+  }
+  T mm = (Mm > (T)0) ? am * Mm * rL : am * Mm * rR;
+  if (mm > (T)0) {
+    f[0] = mm; f[1] = mm * uL + pm; f[2] = mm * vL; f[3] = mm * HL;
+#pragma unroll
+    for (int s = 0; s < NS; s++) f[4 + s] = mm * sL[s];
+  } else {
+    f[0] = mm; f[1] = mm * uR + pm; f[2] = mm * vR; f[3] = mm * HR;
+#pragma unroll
+    for (int s = 0; s < NS; s++) f[4 + s] = mm * sR[s];
+  }
+}
+
+// flux()/flux_sa() dispatch, flux.c:1300-1345 (Roe has no passive-scalar form)
+template <typename T, int NS>
+__device__ __forceinline__ void flux_dispatch(const GasD& g, int kind, T efix, T minf,
+                                              T rL, T uL, T vL, T pL, const T* sL,
+                                              T rR, T uR, T vR, T pR, const T* sR,
+                                              T kL, T kR, T* f) {
+  switch (kind) {
+    case 0: flux_roe<T>(g, efix, rL, uL, vL, pL, rR, uR, vR, pR, f); break;
+    case 1: flux_ausm<T, NS>(g, rL, uL, vL, pL, sL, rR, uR, vR, pR, sR, kL, kR, f); break;
+    case 2: flux_ausmdv<T, NS>(g, rL, uL, vL, pL, sL, rR, uR, vR, pR, sR, kL, kR, f); break;
+    case 3: flux_ausmpup<T, NS>(g, minf, 0, rL, uL, vL, pL, sL, rR, uR, vR, pR, sR, kL, kR, f); break;
+    default: flux_ausmpup<T, NS>(g, minf, 1, rL, uL, vL, pL, sL, rR, uR, vR, pR, sR, kL, kR, f); break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void flux_free(const GasD& g, T rL, T uL, T vL, T pL, T* f) {
+  // flux.c:1369-1380: inviscid wall flux from the ghost state
+  T TL = pL / ((T)g.R * rL);
+  T U3L = gas_T2e<T>(g, TL) * rL + (uL * uL + vL * vL) * rL / (T)2;
+  T HL = (U3L + pL) / rL;
+  f[0] = rL * uL;
+  f[1] = rL * uL * uL + pL;
+  f[2] = rL * uL * vL;
+  f[3] = rL * uL * HL;
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic boundary states, reference boundary.c:17-141.
+// Pd: interior primitive (r u v p), Pb: ghost (r u v p), n: unit normal.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bc_inlet(const GasD& g, const double* Pa, const double* Pd,
+                                         double* Pb, double nx, double ny) {
+  double rd = Pd[0], ud = Pd[1], vd = Pd[2], pd = Pd[3];
+  double Tk = pd / rd / g.R;
+  double c0 = gas_T2c<double>(g, Tk);
+  double m = sqrt(ud * ud + vd * vd) / c0;
+  if (m < 1.) {
+    double ra = Pa[0], ua = Pa[1], va = Pa[2], pa = Pa[3];
+    double pb = 0.5 * (pa + pd - rd * c0 * (nx * (ua - ud) + ny * (va - vd)));
+    double rb = ra + (pb - pa) / (c0 * c0);
+    double ub = ua - nx * (pa - pb) / (rd * c0);
+    double vb = va - ny * (pa - pb) / (rd * c0);
+    Pb[0] = rb; Pb[1] = ub; Pb[2] = vb; Pb[3] = pb;
+  } else {
+    for (int i = 0; i < 4; i++) Pb[i] = Pa[i];
+  }
+}
+
+__device__ __forceinline__ void bc_outlet(const GasD& g, double pout, const double* Pd, double* Pb,
+                                          double nx, double ny) {
+  double rd = Pd[0], ud = Pd[1], vd = Pd[2], pd = Pd[3];
+  double Tk = pd / rd / g.R;
+  double c0 = gas_T2c<double>(g, Tk);
+  double m = sqrt(ud * ud + vd * vd) / c0;
+  if (m < 1.) {
+    double pb = pout;
+    double rb = rd + (pb - pd) / (c0 * c0);
+    double ub = ud + nx * (pd - pb) / (rd * c0);
+    double vb = vd + ny * (pd - pb) / (rd * c0);
+    Pb[0] = rb; Pb[1] = ub; Pb[2] = vb; Pb[3] = pb;
+  } else {
+    for (int i = 0; i < 4; i++) Pb[i] = Pd[i];
+  }
+}
+
+__device__ __forceinline__ void bc_wall(const GasD& g, const double* Pd, double* Pb, double nx, double ny) {
+  double rd = Pd[0], ud = Pd[1], vd = Pd[2], pd = Pd[3];
+  double Tk = pd / rd / g.R;
+  double c0 = gas_T2c<double>(g, Tk);
+  double pb = pd + rd * c0 * (nx * ud + ny * vd);
+  double rb = rd + (pb - pd) / (c0 * c0);
+  double ub = ud - nx * (nx * ud + ny * vd);
+  double vb = vd - ny * (nx * ud + ny * vd);
+  Pb[0] = rb; Pb[1] = ub; Pb[2] = vb; Pb[3] = pb;
+}
+
+// Venkatakrishnan-type limiter function, reference limiter.c:98-147
+template <typename T>
+__device__ __forceinline__ T limiter_v2(T Ui, T Umin, T Umax, T d2, T ee) {
+  T ans;
+  T d1max = Umax - Ui;
+  T d1min = Umin - Ui;
+  if (d2 == (T)0) {
+    ans = (T)1;
+  } else if (d2 > (T)0) {
+    ans = (d1max * d1max + ee) * d2 + (T)2 * d2 * d2 * d1max;
+    ans /= d1max * d1max + (T)2 * d2 * d2 + d1max * d2 + ee;
+    ans /= d2;
+  } else {
+    ans = (d1min * d1min + ee) * d2 + (T)2 * d2 * d2 * d1min;
+    ans /= d1min * d1min + (T)2 * d2 * d2 + d1min * d2 + ee;
+    ans /= d2;
+  }
+  return ans;
+}
+
+}  // namespace csu

@@ -1,0 +1,246 @@
+/*
+ * case.c -- case loading: input.dat options, free-stream state, initial
+ * state or restart, mesh.
+ *
+ * Follows the reference's solverInit / solverSetData / solverInitDomain
+ * (solver.c:1403-1834) for which keys are read, their defaults and the
+ * free-stream state (conditionState, solver.c:40-89), so that one case
+ * directory means the same run to both programs.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "csu_host.h"
+
+/* ---- calorically / thermally perfect gas on the host (gasprop.c:14-171) ---- */
+typedef struct { double gamma, R, Cp, Cv, cc[6]; int tp; } hgas;
+
+static void hgas_init(hgas* g, double gamma, double R, int tp)
+{
+    g->gamma = gamma; g->R = R;
+    g->Cp = g->gamma * g->R / (g->gamma - 1);
+    g->Cv = g->Cp - g->R;
+    g->tp = tp;
+    g->cc[0] = 1.81716e3;
+    g->cc[1] = 9.890504e2 - 287.0530;
+    g->cc[2] = -9.595592e-3;
+    g->cc[3] = 1.041469e-4;
+    g->cc[4] = -4.433065e-8;
+    g->cc[5] = 5.879263e-12;
+}
+
+static double hgas_T2e(const hgas* g, double T)
+{
+    if (!g->tp) return g->Cv * T;
+    double P = 1, aux = g->cc[0];
+    for (int i = 1; i < 6; i++) { P *= T; aux += g->cc[i] * P; }
+    return aux;
+}
+
+static double hgas_T2Cv(const hgas* g, double T)
+{
+    if (!g->tp) return g->Cv;
+    double P = 1, aux = g->cc[1];
+    for (int i = 2; i < 6; i++) { P *= T; aux += i * g->cc[i] * P; }
+    return aux;
+}
+
+static double hgas_T2c(const hgas* g, double T)
+{
+    double Cv = hgas_T2Cv(g, T);
+    double gam = (Cv + g->R) / Cv;
+    return sqrt(gam * g->R * T);
+}
+
+static double sutherland(double T) { return 1.458e-6 * T * sqrt(T) / (T + 110.4); }
+
+/* ---- options ---- */
+static char g_missing[128];
+
+static const char* need(const csu_input* in, const char* key)
+{
+    const char* v = csu_input_get(in, key);
+    if (!v && !g_missing[0]) snprintf(g_missing, sizeof g_missing, "%s", key);
+    return v ? v : "0";
+}
+
+static int opt_i(const csu_input* in, const char* key, int def)
+{
+    const char* v = csu_input_get(in, key);
+    return v ? atoi(v) : def;
+}
+
+static double opt_d(const csu_input* in, const char* key, double def)
+{
+    const char* v = csu_input_get(in, key);
+    return v ? strtod(v, NULL) : def;
+}
+
+static int choice(const char* s, const char* const* names, int n)
+{
+    for (int i = 0; i < n; i++) if (strcmp(s, names[i]) == 0) return i;
+    return -1;
+}
+
+const char* csu_host_missing_key(void) { return g_missing; }
+
+int csu_case_load(const char* wd, csu_case** out)
+{
+    csu_case* c = calloc(1, sizeof(csu_case));
+    if (!c) return CSU_ERR_NOMEM;
+    snprintf(c->wd, sizeof c->wd, "%s", wd);
+    char path[1200];
+    g_missing[0] = '\0';
+
+    snprintf(path, sizeof path, "%sinput.dat", wd);
+    int rc = csu_input_read(path, &c->input);
+    if (rc) { free(c); return rc; }
+    const csu_input* in = c->input;
+    csu_params* p = &c->prm;
+
+    c->threads = atoi(need(in, "threads"));
+    p->sa = opt_i(in, "sa", 0);
+    int sacc = opt_i(in, "saCC", 0);
+    p->sst = opt_i(in, "sst", 0);
+    p->sst_trans = p->sst ? opt_i(in, "sstTrans", 0) : 0;
+    p->nvar = 4;
+    if (p->sa == 1) p->nvar = 5;
+    if (p->sst == 1) p->nvar = 6;
+    int axi = atoi(need(in, "axisymmetric"));
+    if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
+    if (sacc) { csu_case_free(c); return CSU_ERR_UNSUPPORTED; }
+
+    snprintf(path, sizeof path, "%smesh.su2", wd);
+    rc = csu_mesh_read_su2(path, axi, c);
+    if (rc) { csu_case_free(c); return rc; }
+
+    /* solverSetData, solver.c:1403-1658 */
+    p->order = atoi(need(in, "order"));
+    for (int m = 0; m < c->mesh.nmark; m++) {
+        char key[96];
+        snprintf(key, sizeof key, "BC:%s", c->mark_name[m]);
+        const char* v = need(in, key);
+        static const char* const bcn[] = {"symmetry", "inlet", "outlet", "wall", "wallT"};
+        int t = choice(v, bcn, 5);
+        if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
+        if (t < 0) {
+            snprintf(g_missing, sizeof g_missing, "BC:%s=%s", c->mark_name[m], v);
+            csu_case_free(c); return CSU_ERR_INPUT;
+        }
+        c->mark_type[m] = t;
+    }
+    p->tp = opt_i(in, "TP", 0);
+    p->gamma = 1.4; p->gas_r = 287.0530;
+    p->pr = 0.72; p->pr_t = 0.9; p->efix = 0.1;
+    p->laminar = opt_i(in, "laminar", 0);
+    c->restart = opt_i(in, "restart", 0);
+    p->sref = opt_d(in, "Sref", 1.0);
+    p->turb_ratio = opt_d(in, "turbRatio", 10.0);
+    p->dt_local = opt_i(in, "dtLocal", 0);
+    p->dt_local_n = opt_i(in, "dtLocalN", -1);
+    p->wimp = opt_d(in, "wImp", 1.0);
+    p->turb1order = opt_i(in, "turb1order", 0);
+    const char* ts = csu_input_get(in, "timeScheme");
+    static const char* const tsn[] = {"RK", "LUSGS", "DPLUR"};
+    p->time_scheme = ts ? choice(ts, tsn, 3) : 0;
+    if (p->time_scheme < 0) {
+        snprintf(g_missing, sizeof g_missing, "timeScheme=%s", ts);
+        csu_case_free(c); return CSU_ERR_INPUT;
+    }
+    p->nlinear = opt_i(in, "Nlinear", 20);
+    p->rlim = opt_d(in, "rLim", 1e-6);
+    static const char* const fln[] = {"ROE", "AUSM", "AUSMDV", "AUSMpup", "AUSMpup2"};
+    const char* fl = need(in, "flux");
+    p->flux = choice(fl, fln, 5);
+    p->stages = atoi(need(in, "stages"));
+    p->cfl = strtod(need(in, "CFL"), NULL);
+    if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
+    if (p->flux < 0) {
+        snprintf(g_missing, sizeof g_missing, "flux=%s", fl);
+        csu_case_free(c); return CSU_ERR_INPUT;
+    }
+    p->pout = opt_d(in, "pout", 1.0e5);
+    p->visc_blazek = opt_i(in, "viscBlazek", 0);
+    p->twall = opt_d(in, "Twall", 300);
+    const char* ws = csu_input_get(in, "writeSurf");
+    snprintf(c->write_surf, sizeof c->write_surf, "%s", ws ? ws : "wall");
+    p->limiter = opt_i(in, "limiter", 0);
+    p->lim_k = opt_d(in, "limK", 1.0);
+    if (p->sst) {
+        p->sst_kfactor = opt_d(in, "kFactor", 1.125);
+        p->sst_ofactor = opt_d(in, "oFactor", 125.0);
+        p->sst_owallfactor = opt_d(in, "oWallFactor", 10);
+        p->sst_l = opt_d(in, "Lsst", 1.0);
+    }
+    p->precision = opt_i(in, "precision", CSU_PREC_FP64);
+
+    if (p->sa == 1 || p->sst == 1) {
+        rc = csu_wall_distance(c);
+        if (rc) { csu_case_free(c); return rc; }
+    }
+
+    /* solverInitDomain, solver.c:1660-1697 (steady mode only) */
+    c->tube = atoi(need(in, "tube"));
+    if (c->tube != 0) { csu_case_free(c); return CSU_ERR_UNSUPPORTED; }
+    c->nmax = atoi(need(in, "Nmax"));
+    double pinf = strtod(need(in, "pressure"), NULL);
+    double Tinf = strtod(need(in, "temperature"), NULL);
+    double mach = strtod(need(in, "mach"), NULL);
+    double nx = strtod(need(in, "nx"), NULL);
+    double ny = strtod(need(in, "ny"), NULL);
+    if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
+
+    hgas g;
+    hgas_init(&g, p->gamma, p->gas_r, p->tp);
+    /* conditionState, solver.c:40-89 */
+    double r = pinf / (g.R * Tinf);
+    double cs = hgas_T2c(&g, Tinf);
+    double u = nx * mach * cs;
+    double v = ny * mach * cs;
+    double k = 0.0;
+    for (int i = 0; i < 7; i++) p->pin[i] = 0.0;
+    for (int i = 0; i < 6; i++) p->uin[i] = 0.0;
+    if (p->sst == 1) {
+        double L = p->sst_l, Uv = cs * mach;
+        double ReL = (r * Uv * L) / sutherland(Tinf);
+        k = p->sst_kfactor * Uv * Uv / ReL;
+        double om = p->sst_ofactor * Uv / L;
+        p->uin[4] = r * k; p->uin[5] = r * om;
+        p->pin[5] = k; p->pin[6] = om;
+    }
+    double E = hgas_T2e(&g, Tinf) + (u * u + v * v) / 2 + k;
+    p->uin[0] = r; p->uin[1] = r * u; p->uin[2] = r * v; p->uin[3] = r * E;
+    p->pin[0] = r; p->pin[1] = u; p->pin[2] = v; p->pin[3] = g.R * Tinf * r; p->pin[4] = Tinf;
+    if (p->sa == 1) {
+        double n = p->turb_ratio * sutherland(Tinf) / r;
+        p->uin[4] = r * n;
+        p->pin[5] = n;
+    }
+    p->mach_inf = mach;
+
+    int nc = c->mesh.ncell, nv = p->nvar;
+    c->U = malloc(sizeof(double) * (size_t)nv * (size_t)nc);
+    if (!c->U) { csu_case_free(c); return CSU_ERR_NOMEM; }
+    if (c->restart) {
+        snprintf(path, sizeof path, "%srestart.csv", wd);
+        rc = csu_read_restart(c, c->U, path);
+        if (rc) { csu_case_free(c); return rc; }
+    } else {
+        /* solverInitU (4 variables) + saInitU / sstInitU */
+        for (int kk = 0; kk < nv; kk++)
+            for (int i = 0; i < nc; i++) c->U[(size_t)kk * nc + i] = p->uin[kk];
+    }
+    *out = c;
+    return CSU_OK;
+}
+
+void csu_case_free(csu_case* c)
+{
+    if (!c) return;
+    free(c->U); free(c->node_xy); free(c->cell_vert); free(c->face); free(c->bface);
+    free(c->bface_mark); free(c->mark_type); free(c->cell_nbr); free(c->cell_face);
+    free(c->wall_dist); free(c->mark_name);
+    csu_input_free(c->input);
+    free(c);
+}

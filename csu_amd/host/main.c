@@ -1,0 +1,105 @@
+/*
+ * main.c -- the drop-in `executable`: ./executable <case>/
+ *
+ * Same process contract as the reference main.c:18-45 + solverSolve
+ * (solver.c:1836-1911): reads <case>/input.dat, mesh.su2 (and restart.csv
+ * when restart=1), runs Nmax iterations, prints one line per iteration and
+ * appends it to convergence.csv, then writes solution.csv, restart.csv and
+ * friction.csv.  The iterations run on the GPU through csu.h; the host only
+ * formats the per-iteration records, which come back in blocks.
+ *
+ * As the reference, input errors print a message and exit with status 0
+ * (input.c:156-160); device errors print a message and exit with status 1.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include "csu.h"
+#include "csu_host.h"
+
+static double seconds(struct timeval a, struct timeval b)
+{
+    return ((double)b.tv_sec + b.tv_usec / 1000000.) - ((double)a.tv_sec + a.tv_usec / 1000000.);
+}
+
+int main(int argc, char** argv)
+{
+    struct timeval start, stop;
+    gettimeofday(&start, NULL);
+    if (argc < 2) {
+        printf("usage: %s <case directory/>\n", argv[0]);
+        return 0;
+    }
+    const char* wd = argv[1];
+    csu_case* c = NULL;
+    int rc = csu_case_load(wd, &c);
+    if (rc == CSU_ERR_INPUT) {
+        printf("Error: Value not found in the input file: %s.\n", csu_host_missing_key());
+        return 0;
+    }
+    if (rc) {
+        printf("Error: cannot load case %s (code %d).\n", wd, rc);
+        return 0;
+    }
+    printf("mesh: %d cells, %d interior faces, %d boundary faces\n",
+           c->mesh.ncell, c->mesh.nface, c->mesh.nbface);
+
+    csu_dev* d = NULL;
+    rc = csu_dev_create(&c->mesh, &c->prm, &d);
+    if (!rc) rc = csu_dev_set_state(d, c->U);
+    if (rc) {
+        printf("Error: device setup failed: %s\n", csu_last_error());
+        csu_case_free(c);
+        return 1;
+    }
+
+    char path[1200];
+    snprintf(path, sizeof path, "%sconvergence.csv", wd);
+    FILE* conv = fopen(path, c->restart ? "a" : "w");
+    if (!conv) { printf("Error: cannot open %s\n", path); return 1; }
+
+    const int nv = c->prm.nvar, block = 200;
+    double* res = malloc(sizeof(double) * (size_t)block * nv);
+    double* coef = malloc(sizeof(double) * (size_t)block * 4);
+    printf("\nmain: running solution:\n");
+    for (int ii = 0; ii < c->nmax; ii += block) {
+        int n = c->nmax - ii < block ? c->nmax - ii : block;
+        rc = csu_dev_iterate(d, n, res, coef);
+        if (rc) { printf("Error: iteration failed: %s\n", csu_last_error()); return 1; }
+        for (int k = 0; k < n; k++) {
+            const double* r = res + (size_t)k * nv;
+            const double* q = coef + (size_t)k * 4;
+            printf("%i, ", ii + k);
+            for (int v = 0; v < nv; v++) printf(" %+.4e,", r[v]);
+            printf("\n");
+            printf("%f, %f\n", q[0] + q[1], q[2] + q[3]);
+            fprintf(conv, "%i,", ii + k);
+            for (int v = 0; v < nv; v++) fprintf(conv, " %+.4e,", r[v]);
+            fprintf(conv, " %e, %e, %e, %e,", q[0], q[1], q[2], q[3]);
+            fprintf(conv, "\n");
+        }
+    }
+    fclose(conv);
+    free(res); free(coef);
+
+    /* friction.csv is created (empty for inviscid runs), solverCalcCoeff2 */
+    snprintf(path, sizeof path, "%sfriction.csv", wd);
+    FILE* fr = fopen(path, "w");
+    if (fr) fclose(fr);
+
+    rc = csu_dev_get_state(d, c->U);
+    if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }
+    printf("main: saving the solution.\n");
+    snprintf(path, sizeof path, "%ssolution.csv", wd);
+    csu_write_solution(c, c->U, path);
+    printf("main: saving the restart file.\n");
+    snprintf(path, sizeof path, "%srestart.csv", wd);
+    csu_write_restart(c, c->U, path);
+
+    csu_dev_destroy(d);
+    csu_case_free(c);
+    gettimeofday(&stop, NULL);
+    printf("\nDuration %f s\n", seconds(start, stop));
+    return 0;
+}

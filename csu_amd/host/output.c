@@ -1,0 +1,113 @@
+/*
+ * output.c -- restart.csv and solution.csv, in the reference's formats.
+ *
+ *   restart.csv  "1,\n" "0, Nelem, Nvar,\n" then one row per cell of
+ *                "%.10e, " values (reference solver.c:327-355); read back with
+ *                the comma/newline tokenizer of readTables.c:5-142.
+ *   solution.csv "0, Np, Nvar,\n" then per node the inverse-distance average
+ *                of the surrounding cells' U (reference solver.c:222-325),
+ *                velocity zeroed on wall nodes for viscous cases.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "csu_host.h"
+
+int csu_write_restart(const csu_case* c, const double* U, const char* path)
+{
+    FILE* f = fopen(path, "w");
+    if (!f) return CSU_ERR_IO;
+    int nc = c->mesh.ncell, nv = c->prm.nvar;
+    fprintf(f, "1,\n");
+    fprintf(f, "0, %i, %i,\n", nc, nv);
+    for (int i = 0; i < nc; i++) {
+        for (int k = 0; k < nv; k++) fprintf(f, "%.10e, ", U[(size_t)k * nc + i]);
+        fprintf(f, "\n");
+    }
+    fclose(f);
+    return CSU_OK;
+}
+
+/* readTables.c getWord: a token ends at ',' or '\n' */
+static int rt_word(FILE* f, char* s, int cap)
+{
+    int n = 0, ch;
+    for (;;) {
+        ch = fgetc(f);
+        if (ch == EOF || ch == ',' || ch == '\n') break;
+        if (n < cap - 1) s[n++] = (char)ch;
+    }
+    s[n] = '\0';
+    return ch;
+}
+
+int csu_read_restart(const csu_case* c, double* U, const char* path)
+{
+    FILE* f = fopen(path, "r");
+    if (!f) return CSU_ERR_IO;
+    char s[128];
+    int nc = c->mesh.ncell, nv = c->prm.nvar;
+    rt_word(f, s, sizeof s);                 /* number of tables */
+    if (atoi(s) < 1) { fclose(f); return CSU_ERR_IO; }
+    rt_word(f, s, sizeof s);
+    rt_word(f, s, sizeof s);                 /* table number */
+    rt_word(f, s, sizeof s); int nrow = atoi(s);
+    rt_word(f, s, sizeof s); int ncol = atoi(s);
+    rt_word(f, s, sizeof s);
+    if (nrow < nc || ncol < nv) { fclose(f); return CSU_ERR_IO; }
+    for (int i = 0; i < nc; i++) {
+        for (int k = 0; k < ncol; k++) {
+            if (rt_word(f, s, sizeof s) == EOF && k < ncol - 1) { fclose(f); return CSU_ERR_IO; }
+            if (k < nv) U[(size_t)k * nc + i] = strtod(s, NULL);
+        }
+        rt_word(f, s, sizeof s);
+    }
+    fclose(f);
+    return CSU_OK;
+}
+
+int csu_write_solution(const csu_case* c, const double* U, const char* path)
+{
+    const csu_mesh_desc* m = &c->mesh;
+    int nv = c->prm.nvar, np = m->nnode, nc = m->ncell;
+    double* Up = calloc((size_t)nv * (size_t)np, sizeof(double));
+    double* den = calloc((size_t)np, sizeof(double));
+    if (!Up || !den) { free(Up); free(den); return CSU_ERR_NOMEM; }
+    for (int i = 0; i < nc; i++) {
+        const int32_t* v = m->cell_vert + 4 * i;
+        int nvert = v[3] >= 0 ? 4 : 3;
+        double xc = 0.0, yc = 0.0;
+        for (int k = 0; k < nvert; k++) { xc += m->node_xy[2 * v[k]]; yc += m->node_xy[2 * v[k] + 1]; }
+        xc /= nvert; yc /= nvert;
+        for (int j = 0; j < nvert; j++) {
+            int p = v[j];
+            double xp = m->node_xy[2 * p], yp = m->node_xy[2 * p + 1];
+            double L = sqrt((xp - xc) * (xp - xc) + (yp - yc) * (yp - yc));
+            for (int k = 0; k < nv; k++) Up[(size_t)k * np + p] += U[(size_t)k * nc + i] / L;
+            den[p] += 1 / L;
+        }
+    }
+    for (int p = 0; p < np; p++)
+        if (den[p] != 0)
+            for (int k = 0; k < nv; k++) Up[(size_t)k * np + p] /= den[p];
+    if (c->prm.laminar || c->prm.sa || c->prm.sst) {
+        for (int g = 0; g < m->nbface; g++) {
+            int t = m->mark_type[m->bface_mark[g]];
+            if (t != CSU_BC_WALL && t != CSU_BC_WALLT) continue;
+            int p0 = m->bface[3 * g + 1], p1 = m->bface[3 * g + 2];
+            Up[(size_t)1 * np + p0] = 0.0; Up[(size_t)2 * np + p0] = 0.0;
+            Up[(size_t)1 * np + p1] = 0.0; Up[(size_t)2 * np + p1] = 0.0;
+        }
+    }
+    FILE* f = fopen(path, "w");
+    if (!f) { free(Up); free(den); return CSU_ERR_IO; }
+    fprintf(f, "0, %i, %i,\n", np, nv);
+    for (int p = 0; p < np; p++) {
+        for (int k = 0; k < nv; k++) fprintf(f, "%.10e, ", Up[(size_t)k * np + p]);
+        fprintf(f, "\n");
+    }
+    fclose(f);
+    free(Up); free(den);
+    return CSU_OK;
+}

@@ -1,0 +1,277 @@
+"""ctypes binding of libcsu.so (include/csu.h, include/csu_host.h).
+
+Python here is plumbing for tests and benchmarks: it loads the in-tree
+library, fills the C structs and moves numpy arrays across the C ABI.  There
+is no Python or CPU fallback for any device call: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "lib", "libcsu.so")
+EXE = os.path.join(PKG, "lib", "executable")
+
+CSU_OK, CSU_ERR_ARG, CSU_ERR_HIP, CSU_ERR_UNSUPPORTED, CSU_ERR_NOMEM, CSU_ERR_IO, CSU_ERR_INPUT = range(7)
+
+i32p = C.POINTER(C.c_int32)
+f64p = C.POINTER(C.c_double)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("ncell", C.c_int32), ("nnode", C.c_int32), ("nface", C.c_int32),
+                ("nbface", C.c_int32), ("nmark", C.c_int32), ("axi", C.c_int32),
+                ("node_xy", f64p), ("cell_vert", i32p), ("face", i32p), ("bface", i32p),
+                ("bface_mark", i32p), ("mark_type", i32p), ("cell_nbr", i32p),
+                ("cell_face", i32p), ("wall_dist", f64p)]
+
+
+_PARAM_INTS = ["nvar", "order", "flux", "stages", "time_scheme", "dt_local", "dt_local_n",
+               "laminar", "sa", "sst", "sst_trans", "tp", "limiter", "turb1order",
+               "visc_blazek", "nlinear", "precision", "reserved"]
+_PARAM_DBLS = ["cfl", "rlim", "pout", "twall", "lim_k", "efix", "pr", "pr_t", "sref", "wimp",
+               "turb_ratio", "gamma", "gas_r", "mach_inf"]
+
+
+class Params(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in _PARAM_INTS] + [(n, C.c_double) for n in _PARAM_DBLS] +
+                [("pin", C.c_double * 7), ("uin", C.c_double * 6),
+                 ("sst_kfactor", C.c_double), ("sst_ofactor", C.c_double),
+                 ("sst_owallfactor", C.c_double), ("sst_l", C.c_double)])
+
+    def as_dict(self) -> dict:
+        d = {n: getattr(self, n) for n in _PARAM_INTS + _PARAM_DBLS}
+        d["pin"] = list(self.pin)
+        d["uin"] = list(self.uin)
+        return d
+
+
+class Case(C.Structure):
+    _fields_ = [("mesh", MeshDesc), ("prm", Params), ("U", f64p), ("nmax", C.c_int32),
+                ("threads", C.c_int32), ("restart", C.c_int32), ("tube", C.c_int32),
+                ("write_surf", C.c_char * 64), ("wd", C.c_char * 1024),
+                ("mark_name", C.c_void_p),
+                ("node_xy", f64p), ("cell_vert", i32p), ("face", i32p), ("bface", i32p),
+                ("bface_mark", i32p), ("mark_type", i32p), ("cell_nbr", i32p), ("cell_face", i32p),
+                ("wall_dist", f64p), ("input", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcsu.so (build it first with csu_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: run `python -m csu_amd.build` first")
+    L = C.CDLL(LIBPATH)
+    L.csu_last_error.restype = C.c_char_p
+    L.csu_version.restype = C.c_char_p
+    L.csu_dev_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]
+    L.csu_dev_destroy.argtypes = [C.c_void_p]
+    L.csu_dev_destroy.restype = None
+    L.csu_dev_set_state.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_get_state.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_get_dt.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_residual.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p]
+    L.csu_dev_iterate.argtypes = [C.c_void_p, C.c_int32, f64p, f64p]
+    L.csu_dev_iteration.argtypes = [C.c_void_p]
+    L.csu_dev_sync.argtypes = [C.c_void_p]
+    L.csu_dev_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    L.csu_dev_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_char_p), f64p, i32p, C.c_int32]
+    L.csu_case_load.argtypes = [C.c_char_p, C.POINTER(C.POINTER(Case))]
+    L.csu_case_free.argtypes = [C.POINTER(Case)]
+    L.csu_case_free.restype = None
+    L.csu_host_missing_key.restype = C.c_char_p
+    L.csu_input_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    L.csu_input_free.argtypes = [C.c_void_p]
+    L.csu_input_free.restype = None
+    L.csu_input_count.argtypes = [C.c_void_p]
+    L.csu_input_get.argtypes = [C.c_void_p, C.c_char_p]
+    L.csu_input_get.restype = C.c_char_p
+    L.csu_input_name.argtypes = [C.c_void_p, C.c_int]
+    L.csu_input_name.restype = C.c_char_p
+    L.csu_input_value.argtypes = [C.c_void_p, C.c_int]
+    L.csu_input_value.restype = C.c_char_p
+    L.csu_write_restart.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
+    L.csu_read_restart.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
+    L.csu_write_solution.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
+    L.csu_wall_distance.argtypes = [C.POINTER(Case)]
+    _lib = L
+    return L
+
+
+class CsuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"csu error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != CSU_OK:
+        raise CsuError(rc, lib().csu_last_error().decode(errors="replace"))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def read_input(path: str) -> list:
+    L = lib()
+    h = C.c_void_p()
+    check(L.csu_input_read(path.encode(), C.byref(h)))
+    try:
+        return [(L.csu_input_name(h, i).decode(), L.csu_input_value(h, i).decode())
+                for i in range(L.csu_input_count(h))]
+    finally:
+        L.csu_input_free(h)
+
+
+class HostCase:
+    """A case directory loaded by the host C (csu_case_load)."""
+
+    def __init__(self, wd: str):
+        if not wd.endswith("/"):
+            wd += "/"
+        self.wd = wd
+        L = lib()
+        self._c = C.POINTER(Case)()
+        rc = L.csu_case_load(wd.encode(), C.byref(self._c))
+        if rc == CSU_ERR_INPUT:
+            raise CsuError(rc, "input.dat: " + L.csu_host_missing_key().decode())
+        check(rc)
+        c = self._c.contents
+        m = c.mesh
+        self.ncell, self.nnode, self.nface, self.nbface, self.nmark = (
+            m.ncell, m.nnode, m.nface, m.nbface, m.nmark)
+        self.nvar = c.prm.nvar
+        self.nmax = c.nmax
+        self.params = c.prm
+
+    def _arr(self, ptr, n, dtype):
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+    @property
+    def mesh(self) -> MeshDesc:
+        return self._c.contents.mesh
+
+    def arrays(self) -> dict:
+        m = self._c.contents.mesh
+        out = {
+            "node_xy": self._arr(m.node_xy, 2 * m.nnode, np.float64).reshape(-1, 2),
+            "cell_vert": self._arr(m.cell_vert, 4 * m.ncell, np.int32).reshape(-1, 4),
+            "face": self._arr(m.face, 4 * m.nface, np.int32).reshape(-1, 4) if m.nface else np.zeros((0, 4), np.int32),
+            "bface": self._arr(m.bface, 3 * m.nbface, np.int32).reshape(-1, 3),
+            "bface_mark": self._arr(m.bface_mark, m.nbface, np.int32),
+            "mark_type": self._arr(m.mark_type, max(m.nmark, 1), np.int32)[:m.nmark],
+            "cell_nbr": self._arr(m.cell_nbr, 4 * m.ncell, np.int32).reshape(-1, 4),
+            "cell_face": self._arr(m.cell_face, 4 * m.ncell, np.int32).reshape(-1, 4),
+        }
+        if m.wall_dist:
+            out["wall_dist"] = self._arr(m.wall_dist, m.ncell, np.float64)
+        return out
+
+    def initial_state(self) -> np.ndarray:
+        c = self._c.contents
+        return self._arr(c.U, self.nvar * self.ncell, np.float64).reshape(self.nvar, self.ncell)
+
+    def write_restart(self, U: np.ndarray, path: str):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        check(lib().csu_write_restart(self._c, _p(U, C.c_double), path.encode()))
+
+    def read_restart(self, path: str) -> np.ndarray:
+        U = np.zeros((self.nvar, self.ncell))
+        check(lib().csu_read_restart(self._c, _p(U, C.c_double), path.encode()))
+        return U
+
+    def write_solution(self, U: np.ndarray, path: str):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        check(lib().csu_write_solution(self._c, _p(U, C.c_double), path.encode()))
+
+    def close(self):
+        if self._c:
+            lib().csu_case_free(self._c)
+            self._c = C.POINTER(Case)()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Device:
+    """One mesh resident on the GPU (csu_dev_*)."""
+
+    def __init__(self, case: HostCase, params: Params | None = None):
+        L = lib()
+        self.case = case
+        self.prm = params if params is not None else case.params
+        self.nvar, self.ncell, self.nbface = self.prm.nvar, case.ncell, case.nbface
+        self._h = C.c_void_p()
+        check(L.csu_dev_create(C.byref(case.mesh), C.byref(self.prm), C.byref(self._h)))
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nvar, self.ncell)
+        check(lib().csu_dev_set_state(self._h, _p(U, C.c_double)))
+
+    def get_state(self) -> np.ndarray:
+        U = np.empty((self.nvar, self.ncell))
+        check(lib().csu_dev_get_state(self._h, _p(U, C.c_double)))
+        return U
+
+    def get_dt(self) -> np.ndarray:
+        d = np.empty(self.ncell)
+        check(lib().csu_dev_get_dt(self._h, _p(d, C.c_double)))
+        return d
+
+    def residual(self) -> dict:
+        nv, nc, nb = self.nvar, self.ncell, self.nbface
+        out = {k: np.empty((nv, nc)) for k in ("R", "dPx", "dPy", "phi")}
+        out["P"] = np.empty((nc, nv + 1))
+        out["Pb"] = np.empty((max(nb, 1), nv + 1))
+        check(lib().csu_dev_residual(self._h, *[_p(out[k], C.c_double) for k in ("R", "dPx", "dPy", "phi", "P", "Pb")]))
+        out["Pb"] = out["Pb"][:nb]
+        return out
+
+    def iterate(self, n: int, want=True):
+        res = np.empty((n, self.nvar)) if want else None
+        coef = np.empty((n, 4)) if want else None
+        check(lib().csu_dev_iterate(self._h, n, _p(res, C.c_double) if want else None,
+                                    _p(coef, C.c_double) if want else None))
+        return res, coef
+
+    def sync(self):
+        check(lib().csu_dev_sync(self._h))
+
+    def set_profiling(self, on: bool):
+        check(lib().csu_dev_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        names = (C.c_char_p * 64)()
+        ms = np.zeros(64)
+        cnt = np.zeros(64, np.int32)
+        n = lib().csu_dev_kernel_times(self._h, names, _p(ms, C.c_double), _p(cnt, C.c_int32), 64)
+        if n < 0:
+            check(-n)
+        return {names[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    @property
+    def iteration(self) -> int:
+        return lib().csu_dev_iteration(self._h)
+
+    def close(self):
+        if self._h:
+            lib().csu_dev_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,136 @@
+/*
+ * csu.h -- C ABI of the MI355X residual-and-update path (libcsu.so).
+ *
+ * The library replaces the hot loop of the reference solver: everything
+ * below solverSolve's time loop (reference solver.c:1864-1907).  The host
+ * keeps the reference's process contract (input.dat, mesh.su2, restart.csv,
+ * solution.csv, convergence.csv); this header is the thin layer between
+ * that host C and the HIP kernels.  Plain C types only; every call returns
+ * CSU_OK (0) or a CSU_ERR_* code and leaves a message for csu_last_error().
+ * Unlike the reference (printf + exit(0), e.g. input.c:156-160,
+ * solver.c:1973-1977) nothing here terminates the process.
+ *
+ * Layout conventions (reference solver.h:86-170, mesh.h:2-52):
+ *   state U, residual R, gradients, limiter   SoA [nvar][ncell] double
+ *   interior faces (reference mesh->con)      [nface][4] = e0 e1 p0 p1
+ *   per-cell signed face ids (ELEMENT->f)     [ncell][4], 1-based, 0 = none
+ *   per-cell neighbour list (ELEMENT->neiL)   [ncell][4], >= 0 cell id,
+ *                                             < 0 boundary face -(1+g)
+ * Variable order: rho, rho*u, rho*v, rho*E [, rho*nu~ | rho*k, rho*omega].
+ */
+#ifndef CSU_H
+#define CSU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSU_OK              0
+#define CSU_ERR_ARG         1   /* bad argument / inconsistent mesh       */
+#define CSU_ERR_HIP         2   /* HIP runtime failure                    */
+#define CSU_ERR_UNSUPPORTED 3   /* option combination not implemented     */
+#define CSU_ERR_NOMEM       4
+#define CSU_ERR_IO          5   /* host file input/output                 */
+#define CSU_ERR_INPUT       6   /* input.dat key missing or invalid       */
+
+/* boundary types, reference boundary.c:479-511 */
+enum { CSU_BC_SYMMETRY = 0, CSU_BC_INLET = 1, CSU_BC_OUTLET = 2,
+       CSU_BC_WALL = 3, CSU_BC_WALLT = 4 };
+/* convective fluxes, reference flux.c:14-46 */
+enum { CSU_FLUX_ROE = 0, CSU_FLUX_AUSM = 1, CSU_FLUX_AUSMDV = 2,
+       CSU_FLUX_AUSMPUP = 3, CSU_FLUX_AUSMPUP2 = 4 };
+/* time schemes, reference solver.c:1957-1981 */
+enum { CSU_TIME_RK = 0, CSU_TIME_LUSGS = 1, CSU_TIME_DPLUR = 2 };
+/* arithmetic of the residual path */
+enum { CSU_PREC_FP64 = 0,   /* everything FP64 (parity path)            */
+       CSU_PREC_MIXED = 1 };/* FP32 reconstruction/gradient/limiter/flux,
+                               FP64 state, accumulation, update, norms   */
+
+/* Mesh after the reference's preprocessing (mesh.c:115-266): cells CCW,
+ * interior faces in the reference's enumeration order, boundary faces
+ * oriented CCW with respect to their cell.  Built by csu_case_load(). */
+typedef struct csu_mesh_desc {
+    int32_t ncell, nnode, nface, nbface, nmark, axi;
+    const double*  node_xy;     /* [nnode][2]                              */
+    const int32_t* cell_vert;   /* [ncell][4] vertex ids, -1 pads triangles */
+    const int32_t* face;        /* [nface][4] e0 e1 p0 p1                   */
+    const int32_t* bface;       /* [nbface][3] e0 p0 p1                     */
+    const int32_t* bface_mark;  /* [nbface] marker index                    */
+    const int32_t* mark_type;   /* [nmark] CSU_BC_*                         */
+    const int32_t* cell_nbr;    /* [ncell][4] see above; unused slots INT32_MAX */
+    const int32_t* cell_face;   /* [ncell][4] see above                     */
+    const double*  wall_dist;   /* [ncell] wall distance (SA/SST) or NULL   */
+} csu_mesh_desc;
+
+/* Solver options: the input.dat keys of the reference (solver.c:1403-1658,
+ * 1748-1798) after defaults, plus the free-stream state conditionState
+ * derives from them (solver.c:40-89). */
+typedef struct csu_params {
+    int32_t nvar;          /* 4 Euler/laminar, 5 SA, 6 SST                */
+    int32_t order;         /* 1 or 2                                      */
+    int32_t flux;          /* CSU_FLUX_*                                  */
+    int32_t stages;        /* RK stages 1,3,4,5 (2 is a no-op, as in ref) */
+    int32_t time_scheme;   /* CSU_TIME_*                                  */
+    int32_t dt_local;      /* 1: local time step                          */
+    int32_t dt_local_n;    /* iteration after which dt_local switches off */
+    int32_t laminar, sa, sst, sst_trans, tp;
+    int32_t limiter;       /* 0 or 1 (limiter.c:45-94)                    */
+    int32_t turb1order, visc_blazek, nlinear;
+    int32_t precision;     /* CSU_PREC_*                                  */
+    int32_t reserved;
+    double cfl, rlim, pout, twall, lim_k, efix, pr, pr_t, sref, wimp, turb_ratio;
+    double gamma, gas_r;
+    double mach_inf;       /* inlet Mach (AUSM+up reference Mach)         */
+    double pin[7];         /* inlet primitive state r u v p T [nu~ | k om] */
+    double uin[6];         /* inlet conservative state                    */
+    double sst_kfactor, sst_ofactor, sst_owallfactor, sst_l;
+} csu_params;
+
+typedef struct csu_dev csu_dev;
+
+/* Upload mesh + options, allocate all device state on the current device.
+ * The initial state is the free stream prm->uin. */
+int  csu_dev_create(const csu_mesh_desc* mesh, const csu_params* prm, csu_dev** out);
+void csu_dev_destroy(csu_dev* dev);
+
+/* Host <-> device state, SoA [nvar][ncell] FP64. */
+int  csu_dev_set_state(csu_dev* dev, const double* U);
+int  csu_dev_get_state(csu_dev* dev, double* U);
+
+/* One residual evaluation on the current state: reference solverCalcR
+ * (solver.c:685-739), including the positivity clamps it writes back into U.
+ * Any output pointer may be NULL.  R, dPx, dPy, phi: [nvar][ncell];
+ * P: [ncell][nvar+1] primitive; Pb: [nbface][nvar+1] boundary ghost states. */
+int  csu_dev_residual(csu_dev* dev, double* R, double* dPx, double* dPy,
+                      double* phi, double* P, double* Pb);
+
+/* niter iterations of the solverSolve loop body (solver.c:1866-1889),
+ * with the residual norm (solverCalcRes, solver.c:867-892) and the force
+ * coefficients (solverCalcCoeff3, solver.c:1342-1401) after each one.
+ * res: [niter][nvar] or NULL; coef: [niter][4] = Cx_p Cx_v Cy_p Cy_v or NULL.
+ * Iterations are counted across calls (the dt_local_n switch uses it). */
+int  csu_dev_iterate(csu_dev* dev, int32_t niter, double* res, double* coef);
+
+/* Iterations done so far; local time step [ncell] of the last iteration. */
+int  csu_dev_iteration(const csu_dev* dev);
+int  csu_dev_get_dt(csu_dev* dev, double* dtl);
+
+/* Block until all queued device work of this context is done. */
+int  csu_dev_sync(csu_dev* dev);
+
+/* Average duration (ms) of each kernel of the last csu_dev_iterate call,
+ * measured with HIP events on the library's stream when profiling is on.
+ * names: receives up to max entries of static strings. Returns the count. */
+int  csu_dev_set_profiling(csu_dev* dev, int32_t on);
+int  csu_dev_kernel_times(csu_dev* dev, const char** names, double* avg_ms,
+                          int32_t* launches, int32_t max);
+
+const char* csu_last_error(void);
+const char* csu_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CSU_H */

@@ -1,0 +1,878 @@
+/*
+ * csu_oracle.c -- CPU restatement of the reference's residual-and-update
+ * path, used as the parity oracle for the HIP kernels.
+ *
+ * TEST INFRASTRUCTURE: loaded only by tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py, never by the product.
+ *
+ * Written from the reference's algorithm, function by function (citations
+ * are file:line in the reference), with its operation order and its x87
+ * long double least-squares sums, so that on x86-64 it reproduces the
+ * reference's FP64 results bit for bit.  It is pinned against the reference
+ * built from source (oracle/_ref/csu_ref_trace) by the tests.  Data are SoA
+ * over the mesh description of csu.h instead of the reference's per-element
+ * structs.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "csu_oracle.h"
+
+typedef struct {
+    double gamma, R, Cp, Cv, cc[6];
+    int tp;
+} ogas;
+
+struct csu_oracle {
+    int nc, nf, nb, nv, nn, axi, iter, dt_local;
+    csu_params prm;
+    ogas gas;
+    double *xy;
+    int *cv, *face, *bface, *btype, *nbr, *cface;
+    double *omega, *U, *Uaux, *R, *dPx, *dPy, *phi, *flux, *dtL, *P, *Pb;
+    double dt, volMax, Pref20[6];
+};
+
+/* ---------------- gas model: gasprop.c:44-274 ---------------- */
+static void gas_init(ogas* g, double gamma, double R, int tp)
+{
+    g->gamma = gamma; g->R = R;
+    g->Cp = g->gamma * g->R / (g->gamma - 1);     /* gasprop.c:50 */
+    g->Cv = g->Cp - g->R;
+    g->tp = tp;
+    g->cc[0] = 1.81716e3; g->cc[1] = 9.890504e2 - 287.0530; g->cc[2] = -9.595592e-3;
+    g->cc[3] = 1.041469e-4; g->cc[4] = -4.433065e-8; g->cc[5] = 5.879263e-12;
+}
+
+static double gas_T2e(const ogas* g, double T)           /* gasprop.c:77-99 */
+{
+    if (!g->tp) return g->Cv * T;
+    double P = 1, a = g->cc[0];
+    for (int i = 1; i < 6; i++) { P *= T; a += g->cc[i] * P; }
+    return a;
+}
+
+static double gas_T2Cv(const ogas* g, double T)          /* gasprop.c:101-124 */
+{
+    if (!g->tp) return g->Cv;
+    double P = 1, a = g->cc[1];
+    for (int i = 2; i < 6; i++) { P *= T; a += i * g->cc[i] * P; }
+    return a;
+}
+
+static void gas_T2eCv(const ogas* g, double T, double* e, double* Cv)   /* gasprop.c:126-142 */
+{
+    double P = 1;
+    *e = g->cc[0]; *Cv = g->cc[1];
+    for (int i = 1; i < 6; i++) {
+        P *= T;
+        *e += g->cc[i] * P;
+        if (i + 1 < 6) *Cv += (i + 1) * g->cc[i + 1] * P;
+    }
+}
+
+static double gas_e2T(const ogas* g, double e)           /* gasprop.c:144-166 */
+{
+    double T = e / g->Cv;
+    if (g->tp) {
+        double err = 1, e1, Cv;
+        int i = 0;
+        while ((fabs(err) > 1e-14) && (i < 8)) {
+            gas_T2eCv(g, T, &e1, &Cv);
+            err = (e - e1) / Cv;
+            T += err;
+            i++;
+        }
+    }
+    return T;
+}
+
+static double gas_T2c(const ogas* g, double T)           /* gasprop.c:168-184 */
+{
+    double Cv = gas_T2Cv(g, T);
+    return sqrt(((Cv + g->R) / Cv) * g->R * T);
+}
+
+static double gas_critic_H2c(const ogas* g, double H)    /* gasprop.c:219-274 */
+{
+    double c2 = 2 * (g->gamma - 1) * H / (g->gamma + 1);
+    if (!g->tp) return sqrt(c2);
+    double T = c2 / (g->gamma * g->R), e, Cv, H0, dH, err;
+    int i = 0;
+    for (int first = 1;; first = 0) {
+        if (!first && !((fabs(err) > 1.0e-12) && (i < 3))) break;
+        gas_T2eCv(g, T, &e, &Cv);
+        double RT = g->R * T, gam = (Cv + g->R) / Cv;
+        H0 = e + RT + gam * RT * 0.5;
+        dH = Cv + g->R + gam * g->R * 0.5;
+        err = (H - H0) / dH;
+        T += err;
+        if (!first) i++;
+    }
+    return gas_T2c(g, T);
+}
+
+/* ---------------- geometry: mesh.c:371-441, 720-740 ---------------- */
+static int nvert(const csu_oracle* o, int i) { return o->cv[4 * i + 3] >= 0 ? 4 : 3; }
+
+static void cell_center(const csu_oracle* o, int i, double* x, double* y)
+{
+    int n = nvert(o, i);
+    *x = 0.0; *y = 0.0;
+    for (int k = 0; k < n; k++) { *x += o->xy[2 * o->cv[4 * i + k]]; *y += o->xy[2 * o->cv[4 * i + k] + 1]; }
+    *x /= n; *y /= n;
+}
+
+static void bface_center(const csu_oracle* o, int g, double* x, double* y)
+{
+    int p0 = o->bface[3 * g + 1], p1 = o->bface[3 * g + 2];
+    *x = o->xy[2 * p0] + o->xy[2 * p1]; *y = o->xy[2 * p0 + 1] + o->xy[2 * p1 + 1];
+    *x /= 2; *y /= 2;
+}
+
+/* centre of neighbour slot j of cell i (interior cell or boundary element) */
+static void nbr_center(const csu_oracle* o, int i, int j, double* x, double* y)
+{
+    int n = o->nbr[4 * i + j];
+    if (n >= 0) cell_center(o, n, x, y); else bface_center(o, -1 - n, x, y);
+}
+
+static void face_ds(const csu_oracle* o, int p0, int p1, double* dSx, double* dSy)
+{
+    double x0 = o->xy[2 * p0], y0 = o->xy[2 * p0 + 1], x1 = o->xy[2 * p1], y1 = o->xy[2 * p1 + 1];
+    *dSx = y1 - y0;
+    *dSy = -(x1 - x0);
+    double y = (y0 + y1) / 2;
+    if (o->axi == 1) { *dSx = (*dSx) * y; *dSy = (*dSy) * y; }
+}
+
+static double tri_omega(const csu_oracle* o, int a, int b, int c)
+{
+    double x1 = o->xy[2 * a], x2 = o->xy[2 * b], x3 = o->xy[2 * c];
+    double y1 = o->xy[2 * a + 1], y2 = o->xy[2 * b + 1], y3 = o->xy[2 * c + 1];
+    return 0.5 * ((x1 - x2) * (y1 + y2) + (x2 - x3) * (y2 + y3) + (x3 - x1) * (y3 + y1));
+}
+
+static double cell_omega(const csu_oracle* o, int i)
+{
+    const int* v = o->cv + 4 * i;
+    double a = tri_omega(o, v[0], v[1], v[2]);
+    if (nvert(o, i) == 4) a += tri_omega(o, v[2], v[3], v[0]);
+    if (o->axi == 1) { double x, y; cell_center(o, i, &x, &y); a *= y; }
+    return a;
+}
+
+static void rotation(double* U, double dSx, double dSy, double dS)   /* solver.c:441-448 */
+{
+    double aux = (U[1] * dSx + U[2] * dSy) / dS;
+    U[2] = (-U[1] * dSy + U[2] * dSx) / dS;
+    U[1] = aux;
+}
+
+/* ---------------- convective fluxes: flux.c ---------------- */
+static void entropy_fix(double efix, double* l)          /* flux.c:48-57 */
+{
+    if ((*l < efix) && (*l > -efix)) *l = 0.5 * (*l * *l / efix + efix);
+}
+
+static void flux_roe(const csu_oracle* o, const double* L, const double* Rt, double* f)   /* flux.c:59-129 */
+{
+    const double g1 = o->gas.gamma - 1;
+    double rL = L[0], uL = L[1], vL = L[2], pL = L[3], rR = Rt[0], uR = Rt[1], vR = Rt[2], pR = Rt[3];
+    double U1L = rL * uL, U2L = rL * vL, U3L = pL / g1 + (uL * uL + vL * vL) * rL / 2;
+    double HL = (U3L + pL) / rL;
+    double U1R = rR * uR, U2R = rR * vR, U3R = pR / g1 + (uR * uR + vR * vR) * rR / 2;
+    double HR = (U3R + pR) / rR;
+    double sL = sqrt(rL), sR = sqrt(rR);
+    double ub = (sL * uL + sR * uR) / (sL + sR);
+    double vb = (sL * vL + sR * vR) / (sL + sR);
+    double Hb = (sL * HL + sR * HR) / (sL + sR);
+    double ab = sqrt(g1 * (Hb - (ub * ub + vb * vb) / 2));
+    double l1 = ub - ab, l2 = ub, l4 = ub, l5 = ub + ab;
+    double k1[4] = {1.0, ub - ab, vb, Hb - ub * ab};
+    double k2[4] = {0.0, 0.0, 1.0, vb};
+    double k4[4] = {1.0, ub, vb, 0.5 * (ub * ub + vb * vb)};
+    double k5[4] = {1.0, ub + ab, vb, Hb + ub * ab};
+    double d1 = rR - rL, d2 = U1R - U1L, d3 = U2R - U2L, d5 = U3R - U3L;
+    double a4 = (Hb - (ub * ub + vb * vb)) * d1 + ub * d2 + vb * d3 - d5;
+    a4 /= (ab * ab) / g1;
+    double a2 = d3 - d1 * vb;
+    double a5 = ((d1 - a4) + (d2 - ub * d1) / ab) * 0.5;
+    double a1 = ((d1 - a4) - (d2 - ub * d1) / ab) * 0.5;
+    double FL[4] = {U1L, U1L * uL + pL, U1L * vL, uL * (U3L + pL)};
+    double FR[4] = {U1R, U1R * uR + pR, U1R * vR, uR * (U3R + pR)};
+    entropy_fix(o->prm.efix, &l1); entropy_fix(o->prm.efix, &l2);
+    entropy_fix(o->prm.efix, &l4); entropy_fix(o->prm.efix, &l5);
+    for (int i = 0; i < 4; i++)
+        f[i] = 0.5 * (FR[i] + FL[i] - a1 * fabs(l1) * k1[i] - a2 * fabs(l2) * k2[i]
+                      - a4 * fabs(l4) * k4[i] - a5 * fabs(l5) * k5[i]);
+}
+
+/* enthalpy and temperature from a primitive state (flux.c:138-144 pattern) */
+static void HT(const csu_oracle* o, double r, double u, double v, double p, double* H, double* T)
+{
+    *T = p / (o->gas.R * r);
+    double U3 = gas_T2e(&o->gas, *T) * r + (u * u + v * v) * r / 2;
+    *H = (U3 + p) / r;
+}
+
+/* L/R: r u v p [s] (s = passive scalar, used when ns == 1) */
+static void flux_ausm(const csu_oracle* o, const double* L, const double* Rt, int ns, double* f)  /* flux.c:132-203, 512-585 */
+{
+    double HL, TL, HR, TR;
+    HT(o, L[0], L[1], L[2], L[3], &HL, &TL);
+    HT(o, Rt[0], Rt[1], Rt[2], Rt[3], &HR, &TR);
+    double cL = gas_T2c(&o->gas, TL), cR = gas_T2c(&o->gas, TR);
+    double ML = L[1] / cL, MR = Rt[1] / cR, Mp, Mm_, Pp, Pm;
+    if (fabs(ML) >= 1) { Mp = 0.5 * (ML + fabs(ML)); Pp = Mp / ML; }
+    else { double a = 0.25 * (ML + 1) * (ML + 1); Mp = a; Pp = a * (2 - ML); }
+    if (fabs(MR) >= 1) { Mm_ = 0.5 * (MR - fabs(MR)); Pm = Mm_ / MR; }
+    else { double a = -0.25 * (MR - 1) * (MR - 1); Mm_ = a; Pm = -a * (2 + MR); }
+    double M = Mp + Mm_;
+    double pm = Pp * L[3] + Pm * Rt[3];
+    const double* S = M > 0 ? L : Rt;
+    double c = M > 0 ? cL : cR, H = M > 0 ? HL : HR;
+    f[0] = S[0] * c * M;
+    f[1] = f[0] * S[1] + pm;
+    f[2] = f[0] * S[2];
+    f[3] = f[0] * H;
+    if (ns) f[4] = f[0] * S[4];
+}
+
+static void flux_ausmdv(const csu_oracle* o, const double* L, const double* Rt, int ns, double* f)  /* flux.c:206-287, 588-670 */
+{
+    double rL = L[0], uL = L[1], vL = L[2], pL = L[3], rR = Rt[0], uR = Rt[1], vR = Rt[2], pR = Rt[3];
+    double HL, TL, HR, TR, aux;
+    HT(o, rL, uL, vL, pL, &HL, &TL);
+    HT(o, rR, uR, vR, pR, &HR, &TR);
+    double cL = gas_T2c(&o->gas, TL), cR = gas_T2c(&o->gas, TR);
+    double cm = fmax(cL, cR);
+    double aL = (2.0 * pL / rL) / (pL / rL + pR / rR);
+    double aR = (2.0 * pR / rR) / (pL / rL + pR / rR);
+    double uLp, pLp, uRm, pRm;
+    aux = 0.5 * (uL + fabs(uL));
+    if (fabs(uL) < cm) {
+        uLp = aL * (0.25 * (uL + cm) * (uL + cm) / cm - aux) + aux;
+        pLp = 0.25 * pL * (uL + cm) * (uL + cm) * (2.0 - uL / cm) / (cm * cm);
+    } else { uLp = aux; pLp = pL * aux / uL; }
+    aux = 0.5 * (uR - fabs(uR));
+    if (fabs(uR) < cm) {
+        uRm = aR * (-0.25 * (uR - cm) * (uR - cm) / cm - aux) + aux;
+        pRm = 0.25 * pR * (uR - cm) * (uR - cm) * (2.0 + uR / cm) / (cm * cm);
+    } else { uRm = aux; pRm = pR * aux / uR; }
+    double rU = uLp * rL + uRm * rR;
+    int n = 4 + ns;
+    f[0] = rU;
+    f[1] = pLp + pRm;
+    f[2] = 0.5 * (rU * (vR + vL) - fabs(rU) * (vR - vL));
+    f[3] = 0.5 * (rU * (HR + HL) - fabs(rU) * (HR - HL));
+    if (ns) f[4] = 0.5 * (rU * (Rt[4] + L[4]) - fabs(rU) * (Rt[4] - L[4]));
+    double fD = 0.5 * (rU * (uR + uL) - fabs(rU) * (uR - uL));
+    double fV = uLp * rL * uL + uRm * rR * uR;
+    double s = 0.5 * fmin(1, 10 * fabs(pR - pL) / fmin(pL, pR));
+    f[1] += (0.5 + s) * fV + (0.5 - s) * fD;
+    int A = (uL - cL < 0.0) && (uR - cR > 0.0);
+    int B = (uL + cL < 0.0) && (uR + cR > 0.0);
+    double psL[5] = {1.0, uL, vL, HL, ns ? L[4] : 0.0}, psR[5] = {1.0, uR, vR, HR, ns ? Rt[4] : 0.0};
+    if (A && !B) {
+        aux = 0.125 * ((uR - cR) - (uL - cL));
+        for (int k = 0; k < n; k++) f[k] -= aux * (rR * psR[k] - rL * psL[k]);
+    } else if (!A && B) {
+        aux = 0.125 * ((uR + cR) - (uL + cL));
+        for (int k = 0; k < n; k++) f[k] -= aux * (rR * psR[k] - rL * psL[k]);
+    }
+}
+
+static void flux_ausmpup(const csu_oracle* o, const double* L, const double* Rt, int ns, int v2, double* f)
+{
+    /* flux.c:290-398 (v2 == 0) and 401-508 (v2 == 1); _sa 673-899 */
+    double rL = L[0], uL = L[1], vL = L[2], pL = L[3], rR = Rt[0], uR = Rt[1], vR = Rt[2], pR = Rt[3];
+    double HL, TL, HR, TR;
+    HT(o, rL, uL, vL, pL, &HL, &TL);
+    HT(o, rR, uR, vR, pR, &HR, &TR);
+    double as = gas_critic_H2c(&o->gas, HL);
+    double ahL = as * as / fmax(as, fabs(uL));
+    as = gas_critic_H2c(&o->gas, HR);
+    double ahR = as * as / fmax(as, fabs(uR));
+    double am = fmin(ahL, ahR);
+    double Kp = 0.25, Ku = 0.75, sig = 1.0, beta = 1.0 / 8.0;
+    double ML = uL / am, MR = uR / am;
+    double Mb = sqrt((uL * uL + uR * uR) / (2 * am * am));
+    double Minf = o->prm.mach_inf;
+    double M0 = sqrt(fmin(1, fmax(Mb * Mb, Minf * Minf)));
+    double fa = M0 * (2 - M0);
+    if (fa < 1e-3) fa = 1e-3;
+    double alpha = (3.0 / 16.0) * (-4 + 5 * fa * fa);
+    double rhom = 0.5 * (rL + rR);
+    double Mp, Mn, Pp, Pn, a, b;
+    if (fabs(ML) >= 1) { Mp = 0.5 * (ML + fabs(ML)); Pp = Mp / ML; }
+    else {
+        a = 0.25 * (ML + 1) * (ML + 1); b = -0.25 * (ML - 1) * (ML - 1);
+        Mp = a * (1 - 16 * beta * b);
+        Pp = a * ((2 - ML) - 16 * alpha * ML * b);
+    }
+    if (fabs(MR) >= 1) { Mn = 0.5 * (MR - fabs(MR)); Pn = Mn / MR; }
+    else {
+        a = 0.25 * (MR + 1) * (MR + 1); b = -0.25 * (MR - 1) * (MR - 1);
+        Mn = b * (1 + 16 * beta * a);
+        Pn = b * ((-2 - MR) + 16 * alpha * MR * a);
+    }
+    double Mm = Mp + Mn - (Kp / fa) * fmax(1 - sig * Mb * Mb, 0) * (pR - pL) / (rhom * am * am);
+    double pm;
+    if (!v2) pm = Pp * pL + Pn * pR - Ku * Pp * Pn * (rL + rR) * (fa * am) * (uR - uL);
+    else pm = 0.5 * (pL + pR) + 0.5 * (Pp - Pn) * (pL - pR)
+              + sqrt(0.5 * (uL * uL + vL * vL + uR * uR + vR * vR)) * (Pp + Pn - 1) * 0.5 * (pL + pR) / am;
+    double mm = Mm > 0 ? am * Mm * rL : am * Mm * rR;
+    const double* S = mm > 0 ? L : Rt;
+    double H = mm > 0 ? HL : HR;
+    f[0] = mm; f[1] = mm * S[1] + pm; f[2] = mm * S[2]; f[3] = mm * H;
+    if (ns) f[4] = mm * S[4];
+}
+
+static void flux(const csu_oracle* o, const double* L, const double* Rt, double* f)  /* flux.c:1300-1345 */
+{
+    int ns = o->nv == 5 ? 1 : 0;
+    switch (o->prm.flux) {
+        case 0: flux_roe(o, L, Rt, f); break;
+        case 1: flux_ausm(o, L, Rt, ns, f); break;
+        case 2: flux_ausmdv(o, L, Rt, ns, f); break;
+        case 3: flux_ausmpup(o, L, Rt, ns, 0, f); break;
+        default: flux_ausmpup(o, L, Rt, ns, 1, f); break;
+    }
+}
+
+static void flux_free(const csu_oracle* o, const double* P, double* f)   /* flux.c:1369-1380 */
+{
+    double T = P[3] / (o->gas.R * P[0]);
+    double U3 = gas_T2e(&o->gas, T) * P[0] + (P[1] * P[1] + P[2] * P[2]) * P[0] / 2;
+    double H = (U3 + P[3]) / P[0];
+    f[0] = P[0] * P[1];
+    f[1] = P[0] * P[1] * P[1] + P[3];
+    f[2] = P[0] * P[1] * P[2];
+    f[3] = P[0] * P[1] * H;
+}
+
+/* ---------------- boundary states: boundary.c:17-141 ---------------- */
+static void bc_inlet(const csu_oracle* o, const double* Pa, const double* Pd, double* Pb, double nx, double ny)
+{
+    double T = Pd[3] / Pd[0] / o->gas.R;
+    double c0 = gas_T2c(&o->gas, T);
+    double m = sqrt(Pd[1] * Pd[1] + Pd[2] * Pd[2]) / c0;
+    if (m < 1.) {
+        double pb = 0.5 * (Pa[3] + Pd[3] - Pd[0] * c0 * (nx * (Pa[1] - Pd[1]) + ny * (Pa[2] - Pd[2])));
+        Pb[0] = Pa[0] + (pb - Pa[3]) / (c0 * c0);
+        Pb[1] = Pa[1] - nx * (Pa[3] - pb) / (Pd[0] * c0);
+        Pb[2] = Pa[2] - ny * (Pa[3] - pb) / (Pd[0] * c0);
+        Pb[3] = pb;
+    } else {
+        for (int k = 0; k < 4; k++) Pb[k] = Pa[k];
+    }
+}
+
+static void bc_outlet(const csu_oracle* o, const double* Pd, double* Pb, double nx, double ny)
+{
+    double T = Pd[3] / Pd[0] / o->gas.R;
+    double c0 = gas_T2c(&o->gas, T);
+    double m = sqrt(Pd[1] * Pd[1] + Pd[2] * Pd[2]) / c0;
+    if (m < 1.) {
+        double pb = o->prm.pout;
+        Pb[0] = Pd[0] + (pb - Pd[3]) / (c0 * c0);
+        Pb[1] = Pd[1] + nx * (Pd[3] - pb) / (Pd[0] * c0);
+        Pb[2] = Pd[2] + ny * (Pd[3] - pb) / (Pd[0] * c0);
+        Pb[3] = pb;
+    } else {
+        for (int k = 0; k < 4; k++) Pb[k] = Pd[k];
+    }
+}
+
+static void bc_wall(const csu_oracle* o, const double* Pd, double* Pb, double nx, double ny)
+{
+    double T = Pd[3] / Pd[0] / o->gas.R;
+    double c0 = gas_T2c(&o->gas, T);
+    double pb = Pd[3] + Pd[0] * c0 * (nx * Pd[1] + ny * Pd[2]);
+    Pb[0] = Pd[0] + (pb - Pd[3]) / (c0 * c0);
+    Pb[1] = Pd[1] - nx * (nx * Pd[1] + ny * Pd[2]);
+    Pb[2] = Pd[2] - ny * (nx * Pd[1] + ny * Pd[2]);
+    Pb[3] = pb;
+}
+
+/* ---------------- primitive recovery: solver.c:1153-1220 ---------------- */
+static void calc_primitive(csu_oracle* o, double* U)
+{
+    const int nc = o->nc, nv = o->nv, ns = nv + 1;
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++) {
+        double* P = o->P + (size_t)i * ns;
+        P[0] = fmax(U[i], o->prm.rlim);
+        U[i] = P[0];
+        double k = 0;
+        if (o->prm.sst == 1) {
+            k = U[4 * nc + i] / P[0];
+            if (k < 1e-14) { k = 1e-14; U[4 * nc + i] = P[0] * k; }
+            P[5] = k;
+            double om = U[5 * nc + i] / P[0];
+            if (om <= 0.0) { om = 0.0; U[5 * nc + i] = P[0] * om; }
+            P[6] = om;
+        }
+        double u = U[nc + i] / P[0], v = U[2 * nc + i] / P[0];
+        P[1] = u; P[2] = v;
+        double e = U[3 * nc + i] / P[0] - 0.5 * (u * u + v * v) - k;
+        e = fmax(e, 7.176325e+02);
+        double T = gas_e2T(&o->gas, e);
+        if (T < 1) {
+            T = 1;
+            e = gas_T2e(&o->gas, T);
+            U[3 * nc + i] = (e + 0.5 * (u * u + v * v) + k) * P[0];
+        }
+        P[4] = T;
+        P[3] = o->gas.R * P[4] * P[0];
+        if (o->prm.sa == 1) {
+            P[5] = U[4 * nc + i] / P[0];
+            P[5] = fmax(P[5], 0);
+            U[4 * nc + i] = P[5] * P[0];
+        }
+    }
+    /* boundary ghost states, boundary.c:514-693 (marker order = g order) */
+    for (int g = 0; g < o->nb; g++) {
+        const int e0 = o->bface[3 * g];
+        const double* E = o->P + (size_t)e0 * ns;
+        double* B = o->Pb + (size_t)g * ns;
+        double dSx, dSy, dS, PL[4] = {E[0], E[1], E[2], E[3]}, Q[4];
+        face_ds(o, o->bface[3 * g + 1], o->bface[3 * g + 2], &dSx, &dSy);
+        dS = sqrt(dSx * dSx + dSy * dSy);
+        int t = o->btype[g];
+        if (t == CSU_BC_SYMMETRY) {
+            if (dS > 0) { rotation(PL, dSx, dSy, dS); PL[1] = 0.0; rotation(PL, dSx, -dSy, dS); }
+            else PL[2] = 0.0;
+            for (int k = 0; k < 4; k++) B[k] = PL[k];
+            B[4] = B[3] / (B[0] * o->gas.R);
+        } else if (t == CSU_BC_INLET) {
+            bc_inlet(o, o->prm.pin, PL, Q, dSx / dS, dSy / dS);
+            for (int k = 0; k < 4; k++) B[k] = Q[k];
+            B[4] = B[3] / (B[0] * o->gas.R);
+        } else if (t == CSU_BC_OUTLET) {
+            bc_outlet(o, PL, Q, dSx / dS, dSy / dS);
+            for (int k = 0; k < 4; k++) B[k] = Q[k];
+            B[4] = B[3] / (B[0] * o->gas.R);
+        } else if (o->prm.laminar == 1 || o->prm.sa == 1) {
+            if (t == CSU_BC_WALLT) {
+                B[0] = PL[3] / (o->gas.R * o->prm.twall); B[1] = 0.0; B[2] = 0.0; B[3] = PL[3]; B[4] = o->prm.twall;
+            } else {
+                B[0] = PL[0]; B[1] = 0.0; B[2] = 0.0; B[3] = PL[3]; B[4] = B[3] / (B[0] * o->gas.R);
+            }
+        } else {
+            if (dS > 0) { bc_wall(o, PL, Q, dSx / dS, dSy / dS); for (int k = 0; k < 4; k++) B[k] = Q[k]; }
+            else { PL[2] = 0.0; for (int k = 0; k < 4; k++) B[k] = PL[k]; }
+            B[4] = B[3] / (B[0] * o->gas.R);
+        }
+        if (o->prm.sa == 1) {
+            if (t == CSU_BC_SYMMETRY || t == CSU_BC_OUTLET) B[5] = E[5];
+            else if (t == CSU_BC_INLET) B[5] = o->prm.pin[5];
+            else B[5] = 0.0;
+        }
+    }
+}
+
+/* neighbour j of cell i: primitive slot m */
+static double nbr_P(const csu_oracle* o, int i, int j, int m)
+{
+    int n = o->nbr[4 * i + j], ns = o->nv + 1;
+    return n >= 0 ? o->P[(size_t)n * ns + m] : o->Pb[(size_t)(-1 - n) * ns + m];
+}
+
+/* ---------------- gradients: solver.c:451-474, 1031-1074 ---------------- */
+static void grad_lsq(const csu_oracle* o, int i, int m, double* dUx, double* dUy)
+{
+    double x[5], y[5], u[5], xx, yy;
+    int n = nvert(o, i), ns = o->nv + 1;
+    cell_center(o, i, &xx, &yy);
+    x[0] = xx; y[0] = yy; u[0] = o->P[(size_t)i * ns + m];
+    for (int j = 0; j < n; j++) {
+        nbr_center(o, i, j, &xx, &yy);
+        x[j + 1] = xx; y[j + 1] = yy; u[j + 1] = nbr_P(o, i, j, m);
+    }
+    long double a = 0, b = 0, c = 0, A = 0, B = 0;
+    for (int j = 1; j < n + 1; j++) {
+        a += (x[j] - x[0]) * (x[j] - x[0]);
+        b += (x[j] - x[0]) * (y[j] - y[0]);
+        c += (y[j] - y[0]) * (y[j] - y[0]);
+        A += (x[j] - x[0]) * (u[j] - u[0]);
+        B += (y[j] - y[0]) * (u[j] - u[0]);
+    }
+    *dUx = (c * A - b * B) / (a * c - b * b);
+    *dUy = (-b * A + a * B) / (a * c - b * b);
+}
+
+static void update_grad(csu_oracle* o)
+{
+#pragma omp parallel for
+    for (int i = 0; i < o->nc; i++)
+        for (int k = 0; k < o->nv; k++) {
+            double gx, gy;
+            grad_lsq(o, i, k > 3 ? k + 1 : k, &gx, &gy);
+            o->dPx[(size_t)k * o->nc + i] = gx;
+            o->dPy[(size_t)k * o->nc + i] = gy;
+        }
+}
+
+/* ---------------- limiter: limiter.c:34-147 ---------------- */
+static double limiter_v2(double Ui, double Umin, double Umax, double d2, double ee)
+{
+    double ans, dmax = Umax - Ui, dmin = Umin - Ui;
+    if (d2 == 0) return 1;
+    if (d2 > 0) {
+        ans = (dmax * dmax + ee) * d2 + 2 * d2 * d2 * dmax;
+        ans /= dmax * dmax + 2 * d2 * d2 + dmax * d2 + ee;
+    } else {
+        ans = (dmin * dmin + ee) * d2 + 2 * d2 * d2 * dmin;
+        ans /= dmin * dmin + 2 * d2 * d2 + dmin * d2 + ee;
+    }
+    return ans / d2;
+}
+
+static void limiter_update(csu_oracle* o)
+{
+    const int ns = o->nv + 1;
+    if (o->prm.limiter == 0) {
+        o->volMax = 0.0;
+        for (int i = 0; i < o->nc; i++) o->volMax = fmax(o->omega[i], o->volMax);
+        const double* Pin = o->prm.pin;
+        o->Pref20[0] = Pin[0] * Pin[0];
+        o->Pref20[1] = Pin[1] * Pin[1] + Pin[2] * Pin[2];
+        o->Pref20[2] = o->Pref20[1];
+        o->Pref20[3] = Pin[3] * Pin[3];
+        for (int k = 4; k < o->nv; k++) o->Pref20[k] = Pin[k + 1] * Pin[k + 1];
+    } else {
+        double mn[6], mx[6];
+        for (int j = 0; j < o->nv; j++) {
+            int m = j == 4 ? 5 : j;
+            mn[j] = mx[j] = o->P[m];
+        }
+        for (int i = 1; i < o->nc; i++)
+            for (int j = 0; j < o->nv; j++) {
+                int m = j == 4 ? 5 : j;
+                mn[j] = fmin(mn[j], o->P[(size_t)i * ns + m]);
+                mx[j] = fmax(mx[j], o->P[(size_t)i * ns + m]);
+            }
+        for (int j = 0; j < o->nv; j++) {
+            double a = (mx[j] - mn[j]) * o->prm.lim_k;
+            o->Pref20[j] = a * a;
+        }
+    }
+}
+
+/* ---------------- inter(): solver.c:493-668 ---------------- */
+static void inter(csu_oracle* o)
+{
+    const int nc = o->nc, nf = o->nf, nv = o->nv, ns = nv + 1;
+    if (o->prm.order == 2) {
+        limiter_update(o);
+#pragma omp parallel for
+        for (int i = 0; i < nc; i++) {
+            double x0, y0, pref2[6];
+            cell_center(o, i, &x0, &y0);
+            if (o->prm.limiter == 0) {
+                double a = o->prm.lim_k * sqrt(o->omega[i] / o->volMax);
+                a = a * a * a;
+                for (int k = 0; k < nv; k++) pref2[k] = o->Pref20[k] * a;
+            } else {
+                for (int k = 0; k < nv; k++) pref2[k] = o->Pref20[k];
+            }
+            int n = nvert(o, i);
+            for (int k = 0; k < nv; k++) {
+                int m = k > 3 ? k + 1 : k;
+                double Pi = o->P[(size_t)i * ns + m], mn = Pi, mx = Pi, ph = 0;
+                for (int j = 0; j < n; j++) {
+                    mx = fmax(mx, nbr_P(o, i, j, m));
+                    mn = fmin(mn, nbr_P(o, i, j, m));
+                }
+                for (int j = 0; j < n; j++) {
+                    int p0 = o->cv[4 * i + j], p1 = o->cv[4 * i + (j + 1) % n];
+                    double xm = (o->xy[2 * p0] + o->xy[2 * p1]) * 0.5;
+                    double ym = (o->xy[2 * p0 + 1] + o->xy[2 * p1 + 1]) * 0.5;
+                    double d2 = o->dPx[(size_t)k * nc + i] * (xm - x0) + o->dPy[(size_t)k * nc + i] * (ym - y0);
+                    double p = limiter_v2(Pi, mn, mx, d2, pref2[k]);
+                    ph = j == 0 ? p : fmin(ph, p);
+                }
+                o->phi[(size_t)k * nc + i] = ph;
+            }
+        }
+    }
+#pragma omp parallel for
+    for (int f = 0; f < nf; f++) {
+        const int* q = o->face + 4 * f;
+        int e0 = q[0], e1 = q[1];
+        double dSx, dSy, dS, L[6] = {0}, Rr[6] = {0}, fl[6];
+        face_ds(o, q[2], q[3], &dSx, &dSy);
+        dS = sqrt(dSx * dSx + dSy * dSy);
+        const double* P0 = o->P + (size_t)e0 * ns;
+        const double* P1 = o->P + (size_t)e1 * ns;
+        if (o->prm.order == 2) {
+            double xm = (o->xy[2 * q[2]] + o->xy[2 * q[3]]) * 0.5, ym = (o->xy[2 * q[2] + 1] + o->xy[2 * q[3] + 1]) * 0.5;
+            double x0, y0, x1, y1;
+            cell_center(o, e0, &x0, &y0);
+            cell_center(o, e1, &x1, &y1);
+            for (int k = 0; k < nv; k++) {
+                int m = k > 3 ? k + 1 : k;
+                if (k > 3 && o->prm.turb1order) { L[k] = P0[m]; Rr[k] = P1[m]; continue; }
+                L[k] = P0[m] + (o->dPx[(size_t)k * nc + e0] * (xm - x0) + o->dPy[(size_t)k * nc + e0] * (ym - y0)) * o->phi[(size_t)k * nc + e0];
+                Rr[k] = P1[m] + (o->dPx[(size_t)k * nc + e1] * (xm - x1) + o->dPy[(size_t)k * nc + e1] * (ym - y1)) * o->phi[(size_t)k * nc + e1];
+            }
+        } else {
+            for (int k = 0; k < nv; k++) { int m = k > 3 ? k + 1 : k; L[k] = P0[m]; Rr[k] = P1[m]; }
+        }
+        rotation(L, dSx, dSy, dS);
+        rotation(Rr, dSx, dSy, dS);
+        flux(o, L, Rr, fl);
+        rotation(fl, dSx, -dSy, dS);
+        for (int k = 0; k < nv; k++) o->flux[(size_t)k * nf + f] = fl[k] * dS;
+    }
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < 4; j++) {
+            int f = o->cface[4 * i + j];
+            if (f > 0) for (int k = 0; k < nv; k++) o->R[(size_t)k * nc + i] += o->flux[(size_t)k * nf + f - 1];
+            else if (f < 0) for (int k = 0; k < nv; k++) o->R[(size_t)k * nc + i] -= o->flux[(size_t)k * nf - f - 1];
+        }
+}
+
+/* ---------------- boundary fluxes: boundary.c:143-464 ---------------- */
+static void boundary(csu_oracle* o)
+{
+    const int nc = o->nc, nv = o->nv, ns = nv + 1;
+    for (int g = 0; g < o->nb; g++) {
+        const int e0 = o->bface[3 * g];
+        const double* E = o->P + (size_t)e0 * ns;
+        double dSx, dSy, dS, L[5] = {E[0], E[1], E[2], E[3], nv == 5 ? E[5] : 0.0}, B[5], M[5], fl[5];
+        face_ds(o, o->bface[3 * g + 1], o->bface[3 * g + 2], &dSx, &dSy);
+        dS = sqrt(dSx * dSx + dSy * dSy);
+        int t = o->btype[g];
+        if (t == CSU_BC_SYMMETRY) {
+            rotation(L, dSx, dSy, dS);
+            M[0] = L[0]; M[1] = -L[1]; M[2] = L[2]; M[3] = L[3]; M[4] = L[4];
+            flux(o, L, M, fl);
+        } else if (t == CSU_BC_INLET || t == CSU_BC_OUTLET) {
+            if (t == CSU_BC_INLET) bc_inlet(o, o->prm.pin, L, B, dSx / dS, dSy / dS);
+            else bc_outlet(o, L, B, dSx / dS, dSy / dS);
+            B[4] = t == CSU_BC_INLET ? o->prm.pin[5] : L[4];
+            rotation(L, dSx, dSy, dS);
+            rotation(B, dSx, dSy, dS);
+            flux(o, L, B, fl);
+        } else if (o->prm.laminar == 1 || nv == 5) {
+            rotation(L, dSx, dSy, dS);
+            M[0] = L[0]; M[1] = -L[1]; M[2] = -L[2]; M[3] = L[3]; M[4] = L[4];
+            flux(o, L, M, fl);
+        } else {
+            bc_wall(o, L, B, dSx / dS, dSy / dS);
+            rotation(L, dSx, dSy, dS);
+            rotation(B, dSx, dSy, dS);
+            flux_free(o, B, fl);
+        }
+        rotation(fl, dSx, -dSy, dS);
+        if (dS > 0)
+            for (int k = 0; k < nv; k++) o->R[(size_t)k * nc + e0] += fl[k] * dS;
+    }
+}
+
+/* solverCalcR, solver.c:685-739 (Euler part) */
+static void calc_R(csu_oracle* o, double* U)
+{
+    memset(o->R, 0, sizeof(double) * (size_t)o->nv * o->nc);
+    calc_primitive(o, U);
+    update_grad(o);
+    inter(o);
+    boundary(o);
+}
+
+/* ---------------- time step: solver.c:895-950 ---------------- */
+static double local_time_step(const csu_oracle* o, int i)
+{
+    int n = nvert(o, i), nc = o->nc;
+    double sx = 0., sy = 0.;
+    for (int j = 0; j < n; j++) {
+        double dSx, dSy;
+        face_ds(o, o->cv[4 * i + j], o->cv[4 * i + (j + 1) % n], &dSx, &dSy);
+        sx += fabs(dSx); sy += fabs(dSy);
+    }
+    sx *= 0.5; sy *= 0.5;
+    const double* U = o->U;
+    double k = o->prm.sst ? U[4 * nc + i] / U[i] : 0;
+    double E = U[3 * nc + i] / U[i];
+    double u = U[nc + i] / U[i], v = U[2 * nc + i] / U[i];
+    double aux = (E - (u * u + v * v) / 2 - k);
+    double c = gas_T2c(&o->gas, gas_e2T(&o->gas, aux));
+    double Lx = (fabs(u) + c) * sx, Ly = (fabs(v) + c) * sy;
+    return o->omega[i] / (Lx + Ly);
+}
+
+static void calc_dt(csu_oracle* o)
+{
+    if (o->dt_local == 1) {
+#pragma omp parallel for
+        for (int i = 0; i < o->nc; i++) o->dtL[i] = o->prm.cfl * local_time_step(o, i);
+    } else {
+        double dt = local_time_step(o, 0);
+        for (int i = 1; i < o->nc; i++) {
+            double d = local_time_step(o, i);
+            if (d < dt) dt = d;
+        }
+        o->dt = dt * o->prm.cfl;
+    }
+}
+
+/* ---------------- RK: solver.c:741-840 ---------------- */
+static void rk_stage(csu_oracle* o, double a)
+{
+    double sigma = 0.5 * o->prm.stages;
+    const int nc = o->nc;
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < o->nv; k++) {
+            double dt = o->dt_local == 1 ? o->dtL[i] : o->dt;
+            o->Uaux[(size_t)k * nc + i] = o->U[(size_t)k * nc + i] - dt * sigma * a * o->R[(size_t)k * nc + i] / o->omega[i];
+        }
+}
+
+static void step_rk(csu_oracle* o)
+{
+    static const double a1[] = {1.0}, a3[] = {0.1481, 0.4, 1.0}, a4[] = {0.0833, 0.2069, 0.4265, 1.0},
+                        a5[] = {0.0533, 0.1263, 0.2375, 0.4414, 1.0};
+    int s = o->prm.stages;
+    const double* a = s == 1 ? a1 : s == 3 ? a3 : s == 4 ? a4 : a5;
+    for (int k = 0; k < s; k++) {
+        calc_R(o, k == 0 ? o->U : o->Uaux);
+        rk_stage(o, a[k]);
+    }
+    double* t = o->U; o->U = o->Uaux; o->Uaux = t;
+}
+
+/* ---------------- residual norm and forces ---------------- */
+static void calc_res(const csu_oracle* o, double* res)       /* solver.c:867-892 */
+{
+    for (int k = 0; k < o->nv; k++) res[k] = 0.0;
+    for (int i = 0; i < o->nc; i++)
+        for (int k = 0; k < o->nv; k++) res[k] += o->R[(size_t)k * o->nc + i] * o->R[(size_t)k * o->nc + i];
+    for (int k = 0; k < o->nv; k++) res[k] = sqrt(res[k] / o->nc);
+}
+
+static void calc_coeff(const csu_oracle* o, double* c4)      /* solver.c:1342-1401, pressure part */
+{
+    const double* Pin = o->prm.pin;
+    double q = 0.5 * Pin[0] * (Pin[1] * Pin[1] + Pin[2] * Pin[2]);
+    double cxp = 0, cxv = 0, cyp = 0, cyv = 0;
+    const int ns = o->nv + 1;
+    for (int g = 0; g < o->nb; g++) {
+        int t = o->btype[g];
+        if (t != CSU_BC_WALL && t != CSU_BC_WALLT) continue;
+        double cp = (o->Pb[(size_t)g * ns + 3] - Pin[3]) / q, dSx, dSy;
+        face_ds(o, o->bface[3 * g + 1], o->bface[3 * g + 2], &dSx, &dSy);
+        cxp += cp * dSx;
+        cyp += cp * dSy;
+    }
+    cxp /= o->prm.sref; cxv /= o->prm.sref; cyp /= o->prm.sref; cyv /= o->prm.sref;
+    if (o->axi) { cxp *= 2 * M_PI; cxv *= 2 * M_PI; }
+    c4[0] = cxp; c4[1] = cxv; c4[2] = cyp; c4[3] = cyv;
+}
+
+/* ---------------- API ---------------- */
+int oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+static void* dup(const void* p, size_t n) { void* q = malloc(n ? n : 1); if (n) memcpy(q, p, n); return q; }
+
+int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
+{
+    if (p->laminar || p->sa || p->sst || p->time_scheme != CSU_TIME_RK) return CSU_ERR_UNSUPPORTED;
+    csu_oracle* o = calloc(1, sizeof(*o));
+    o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;
+    o->prm = *p; o->dt_local = p->dt_local;
+    gas_init(&o->gas, p->gamma, p->gas_r, p->tp);
+    const int nc = o->nc, nv = o->nv;
+    o->xy = dup(m->node_xy, sizeof(double) * 2 * (size_t)m->nnode);
+    o->cv = dup(m->cell_vert, sizeof(int) * 4 * (size_t)nc);
+    o->face = dup(m->face, sizeof(int) * 4 * (size_t)o->nf);
+    o->bface = dup(m->bface, sizeof(int) * 3 * (size_t)o->nb);
+    o->nbr = dup(m->cell_nbr, sizeof(int) * 4 * (size_t)nc);
+    o->cface = dup(m->cell_face, sizeof(int) * 4 * (size_t)nc);
+    o->btype = malloc(sizeof(int) * (size_t)(o->nb ? o->nb : 1));
+    for (int g = 0; g < o->nb; g++) o->btype[g] = m->mark_type[m->bface_mark[g]];
+    o->omega = malloc(sizeof(double) * nc);
+    for (int i = 0; i < nc; i++) o->omega[i] = cell_omega(o, i);
+    size_t nvc = (size_t)nv * nc;
+    o->U = malloc(sizeof(double) * nvc); o->Uaux = calloc(nvc, sizeof(double));
+    o->R = calloc(nvc, sizeof(double)); o->dPx = calloc(nvc, sizeof(double));
+    o->dPy = calloc(nvc, sizeof(double)); o->phi = calloc(nvc, sizeof(double));
+    o->flux = calloc((size_t)nv * (o->nf ? o->nf : 1), sizeof(double));
+    o->dtL = calloc(nc, sizeof(double));
+    o->P = calloc((size_t)(nv + 1) * nc, sizeof(double));
+    o->Pb = calloc((size_t)(nv + 1) * (o->nb ? o->nb : 1), sizeof(double));
+    for (int k = 0; k < nv; k++)
+        for (int i = 0; i < nc; i++) o->U[(size_t)k * nc + i] = p->uin[k];
+    *out = o;
+    return CSU_OK;
+}
+
+void oracle_destroy(csu_oracle* o)
+{
+    if (!o) return;
+    free(o->xy); free(o->cv); free(o->face); free(o->bface); free(o->nbr); free(o->cface); free(o->btype);
+    free(o->omega); free(o->U); free(o->Uaux); free(o->R); free(o->dPx); free(o->dPy); free(o->phi);
+    free(o->flux); free(o->dtL); free(o->P); free(o->Pb);
+    free(o);
+}
+
+int oracle_set_state(csu_oracle* o, const double* U)
+{
+    memcpy(o->U, U, sizeof(double) * (size_t)o->nv * o->nc);
+    return CSU_OK;
+}
+
+int oracle_get_state(csu_oracle* o, double* U)
+{
+    memcpy(U, o->U, sizeof(double) * (size_t)o->nv * o->nc);
+    return CSU_OK;
+}
+
+int oracle_get_dt(csu_oracle* o, double* dtl)
+{
+    memcpy(dtl, o->dtL, sizeof(double) * (size_t)o->nc);
+    return CSU_OK;
+}
+
+int oracle_residual(csu_oracle* o, double* R, double* dPx, double* dPy, double* phi, double* P, double* Pb)
+{
+    calc_R(o, o->U);
+    size_t nvc = sizeof(double) * (size_t)o->nv * o->nc;
+    if (R) memcpy(R, o->R, nvc);
+    if (dPx) memcpy(dPx, o->dPx, nvc);
+    if (dPy) memcpy(dPy, o->dPy, nvc);
+    if (phi) memcpy(phi, o->phi, nvc);
+    if (P) memcpy(P, o->P, sizeof(double) * (size_t)(o->nv + 1) * o->nc);
+    if (Pb) memcpy(Pb, o->Pb, sizeof(double) * (size_t)(o->nv + 1) * o->nb);
+    return CSU_OK;
+}
+
+int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
+{
+    for (int it = 0; it < niter; it++) {
+        calc_dt(o);
+        step_rk(o);
+        if (o->iter == o->prm.dt_local_n) o->dt_local = 0;   /* solver.c:1886-1889 */
+        if (res) calc_res(o, res + (size_t)it * o->nv);
+        if (coef) calc_coeff(o, coef + (size_t)it * 4);
+        o->iter++;
+    }
+    return CSU_OK;
+}

@@ -1,0 +1,28 @@
+/*
+ * csu_oracle.h -- CPU restatement of the reference's residual-and-update path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg may load liboracle (oracle/lib/); the product
+ * (libcsu.so) never does.  It takes the same csu_mesh_desc / csu_params as
+ * the device library and restates, in plain C with x87 long double where the
+ * reference uses it, the reference functions cited in csu_oracle.c.
+ */
+#ifndef CSU_ORACLE_H
+#define CSU_ORACLE_H
+#include "csu.h"
+
+typedef struct csu_oracle csu_oracle;
+
+int  oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out);
+void oracle_destroy(csu_oracle* o);
+int  oracle_set_state(csu_oracle* o, const double* U);
+int  oracle_get_state(csu_oracle* o, double* U);
+int  oracle_get_dt(csu_oracle* o, double* dtl);
+/* R, dPx, dPy, phi [nvar][ncell]; P [ncell][nvar+1]; Pb [nbface][nvar+1] */
+int  oracle_residual(csu_oracle* o, double* R, double* dPx, double* dPy, double* phi,
+                     double* P, double* Pb);
+/* res [niter][nvar], coef [niter][4] (Cx_p Cx_v Cy_p Cy_v) */
+int  oracle_iterate(csu_oracle* o, int niter, double* res, double* coef);
+int  oracle_set_threads(int n);
+
+#endif

@@ -1,0 +1,131 @@
+"""ctypes binding of the CPU oracle (oracle/lib/libcsu_oracle.so) and of the
+reference trace driver (oracle/_ref/csu_ref_trace).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "libcsu_oracle.so")
+REF_TRACE = os.path.join(HERE, "_ref", "csu_ref_trace")
+REF_EXE = os.path.join(HERE, "_ref", "csu_ref")
+
+_lib = None
+
+
+def build():
+    """Compile the oracle (and oracle/_ref where the reference sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        f64 = C.POINTER(C.c_double)
+        L.oracle_create.argtypes = [vp, vp, C.POINTER(vp)]
+        L.oracle_destroy.argtypes = [vp]
+        L.oracle_destroy.restype = None
+        L.oracle_set_state.argtypes = [vp, f64]
+        L.oracle_get_state.argtypes = [vp, f64]
+        L.oracle_get_dt.argtypes = [vp, f64]
+        L.oracle_residual.argtypes = [vp, f64, f64, f64, f64, f64, f64]
+        L.oracle_iterate.argtypes = [vp, C.c_int, f64, f64]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """CPU restatement over the same mesh description / params as the device."""
+
+    def __init__(self, case, params=None, threads: int = 0):
+        L = lib()
+        self.case = case
+        self.prm = params if params is not None else case.params
+        self.nvar, self.ncell, self.nbface = self.prm.nvar, case.ncell, case.nbface
+        if threads:
+            L.oracle_set_threads(threads)
+        self._h = C.c_void_p()
+        rc = L.oracle_create(C.addressof(case.mesh), C.addressof(self.prm), C.byref(self._h))
+        if rc != 0:
+            raise RuntimeError(f"oracle_create failed ({rc})")
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nvar, self.ncell)
+        lib().oracle_set_state(self._h, _p(U))
+
+    def get_state(self):
+        U = np.empty((self.nvar, self.ncell))
+        lib().oracle_get_state(self._h, _p(U))
+        return U
+
+    def get_dt(self):
+        d = np.empty(self.ncell)
+        lib().oracle_get_dt(self._h, _p(d))
+        return d
+
+    def residual(self) -> dict:
+        nv, nc, nb = self.nvar, self.ncell, self.nbface
+        out = {k: np.empty((nv, nc)) for k in ("R", "dPx", "dPy", "phi")}
+        out["P"] = np.empty((nc, nv + 1))
+        out["Pb"] = np.empty((max(nb, 1), nv + 1))
+        lib().oracle_residual(self._h, *[_p(out[k]) for k in ("R", "dPx", "dPy", "phi", "P", "Pb")])
+        out["Pb"] = out["Pb"][:nb]
+        return out
+
+    def iterate(self, n: int):
+        res = np.empty((n, self.nvar))
+        coef = np.empty((n, 4))
+        lib().oracle_iterate(self._h, n, _p(res), _p(coef))
+        return res, coef
+
+    def close(self):
+        if self._h:
+            lib().oracle_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_TRACE)
+
+
+def ref_trace(case_dir: str, mode: str, *args, U: np.ndarray | None = None) -> dict:
+    """Run the reference (built from its sources) and read its binary dumps."""
+    if not case_dir.endswith("/"):
+        case_dir += "/"
+    with tempfile.TemporaryDirectory(prefix="rt", dir="/tmp") as out:
+        out += "/"
+        cmd = [REF_TRACE, case_dir, out, mode, *[str(a) for a in args]]
+        if U is not None:
+            np.ascontiguousarray(U, dtype=np.float64).tofile(out + "Uin.f64")
+            cmd.append(out + "Uin.f64")
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        res = {}
+        for f in os.listdir(out):
+            if f == "Uin.f64":
+                continue
+            name, ext = os.path.splitext(f)
+            res[name] = np.fromfile(out + f, dtype=np.int32 if ext == ".i32" else np.float64)
+        return res
