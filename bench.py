@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""bench.py -- throughput of the residual-and-update path on one MI355X.
+
+Workload: the inviscid Mach 2.5 compression-ramp case (15 deg ramp at x=0.3
+on [0,1]x[0,0.5], seeded +-10% node jitter, quads split into triangles),
+Roe flux, least-squares gradients, Venkatakrishnan limiter (K=5), 5-stage
+RK with local time stepping at CFL 0.5, FP64 -- the plain method of the
+reference's CPU-runnable configuration, on a mesh scaled to 2000x1000x2 =
+4M cells so the working set (~2 GB) is far above the 256 MB Infinity
+Cache.  One step = one solver iteration (5 residual evaluations + update).
+The 200x100x2 (40k-cell) size of the reference's own configuration is timed
+too and reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  Inputs are resident in HBM before the timed region;
+the timed region is K iterations bracketed by device synchronisation.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-iters/s + solves/hr, p50/p99 time-to-converge vs plain FP64 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int) -> float:
+    """Compulsory HBM bytes of one launch: every array the kernel must touch,
+    each element counted once (neighbour re-reads are cache traffic)."""
+    ns = nv + 1
+    if kernel == "k_prim":
+        return nc * (8 * nv + 8 * ns)
+    if kernel == "k_grad":
+        return (nc * (8 * ns + 4 + 16 + 16 + 8 + 64 + 16 * nv + 8 * nv)
+                + nb * (8 * ns + 44))
+    if kernel == "k_flux":
+        return nf * (8 + 56 + 8 * nv) + nc * (8 * ns + 16 * nv + 8 * nv)
+    if kernel == "k_update":
+        return nc * (16 + 16 + 4 + 8 * nv * 3 + 16) + nf * 8 * nv + nb * 44
+    return 0.0
+
+
+def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool):
+    from csu_amd import cases, native
+    d = tempfile.mkdtemp(prefix="bn", dir="/tmp") + "/"
+    cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=steps)
+    case = native.HostCase(d)
+    dev = native.Device(case)
+    dev.iterate(warmup, want=False)
+    dev.sync()
+    dev.set_profiling(profile)
+    t0 = time.perf_counter()
+    dev.iterate(steps, want=False)
+    dev.sync()
+    t1 = time.perf_counter()
+    kt = dev.kernel_times() if profile else {}
+    res, _ = dev.iterate(1)
+    info = dict(nc=case.ncell, nf=case.nface, nb=case.nbface, nv=case.nvar,
+                seconds=t1 - t0, kernels=kt, res=res[0].tolist())
+    dev.close()
+    case.close()
+    return info
+
+
+def cpu_baseline(nx: int, ny: int, iters: int):
+    """Reference OpenMP build (oracle/_ref/csu_ref, compiled from the
+    reference sources) on the host cores: Nc*N / (T(N) - T(0))."""
+    from oracle import pyoracle
+    from csu_amd import cases
+    threads = min(os.cpu_count() or 1, 16)
+    exe = pyoracle.REF_EXE
+    if not os.path.exists(exe):
+        return None
+
+    def run(n):
+        d = tempfile.mkdtemp(prefix="cb", dir="/tmp") + "/"
+        cases.ramp_case(d, nx=nx, ny=ny, threads=threads, nmax=n)
+        out = subprocess.run([exe, d], capture_output=True, text=True, check=True).stdout
+        return float(re.findall(r"Duration ([0-9.eE+-]+) s", out)[-1])
+    t0 = min(run(0) for _ in range(2))
+    t1 = run(iters)
+    nc = 2 * nx * ny
+    return {"value": nc * iters / (t1 - t0), "unit": "cell-iter/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"reference OpenMP build (CSU.py gcc flags), ramp {nx}x{ny}x2={nc} cells, "
+                      f"{iters} iterations minus a 0-iteration run, threads={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=2000)
+    ap.add_argument("--ny", type=int, default=1000)
+    ap.add_argument("--cpu-iters", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
+              "multi-GPU sharding is a design note, not a benchmark", file=sys.stderr)
+        sys.exit(2)
+
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    log(f"ramp {a.nx}x{a.ny}x2: generating, loading, {a.warmup}+{a.steps} iterations")
+    big = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=True)
+    log(f"big done: {big['seconds']:.3f} s; ramp 200x100x2")
+    small = gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False)
+    log("small done; cpu baseline" if not a.no_cpu else "small done")
+    nc = big["nc"]
+    ms = 1e3 * big["seconds"] / a.steps
+    value = nc * a.steps / big["seconds"]
+
+    # dominant kernel by total time, its algorithmic bytes per launch
+    kt = big["kernels"]
+    dom = max((k for k in kt if alg_bytes(k, 1, 1, 1, 4) > 0), key=lambda k: kt[k][0] * kt[k][1])
+    avg_ms = kt[dom][0]
+    traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"])
+    achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "kernel_avg_ms": avg_ms,
+            "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
+            "iteration_alg_bytes_per_cell": 2334.5,
+            "iteration_frac": value * 2334.5 / (HBM_PEAK_GBS * 1e9)}
+    cpu = None if a.no_cpu else cpu_baseline(200, 100, a.cpu_iters)
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"euler-ramp M2.5 Roe/LSQ/Venkat(K=5) RK5 CFL0.5 FP64, "
+                               f"{a.nx}x{a.ny}x2 tri mesh",
+                   "cells": nc, "faces": big["nf"], "boundary_faces": big["nb"],
+                   "config0_40k_cell_iter_per_s": small["nc"] * max(a.steps, 50) / small["seconds"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

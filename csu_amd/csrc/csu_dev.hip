@@ -20,6 +20,7 @@
 #include <string.h>
 #include <math.h>
 #include <vector>
+#include <algorithm>
 #include <string>
 #include "csu.h"
 #include "device_math.h"
@@ -83,6 +84,8 @@ struct csu_dev {
   double *pref2;                           // [nv] limiter reference (type 0 const, type 1 per stage)
   double *red;                             // reduction scratch
   int nred;                                // number of partial blocks
+  double* dtg;                             // global time step (device scalar)
+  int fresh;                               // state set from the host: P and dt not yet derived
   double *res_hist, *coef_hist;            // [cap][nv], [cap][4]
   int hist_cap;
   // profiling
@@ -101,47 +104,59 @@ static inline int slot_of(int kk) { return kk > 3 ? kk + 1 : kk; }
 // kernels
 // ---------------------------------------------------------------------------
 
-// Primitive recovery with clamps, reference solverCalcPrimitive
-// (solver.c:1153-1213).  U is the stage state; clamped values are written
-// back to it, as the reference does.
-__global__ __launch_bounds__(CSU_BLOCK) void k_prim(KP kp, double* __restrict__ U,
-                                                    double* __restrict__ P) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nc = kp.nc;
-  if (i >= nc) return;
-  double r = U[i];
-  double r0 = fmax(r, kp.rlim);
-  if (r0 != r) U[i] = r0;
+// Primitive recovery of one cell with the positivity clamps, reference
+// solverCalcPrimitive (solver.c:1153-1213).  u[] (conservative, in
+// registers) receives the clamped values the reference writes back into U;
+// p[] receives r u v p T [nu~ | k om].
+template <int NV>
+__device__ __forceinline__ void prim_point(const KP& kp, double* u, double* p) {
+  const double r0 = fmax(u[0], kp.rlim);
+  u[0] = r0;
   double k = 0;
-  if (kp.sst == 1) {
-    k = U[4 * nc + i] / r0;
-    if (k < 1e-14) { k = 1e-14; U[4 * nc + i] = r0 * k; }
-    P[5 * nc + i] = k;
-    double om = U[5 * nc + i] / r0;
-    if (om <= 0.0) { om = 0.0; U[5 * nc + i] = r0 * om; }
-    P[6 * nc + i] = om;
+  if (NV == 6) {
+    k = u[4] / r0;
+    if (k < 1e-14) { k = 1e-14; u[4] = r0 * k; }
+    p[5] = k;
+    double om = u[5] / r0;
+    if (om <= 0.0) { om = 0.0; u[5] = r0 * om; }
+    p[6] = om;
   }
-  double u = U[nc + i] / r0;
-  double v = U[2 * nc + i] / r0;
-  double e = U[3 * nc + i] / r0 - 0.5 * (u * u + v * v) - k;
+  const double uu = u[1] / r0, vv = u[2] / r0;
+  double e = u[3] / r0 - 0.5 * (uu * uu + vv * vv) - k;
   e = fmax(e, 7.176325e+02);
   double T = gas_e2T<double>(kp.gas, e);
   if (T < 1) {
     T = 1;
     e = gas_T2e<double>(kp.gas, T);
-    U[3 * nc + i] = (e + 0.5 * (u * u + v * v) + k) * r0;
+    u[3] = (e + 0.5 * (uu * uu + vv * vv) + k) * r0;
   }
-  P[i] = r0;
-  P[nc + i] = u;
-  P[2 * nc + i] = v;
-  P[3 * nc + i] = kp.gas.R * T * r0;
-  P[4 * nc + i] = T;
-  if (kp.sa == 1) {
-    double n = U[4 * nc + i] / r0;
+  p[0] = r0; p[1] = uu; p[2] = vv;
+  p[3] = kp.gas.R * T * r0;
+  p[4] = T;
+  if (NV == 5) {
+    double n = u[4] / r0;
     n = fmax(n, 0.0);
-    U[4 * nc + i] = n * r0;
-    P[5 * nc + i] = n;
+    u[4] = n * r0;
+    p[5] = n;
   }
+}
+
+// Standalone primitive pass (first stage after the state was set from the
+// host).  Clamped values are written back to U, as the reference does.
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_prim(KP kp, double* __restrict__ U, double* __restrict__ P) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc;
+  if (i >= nc) return;
+  double u[NV], u0[NV], p[NV + 1];
+#pragma unroll
+  for (int k = 0; k < NV; k++) u0[k] = u[k] = U[k * nc + i];
+  prim_point<NV>(kp, u, p);
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+    if (__double_as_longlong(u[k]) != __double_as_longlong(u0[k])) U[k * nc + i] = u[k];
+#pragma unroll
+  for (int m = 0; m < NV + 1; m++) P[m * nc + i] = p[m];
 }
 
 // Ghost state of a boundary face from the interior primitive Pi (nv+1 slots),
@@ -440,58 +455,130 @@ __device__ __forceinline__ void bc_flux(const KP& kp, int type, double dSx, doub
   }
 }
 
-// Residual gather + boundary fluxes (+ RK stage update when mode == 1).
-// Reference: gather loop solver.c:646-667, boundary() boundary.c:447-464,
-// solverRK solver.c:741-766.
+// Residual gather + boundary fluxes, then (flags) the RK stage update fused
+// with the next stage's primitive recovery, and on the last stage the next
+// iteration's time step and the residual-norm partial sums.
+// Reference: gather solver.c:646-667, boundary() boundary.c:447-464,
+// solverRK solver.c:741-766, solverCalcPrimitive solver.c:1153-1213,
+// solverLocalTimeStep solver.c:895-923, solverCalcRes solver.c:867-892.
+enum { F_RK = 1, F_CLAMP_UN = 2, F_LAST = 4, F_NEXT_LOCAL = 8, F_STORE_R = 16, F_CUR_LOCAL = 32 };
+
+template <int NV>
+__device__ __forceinline__ double cell_dt(const KP& kp, const double* u, double om, double sx, double sy) {
+  // solverLocalTimeStep + solverCalcVel, from the unclamped conservative state
+  double k = 0;
+  if (NV == 6) k = u[4] / u[0];
+  double E = u[3] / u[0];
+  double uu = u[1] / u[0];
+  double vv = u[2] / u[0];
+  double aux = (E - (uu * uu + vv * vv) / 2 - k);
+  double T = gas_e2T<double>(kp.gas, aux);
+  double c = gas_T2c<double>(kp.gas, T);
+  double Lx = (fabs(uu) + c) * sx;
+  double Ly = (fabs(vv) + c) * sy;
+  return om / (Lx + Ly);
+}
+
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_update(
-    KP kp, int mode, int dt_local, double sigma, double rk_a, const int* __restrict__ cnp,
+    KP kp, int flags, double sigma, double rk_a, double cfl, const int* __restrict__ cnp,
     const int* __restrict__ cnbr, const int* __restrict__ cface,
     const double* __restrict__ bdsx, const double* __restrict__ bdsy, const double* __restrict__ bds,
-    const int* __restrict__ btype, const double* __restrict__ P, const double* __restrict__ fflux,
-    const double* __restrict__ omega, const double* __restrict__ dtl, const double* __restrict__ Un,
-    double* __restrict__ Uout, double* __restrict__ Rout) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int* __restrict__ btype, double* P, const double* __restrict__ fflux,
+    const double* __restrict__ omega, const double* __restrict__ dsxm, const double* __restrict__ dsym,
+    double* dtl, const double* dtg, double* Un, double* __restrict__ Uout, double* __restrict__ Rout,
+    double* __restrict__ red_res, double* __restrict__ red_dt) {
+  __shared__ double sred[NV + 1][CSU_BLOCK];
+  const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc, nf = kp.nf;
-  if (i >= nc) return;
+  const bool active = i < nc;
   double R[NV];
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) R[kk] = 0.0;
-  const int np = cnp[i];
+  double dt_next = INFINITY;
+  if (active) {
+    const int np = cnp[i];
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    int f = cface[j * nc + i];
-    if (f > 0) {
+    for (int j = 0; j < 4; j++) {
+      int f = cface[j * nc + i];
+      if (f > 0) {
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) R[kk] += fflux[kk * nf + f - 1];
-    } else if (f < 0) {
+        for (int kk = 0; kk < NV; kk++) R[kk] += fflux[kk * nf + f - 1];
+      } else if (f < 0) {
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) R[kk] -= fflux[kk * nf - f - 1];
-    }
-  }
-  double Pi[NV + 1];
-  bool loaded = false;
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    if (j >= np) break;
-    int n = cnbr[j * nc + i];
-    if (n < 0) {
-      int g = -1 - n;
-      if (!loaded) {
-#pragma unroll
-        for (int m = 0; m < NV + 1; m++) Pi[m] = P[m * nc + i];
-        loaded = true;
+        for (int kk = 0; kk < NV; kk++) R[kk] -= fflux[kk * nf - f - 1];
       }
-      bc_flux<NV>(kp, btype[g], bdsx[g], bdsy[g], bds[g], Pi, R);
+    }
+    double Pi[NV + 1];
+    bool loaded = false;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (j >= np) break;
+      int n = cnbr[j * nc + i];
+      if (n < 0) {
+        int g = -1 - n;
+        if (!loaded) {
+#pragma unroll
+          for (int m = 0; m < NV + 1; m++) Pi[m] = P[m * nc + i];
+          loaded = true;
+        }
+        bc_flux<NV>(kp, btype[g], bdsx[g], bdsy[g], bds[g], Pi, R);
+      }
+    }
+    if (flags & F_STORE_R) {
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) Rout[kk * nc + i] = R[kk];
+    }
+    if (flags & F_RK) {
+      double un[NV], un0[NV], p[NV + 1];
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) un0[kk] = un[kk] = Un[kk * nc + i];
+      if (flags & F_CLAMP_UN) {
+        // first stage: the reference clamps U^n in place (solverCalcR(U))
+        prim_point<NV>(kp, un, p);
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++)
+          if (__double_as_longlong(un[kk]) != __double_as_longlong(un0[kk])) Un[kk * nc + i] = un[kk];
+      }
+      const double dt = dtl && (flags & F_CUR_LOCAL) ? dtl[i] : dtg[0];
+      const double om = omega[i];
+      double unew[NV];
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) unew[kk] = un[kk] - dt * sigma * rk_a * R[kk] / om;
+      if (flags & F_LAST) {
+        // U^{n+1} stays unclamped until the next residual, as in the reference
+        dt_next = cell_dt<NV>(kp, unew, om, dsxm[i], dsym[i]);
+        if (flags & F_NEXT_LOCAL) dtl[i] = cfl * dt_next;
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++) Uout[kk * nc + i] = unew[kk];
+        prim_point<NV>(kp, unew, p);
+      } else {
+        prim_point<NV>(kp, unew, p);
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++) Uout[kk * nc + i] = unew[kk];
+      }
+#pragma unroll
+      for (int m = 0; m < NV + 1; m++) P[m * nc + i] = p[m];
     }
   }
+  if (!(flags & F_LAST)) return;
+  // residual-norm partials (and global time-step minimum): fixed tree
 #pragma unroll
-  for (int kk = 0; kk < NV; kk++) Rout[kk * nc + i] = R[kk];
-  if (mode == 1) {
-    const double dt = dtl[dt_local ? i : 0];
-    const double om = omega[i];
+  for (int kk = 0; kk < NV; kk++) sred[kk][t] = R[kk] * R[kk];
+  sred[NV][t] = dt_next;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st) {
 #pragma unroll
-    for (int kk = 0; kk < NV; kk++) Uout[kk * nc + i] = Un[kk * nc + i] - dt * sigma * rk_a * R[kk] / om;
+      for (int kk = 0; kk < NV; kk++) sred[kk][t] += sred[kk][t + st];
+      sred[NV][t] = fmin(sred[NV][t], sred[NV][t + st]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) red_res[kk * gridDim.x + blockIdx.x] = sred[kk][0];
+    red_dt[blockIdx.x] = sred[NV][0];
   }
 }
 
@@ -549,43 +636,33 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_dt_final(int nblocks, double cfl,
 
 // Residual norm sqrt(sum R^2 / Nc) per variable, reference solverCalcRes
 // (solver.c:867-892); fixed two-level tree, independent of scheduling.
+// Final level of the residual norm: one block, each thread sums a strided
+// slice of the block partials for all variables at once (independent
+// chains), then a fixed tree.  Deterministic for a given launch shape.
+#define RES_THREADS 1024
 template <int NV>
-__global__ __launch_bounds__(CSU_BLOCK) void k_res_partial(int nc, const double* __restrict__ R,
-                                                           double* __restrict__ red) {
-  __shared__ double s[NV][CSU_BLOCK];
-  int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
+__global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, const double* __restrict__ red,
+                                                           double* __restrict__ res) {
+  __shared__ double s[NV][RES_THREADS];
+  const int t = threadIdx.x;
+  double a[NV];
 #pragma unroll
-  for (int kk = 0; kk < NV; kk++) {
-    double r = i < nc ? R[kk * nc + i] : 0.0;
-    s[kk][t] = r * r;
+  for (int kk = 0; kk < NV; kk++) a[kk] = 0.0;
+  for (int b = t; b < nblocks; b += RES_THREADS) {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) a[kk] += red[kk * nblocks + b];
   }
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) s[kk][t] = a[kk];
   __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st)
+  for (int st = RES_THREADS / 2; st > 0; st >>= 1) {
+    if (t < st) {
+#pragma unroll
       for (int kk = 0; kk < NV; kk++) s[kk][t] += s[kk][t + st];
-    __syncthreads();
-  }
-  if (t == 0)
-    for (int kk = 0; kk < NV; kk++) red[kk * gridDim.x + blockIdx.x] = s[kk][0];
-}
-
-template <int NV>
-__global__ __launch_bounds__(CSU_BLOCK) void k_res_final(int nc, int nblocks, const double* __restrict__ red,
-                                                         double* __restrict__ res) {
-  __shared__ double s[CSU_BLOCK];
-  int t = threadIdx.x;
-  for (int kk = 0; kk < NV; kk++) {
-    double a = 0.0;
-    for (int b = t; b < nblocks; b += CSU_BLOCK) a += red[kk * nblocks + b];
-    s[t] = a;
-    __syncthreads();
-    for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-      if (t < st) s[t] += s[t + st];
-      __syncthreads();
     }
-    if (t == 0) res[kk] = sqrt(s[0] / nc);
     __syncthreads();
   }
+  if (t < NV) res[t] = sqrt(s[t][0] / nc);
 }
 
 // Wall pressure forces, reference solverCalcCoeff3 pressure part
@@ -690,7 +767,7 @@ static void free_all(csu_dev* d) {
                   d->fe0, d->fe1, d->fdsx, d->fdsy, d->fds, d->fdx0, d->fdy0, d->fdx1, d->fdy1,
                   d->be0, d->btype, d->bdsx, d->bdsy, d->bds, d->bxm, d->bym, d->wall,
                   d->U, d->Uaux, d->P, d->Pb, d->dPx, d->dPy, d->phi, d->R, d->fflux, d->dtl,
-                  d->pref2, d->red, d->res_hist, d->coef_hist};
+                  d->pref2, d->red, d->res_hist, d->coef_hist, d->dtg};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : d->evpool) (void)hipEventDestroy(e);
@@ -746,6 +823,8 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   d->U = d->Uaux = d->P = d->Pb = d->dPx = d->dPy = d->phi = d->R = d->fflux = d->dtl = nullptr;
   d->pref2 = d->red = d->res_hist = d->coef_hist = nullptr;
   d->hist_cap = 0;
+  d->dtg = nullptr;
+  d->fresh = 1;
   const int nc = d->nc, nf = d->nf, nb = d->nb, nv = d->nv;
 
   // ---- host-side geometry ----
@@ -781,10 +860,26 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
       cface[(size_t)j * nc + i] = m->cell_face[4 * i + j];
     }
   }
+  // Device face order: sorted by (e0, e1) so that consecutive faces touch
+  // neighbouring cells (the reference's hash order scatters them).  Cells
+  // keep their face lists in the reference's order; only the ids change, so
+  // every per-cell sum keeps its order.
+  std::vector<int> ford(nf), finv(nf);
+  for (int f = 0; f < nf; f++) ford[f] = f;
+  std::stable_sort(ford.begin(), ford.end(), [m](int a, int b) {
+    const int32_t *qa = m->face + 4 * a, *qb = m->face + 4 * b;
+    return qa[0] != qb[0] ? qa[0] < qb[0] : qa[1] < qb[1];
+  });
+  for (int f = 0; f < nf; f++) finv[ford[f]] = f;
+  for (size_t t = 0; t < cface.size(); t++) {
+    int f = cface[t];
+    if (f > 0) cface[t] = finv[f - 1] + 1;
+    else if (f < 0) cface[t] = -(finv[-f - 1] + 1);
+  }
   std::vector<int> fe0(nf), fe1(nf);
   std::vector<double> fdsx(nf), fdsy(nf), fds(nf), fdx0(nf), fdy0(nf), fdx1(nf), fdy1(nf);
   for (int f = 0; f < nf; f++) {
-    const int32_t* q = m->face + 4 * f;
+    const int32_t* q = m->face + 4 * ford[f];
     fe0[f] = q[0]; fe1[f] = q[1];
     double dSx, dSy;
     calc_ds(m, q[2], q[3], &dSx, &dSy);
@@ -849,7 +944,8 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
       (rc = dalloc(&d->dPx, (size_t)nv * nc)) || (rc = dalloc(&d->dPy, (size_t)nv * nc)) ||
       (rc = dalloc(&d->phi, (size_t)nv * nc)) || (rc = dalloc(&d->R, (size_t)nv * nc)) ||
       (rc = dalloc(&d->fflux, (size_t)nv * nf)) || (rc = dalloc(&d->dtl, nc + 1)) ||
-      (rc = dalloc(&d->pref2, nv)) || (rc = dalloc(&d->red, 2 * (size_t)nv * nblocks + 16))) {
+      (rc = dalloc(&d->pref2, nv)) || (rc = dalloc(&d->red, 2 * (size_t)nv * nblocks + 16)) ||
+      (rc = dalloc(&d->dtg, 1))) {
     csu_dev_destroy(d);
     return rc;
   }
@@ -884,6 +980,7 @@ extern "C" int csu_dev_set_state(csu_dev* d, const double* U) {
   if (!d || !U) return set_err(CSU_ERR_ARG, "null argument");
   HIPCHK(hipMemcpyAsync(d->U, U, sizeof(double) * (size_t)d->nv * d->nc, hipMemcpyHostToDevice, d->stream));
   HIPCHK(hipStreamSynchronize(d->stream));
+  d->fresh = 1;
   return CSU_OK;
 }
 
@@ -972,13 +1069,12 @@ extern "C" int csu_dev_kernel_times(csu_dev* d, const char** names, double* avg_
   return n;
 }
 
-// ---- one residual evaluation (solverCalcR) on the buffer Us ----
+// ---- one residual evaluation (solverCalcR) with P already valid ----
+// Launches gradient/limiter, face flux and the (fused) cell update.
 template <int NV>
-static int calc_r(csu_dev* d, double* Us, int mode, double* Un, double* Uout, int dt_local, double sigma,
-                  double rk_a, const double* dtl) {
+static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, double* Uout) {
   const KP& kp = d->kp;
   const int nc = d->nc;
-  { Prof p(d, "k_prim"); hipLaunchKernelGGL(k_prim, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, Us, d->P); }
   if (kp.order == 2 && kp.limiter == 1) {
     Prof p(d, "k_lim1");
     hipLaunchKernelGGL(k_lim1_partial<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->P, d->red);
@@ -997,16 +1093,29 @@ static int calc_r(csu_dev* d, double* Us, int mode, double* Un, double* Uout, in
   }
   {
     Prof p(d, "k_update");
-    hipLaunchKernelGGL(k_update<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, mode, dt_local, sigma, rk_a, d->cnp,
-                       d->cnbr, d->cface, d->bdsx, d->bdsy, d->bds, d->btype, d->P, d->fflux, d->omega, dtl, Un, Uout,
-                       d->R);
+    hipLaunchKernelGGL(k_update<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, flags, sigma, rk_a,
+                       d->prm.cfl, d->cnp, d->cnbr, d->cface, d->bdsx, d->bdsy, d->bds, d->btype, d->P, d->fflux,
+                       d->omega, d->dsxm, d->dsym, d->dtl, d->dtg, Un, Uout, d->R, d->red, d->red + (size_t)NV * d->nred);
   }
   HIPCHK(hipGetLastError());
   return CSU_OK;
 }
 
 template <int NV>
-static int residual_t(csu_dev* d) { return calc_r<NV>(d, d->U, 0, nullptr, nullptr, 0, 0.0, 0.0, nullptr); }
+static int prim_pass(csu_dev* d) {
+  Prof p(d, "k_prim");
+  hipLaunchKernelGGL(k_prim<NV>, dim3(nblk(d->nc)), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->U, d->P);
+  HIPCHK(hipGetLastError());
+  return CSU_OK;
+}
+
+template <int NV>
+static int residual_t(csu_dev* d) {
+  int rc = prim_pass<NV>(d);
+  if (!rc) rc = stage<NV>(d, F_STORE_R, 0.0, 0.0, nullptr, nullptr);
+  d->fresh = 1;
+  return rc;
+}
 
 extern "C" int csu_dev_residual(csu_dev* d, double* R, double* dPx, double* dPy, double* phi, double* P,
                                 double* Pb) {
@@ -1057,28 +1166,39 @@ static int iterate_t(csu_dev* d, int niter) {
   const int nc = d->nc;
   const double* a = d->prm.stages == 1 ? RK1 : d->prm.stages == 3 ? RK3 : d->prm.stages == 4 ? RK4 : RK5;
   const double sigma = 0.5 * d->prm.stages;
-  double* dtg = d->red + 2 * (size_t)d->nv * d->nred + 8;  // global dt slot
+  double* red_dt = d->red + (size_t)NV * d->nred;
+  auto is_local = [d](int it) {
+    return d->prm.dt_local == 1 && !(d->prm.dt_local_n >= 0 && it > d->prm.dt_local_n);
+  };
   for (int it = 0; it < niter; it++) {
-    const int local = d->prm.dt_local == 1 && !(d->prm.dt_local_n >= 0 && d->iter > d->prm.dt_local_n);
-    {
+    const int cur_local = is_local(d->iter), next_local = is_local(d->iter + 1);
+    int first_flags = F_CLAMP_UN;
+    if (d->fresh) {
+      // time step from the unclamped state (solverCalcDt), then primitives
       Prof p(d, "k_dt");
-      hipLaunchKernelGGL(k_dt<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, local, d->prm.cfl, d->U,
-                         d->omega, d->dsxm, d->dsym, d->dtl, d->red);
-      if (!local)
-        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nblk(nc), d->prm.cfl, d->red, dtg);
+      hipLaunchKernelGGL(k_dt<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, cur_local, d->prm.cfl, d->U,
+                         d->omega, d->dsxm, d->dsym, d->dtl, red_dt);
+      if (!cur_local)
+        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nblk(nc), d->prm.cfl, red_dt, d->dtg);
+      int rc = prim_pass<NV>(d);
+      if (rc) return rc;
+      first_flags = 0;
+      d->fresh = 0;
     }
-    // the global time step stays on the device (dtg[0]); stages read it there
     for (int s = 0; s < d->prm.stages; s++) {
-      double* src = s == 0 ? d->U : d->Uaux;
-      int rc = calc_r<NV>(d, src, 1, d->U, d->Uaux, local, sigma, a[s], local ? d->dtl : dtg);
+      const bool last = s == d->prm.stages - 1;
+      int flags = F_RK | (cur_local ? F_CUR_LOCAL : 0) | (s == 0 ? first_flags : 0);
+      if (last) flags |= F_LAST | F_STORE_R | (next_local ? F_NEXT_LOCAL : 0);
+      int rc = stage<NV>(d, flags, sigma, a[s], d->U, d->Uaux);
       if (rc) return rc;
     }
     std::swap(d->U, d->Uaux);
+    if (!next_local)
+      hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nblk(nc), d->prm.cfl, red_dt, d->dtg);
     const int slot = d->iter % d->hist_cap;
     {
       Prof p(d, "k_res");
-      hipLaunchKernelGGL(k_res_partial<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, nc, d->R, d->red);
-      hipLaunchKernelGGL(k_res_final<NV>, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nc, nblk(nc), d->red,
+      hipLaunchKernelGGL(k_res_final<NV>, dim3(1), dim3(RES_THREADS), 0, d->stream, nc, nblk(nc), d->red,
                          d->res_hist + (size_t)slot * NV);
     }
     {

@@ -1,0 +1,94 @@
+"""Write the golden fixtures of the parity tests from the reference solver.
+
+Runs oracle/_ref/csu_ref_trace (the reference compiled from its own C
+sources by oracle/Makefile; our driver calls its functions unchanged) on
+seeded cases and stores its full-precision outputs as small .npz files next
+to this script.  Only this container can run it (the reference sources are
+not on the GPU box); the fixtures are committed.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from csu_amd import cases  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+
+# (name, mesh kwargs, input overrides, history iterations)
+SMALL = [
+    ("ramp_roe", dict(nx=12, ny=6), {}, 20),
+    ("ramp_ausm_o1", dict(nx=12, ny=6), {"flux": "AUSM", "order": 1}, 20),
+    ("ramp_ausmdv_lim1", dict(nx=12, ny=6), {"flux": "AUSMDV", "limiter": 1, "limK": 0.5}, 20),
+    ("ramp_ausmpup", dict(nx=12, ny=6), {"flux": "AUSMpup", "stages": 3}, 20),
+    ("ramp_ausmpup2_sub", dict(nx=12, ny=6), {"flux": "AUSMpup2", "mach": 0.6, "stages": 4}, 20),
+    ("ramp_quad_global_dt", dict(nx=12, ny=6, split=False), {"dtLocal": 0, "stages": 1, "CFL": 0.4}, 20),
+    ("ramp_tp_dtswitch", dict(nx=12, ny=6), {"TP": 1, "dtLocalN": 5}, 20),
+]
+
+
+def case_dir(tag):
+    return tempfile.mkdtemp(prefix="gd", dir="/tmp") + "/"
+
+
+def make_small():
+    for name, mk, inp, nit in SMALL:
+        d = case_dir(name)
+        cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nit, **inp))
+        mesh = po.ref_trace(d, "mesh")
+        nv = 4
+        nc = mesh["cellv"].size // 4
+        rng = np.random.default_rng(5)
+        # perturbed free stream: the reference's own initial state, +-5 %
+        init = po.ref_trace(d, "resid")
+        U0 = init["U"].reshape(nv, nc)
+        U = U0 * (1 + 0.05 * rng.standard_normal(U0.shape))
+        r = po.ref_trace(d, "resid", U=U)
+        h = po.ref_trace(d, "hist", nit)
+        out = dict(
+            mesh_kwargs=np.array(repr(mk)), inputs=np.array(repr(inp)),
+            con=mesh["con"].reshape(-1, 4), nbr=mesh["nbr"].reshape(-1, 4), cface=mesh["cface"].reshape(-1, 4),
+            cellv=mesh["cellv"].reshape(-1, 4), bface=mesh["bface"].reshape(-1, 3),
+            U_in=U, R=r["R"].reshape(nv, nc), dPx=r["dPx"].reshape(nv, nc), dPy=r["dPy"].reshape(nv, nc),
+            phi=r["phi"].reshape(nv, nc), P=r["P"].reshape(nc, nv + 1), Pb=r["Pb"].reshape(-1, nv + 1),
+            U_clamped=r["U"].reshape(nv, nc),
+            hist_res=h["res"].reshape(nit, nv), hist_coef=h["coef"].reshape(nit, 2),
+            hist_U=h["U"].reshape(nv, nc),
+        )
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print("wrote", name, nc, "cells")
+
+
+def make_config0(n_iter=100, n_sample=1000):
+    """Config-0 plain method (200x100x2 ramp, Roe/LSQ/Venkat K=5, RK5,
+    CFL 0.5, local dt): first n_iter residuals, forces, and the final state
+    at n_sample seeded cells plus per-variable norms."""
+    d = case_dir("c0")
+    cases.ramp_case(d, nx=200, ny=100, seed=0, threads=8, nmax=n_iter)
+    h = po.ref_trace(d, "hist", n_iter)
+    nv, nc = 4, 40000
+    U = h["U"].reshape(nv, nc)
+    idx = np.sort(np.random.default_rng(0).choice(nc, n_sample, replace=False))
+    with open(d + "mesh.su2", "rb") as f:
+        import hashlib
+        digest = hashlib.sha256(f.read()).hexdigest()
+    np.savez_compressed(os.path.join(HERE, "config0_hist100.npz"),
+                        res=h["res"].reshape(n_iter, nv), coef=h["coef"].reshape(n_iter, 2),
+                        U_idx=idx, U_sample=U[:, idx], U_l2=np.sqrt((U ** 2).sum(axis=1)),
+                        U_linf=np.abs(U).max(axis=1), mesh_sha256=np.array(digest))
+    print("wrote config0_hist100", digest)
+
+
+if __name__ == "__main__":
+    if not po.have_ref():
+        sys.exit("oracle/_ref is not built (needs the reference sources): make -C oracle ref")
+    make_small()
+    make_config0()

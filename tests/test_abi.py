@@ -1,0 +1,38 @@
+"""The C ABI library loads and exports every symbol its headers declare; the
+drop-in executable keeps the reference's error behaviour (CPU only)."""
+import os
+import re
+import subprocess
+
+from csu_amd import cases, native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("csu.h", "csu_host.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(csu_\w+)\s*\(", src, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_exports_every_declared_symbol():
+    lib = native.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.csu_version().startswith(b"csu-mi355x")
+
+
+def test_executable_missing_key_exits_zero(tmp_path):
+    d = str(tmp_path) + "/"
+    inp = cases.ramp_input(nmax=1)
+    del inp["flux"]
+    cases.write_case(d, cases.ramp_mesh(4, 2), inp)
+    r = subprocess.run([native.EXE, d], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    assert "Error: Value not found in the input file: flux." in r.stdout

@@ -1,0 +1,107 @@
+"""HIP Euler path vs the CPU oracle on the same seeded inputs (GPU)."""
+import numpy as np
+import pytest
+
+from csu_amd import cases, native
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(tmpcase, nx=12, ny=6, **inp):
+    d = tmpcase("g")
+    cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=10, **inp)
+    return native.HostCase(d)
+
+
+def _rel(a, b):
+    scale = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-300)
+    return float(np.max(np.abs(a - b) / scale))
+
+
+@pytest.mark.parametrize("inp", [
+    {},
+    {"flux": "AUSM"}, {"flux": "AUSMDV"}, {"flux": "AUSMpup"}, {"flux": "AUSMpup2"},
+    {"order": 1}, {"limiter": 1, "limK": 0.5}, {"mach": 0.6, "BC:outlet": "outlet"},
+])
+def test_single_residual(tmpcase, inp):
+    c = _case(tmpcase, **inp)
+    print("case", inp, flush=True)
+    rng = np.random.default_rng(11)
+    U = c.initial_state() * (1 + 0.05 * rng.standard_normal((c.nvar, c.ncell)))
+    dev = native.Device(c)
+    orc = po.Oracle(c)
+    dev.set_state(U)
+    orc.set_state(U)
+    g = dev.residual()
+    o = orc.residual()
+    assert np.array_equal(g["P"], o["P"])
+    assert np.array_equal(g["Pb"], o["Pb"])
+    for k in ("dPx", "dPy"):
+        assert _rel(g[k], o[k]) < 1e-13, k
+    if c.params.order == 2:
+        assert _rel(g["phi"], o["phi"]) < 1e-12
+    assert _rel(g["R"], o["R"]) < 1e-12
+
+
+def test_history_ramp(tmpcase):
+    c = _case(tmpcase, nx=40, ny=20)
+    print("ramp 40x20 history", flush=True)
+    dev = native.Device(c)
+    orc = po.Oracle(c)
+    rg, cg = dev.iterate(100)
+    ro, co = orc.iterate(100)
+    rel = np.max(np.abs(rg[:, 0] / ro[:, 0] - 1))
+    print("max rel diff rho residual over 100 iterations:", rel)
+    assert rel < 1e-10
+    assert np.max(np.abs(cg - co)) < 1e-10 * max(1.0, np.abs(co).max())
+    Ug, Uo = dev.get_state(), orc.get_state()
+    assert _rel(Ug, Uo) < 1e-10
+
+
+GOLD = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+SMALL = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
+         "ramp_ausmpup2_sub", "ramp_quad_global_dt", "ramp_tp_dtswitch"]
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_vs_reference_fixture(tmpcase, name):
+    """HIP path against the reference's own outputs (golden fixtures)."""
+    g = np.load(f"{GOLD}/{name}.npz")
+    print("fixture", name, flush=True)
+    d = tmpcase("f")
+    cases.write_case(d, cases.ramp_mesh(seed=0, **eval(str(g["mesh_kwargs"]))),
+                     cases.ramp_input(threads=1, nmax=1, **eval(str(g["inputs"]))))
+    c = native.HostCase(d)
+    dev = native.Device(c)
+    dev.set_state(g["U_in"])
+    r = dev.residual()
+    assert np.array_equal(r["P"], g["P"]) and np.array_equal(r["Pb"], g["Pb"])
+    assert _rel(r["R"], g["R"]) < 1e-12
+    dev2 = native.Device(c)
+    n = g["hist_res"].shape[0]
+    res, coef = dev2.iterate(n)
+    assert np.max(np.abs(res / g["hist_res"] - 1)) < 1e-10
+    assert np.max(np.abs(coef[:, [0, 2]] - g["hist_coef"])) <= 1e-10 * np.abs(g["hist_coef"]).max()
+    assert _rel(dev2.get_state(), g["hist_U"]) < 1e-10
+
+
+def test_config0_residual_history_1e10(tmpcase):
+    """The parity bar: config-0 plain FP64 path (200x100x2 ramp, Roe, LSQ,
+    Venkatakrishnan K=5, RK5, CFL 0.5, local dt) reproduces the reference's
+    per-iteration L2 density residual to 1e-10 relative over the first 100
+    iterations, and its state at 1000 seeded cells."""
+    g = np.load(f"{GOLD}/config0_hist100.npz")
+    d = tmpcase("c")
+    cases.ramp_case(d, nx=200, ny=100, seed=0, threads=1, nmax=100)
+    import hashlib
+    assert hashlib.sha256(open(d + "mesh.su2", "rb").read()).hexdigest() == str(g["mesh_sha256"])
+    dev = native.Device(native.HostCase(d))
+    res, coef = dev.iterate(100)
+    rel = np.abs(res / g["res"] - 1)
+    print("config0: max rel diff per variable over 100 iterations:", rel.max(axis=0), flush=True)
+    assert rel[:, 0].max() < 1e-10
+    assert rel.max() < 1e-10
+    assert np.max(np.abs(coef[:, [0, 2]] - g["coef"])) <= 1e-10 * np.abs(g["coef"]).max()
+    U = dev.get_state()
+    assert np.max(np.abs(U[:, g["U_idx"]] / g["U_sample"] - 1)) < 1e-10
