@@ -132,3 +132,72 @@ def ramp_case(dirpath: str, nx: int = 200, ny: int = 100, seed: int = 0, threads
               nmax: int = 2000, split: bool = True, **inputs) -> str:
     return write_case(dirpath, ramp_mesh(nx, ny, seed=seed, split=split),
                       ramp_input(threads=threads, nmax=nmax, **inputs))
+
+
+# --------------------------------------------------------------------------
+# Laminar flat plate: x in [x0, 1] (plate = wall for x >= 0, symmetry
+# upstream), y in [0, H] with geometric wall-normal stretching; a seeded
+# fraction of the far-field quads (rows j >= tri_from) split into triangles.
+# Markers: inlet (x=x0), outlet (x=1), top (outlet), sym (y=0, x<0),
+# plate (y=0, x>=0).
+# --------------------------------------------------------------------------
+def plate_mesh(nx: int = 500, ny: int = 100, x0: float = -0.1, y1: float = 5e-5,
+               growth: float = 1.08, tri_frac: float = 0.2, tri_from: int | None = None,
+               jitter: float = 0.0, seed: int = 1, y_shift: float = 0.0) -> Mesh2D:
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(x0, 1.0, nx + 1)
+    # snap the leading edge onto a node
+    i_le = int(np.argmin(np.abs(xs)))
+    xs[i_le] = 0.0
+    ys = np.concatenate([[0.0], y1 * (growth ** np.arange(1, ny + 1) - 1.0) / (growth - 1.0)])
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    if jitter > 0:
+        dx = np.diff(xs).min()
+        for i in range(1, nx):
+            for j in range(1, ny):
+                X[i, j] += rng.uniform(-jitter, jitter) * dx
+                Y[i, j] += rng.uniform(-jitter, jitter) * (ys[j + 1] - ys[j - 1]) * 0.5
+    Y = Y + y_shift        # off the axis for axisymmetric runs
+    nid = lambda i, j: i * (ny + 1) + j          # noqa: E731
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+    if tri_from is None:
+        tri_from = int(0.4 * ny)
+    split = rng.random((nx, ny)) < tri_frac
+    cells = []
+    for i in range(nx):
+        for j in range(ny):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            if j >= tri_from and split[i, j]:
+                cells.append((a, b, c))
+                cells.append((a, c, d))
+            else:
+                cells.append((a, b, c, d))
+    inlet = [(nid(0, j + 1), nid(0, j)) for j in range(ny)]
+    outlet = [(nid(nx, j), nid(nx, j + 1)) for j in range(ny)]
+    top = [(nid(i + 1, ny), nid(i, ny)) for i in range(nx)]
+    sym = [(nid(i, 0), nid(i + 1, 0)) for i in range(i_le)]
+    wall = [(nid(i, 0), nid(i + 1, 0)) for i in range(i_le, nx)]
+    markers = [("inlet", inlet), ("outlet", outlet), ("top", top), ("sym", sym), ("plate", wall)]
+    return Mesh2D(nodes=nodes, cells=cells, markers=[m for m in markers if m[1]])
+
+
+def plate_input(threads: int = 8, nmax: int = 100, **over) -> dict:
+    """Mach 0.3, Re_L = 1e5 at T = 300 K and L = 1 m (p = 1526.1 Pa with
+    Sutherland mu(300 K) = 1.846e-5)."""
+    d = {
+        "pressure": "1526.1", "temperature": "300.", "mach": "0.3", "nx": "1.", "ny": "0.",
+        "pout": "1526.1",
+        "BC:inlet": "inlet", "BC:outlet": "outlet", "BC:top": "outlet", "BC:sym": "symmetry",
+        "BC:plate": "wall",
+        "Nmax": str(nmax), "order": "2", "stages": "5", "threads": str(threads),
+        "flux": "AUSM", "tube": "0", "axisymmetric": "0", "CFL": "0.5", "laminar": "1",
+        "limiter": "0", "limK": "5", "dtLocal": "1", "timeScheme": "RK",
+    }
+    d.update({k: str(v) for k, v in over.items()})
+    return d
+
+
+def plate_case(dirpath: str, nx: int = 500, ny: int = 100, seed: int = 1, threads: int = 8,
+               nmax: int = 100, mesh_kw: dict | None = None, **inputs) -> str:
+    return write_case(dirpath, plate_mesh(nx, ny, seed=seed, **(mesh_kw or {})),
+                      plate_input(threads=threads, nmax=nmax, **inputs))

@@ -1,0 +1,190 @@
+// kern_common.h -- kernel parameter blocks and per-cell / per-boundary-face
+// routines shared by the residual kernels.
+#pragma once
+#include "csu.h"
+#include "device_math.h"
+
+namespace csu {
+
+#define CSU_BLOCK 256
+
+// Scalar options and free-stream data, passed by value.
+struct KP {
+  int nc, nf, nb, nv, order, flux, laminar, sa, sst, limiter, turb1order, axi, visc_blazek;
+  double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t;
+  double pin[7];
+  GasD gas;
+};
+
+// Geometry (device pointers, read-only), passed by value.
+struct Geo {
+  const int *cnp, *cnbr, *cface;                        // [nc], [4][nc], [4][nc]
+  const double *xc, *yc, *omega, *dsxm, *dsym, *dslat;  // [nc]
+  const double *edx, *edy;                              // [4][nc] own-edge midpoint offsets
+  const double *wdist;                                  // [nc] wall distance (SA/SST)
+  const int *fe0, *fe1;                                 // [nf]
+  const double *fdsx, *fdsy, *fds, *fdx0, *fdy0, *fdx1, *fdy1;
+  const int *be0, *btype, *bdonor;                      // [nb]
+  const double *bdsx, *bdsy, *bds, *bxm, *bym, *bnx, *bny, *bds2;
+};
+
+// Mutable per-stage arrays, passed by value.
+struct St {
+  double *P, *Pb, *dPx, *dPy, *phi, *dTx, *dTy;
+  double *fflux, *vflux, *bvflux, *wfric, *pref2, *miT;
+};
+
+// Primitive recovery of one cell with the positivity clamps, reference
+// solverCalcPrimitive (solver.c:1153-1213).  u[] (conservative, in
+// registers) receives the clamped values the reference writes back into U;
+// p[] receives r u v p T [nu~ | k om].
+template <int NV>
+__device__ __forceinline__ void prim_point(const KP& kp, double* u, double* p) {
+  const double r0 = fmax(u[0], kp.rlim);
+  u[0] = r0;
+  double k = 0;
+  if (NV == 6) {
+    k = u[4] / r0;
+    if (k < 1e-14) { k = 1e-14; u[4] = r0 * k; }
+    p[5] = k;
+    double om = u[5] / r0;
+    if (om <= 0.0) { om = 0.0; u[5] = r0 * om; }
+    p[6] = om;
+  }
+  const double uu = u[1] / r0, vv = u[2] / r0;
+  double e = u[3] / r0 - 0.5 * (uu * uu + vv * vv) - k;
+  e = fmax(e, 7.176325e+02);
+  double T = gas_e2T<double>(kp.gas, e);
+  if (T < 1) {
+    T = 1;
+    e = gas_T2e<double>(kp.gas, T);
+    u[3] = (e + 0.5 * (uu * uu + vv * vv) + k) * r0;
+  }
+  p[0] = r0; p[1] = uu; p[2] = vv;
+  p[3] = kp.gas.R * T * r0;
+  p[4] = T;
+  if (NV == 5) {
+    double n = u[4] / r0;
+    n = fmax(n, 0.0);
+    u[4] = n * r0;
+    p[5] = n;
+  }
+}
+
+// Ghost state of a boundary face from the interior primitive Pi (nv+1
+// slots), reference boundaryCalcPrimitive (boundary.c:514-693).
+template <int NV>
+__device__ __forceinline__ void ghost_state(const KP& kp, int type, double dSx, double dSy, double dS,
+                                            const double* Pi, double wall_d, double* Pg) {
+  double PL[4] = {Pi[0], Pi[1], Pi[2], Pi[3]};
+  double Pb[4];
+  const double R = kp.gas.R;
+  if (type == CSU_BC_SYMMETRY) {
+    if (dS > 0) {
+      rotate<double>(PL[1], PL[2], dSx, dSy, dS);
+      PL[1] = 0.0;
+      rotate<double>(PL[1], PL[2], dSx, -dSy, dS);
+    } else {
+      PL[2] = 0.0;
+    }
+    for (int k = 0; k < 4; k++) Pg[k] = PL[k];
+    Pg[4] = Pg[3] / (Pg[0] * R);
+  } else if (type == CSU_BC_INLET) {
+    bc_inlet(kp.gas, kp.pin, PL, Pb, dSx / dS, dSy / dS);
+    for (int k = 0; k < 4; k++) Pg[k] = Pb[k];
+    Pg[4] = Pg[3] / (Pg[0] * R);
+  } else if (type == CSU_BC_OUTLET) {
+    bc_outlet(kp.gas, kp.pout, PL, Pb, dSx / dS, dSy / dS);
+    for (int k = 0; k < 4; k++) Pg[k] = Pb[k];
+    Pg[4] = Pg[3] / (Pg[0] * R);
+  } else {
+    if (kp.laminar == 1 || kp.sa == 1) {
+      if (type == CSU_BC_WALLT) {
+        Pg[0] = PL[3] / (R * kp.twall);
+        Pg[1] = 0.0; Pg[2] = 0.0; Pg[3] = PL[3]; Pg[4] = kp.twall;
+      } else {
+        Pg[0] = PL[0]; Pg[1] = 0.0; Pg[2] = 0.0; Pg[3] = PL[3];
+        Pg[4] = Pg[3] / (Pg[0] * R);
+      }
+    } else {
+      if (dS > 0) {
+        bc_wall(kp.gas, PL, Pb, dSx / dS, dSy / dS);
+        for (int k = 0; k < 4; k++) Pg[k] = Pb[k];
+      } else {
+        PL[2] = 0.0;
+        for (int k = 0; k < 4; k++) Pg[k] = PL[k];
+      }
+      Pg[4] = Pg[3] / (Pg[0] * R);
+    }
+  }
+  if (NV == 5) {  // SA nu~
+    if (type == CSU_BC_SYMMETRY || type == CSU_BC_OUTLET) Pg[5] = Pi[5];
+    else if (type == CSU_BC_INLET) Pg[5] = kp.pin[5];
+    else Pg[5] = 0.0;
+  }
+  (void)wall_d;
+}
+
+// Convective boundary flux of one boundary face, reference boundaryCalc /
+// boundaryCalc_sa (boundary.c:143-341).  Adds f*dS to R when dS > 0.
+template <int NV>
+__device__ __forceinline__ void bc_flux(const KP& kp, int type, double dSx, double dSy, double dS,
+                                        const double* Pi, double* R) {
+  double PL[NV], Pb[NV], fl[NV];
+  PL[0] = Pi[0]; PL[1] = Pi[1]; PL[2] = Pi[2]; PL[3] = Pi[3];
+  if (NV == 5) PL[4] = Pi[5];
+  constexpr int NSC = NV == 5 ? 1 : 0;
+  if (type == CSU_BC_SYMMETRY) {
+    rotate<double>(PL[1], PL[2], dSx, dSy, dS);
+    double Pm[NV];
+    Pm[0] = PL[0]; Pm[1] = -PL[1]; Pm[2] = PL[2]; Pm[3] = PL[3];
+    if (NV == 5) Pm[4] = PL[4];
+    flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
+                               Pm[0], Pm[1], Pm[2], Pm[3], Pm + 4, 0.0, 0.0, fl);
+  } else if (type == CSU_BC_INLET || type == CSU_BC_OUTLET) {
+    if (type == CSU_BC_INLET) bc_inlet(kp.gas, kp.pin, PL, Pb, dSx / dS, dSy / dS);
+    else bc_outlet(kp.gas, kp.pout, PL, Pb, dSx / dS, dSy / dS);
+    if (NV == 5) Pb[4] = (type == CSU_BC_INLET) ? kp.pin[5] : PL[4];
+    rotate<double>(PL[1], PL[2], dSx, dSy, dS);
+    rotate<double>(Pb[1], Pb[2], dSx, dSy, dS);
+    flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
+                               Pb[0], Pb[1], Pb[2], Pb[3], Pb + 4, 0.0, 0.0, fl);
+  } else {
+    if (kp.laminar == 1 || NV == 5) {
+      rotate<double>(PL[1], PL[2], dSx, dSy, dS);
+      double Pm[NV];
+      Pm[0] = PL[0]; Pm[1] = -PL[1]; Pm[2] = -PL[2]; Pm[3] = PL[3];
+      if (NV == 5) Pm[4] = PL[4];
+      flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
+                                 Pm[0], Pm[1], Pm[2], Pm[3], Pm + 4, 0.0, 0.0, fl);
+    } else {
+      bc_wall(kp.gas, PL, Pb, dSx / dS, dSy / dS);
+      rotate<double>(Pb[1], Pb[2], dSx, dSy, dS);
+      flux_free<double>(kp.gas, Pb[0], Pb[1], Pb[2], Pb[3], fl);
+    }
+  }
+  rotate<double>(fl[1], fl[2], dSx, -dSy, dS);
+  if (dS > 0) {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) R[kk] += fl[kk] * dS;
+  }
+}
+
+// Local time step of one cell, reference solverLocalTimeStep + solverCalcVel
+// (solver.c:895-923, 420-439), from the unclamped conservative state.
+template <int NV>
+__device__ __forceinline__ double cell_dt(const KP& kp, const double* u, double om, double sx, double sy) {
+  double k = 0;
+  if (NV == 6) k = u[4] / u[0];
+  double E = u[3] / u[0];
+  double uu = u[1] / u[0];
+  double vv = u[2] / u[0];
+  double aux = (E - (uu * uu + vv * vv) / 2 - k);
+  double T = gas_e2T<double>(kp.gas, aux);
+  double c = gas_T2c<double>(kp.gas, T);
+  double Lx = (fabs(uu) + c) * sx;
+  double Ly = (fabs(vv) + c) * sy;
+  return om / (Lx + Ly);
+}
+
+}  // namespace csu

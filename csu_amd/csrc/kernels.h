@@ -1,0 +1,535 @@
+// kernels.h -- the residual-and-update kernels (one mesh, SoA FP64 state).
+//
+// Per explicit stage (reference solverCalcR + solverRK, solver.c:685-766):
+//   k_grad    cells  ghost states of the cell's boundary faces, least-squares
+//                    gradients (double-double sums), Venkatakrishnan phi;
+//                    viscous: T gradient, boundary viscous fluxes, wall
+//                    friction (all need only the cell's own fresh data)
+//   k_flux    faces  reconstruction + Riemann flux (+ viscous face flux)
+//   k_update  cells  per-cell sums in the reference's order, then the RK
+//                    update fused with the next stage's primitive recovery;
+//                    on the last stage also the next time step and the
+//                    residual-norm block partials
+// Every cell gathers its own faces: no atomics, fixed summation order.
+#pragma once
+#include "kern_common.h"
+#include "kern_visc.h"
+
+namespace csu {
+
+enum { F_RK = 1, F_CLAMP_UN = 2, F_LAST = 4, F_NEXT_LOCAL = 8, F_STORE_R = 16, F_CUR_LOCAL = 32 };
+
+// Standalone primitive pass (first stage after the state was set from the
+// host).  Clamped values are written back to U, as the reference does.
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_prim(KP kp, double* __restrict__ U, double* __restrict__ P) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc;
+  if (i >= nc) return;
+  double u[NV], u0[NV], p[NV + 1];
+#pragma unroll
+  for (int k = 0; k < NV; k++) u0[k] = u[k] = U[k * nc + i];
+  prim_point<NV>(kp, u, p);
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+    if (__double_as_longlong(u[k]) != __double_as_longlong(u0[k])) U[k * nc + i] = u[k];
+#pragma unroll
+  for (int m = 0; m < NV + 1; m++) P[m * nc + i] = p[m];
+}
+
+// LSQ gradient of one variable from its differences to the neighbours, with
+// the normal-equation sums in double-double (the reference uses x87 long
+// double accumulators, solver.c:1041-1073).
+template <int NP>
+__device__ __forceinline__ void lsq_grad(int np, const double* dx, const double* dy, const double* du,
+                                         const dd& a, const dd& b, const dd& nb, const dd& c, const dd& det,
+                                         double& gx, double& gy) {
+  dd A = {0, 0}, B = {0, 0};
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    if (j >= np) break;
+    A = dd_add_d(A, dx[j] * du[j]);
+    B = dd_add_d(B, dy[j] * du[j]);
+  }
+  gx = dd_div(dd_sub(dd_mul(c, A), dd_mul(b, B)), det).hi;
+  gy = dd_div(dd_add(dd_mul(nb, A), dd_mul(a, B)), det).hi;
+}
+
+// Gradient + limiter (+ boundary work) kernel.  Reference solverUpdateGrad /
+// solverCalcGrad3 (solver.c:451-474, 1031-1074), solverGrad_T
+// (solver.c:477-490), the limiter loop of inter() (solver.c:496-548,
+// limiter.c:98-147), boundaryCalcPrimitive (boundary.c:514-693),
+// boundaryFaceViscFlux (laminar.c:128-346), boundaryCalcFrictionWall
+// (boundary.c:695-742).
+template <int NV, bool VISC>
+__global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
+  constexpr int NS = NV + 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc, nb = kp.nb;
+  if (i >= nc) return;
+  const int np = g.cnp[i];
+  const double x0 = g.xc[i], y0 = g.yc[i];
+  double Pi[NS];
+#pragma unroll
+  for (int m = 0; m < NS; m++) Pi[m] = s.P[m * nc + i];
+  double Pn[4][NS];
+  double dx[4], dy[4];
+  int gb[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    gb[j] = -1;
+    if (j >= np) break;
+    int n = g.cnbr[j * nc + i];
+    if (n >= 0) {
+      dx[j] = g.xc[n] - x0;
+      dy[j] = g.yc[n] - y0;
+#pragma unroll
+      for (int m = 0; m < NS; m++) Pn[j][m] = s.P[m * nc + n];
+    } else {
+      int q = -1 - n;
+      gb[j] = q;
+      dx[j] = g.bxm[q] - x0;
+      dy[j] = g.bym[q] - y0;
+      ghost_state<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, 0.0, Pn[j]);
+#pragma unroll
+      for (int m = 0; m < NS; m++) s.Pb[m * nb + q] = Pn[j][m];
+    }
+  }
+  dd a = {0, 0}, b = {0, 0}, c = {0, 0};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (j >= np) break;
+    a = dd_add_d(a, dx[j] * dx[j]);
+    b = dd_add_d(b, dx[j] * dy[j]);
+    c = dd_add_d(c, dy[j] * dy[j]);
+  }
+  const dd det = dd_sub(dd_mul(a, c), dd_mul(b, b));
+  const dd nb_ = dd_neg(b);
+  double lim_aux = 0.0;
+  if (kp.limiter == 0) {
+    lim_aux = kp.limK * sqrt(g.omega[i] / kp.volmax);
+    lim_aux = lim_aux * lim_aux * lim_aux;
+  }
+  double gux = 0, guy = 0, gvx = 0, gvy = 0;
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) {
+    const int m = kk > 3 ? kk + 1 : kk;
+    const double u0 = Pi[m];
+    double du[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) du[j] = j < np ? Pn[j][m] - u0 : 0.0;
+    double gx, gy;
+    lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gx, gy);
+    s.dPx[kk * nc + i] = gx;
+    s.dPy[kk * nc + i] = gy;
+    if (kk == 1) { gux = gx; guy = gy; }
+    if (kk == 2) { gvx = gx; gvy = gy; }
+    if (kp.order == 2) {
+      double pref2 = kp.limiter == 0 ? s.pref2[kk] * lim_aux : s.pref2[kk];
+      double Pmin = u0, Pmax = u0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (j >= np) break;
+        Pmax = fmax(Pmax, Pn[j][m]);
+        Pmin = fmin(Pmin, Pn[j][m]);
+      }
+      double ph = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (j >= np) break;
+        double d2 = gx * g.edx[j * nc + i] + gy * g.edy[j * nc + i];
+        double p0 = limiter_v2<double>(u0, Pmin, Pmax, d2, pref2);
+        ph = (j == 0) ? p0 : fmin(ph, p0);
+      }
+      s.phi[kk * nc + i] = ph;
+    }
+  }
+  if (VISC) {
+    double du[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) du[j] = j < np ? Pn[j][4] - Pi[4] : 0.0;
+    double gTx, gTy;
+    lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gTx, gTy);
+    s.dTx[i] = gTx;
+    s.dTy[i] = gTy;
+    const double gx3[3] = {gux, gvx, gTx}, gy3[3] = {guy, gvy, gTy};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int q = gb[j];
+      if (q < 0) continue;
+      const int t = g.btype[q];
+      double f[3];
+      laminar_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, f);
+      s.bvflux[0 * nb + q] = f[0];
+      s.bvflux[1 * nb + q] = f[1];
+      s.bvflux[2 * nb + q] = f[2];
+      if (t == CSU_BC_WALL || t == CSU_BC_WALLT) {
+        double fx, fy;
+        wall_friction(g.bdsx[q], g.bdsy[q], dx[j], dy[j], Pi, gx3, gy3, fx, fy);
+        s.wfric[q] = fx;
+        s.wfric[nb + q] = fy;
+      }
+    }
+  }
+}
+
+// Limiter type 1 global range (limiter.c:64-94): per-block min/max partials.
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_lim1_partial(KP kp, const double* __restrict__ P,
+                                                            double* __restrict__ red) {
+  __shared__ double smin[NV][CSU_BLOCK], smax[NV][CSU_BLOCK];
+  int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
+  const int nc = kp.nc;
+  int ii = i < nc ? i : 0;
+#pragma unroll
+  for (int jj = 0; jj < NV; jj++) {
+    int mm = jj == 4 ? 5 : jj;  // limiter.c:350-357 (slot 5 also for jj == 5)
+    double v = P[mm * nc + ii];
+    smin[jj][t] = v; smax[jj][t] = v;
+  }
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st)
+      for (int jj = 0; jj < NV; jj++) {
+        smin[jj][t] = fmin(smin[jj][t], smin[jj][t + st]);
+        smax[jj][t] = fmax(smax[jj][t], smax[jj][t + st]);
+      }
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int jj = 0; jj < NV; jj++) {
+      red[(2 * jj) * gridDim.x + blockIdx.x] = smin[jj][0];
+      red[(2 * jj + 1) * gridDim.x + blockIdx.x] = smax[jj][0];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, const double* __restrict__ red,
+                                                          double* __restrict__ pref2) {
+  __shared__ double smin[CSU_BLOCK], smax[CSU_BLOCK];
+  int t = threadIdx.x;
+  for (int jj = 0; jj < NV; jj++) {
+    double mn = red[(2 * jj) * nblocks], mx = red[(2 * jj + 1) * nblocks];
+    for (int b = t; b < nblocks; b += CSU_BLOCK) {
+      mn = fmin(mn, red[(2 * jj) * nblocks + b]);
+      mx = fmax(mx, red[(2 * jj + 1) * nblocks + b]);
+    }
+    smin[t] = mn; smax[t] = mx;
+    __syncthreads();
+    for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+      if (t < st) { smin[t] = fmin(smin[t], smin[t + st]); smax[t] = fmax(smax[t], smax[t + st]); }
+      __syncthreads();
+    }
+    if (t == 0) {
+      double aux = (smax[0] - smin[0]) * kp.limK;
+      pref2[jj] = aux * aux;
+    }
+    __syncthreads();
+  }
+}
+
+// Interior face flux, reference inter() face loop (solver.c:550-644), and
+// for viscous runs the laminar face flux (laminar.c:17-75).
+template <int NV, bool VISC>
+__global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc, nf = kp.nf;
+  if (f >= nf) return;
+  const int e0 = g.fe0[f], e1 = g.fe1[f];
+  const double dSx = g.fdsx[f], dSy = g.fdsy[f], dS = g.fds[f];
+  double PL[NV], PR[NV];
+  if (kp.order == 2) {
+    const double dx0 = g.fdx0[f], dy0 = g.fdy0[f], dx1 = g.fdx1[f], dy1 = g.fdy1[f];
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) {
+      const int m = kk > 3 ? kk + 1 : kk;
+      if (kk > 3 && kp.turb1order) {
+        PL[kk] = s.P[m * nc + e0];
+        PR[kk] = s.P[m * nc + e1];
+      } else {
+        PL[kk] = s.P[m * nc + e0] + (s.dPx[kk * nc + e0] * dx0 + s.dPy[kk * nc + e0] * dy0) * s.phi[kk * nc + e0];
+        PR[kk] = s.P[m * nc + e1] + (s.dPx[kk * nc + e1] * dx1 + s.dPy[kk * nc + e1] * dy1) * s.phi[kk * nc + e1];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) {
+      const int m = kk > 3 ? kk + 1 : kk;
+      PL[kk] = s.P[m * nc + e0];
+      PR[kk] = s.P[m * nc + e1];
+    }
+  }
+  rotate<double>(PL[1], PL[2], dSx, dSy, dS);
+  rotate<double>(PR[1], PR[2], dSx, dSy, dS);
+  double fl[NV];
+  constexpr int NSC = NV == 5 ? 1 : 0;
+  flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
+                             PR[0], PR[1], PR[2], PR[3], PR + 4, 0.0, 0.0, fl);
+  rotate<double>(fl[1], fl[2], dSx, -dSy, dS);
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) s.fflux[kk * nf + f] = fl[kk] * dS;
+  if (VISC && NV == 4) {
+    double P0[5], P1[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) { P0[m] = s.P[m * nc + e0]; P1[m] = s.P[m * nc + e1]; }
+    const double g0x[3] = {s.dPx[nc + e0], s.dPx[2 * nc + e0], s.dTx[e0]};
+    const double g0y[3] = {s.dPy[nc + e0], s.dPy[2 * nc + e0], s.dTy[e0]};
+    const double g1x[3] = {s.dPx[nc + e1], s.dPx[2 * nc + e1], s.dTx[e1]};
+    const double g1y[3] = {s.dPy[nc + e1], s.dPy[2 * nc + e1], s.dTy[e1]};
+    double F[3];
+    laminar_face(kp, dSx, dSy, g.xc[e1] - g.xc[e0], g.yc[e1] - g.yc[e0], P0, P1, g0x, g0y, g1x, g1y, F);
+    s.vflux[0 * nf + f] = F[0];
+    s.vflux[1 * nf + f] = F[1];
+    s.vflux[2 * nf + f] = F[2];
+  }
+}
+
+// Residual sums + RK update.  Order of the per-cell sums follows
+// solverCalcR: interior convective faces (solver.c:646-667), boundary
+// convective fluxes (boundary.c:447-464), axisymmetric pressure
+// (solver.c:670-683), interior viscous faces (laminar.c:77-98), boundary
+// viscous fluxes (laminar.c:101-125).  Then solverRK (solver.c:741-766)
+// fused with the next stage's solverCalcPrimitive, and on the last stage
+// the next time step (solver.c:895-950) and solverCalcRes partials
+// (solver.c:867-892).
+template <int NV, bool VISC>
+__global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int flags, double sigma, double rk_a,
+                                                      double cfl, double* dtl, const double* dtg, double* Un,
+                                                      double* __restrict__ Uout, double* __restrict__ Rout,
+                                                      double* __restrict__ red_res, double* __restrict__ red_dt) {
+  __shared__ double sred[NV + 1][CSU_BLOCK];
+  const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
+  const int nc = kp.nc, nf = kp.nf, nb = kp.nb;
+  const bool active = i < nc;
+  double R[NV];
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) R[kk] = 0.0;
+  double dt_next = INFINITY;
+  if (active) {
+    const int np = g.cnp[i];
+    int cf[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      cf[j] = g.cface[j * nc + i];
+      const int f = cf[j];
+      if (f > 0) {
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++) R[kk] += s.fflux[kk * nf + f - 1];
+      } else if (f < 0) {
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++) R[kk] -= s.fflux[kk * nf - f - 1];
+      }
+    }
+    double Pi[NV + 1];
+    bool loaded = false;
+    int gbq[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      gbq[j] = -1;
+      if (j >= np) break;
+      const int n = g.cnbr[j * nc + i];
+      if (n < 0) {
+        const int q = -1 - n;
+        gbq[j] = q;
+        if (!loaded) {
+#pragma unroll
+          for (int m = 0; m < NV + 1; m++) Pi[m] = s.P[m * nc + i];
+          loaded = true;
+        }
+        bc_flux<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, R);
+      }
+    }
+    if (kp.axi == 1) {
+      R[2] -= s.P[3 * nc + i] * g.dslat[i];
+      if (NV == 6) R[2] -= (2. / 3.) * s.P[i] * s.P[5 * nc + i] * g.dslat[i];
+    }
+    if (VISC && NV == 4) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int f = cf[j];
+        if (f > 0) {
+#pragma unroll
+          for (int kk = 1; kk < 4; kk++) R[kk] -= s.vflux[(kk - 1) * nf + f - 1];
+        } else if (f < 0) {
+#pragma unroll
+          for (int kk = 1; kk < 4; kk++) R[kk] += s.vflux[(kk - 1) * nf - f - 1];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int q = gbq[j];
+        if (q < 0) continue;
+        R[1] -= s.bvflux[q];
+        R[2] -= s.bvflux[nb + q];
+        if (g.btype[q] == CSU_BC_WALL) {
+          // adiabatic wall: the reference subtracts the energy flux of the
+          // boundary face processed before it (laminar.c:101-117, 230-269)
+          const int dn = g.bdonor[q];
+          R[3] -= dn >= 0 ? s.bvflux[2 * nb + dn] : 0.0;
+        } else {
+          R[3] -= s.bvflux[2 * nb + q];
+        }
+      }
+    }
+    if (flags & F_STORE_R) {
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) Rout[kk * nc + i] = R[kk];
+    }
+    if (flags & F_RK) {
+      double un[NV], un0[NV], p[NV + 1];
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) un0[kk] = un[kk] = Un[kk * nc + i];
+      if (flags & F_CLAMP_UN) {
+        // first stage: the reference clamps U^n in place (solverCalcR(U))
+        prim_point<NV>(kp, un, p);
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++)
+          if (__double_as_longlong(un[kk]) != __double_as_longlong(un0[kk])) Un[kk * nc + i] = un[kk];
+      }
+      const double dt = (flags & F_CUR_LOCAL) ? dtl[i] : dtg[0];
+      const double om = g.omega[i];
+      double unew[NV];
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) unew[kk] = un[kk] - dt * sigma * rk_a * R[kk] / om;
+      if (flags & F_LAST) {
+        // U^{n+1} stays unclamped until the next residual, as in the reference
+        dt_next = cell_dt<NV>(kp, unew, om, g.dsxm[i], g.dsym[i]);
+        if (flags & F_NEXT_LOCAL) dtl[i] = cfl * dt_next;
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++) Uout[kk * nc + i] = unew[kk];
+        prim_point<NV>(kp, unew, p);
+      } else {
+        prim_point<NV>(kp, unew, p);
+#pragma unroll
+        for (int kk = 0; kk < NV; kk++) Uout[kk * nc + i] = unew[kk];
+      }
+#pragma unroll
+      for (int m = 0; m < NV + 1; m++) s.P[m * nc + i] = p[m];
+    }
+  }
+  if (!(flags & F_LAST)) return;
+  // residual-norm partials (and global time-step minimum): fixed tree
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) sred[kk][t] = R[kk] * R[kk];
+  sred[NV][t] = dt_next;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st) {
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) sred[kk][t] += sred[kk][t + st];
+      sred[NV][t] = fmin(sred[NV][t], sred[NV][t + st]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) red_res[kk * gridDim.x + blockIdx.x] = sred[kk][0];
+    red_dt[blockIdx.x] = sred[NV][0];
+  }
+}
+
+// Time step from the host-set state (solverCalcDt, solver.c:925-950):
+// local values or per-block minima for the global step.
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_dt(KP kp, Geo g, int local, double cfl, const double* __restrict__ U,
+                                                  double* __restrict__ dtl, double* __restrict__ red) {
+  __shared__ double smin[CSU_BLOCK];
+  int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
+  const int nc = kp.nc;
+  double val = INFINITY;
+  if (i < nc) {
+    double u[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) u[k] = U[k * nc + i];
+    val = cell_dt<NV>(kp, u, g.omega[i], g.dsxm[i], g.dsym[i]);
+    if (local) dtl[i] = cfl * val;
+  }
+  if (local) return;
+  smin[t] = val;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st) smin[t] = fmin(smin[t], smin[t + st]);
+    __syncthreads();
+  }
+  if (t == 0) red[blockIdx.x] = smin[0];
+}
+
+__global__ __launch_bounds__(CSU_BLOCK) void k_dt_final(int nblocks, double cfl, const double* __restrict__ red,
+                                                        double* __restrict__ dtglob) {
+  __shared__ double smin[CSU_BLOCK];
+  int t = threadIdx.x;
+  double m = INFINITY;
+  for (int b = t; b < nblocks; b += CSU_BLOCK) m = fmin(m, red[b]);
+  smin[t] = m;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st) smin[t] = fmin(smin[t], smin[t + st]);
+    __syncthreads();
+  }
+  if (t == 0) dtglob[0] = smin[0] * cfl;
+}
+
+// Final level of the residual norm: one block, each thread sums a strided
+// slice of the block partials for all variables at once, then a fixed tree.
+#define RES_THREADS 1024
+template <int NV>
+__global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, const double* __restrict__ red,
+                                                           double* __restrict__ res) {
+  __shared__ double s[NV][RES_THREADS];
+  const int t = threadIdx.x;
+  double a[NV];
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) a[kk] = 0.0;
+  for (int b = t; b < nblocks; b += RES_THREADS) {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) a[kk] += red[kk * nblocks + b];
+  }
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) s[kk][t] = a[kk];
+  __syncthreads();
+  for (int st = RES_THREADS / 2; st > 0; st >>= 1) {
+    if (t < st) {
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) s[kk][t] += s[kk][t + st];
+    }
+    __syncthreads();
+  }
+  if (t < NV) res[t] = sqrt(s[t][0] / nc);
+}
+
+// Wall force coefficients Cx_p, Cx_v, Cy_p, Cy_v, reference solverCalcCoeff3
+// (solver.c:1342-1401); friction per face comes from k_grad.
+__global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const int* __restrict__ wall,
+                                                      const double* __restrict__ Pb, const double* __restrict__ wfric,
+                                                      const double* __restrict__ bdsx, const double* __restrict__ bdsy,
+                                                      double sref, int visc, double* __restrict__ coef) {
+  __shared__ double sh[4][CSU_BLOCK];
+  int t = threadIdx.x;
+  const double r = kp.pin[0], u = kp.pin[1], v = kp.pin[2], Pinf = kp.pin[3];
+  const double q = 0.5 * r * (u * u + v * v);
+  double ax = 0.0, ay = 0.0, vx = 0.0, vy = 0.0;
+  for (int w = t; w < nwall; w += CSU_BLOCK) {
+    int gq = wall[w];
+    double cp = (Pb[3 * kp.nb + gq] - Pinf) / q;
+    ax += cp * bdsx[gq];
+    ay += cp * bdsy[gq];
+    if (visc) {
+      vx -= wfric[gq] / q;
+      vy -= wfric[kp.nb + gq] / q;
+    }
+  }
+  sh[0][t] = ax; sh[1][t] = vx; sh[2][t] = ay; sh[3][t] = vy;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st)
+      for (int k = 0; k < 4; k++) sh[k][t] += sh[k][t + st];
+    __syncthreads();
+  }
+  if (t == 0) {
+    double c[4] = {sh[0][0] / sref, sh[1][0] / sref, sh[2][0] / sref, sh[3][0] / sref};
+    if (kp.axi) { c[0] *= 2 * M_PI; c[1] *= 2 * M_PI; }
+    for (int k = 0; k < 4; k++) coef[k] = c[k];
+  }
+}
+
+}  // namespace csu

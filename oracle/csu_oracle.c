@@ -97,6 +97,10 @@ static double gas_T2c(const ogas* g, double T)           /* gasprop.c:168-184 */
     return sqrt(((Cv + g->R) / Cv) * g->R * T);
 }
 
+static double gas_T2Cp(const ogas* g, double T) { return gas_T2Cv(g, T) + g->R; }   /* gasprop.c:174-177 */
+
+static double sutherland(double T) { return 1.458e-6 * T * sqrt(T) / (T + 110.4); }  /* solver.c:1222-1225 */
+
 static double gas_critic_H2c(const ogas* g, double H)    /* gasprop.c:219-274 */
 {
     double c2 = 2 * (g->gamma - 1) * H / (g->gamma + 1);
@@ -678,7 +682,185 @@ static void boundary(csu_oracle* o)
     }
 }
 
-/* solverCalcR, solver.c:685-739 (Euler part) */
+/* interAxisPressure, solver.c:670-683 (lateral area: mesh.c:404-418) */
+static void axis_pressure(csu_oracle* o)
+{
+    const int nc = o->nc, ns = o->nv + 1;
+    for (int i = 0; i < nc; i++) {
+        const int* v = o->cv + 4 * i;
+        double dS = tri_omega(o, v[0], v[1], v[2]);
+        if (nvert(o, i) == 4) dS += tri_omega(o, v[2], v[3], v[0]);
+        const double* P = o->P + (size_t)i * ns;
+        o->R[2 * (size_t)nc + i] -= P[3] * dS;
+        if (o->prm.sst) o->R[2 * (size_t)nc + i] -= (2. / 3.) * P[0] * P[5] * dS;
+    }
+}
+
+/* solverGrad_T, solver.c:477-490: T gradient into slot 3 */
+static void grad_T(csu_oracle* o)
+{
+#pragma omp parallel for
+    for (int i = 0; i < o->nc; i++) {
+        double gx, gy;
+        grad_lsq(o, i, 4, &gx, &gy);
+        o->dPx[3 * (size_t)o->nc + i] = gx;
+        o->dPy[3 * (size_t)o->nc + i] = gy;
+    }
+}
+
+/* face stress/heat flux with the edge-direction correction, laminar.c:33-74 */
+static void laminar_face(const csu_oracle* o, int f, double* F)
+{
+    const int nc = o->nc, ns = o->nv + 1;
+    const int* q = o->face + 4 * f;
+    int e0 = q[0], e1 = q[1];
+    const double *P0 = o->P + (size_t)e0 * ns, *P1 = o->P + (size_t)e1 * ns;
+    const double *gx = o->dPx, *gy = o->dPy;
+    double dSx, dSy, x0, y0, x1, y1;
+    face_ds(o, q[2], q[3], &dSx, &dSy);
+    double duxm = (gx[nc + e0] + gx[nc + e1]) * 0.5, dvxm = (gx[2 * nc + e0] + gx[2 * nc + e1]) * 0.5;
+    double dTxm = (gx[3 * nc + e0] + gx[3 * nc + e1]) * 0.5;
+    double duym = (gy[nc + e0] + gy[nc + e1]) * 0.5, dvym = (gy[2 * nc + e0] + gy[2 * nc + e1]) * 0.5;
+    double dTym = (gy[3 * nc + e0] + gy[3 * nc + e1]) * 0.5;
+    cell_center(o, e0, &x0, &y0);
+    cell_center(o, e1, &x1, &y1);
+    double dx = x1 - x0, dy = y1 - y0, L = sqrt(dx * dx + dy * dy);
+    double dul = (P1[1] - P0[1]) / L, dvl = (P1[2] - P0[2]) / L, dTl = (P1[4] - P0[4]) / L;
+    double dux = duxm + (dul - (duxm * dx + duym * dy) / L) * dx / L;
+    double duy = duym + (dul - (duxm * dx + duym * dy) / L) * dy / L;
+    double dvx = dvxm + (dvl - (dvxm * dx + dvym * dy) / L) * dx / L;
+    double dvy = dvym + (dvl - (dvxm * dx + dvym * dy) / L) * dy / L;
+    double dTx = dTxm + (dTl - (dTxm * dx + dTym * dy) / L) * dx / L;
+    double dTy = dTym + (dTl - (dTxm * dx + dTym * dy) / L) * dy / L;
+    double T = (P1[4] + P0[4]) * 0.5, mi = sutherland(T);
+    double k = gas_T2Cp(&o->gas, T) * mi / o->prm.pr;
+    double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+    double u = (P1[1] + P0[1]) * 0.5, v = (P1[2] + P0[2]) * 0.5;
+    F[1] = txx * dSx + txy * dSy;
+    F[2] = txy * dSx + tyy * dSy;
+    F[3] = (txx * dSx + txy * dSy) * u + (txy * dSx + tyy * dSy) * v + k * (dTx * dSx + dTy * dSy);
+}
+
+/* laminarInter, laminar.c:15-99 */
+static void laminar_inter(csu_oracle* o)
+{
+    const int nc = o->nc, nf = o->nf;
+#pragma omp parallel for
+    for (int f = 0; f < nf; f++) {
+        double F[4];
+        laminar_face(o, f, F);
+        for (int k = 1; k < 4; k++) o->flux[(size_t)k * nf + f] = F[k];
+    }
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < 4; j++) {
+            int f = o->cface[4 * i + j];
+            if (f > 0) for (int k = 1; k < 4; k++) o->R[(size_t)k * nc + i] -= o->flux[(size_t)k * nf + f - 1];
+            else if (f < 0) for (int k = 1; k < 4; k++) o->R[(size_t)k * nc + i] += o->flux[(size_t)k * nf - f - 1];
+        }
+}
+
+/* wall-corrected velocity gradients of boundary face g, laminar.c:230-257 */
+static void wall_grads(const csu_oracle* o, int g, double ub, double vb, double* du, double* dv,
+                       double* L_out, double* dxy)
+{
+    const int nc = o->nc, ns = o->nv + 1, e0 = o->bface[3 * g];
+    const double* P = o->P + (size_t)e0 * ns;
+    double x0, y0, x1, y1;
+    cell_center(o, e0, &x0, &y0);
+    bface_center(o, g, &x1, &y1);
+    double dx = x1 - x0, dy = y1 - y0, L = sqrt(dx * dx + dy * dy);
+    double dul = (ub - P[1]) / L, dvl = (vb - P[2]) / L;
+    double duxm = o->dPx[nc + e0], dvxm = o->dPx[2 * nc + e0], duym = o->dPy[nc + e0], dvym = o->dPy[2 * nc + e0];
+    du[0] = duxm + (dul - (duxm * dx + duym * dy) / L) * dx / L;
+    du[1] = duym + (dul - (duxm * dx + duym * dy) / L) * dy / L;
+    dv[0] = dvxm + (dvl - (dvxm * dx + dvym * dy) / L) * dx / L;
+    dv[1] = dvym + (dvl - (dvxm * dx + dvym * dy) / L) * dy / L;
+    *L_out = L; dxy[0] = dx; dxy[1] = dy;
+}
+
+/* boundaryFaceViscFlux, laminar.c:128-346 (f[3] of an adiabatic wall is 0:
+   u = v = 0 there and no heat flux; the reference leaves it unset) */
+static void laminar_bc_flux(const csu_oracle* o, int g, double* f)
+{
+    const int nc = o->nc, ns = o->nv + 1, e0 = o->bface[3 * g], t = o->btype[g];
+    const int p0 = o->bface[3 * g + 1], p1 = o->bface[3 * g + 2];
+    const double* P = o->P + (size_t)e0 * ns;
+    double dux, duy, dvx, dvy, dTx, dTy, dSx, dSy;
+    f[3] = 0.0;
+    if (t == CSU_BC_SYMMETRY) {
+        double x0 = o->xy[2 * p0], y0 = o->xy[2 * p0 + 1], x1 = o->xy[2 * p1], y1 = o->xy[2 * p1 + 1];
+        double sx = y1 - y0, sy = -(x1 - x0), dS = sqrt(sx * sx + sy * sy);    /* meshCalcDS2 */
+        double nx = sx / dS, ny = sy / dS;
+        if (o->axi == 1) dS = dS * (y0 + y1) * 0.5;
+        double duxm = o->dPx[nc + e0], dvxm = o->dPx[2 * nc + e0], dTxm = o->dPx[3 * nc + e0];
+        double duym = o->dPy[nc + e0], dvym = o->dPy[2 * nc + e0], dTym = o->dPy[3 * nc + e0];
+        dux = duxm - (duxm * nx + duym * ny) * nx; duy = duym - (duxm * nx + duym * ny) * ny;
+        dvx = dvxm - (dvxm * nx + dvym * ny) * nx; dvy = dvym - (dvxm * nx + dvym * ny) * ny;
+        dTx = dTxm - (dTxm * nx + dTym * ny) * nx; dTy = dTym - (dTxm * nx + dTym * ny) * ny;
+        double T = P[4], mi = sutherland(T), k = gas_T2Cp(&o->gas, T) * mi / o->prm.pr;
+        double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+        double u = P[1], v = P[2];
+        f[1] = (txx * nx + txy * ny) * dS;
+        f[2] = (txy * nx + tyy * ny) * dS;
+        f[3] = (u * (txx * nx + txy * ny) + v * (txy * nx + tyy * ny) + k * (dTx * nx + dTy * ny)) * dS;
+        return;
+    }
+    face_ds(o, p0, p1, &dSx, &dSy);
+    if (t == CSU_BC_INLET || t == CSU_BC_WALL || t == CSU_BC_WALLT) {
+        const double* Pin = o->prm.pin;
+        double du[2], dv[2], L, dxy[2];
+        wall_grads(o, g, t == CSU_BC_INLET ? Pin[1] : 0, t == CSU_BC_INLET ? Pin[2] : 0, du, dv, &L, dxy);
+        dux = du[0]; duy = du[1]; dvx = dv[0]; dvy = dv[1];
+        double T = t == CSU_BC_WALLT ? o->prm.twall : P[4];
+        double mi = sutherland(T);
+        double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+        f[1] = txx * dSx + txy * dSy;
+        f[2] = txy * dSx + tyy * dSy;
+        if (t == CSU_BC_WALL) return;
+        double Tb = t == CSU_BC_INLET ? Pin[4] : o->prm.twall;
+        double dTl = (Tb - P[4]) / L, dx = dxy[0], dy = dxy[1];
+        double dTxm = o->dPx[3 * nc + e0], dTym = o->dPy[3 * nc + e0];
+        dTx = dTxm + (dTl - (dTxm * dx + dTym * dy) / L) * dx / L;
+        dTy = dTym + (dTl - (dTxm * dx + dTym * dy) / L) * dy / L;
+        double k = gas_T2Cp(&o->gas, T) * mi / o->prm.pr;
+        if (t == CSU_BC_WALLT) { f[3] = k * (dTx * dSx + dTy * dSy); return; }
+        double u = P[1], v = P[2];
+        f[3] = u * (txx * dSx + txy * dSy) + v * (txy * dSx + tyy * dSy) + k * (dTx * dSx + dTy * dSy);
+        return;
+    }
+    /* outlet: the cell's own gradients */
+    dux = o->dPx[nc + e0]; dvx = o->dPx[2 * nc + e0]; dTx = o->dPx[3 * nc + e0];
+    duy = o->dPy[nc + e0]; dvy = o->dPy[2 * nc + e0]; dTy = o->dPy[3 * nc + e0];
+    double T = P[4], mi = sutherland(T), k = gas_T2Cp(&o->gas, T) * mi / o->prm.pr;
+    double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+    double u = P[1], v = P[2];
+    f[1] = txx * dSx + txy * dSy;
+    f[2] = txy * dSx + tyy * dSy;
+    f[3] = u * (txx * dSx + txy * dSy) + v * (txy * dSx + tyy * dSy) + k * (dTx * dSx + dTy * dSy);
+}
+
+/* laminarBoundary, laminar.c:101-125.  The reference's adiabatic-wall
+   branch (laminar.c:230-269) never writes f[3], and f[] lives in the frame
+   of laminarBoundaryVisc, so a wall face subtracts the energy flux of the
+   boundary face processed before it (the last face of the preceding marker
+   that writes one).  Restated as such; with no such face before it (wall
+   marker first) the reference reads an undefined stack value, here 0. */
+static void laminar_boundary(csu_oracle* o)
+{
+    const int nc = o->nc;
+    double carry = 0.0;
+    for (int g = 0; g < o->nb; g++) {
+        double f[4];
+        laminar_bc_flux(o, g, f);
+        if (o->btype[g] == CSU_BC_WALL) f[3] = carry;
+        else carry = f[3];
+        int e0 = o->bface[3 * g];
+        for (int k = 1; k < 4; k++) o->R[(size_t)k * nc + e0] -= f[k];
+    }
+}
+
+/* solverCalcR, solver.c:685-739 */
 static void calc_R(csu_oracle* o, double* U)
 {
     memset(o->R, 0, sizeof(double) * (size_t)o->nv * o->nc);
@@ -686,6 +868,12 @@ static void calc_R(csu_oracle* o, double* U)
     update_grad(o);
     inter(o);
     boundary(o);
+    if (o->axi == 1) axis_pressure(o);
+    if (o->prm.laminar == 1) {
+        grad_T(o);
+        laminar_inter(o);
+        laminar_boundary(o);
+    }
 }
 
 /* ---------------- time step: solver.c:895-950 ---------------- */
@@ -772,6 +960,16 @@ static void calc_coeff(const csu_oracle* o, double* c4)      /* solver.c:1342-14
         face_ds(o, o->bface[3 * g + 1], o->bface[3 * g + 2], &dSx, &dSy);
         cxp += cp * dSx;
         cyp += cp * dSy;
+        if (o->prm.laminar || o->prm.sa || o->prm.sst) {
+            /* boundaryCalcFrictionWall, boundary.c:695-742 */
+            double du[2], dv[2], L, dxy[2];
+            wall_grads(o, g, 0.0, 0.0, du, dv, &L, dxy);
+            double mi = sutherland(o->P[(size_t)o->bface[3 * g] * ns + 4]);
+            double txx = 2 * mi * (du[0] - (du[0] + dv[1]) / 3), tyy = 2 * mi * (dv[1] - (du[0] + dv[1]) / 3);
+            double txy = mi * (du[1] + dv[0]);
+            cxv -= (txx * dSx + txy * dSy) / q;
+            cyv -= (txy * dSx + tyy * dSy) / q;
+        }
     }
     cxp /= o->prm.sref; cxv /= o->prm.sref; cyp /= o->prm.sref; cyv /= o->prm.sref;
     if (o->axi) { cxp *= 2 * M_PI; cxv *= 2 * M_PI; }
@@ -794,7 +992,7 @@ static void* dup(const void* p, size_t n) { void* q = malloc(n ? n : 1); if (n) 
 
 int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
 {
-    if (p->laminar || p->sa || p->sst || p->time_scheme != CSU_TIME_RK) return CSU_ERR_UNSUPPORTED;
+    if (p->sa || p->sst || p->time_scheme != CSU_TIME_RK) return CSU_ERR_UNSUPPORTED;
     csu_oracle* o = calloc(1, sizeof(*o));
     o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;
     o->prm = *p; o->dt_local = p->dt_local;

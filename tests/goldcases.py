@@ -1,0 +1,32 @@
+"""Golden fixture catalogue shared by the tests: which fixtures exist and how
+to rebuild the case directory each one was made from."""
+import os
+
+import numpy as np
+
+from csu_amd import cases, native
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RAMP = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
+        "ramp_ausmpup2_sub", "ramp_quad_global_dt", "ramp_tp_dtswitch"]
+PLATE = ["plate_lam", "plate_lamT_ausmdv", "plate_lam_axi_roe", "plate_euler_axi",
+         "plate_lam_pup_lim1"]
+ALL = RAMP + PLATE
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name + ".npz"))
+
+
+def write(name, g, d, nmax=1):
+    mk = eval(str(g["mesh_kwargs"]))
+    inp = eval(str(g["inputs"]))
+    if name.startswith("plate"):
+        cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nmax, **inp))
+    else:
+        cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nmax, **inp))
+    return d
+
+
+def host_case(name, g, d, nmax=1):
+    return native.HostCase(write(name, g, d, nmax))

@@ -35,14 +35,28 @@ SMALL = [
 ]
 
 
+_PL = dict(nx=14, ny=6, y1=0.01, growth=1.3, tri_from=2, tri_frac=0.3)
+PLATE = [
+    ("plate_lam", dict(_PL, jitter=0.1), {}, 20),
+    ("plate_lamT_ausmdv", _PL, {"BC:plate": "wallT", "Twall": "350", "flux": "AUSMDV"}, 20),
+    ("plate_lam_axi_roe", dict(_PL, y_shift=0.05), {"axisymmetric": "1", "flux": "ROE"}, 20),
+    ("plate_euler_axi", dict(_PL, y_shift=0.05), {"axisymmetric": "1", "laminar": "0"}, 20),
+    ("plate_lam_pup_lim1", dict(_PL, jitter=0.1), {"flux": "AUSMpup", "limiter": "1", "limK": "0.3",
+                                                   "BC:top": "symmetry"}, 20),
+]
+
+
 def case_dir(tag):
     return tempfile.mkdtemp(prefix="gd", dir="/tmp") + "/"
 
 
 def make_small():
-    for name, mk, inp, nit in SMALL:
+    for name, mk, inp, nit in SMALL + PLATE:
         d = case_dir(name)
-        cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nit, **inp))
+        if name.startswith("plate"):
+            cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nit, **inp))
+        else:
+            cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nit, **inp))
         mesh = po.ref_trace(d, "mesh")
         nv = 4
         nc = mesh["cellv"].size // 4

@@ -59,30 +59,29 @@ def test_history_ramp(tmpcase):
     assert _rel(Ug, Uo) < 1e-10
 
 
-GOLD = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
-SMALL = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
-         "ramp_ausmpup2_sub", "ramp_quad_global_dt", "ramp_tp_dtswitch"]
+import goldcases
+GOLD = goldcases.GOLD
 
 
-@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("name", goldcases.ALL)
 def test_vs_reference_fixture(tmpcase, name):
     """HIP path against the reference's own outputs (golden fixtures)."""
-    g = np.load(f"{GOLD}/{name}.npz")
+    g = goldcases.load(name)
     print("fixture", name, flush=True)
-    d = tmpcase("f")
-    cases.write_case(d, cases.ramp_mesh(seed=0, **eval(str(g["mesh_kwargs"]))),
-                     cases.ramp_input(threads=1, nmax=1, **eval(str(g["inputs"]))))
-    c = native.HostCase(d)
+    c = goldcases.host_case(name, g, tmpcase("f"))
     dev = native.Device(c)
     dev.set_state(g["U_in"])
     r = dev.residual()
     assert np.array_equal(r["P"], g["P"]) and np.array_equal(r["Pb"], g["Pb"])
     assert _rel(r["R"], g["R"]) < 1e-12
+    for k in ("dPx", "dPy"):
+        assert _rel(r[k], g[k]) < 1e-12, k
     dev2 = native.Device(c)
     n = g["hist_res"].shape[0]
     res, coef = dev2.iterate(n)
     assert np.max(np.abs(res / g["hist_res"] - 1)) < 1e-10
-    assert np.max(np.abs(coef[:, [0, 2]] - g["hist_coef"])) <= 1e-10 * np.abs(g["hist_coef"]).max()
+    tot = np.stack([coef[:, 0] + coef[:, 1], coef[:, 2] + coef[:, 3]], axis=1)
+    assert np.max(np.abs(tot - g["hist_coef"])) <= 1e-10 * np.abs(g["hist_coef"]).max()
     assert _rel(dev2.get_state(), g["hist_U"]) < 1e-10
 
 
@@ -107,3 +106,27 @@ def test_config0_residual_history_1e10(tmpcase):
     # per-variable scale: rho*v is rounding noise around 0 in the free stream
     assert _rel(U[:, g["U_idx"]], g["U_sample"]) < 1e-10
     assert np.max(np.abs(np.sqrt((U ** 2).sum(axis=1)) / g["U_l2"] - 1)) < 1e-10
+
+
+@pytest.mark.parametrize("inp", [
+    {},                                               # adiabatic plate, AUSM
+    {"BC:plate": "wallT", "Twall": "320", "flux": "AUSMDV"},
+    {"axisymmetric": "1", "flux": "ROE"},
+])
+def test_laminar_history_vs_oracle(tmpcase, inp):
+    """Viscous path (laminar faces with the edge-direction correction,
+    boundary viscous fluxes, wall friction, axisymmetric source) over 100
+    iterations on a 60x24 hybrid plate mesh: GPU against the oracle."""
+    d = tmpcase("l")
+    print("laminar plate", inp, flush=True)
+    cases.write_case(d, cases.plate_mesh(nx=60, ny=24, y1=2e-3, growth=1.15, jitter=0.1,
+                                         y_shift=0.05 if inp.get("axisymmetric") else 0.0),
+                     cases.plate_input(threads=1, nmax=100, **inp))
+    c = native.HostCase(d)
+    dev = native.Device(c)
+    orc = po.Oracle(c)
+    rg, cg = dev.iterate(100)
+    ro, co = orc.iterate(100)
+    assert np.max(np.abs(rg / ro - 1)) < 1e-10
+    assert np.max(np.abs(cg - co)) <= 1e-10 * np.abs(co).max()
+    assert _rel(dev.get_state(), orc.get_state()) < 1e-10

@@ -8,9 +8,7 @@ import pytest
 from csu_amd import cases, native
 from oracle import pyoracle as po
 
-GOLD = os.path.join(os.path.dirname(__file__), "golden")
-SMALL = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
-         "ramp_ausmpup2_sub", "ramp_quad_global_dt", "ramp_tp_dtswitch"]
+import goldcases
 
 
 def _write(path, text):
@@ -49,14 +47,10 @@ def test_missing_key_reported(tmp_path):
     assert "CFL" in str(e.value)
 
 
-@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("name", goldcases.ALL)
 def test_connectivity_matches_reference_fixture(tmp_path, name):
-    g = np.load(os.path.join(GOLD, name + ".npz"))
-    mk = eval(str(g["mesh_kwargs"]))
-    inp = eval(str(g["inputs"]))
-    d = str(tmp_path) + "/"
-    cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=1, **inp))
-    a = native.HostCase(d).arrays()
+    g = goldcases.load(name)
+    a = goldcases.host_case(name, g, str(tmp_path) + "/").arrays()
     assert np.array_equal(a["face"], g["con"])
     assert np.array_equal(a["cell_nbr"], g["nbr"])
     assert np.array_equal(a["cell_face"], g["cface"])
