@@ -12,6 +12,7 @@
 #include <vector>
 #include "csu.h"
 #include "kernels.h"
+#include "kern_implicit.h"
 
 using namespace csu;
 
@@ -66,6 +67,11 @@ struct csu_dev {
   double *res_hist, *coef_hist;
   int hist_cap;
   int fresh;                   // state set from the host: P and dt not yet derived
+  // implicit schemes
+  double *Dp, *dW0, *dW1, *A;  // diagonal (without the dt term), increments, DPLUR blocks
+  int* jstate;                 // DPLUR: [converged, current buffer]
+  std::vector<int> lvlL, lvlU; // LU-SGS level offsets (forward / backward)
+  int *cellsL, *cellsU;        // cells sorted by level
   // profiling
   int prof;
   std::vector<std::string> kname;
@@ -145,7 +151,8 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
   if (p->limiter != 0 && p->limiter != 1) return set_err(CSU_ERR_ARG, "limiter %d", p->limiter);
   if (p->sa || p->sst) return set_err(CSU_ERR_UNSUPPORTED, "turbulence models not built yet");
   if (p->nvar != 4) return set_err(CSU_ERR_ARG, "nvar %d for a laminar/Euler run", p->nvar);
-  if (p->time_scheme != CSU_TIME_RK) return set_err(CSU_ERR_UNSUPPORTED, "implicit schemes not built yet");
+  if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);
+  if (p->time_scheme == CSU_TIME_DPLUR && p->nlinear < 0) return set_err(CSU_ERR_ARG, "Nlinear %d", p->nlinear);
   if (p->precision != CSU_PREC_FP64) return set_err(CSU_ERR_UNSUPPORTED, "mixed precision not built yet");
   return CSU_OK;
 }
@@ -194,6 +201,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   d->nmark = m->nmark; d->iter = 0; d->prm = *p; d->stream = nullptr; d->prof = 0;
   d->visc = (p->laminar || p->sa || p->sst) ? 1 : 0;
   d->hist_cap = 0; d->res_hist = d->coef_hist = nullptr; d->fresh = 1;
+  d->Dp = d->dW0 = d->dW1 = d->A = nullptr; d->jstate = nullptr; d->cellsL = d->cellsU = nullptr;
   memset((void*)&d->kp, 0, sizeof(KP));
   memset((void*)&d->geo, 0, sizeof(Geo));
   memset((void*)&d->st, 0, sizeof(St));
@@ -296,7 +304,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   kp.laminar = p->laminar; kp.sa = p->sa; kp.sst = p->sst; kp.limiter = p->limiter;
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
   kp.rlim = p->rlim; kp.pout = p->pout; kp.twall = p->twall; kp.efix = p->efix; kp.minf = p->mach_inf;
-  kp.limK = p->lim_k; kp.volmax = volmax; kp.pr = p->pr; kp.pr_t = p->pr_t;
+  kp.limK = p->lim_k; kp.volmax = volmax; kp.pr = p->pr; kp.pr_t = p->pr_t; kp.wimp = p->wimp;
   for (int k = 0; k < 7; k++) kp.pin[k] = p->pin[k];
   kp.gas.gamma = p->gamma; kp.gas.R = p->gas_r;
   kp.gas.Cp = kp.gas.gamma * kp.gas.R / (kp.gas.gamma - 1);
@@ -355,6 +363,52 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
       return rc;
     }
     HIPCHK(hipMemset(s.wfric, 0, sizeof(double) * 2 * (size_t)(nb ? nb : 1)));
+  }
+  if (p->time_scheme != CSU_TIME_RK) {
+    if ((rc = dalloc(d, &d->Dp, nc)) || (rc = dalloc(d, &d->dW0, nvc)) || (rc = dalloc(d, &d->dW1, nvc))) {
+      csu_dev_destroy(d);
+      return rc;
+    }
+    HIPCHK(hipMemset(d->dW0, 0, nvc * sizeof(double)));
+    HIPCHK(hipMemset(d->dW1, 0, nvc * sizeof(double)));   // implicitInitDPLUR
+  }
+  if (p->time_scheme == CSU_TIME_DPLUR) {
+    if ((rc = dalloc(d, &d->A, (size_t)4 * nv * nv * nc)) || (rc = dalloc(d, &d->jstate, 2))) {
+      csu_dev_destroy(d);
+      return rc;
+    }
+    HIPCHK(hipMemset(d->A, 0, (size_t)4 * nv * nv * nc * sizeof(double)));
+    HIPCHK(hipMemset(d->jstate, 0, 2 * sizeof(int)));
+  }
+  if (p->time_scheme == CSU_TIME_LUSGS) {
+    // levels of the sweeps' dependency DAGs (lower / upper cell index)
+    std::vector<int> lev(nc);
+    auto build = [&](bool fwd, std::vector<int>& off, int** dcells) -> int {
+      int depth = 0;
+      for (int t = 0; t < nc; t++) {
+        int i = fwd ? t : nc - 1 - t, l = 0;
+        for (int j = 0; j < 4; j++) {
+          int n = m->cell_nbr[4 * i + j];
+          if (m->cell_face[4 * i + j] == 0 || n < 0) continue;
+          if (fwd ? n < i : n > i) l = std::max(l, lev[n] + 1);
+        }
+        lev[i] = l;
+        depth = std::max(depth, l + 1);
+      }
+      off.assign(depth + 1, 0);
+      for (int i = 0; i < nc; i++) off[lev[i] + 1]++;
+      for (int l = 0; l < depth; l++) off[l + 1] += off[l];
+      std::vector<int> cells(nc), fill(off.begin(), off.end() - 1);
+      for (int i = 0; i < nc; i++) cells[fill[lev[i]]++] = i;
+      const int* dc = nullptr;
+      int r = upload(d, &dc, cells);
+      *dcells = const_cast<int*>(dc);
+      return r;
+    };
+    if ((rc = build(true, d->lvlL, &d->cellsL)) || (rc = build(false, d->lvlU, &d->cellsU))) {
+      csu_dev_destroy(d);
+      return rc;
+    }
   }
   // free-stream initial state (solverInitU)
   std::vector<double> U0(nvc);
@@ -606,6 +660,63 @@ static int iterate_t(csu_dev* d, int niter) {
   return CSU_OK;
 }
 
+// implicit iteration: solverCalcR, then LU-SGS (implicitCalcD + sweeps) or
+// DPLUR (implicitCalcDPLUR), then U += dW (solver.c:1871-1884)
+template <int NV, bool VISC>
+static int iterate_implicit(csu_dev* d, int niter) {
+  const KP& kp = d->kp;
+  const int nc = d->nc, nb = nblk(nc);
+  double* red_min = d->red + (size_t)NV * d->nred;
+  for (int it = 0; it < niter; it++) {
+    int rc = prim_pass<NV>(d);
+    if (!rc) rc = stage<NV, VISC>(d, F_STORE_R | F_RES, 0.0, 0.0, nullptr, nullptr);
+    if (rc) return rc;
+    {
+      Prof p(d, "k_impD");
+      hipLaunchKernelGGL((k_impD<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->prm.wimp,
+                         d->Dp, red_min);
+      hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, d->prm.cfl, red_min, d->dtg);
+    }
+    if (d->prm.time_scheme == CSU_TIME_LUSGS) {
+      Prof p(d, "k_lusgs");
+      for (size_t l = 0; l + 1 < d->lvlL.size(); l++) {
+        int off = d->lvlL[l], cnt = d->lvlL[l + 1] - off;
+        hipLaunchKernelGGL((k_lusgs<NV, VISC, 0>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+                           d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0);
+      }
+      hipLaunchKernelGGL(k_lusgs_uinit<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->Dp, d->dtg, d->dW0,
+                         d->dW1);
+      for (size_t l = 0; l + 1 < d->lvlU.size(); l++) {
+        int off = d->lvlU[l], cnt = d->lvlU[l + 1] - off;
+        hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+                           d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1);
+      }
+      hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW1, d->dW1,
+                         (const int*)nullptr);
+    } else {
+      Prof p(d, "k_dplur");
+      hipLaunchKernelGGL((k_blocks<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->U, d->A);
+      hipLaunchKernelGGL(k_jac_reset, dim3(1), dim3(1), 0, d->stream, d->jstate);
+      for (int mm = 0; mm < d->prm.nlinear; mm++) {
+        hipLaunchKernelGGL(k_jacobi<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->A, d->Dp, d->dtg,
+                           d->R, d->dW0, d->dW1, d->jstate, red_min);
+        hipLaunchKernelGGL(k_jac_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, red_min, d->jstate);
+      }
+      hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW0, d->dW1,
+                         (const int*)d->jstate);
+    }
+    const int slot = d->iter % d->hist_cap;
+    hipLaunchKernelGGL(k_res_final<NV>, dim3(1), dim3(RES_THREADS), 0, d->stream, nc, nb, d->red,
+                       d->res_hist + (size_t)slot * NV);
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(CSU_BLOCK), 0, d->stream, kp, d->nwall, d->wall, d->st.Pb, d->st.wfric,
+                       d->geo.bdsx, d->geo.bdsy, d->prm.sref, d->visc, d->coef_hist + (size_t)slot * 4);
+    HIPCHK(hipGetLastError());
+    d->iter++;
+  }
+  d->fresh = 1;
+  return CSU_OK;
+}
+
 extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* coef) {
   if (!d) return set_err(CSU_ERR_ARG, "null argument");
   if (niter < 0) return set_err(CSU_ERR_ARG, "niter %d", niter);
@@ -617,7 +728,11 @@ extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* c
     d->hist_cap = niter;
   }
   const int start = d->iter;
-  int rc = d->visc ? iterate_t<4, true>(d, niter) : iterate_t<4, false>(d, niter);
+  int rc;
+  if (d->prm.time_scheme == CSU_TIME_RK)
+    rc = d->visc ? iterate_t<4, true>(d, niter) : iterate_t<4, false>(d, niter);
+  else
+    rc = d->visc ? iterate_implicit<4, true>(d, niter) : iterate_implicit<4, false>(d, niter);
   if (rc) return rc;
   if (res || coef) {
     std::vector<double> rh((size_t)d->hist_cap * d->nv), ch((size_t)d->hist_cap * 4);

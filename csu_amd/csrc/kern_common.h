@@ -11,7 +11,7 @@ namespace csu {
 // Scalar options and free-stream data, passed by value.
 struct KP {
   int nc, nf, nb, nv, order, flux, laminar, sa, sst, limiter, turb1order, axi, visc_blazek;
-  double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t;
+  double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t, wimp;
   double pin[7];
   GasD gas;
 };

@@ -1,0 +1,326 @@
+// kern_implicit.h -- implicit time stepping: LU-SGS by level scheduling and
+// DPLUR by damped Jacobi (reference implicit.c).
+//
+// LU-SGS: the reference sweeps cells in index order, a cell using the final
+// increments of its lower-index (forward) or higher-index (backward)
+// neighbours.  The host groups cells into levels of that dependency DAG; each
+// level is one launch, cells of a level are independent.  Every cell does the
+// reference's arithmetic on the same inputs, so the result equals the serial
+// sweep for any cell order; a colour-sorted numbering makes the number of
+// levels equal to the number of colours.
+#pragma once
+#include "kern_common.h"
+
+namespace csu {
+
+__device__ __forceinline__ double gamma_of(const GasD& g, double T) { return gas_T2gamma<double>(g, T); }
+
+// implicitCalcD (implicit.c:19-181), everything but the global-time-step
+// term: writes Dp = 0.5 w Lc (+ Lv/Omega) and per-block min of Omega/Lc.
+template <int NV, bool VISC>
+__global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double wimp, double* __restrict__ Dp,
+                                                    double* __restrict__ red_min) {
+  __shared__ double smin[CSU_BLOCK];
+  const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
+  const int nc = kp.nc;
+  double val = INFINITY;
+  if (i < nc) {
+    const double* P = s.P;
+    double Lc = 0, Lv = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int f = g.cface[j * nc + i];
+      if (f == 0) continue;
+      const int fi = f > 0 ? f - 1 : -f - 1;
+      const int e0 = g.fe0[fi], e1 = g.fe1[fi];
+      const double dSx = g.fdsx[fi], dSy = g.fdsy[fi], dS = g.fds[fi];
+      double r = (P[e0] + P[e1]) * 0.5;
+      double u = (P[nc + e0] + P[nc + e1]) * 0.5;
+      double v = (P[2 * nc + e0] + P[2 * nc + e1]) * 0.5;
+      double T = (P[4 * nc + e0] + P[4 * nc + e1]) * 0.5;
+      double c = gas_T2c<double>(kp.gas, T);
+      double nx = dSx / dS, ny = dSy / dS;
+      Lc += (fabs(nx * u + ny * v) + c) * dS;
+      if (VISC) {
+        double mi = sutherland(T);
+        double mt = NV == 4 ? mi / kp.pr : mi / kp.pr + s.miT[fi] / kp.pr_t;
+        Lv += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * mt * dS * dS;
+      }
+    }
+    const double om = g.omega[i];
+    double dtl = Lc;
+    double D = 0.5 * wimp * Lc;
+    if (VISC) D += Lv / om;
+    const int np = g.cnp[i];
+    for (int j = 0; j < np; j++) {
+      const int n = g.cnbr[j * nc + i];
+      if (n >= 0) continue;
+      const int q = -1 - n;
+      const double dSx = g.bdsx[q], dSy = g.bdsy[q], dS = g.bds[q];
+      double r = P[i], u = P[nc + i], v = P[2 * nc + i], T = P[4 * nc + i];
+      double c = gas_T2c<double>(kp.gas, T);
+      double nx, ny;
+      if (dS > 0) { nx = dSx / dS; ny = dSy / dS; } else { nx = 0.0; ny = 1.0; }
+      double Lcb = (fabs(nx * u + ny * v) + c) * dS;
+      dtl += Lcb;
+      D += 0.5 * wimp * Lcb;
+      if (VISC) {
+        double mi = sutherland(T);
+        double Lvb = fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr) * dS * dS;
+        D += Lvb / om;
+      }
+    }
+    Dp[i] = D;
+    val = om / dtl;
+  }
+  smin[t] = val;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st) smin[t] = fmin(smin[t], smin[t + st]);
+    __syncthreads();
+  }
+  if (t == 0) red_min[blockIdx.x] = smin[0];
+}
+
+// diagonal with the global-time-step term (implicit.c:176-180)
+__device__ __forceinline__ double diag(const double* Dp, const double* omega, const double* dtg, int i) {
+  return Dp[i] + omega[i] / dtg[0];
+}
+
+// implicitCalcDeltaFlux (implicit.c:194-231) at state Pj with increment d
+template <int NV>
+__device__ __forceinline__ void delta_flux(const KP& kp, const double* Pj, const double* d, double nx, double ny,
+                                           double* dF) {
+  double rho = Pj[0], u = Pj[1], v = Pj[2], p = Pj[3];
+  double T = p / (kp.gas.R * rho);
+  double e0 = gas_T2e<double>(kp.gas, T);
+  double E = e0 + (u * u + v * v) / 2;
+  double U[4] = {rho, rho * u, rho * v, rho * E};
+  double V = (nx * U[1] + ny * U[2]) / U[0];
+  double F0[4] = {U[0] * V, U[1] * V + nx * p, U[2] * V + ny * p, (U[3] + p) * V};
+#pragma unroll
+  for (int k = 0; k < 4; k++) U[k] = U[k] + d[k];
+  rho = U[0]; u = U[1] / rho; v = U[2] / rho; E = U[3] / rho;
+  double e1 = E - 0.5 * (u * u + v * v);
+  T = T + (e1 - e0) / gas_T2Cv<double>(kp.gas, T);
+  p = kp.gas.R * T * rho;
+  V = (nx * U[1] + ny * U[2]) / U[0];
+  dF[0] = U[0] * V - F0[0];
+  dF[1] = (U[1] * V + nx * p) - F0[1];
+  dF[2] = (U[2] * V + ny * p) - F0[2];
+  dF[3] = (U[3] + p) * V - F0[3];
+}
+
+// Face slot j of cell i seen from i: neighbour id and outward face vector
+// (the reference flips p0/p1 when i is the face's e1, implicit.c:288-305;
+// the flipped vector is the exact negation).
+__device__ __forceinline__ int face_from(const Geo& g, int nc, int i, int j, double& dSx, double& dSy, double& dS) {
+  const int f = g.cface[j * nc + i];
+  if (f == 0) return -1;
+  const int fi = f > 0 ? f - 1 : -f - 1;
+  dS = g.fds[fi];
+  if (f > 0) { dSx = g.fdsx[fi]; dSy = g.fdsy[fi]; return g.fe1[fi]; }
+  dSx = -g.fdsx[fi]; dSy = -g.fdsy[fi];
+  return g.fe0[fi];
+}
+
+// One LU-SGS level.  DIR 0: forward sweep (implicitLUSGS_L, implicit.c:274-363),
+// dW = -R then lower neighbours; DIR 1: backward sweep (implicitLUSGS_U,
+// implicit.c:365-482), dW holds D*dW0 and upper neighbours are used.
+template <int NV, bool VISC, int DIR>
+__global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const int* __restrict__ cells, int ncell_lvl,
+                                                     const double* __restrict__ Dp, const double* __restrict__ dtg,
+                                                     const double* __restrict__ R, double* dW) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell_lvl) return;
+  const int nc = kp.nc;
+  const int i = cells[t];
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = DIR == 0 ? -R[k * nc + i] : dW[k * nc + i];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double dSx, dSy, dS;
+    const int nb = face_from(g, nc, i, j, dSx, dSy, dS);
+    if (nb < 0) continue;
+    if (DIR == 0 ? !(nb < i) : !(nb > i)) continue;
+    const double nx = dSx / dS, ny = dSy / dS;
+    double Pj[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) Pj[m] = s.P[m * nc + nb];
+    double d[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) d[k] = dW[k * nc + nb];
+    double dF[NV];
+    delta_flux<NV>(kp, Pj, d, nx, ny, dF);
+    double c = gas_T2c<double>(kp.gas, Pj[4]);
+    double ra = kp.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
+    if (VISC && kp.laminar) {
+      double r = Pj[0], T = Pj[4], mi = sutherland(T);
+      double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
+      double dd_ = sqrt(ddx * ddx + ddy * ddy);
+      ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr) * dS / dd_;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] -= 0.5 * (dF[k] * dS - ra * d[k]);
+  }
+  const double D = diag(Dp, g.omega, dtg, i);
+#pragma unroll
+  for (int k = 0; k < NV; k++) dW[k * nc + i] = acc[k] / D;
+}
+
+// backward-sweep initial value dW1 = D dW0 (implicit.c:369-399)
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_lusgs_uinit(KP kp, Geo g, const double* __restrict__ Dp,
+                                                           const double* __restrict__ dtg, const double* __restrict__ dW0,
+                                                           double* __restrict__ dW1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc;
+  if (i >= nc) return;
+  const double D = diag(Dp, g.omega, dtg, i);
+#pragma unroll
+  for (int k = 0; k < NV; k++) dW1[k * nc + i] = D * dW0[k * nc + i];
+}
+
+// DPLUR finite-difference blocks of every interior neighbour
+// (implicitUpdateA, implicit.c:689-813), h = 1e-6.  Layout [4 slots][NV*NV][nc].
+template <int NV, bool VISC>
+__global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const double* __restrict__ U,
+                                                      double* __restrict__ A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc;
+  if (i >= nc) return;
+  const double h = 1.0e-6;
+#pragma unroll 1
+  for (int j = 0; j < 4; j++) {
+    double dSx, dSy, dS;
+    const int nb = face_from(g, nc, i, j, dSx, dSy, dS);
+    if (nb < 0) continue;
+    const double nx = dSx / dS, ny = dSy / dS;
+    const double P3 = s.P[3 * nc + nb], P4 = s.P[4 * nc + nb];
+    double Uj[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) Uj[k] = U[k * nc + nb];
+    double rho = Uj[0], u = Uj[1] / rho, v = Uj[2] / rho, E = Uj[3] / rho;
+    double V = (nx * Uj[1] + ny * Uj[2]) / Uj[0];
+    const double F0[4] = {Uj[0] * V, Uj[1] * V + nx * P3, Uj[2] * V + ny * P3, (Uj[3] + P3) * V};
+    const double ee0 = E - 0.5 * (u * u + v * v);
+    const double cv = gas_T2Cv<double>(kp.gas, P4);
+    double* Aj = A + (size_t)j * NV * NV * nc;
+#pragma unroll
+    for (int mm = 0; mm < 4; mm++) {
+      double Up[4];
+#pragma unroll
+      for (int n = 0; n < 4; n++) Up[n] = mm == n ? Uj[n] + h : Uj[n];
+      rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
+      double ee1 = E - 0.5 * (u * u + v * v);
+      double T = P4 + (ee1 - ee0) / cv;
+      double p = kp.gas.R * rho * T;
+      double Vp = (nx * Up[1] + ny * Up[2]) / Up[0];
+      double F[4] = {Up[0] * Vp, Up[1] * Vp + nx * p, Up[2] * Vp + ny * p, (Up[3] + p) * Vp};
+#pragma unroll
+      for (int n = 0; n < 4; n++) Aj[(size_t)(n * NV + mm) * nc + i] = 0.5 * (F[n] - F0[n]) * dS / h;
+    }
+    double c = gas_T2c<double>(kp.gas, P4);
+    double ra = kp.wimp * (fabs(nx * s.P[nc + nb] + ny * s.P[2 * nc + nb]) + c) * dS;
+    if (VISC && kp.laminar) {
+      double r = (s.P[i] + s.P[nb]) * 0.5;
+      double T = (s.P[4 * nc + i] + P4) * 0.5;
+      double mi = sutherland(T);
+      double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
+      double dd_ = sqrt(ddx * ddx + ddy * ddy);
+      ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr) * dS / dd_;
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) Aj[(size_t)(n * NV + n) * nc + i] -= 0.5 * ra;
+  }
+}
+
+// One damped Jacobi sweep (implicitMultA + update, implicit.c:938-1022):
+// y = D x + sum_j A_j x_j;  x_new = w (-R - y)/D + x.  Buffers buf[cur] ->
+// buf[1-cur]; state[0] = converged flag, state[1] = cur.  A sweep after
+// convergence in the same implicit step does nothing (the reference breaks).
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double* __restrict__ A,
+                                                      const double* __restrict__ Dp, const double* __restrict__ dtg,
+                                                      const double* __restrict__ R, double* buf0, double* buf1,
+                                                      const int* __restrict__ state, double* __restrict__ red_max) {
+  __shared__ double smax[CSU_BLOCK];
+  if (state[0]) return;
+  const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
+  const int nc = kp.nc;
+  const double* x = state[1] ? buf1 : buf0;
+  double* y = state[1] ? buf0 : buf1;
+  double diff = 0.0;
+  if (i < nc) {
+    const double D = diag(Dp, g.omega, dtg, i);
+    double xi[NV], acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) { xi[k] = x[k * nc + i]; acc[k] = D * xi[k]; }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      double dSx, dSy, dS;
+      const int nb = face_from(g, nc, i, j, dSx, dSy, dS);
+      if (nb < 0) continue;
+      const double* Aj = A + (size_t)j * NV * NV * nc;
+      double xj[NV];
+#pragma unroll
+      for (int n = 0; n < NV; n++) xj[n] = x[n * nc + nb];
+#pragma unroll
+      for (int k = 0; k < NV; k++)
+#pragma unroll
+        for (int n = 0; n < NV; n++) acc[k] += Aj[(size_t)(k * NV + n) * nc + i] * xj[n];
+    }
+    const double w = 2. / 3.;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double yk = w * (-R[k * nc + i] - acc[k]) / D + xi[k];
+      y[k * nc + i] = yk;
+      if (k == 0) diff = fabs(yk - xi[0]);
+    }
+  }
+  smax[t] = diff;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
+    __syncthreads();
+  }
+  if (t == 0) red_max[blockIdx.x] = smax[0];
+}
+
+// After a Jacobi sweep: flip the current buffer and test max|dx_rho| < 1e-14.
+__global__ __launch_bounds__(CSU_BLOCK) void k_jac_final(int nblocks, const double* __restrict__ red_max, int* state) {
+  __shared__ double smax[CSU_BLOCK];
+  if (state[0]) return;
+  const int t = threadIdx.x;
+  double m = 0.0;
+  for (int b = t; b < nblocks; b += CSU_BLOCK) m = fmax(m, red_max[b]);
+  smax[t] = m;
+  __syncthreads();
+  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+    if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    state[1] = 1 - state[1];
+    if (smax[0] < 1e-14) state[0] = 1;
+  }
+}
+
+__global__ void k_jac_reset(int* state) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) state[0] = 0;
+}
+
+// solverUpdateUImplicit (solver.c:1945-1955): U += dW; for DPLUR dW is
+// buf[state[1]].
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_upd_imp(int nc, double* __restrict__ U, const double* buf0,
+                                                       const double* buf1, const int* __restrict__ state) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const double* dW = (state && state[1]) ? buf1 : buf0;
+#pragma unroll
+  for (int k = 0; k < NV; k++) U[k * nc + i] += dW[k * nc + i];
+}
+
+}  // namespace csu

@@ -17,7 +17,7 @@
 
 namespace csu {
 
-enum { F_RK = 1, F_CLAMP_UN = 2, F_LAST = 4, F_NEXT_LOCAL = 8, F_STORE_R = 16, F_CUR_LOCAL = 32 };
+enum { F_RK = 1, F_CLAMP_UN = 2, F_LAST = 4, F_NEXT_LOCAL = 8, F_STORE_R = 16, F_CUR_LOCAL = 32, F_RES = 64 };
 
 // Standalone primitive pass (first stage after the state was set from the
 // host).  Clamped values are written back to U, as the reference does.
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
       for (int m = 0; m < NV + 1; m++) s.P[m * nc + i] = p[m];
     }
   }
-  if (!(flags & F_LAST)) return;
+  if (!(flags & (F_LAST | F_RES))) return;
   // residual-norm partials (and global time-step minimum): fixed tree
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) sred[kk][t] = R[kk] * R[kk];

@@ -35,6 +35,8 @@ struct csu_oracle {
     int *cv, *face, *bface, *btype, *nbr, *cface;
     double *omega, *U, *Uaux, *R, *dPx, *dPy, *phi, *flux, *dtL, *P, *Pb;
     double dt, volMax, Pref20[6];
+    /* implicit schemes */
+    double *D, *dW0, *dW1, *A;    /* A: DPLUR blocks [nc][4][nv*nv] */
 };
 
 /* ---------------- gas model: gasprop.c:44-274 ---------------- */
@@ -938,6 +940,229 @@ static void step_rk(csu_oracle* o)
     double* t = o->U; o->U = o->Uaux; o->Uaux = t;
 }
 
+
+/* ---------------- implicit schemes: implicit.c ---------------- */
+static double gas_T2gamma(const ogas* g, double T) { double Cv = gas_T2Cv(g, T); return (Cv + g->R) / Cv; }
+
+/* implicitCalcD, implicit.c:19-181 */
+static void implicit_D(csu_oracle* o)
+{
+    const int nc = o->nc, ns = o->nv + 1;
+    const int visc = o->prm.laminar || o->prm.sa || o->prm.sst;
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++) {
+        double Lc = 0, Lv = 0;
+        for (int j = 0; j < 4; j++) {
+            int f = o->cface[4 * i + j];
+            if (f == 0) continue;
+            int f1 = f > 0 ? f - 1 : -f - 1;
+            const int* q = o->face + 4 * f1;
+            const double *P0 = o->P + (size_t)q[0] * ns, *P1 = o->P + (size_t)q[1] * ns;
+            double dSx, dSy;
+            face_ds(o, q[2], q[3], &dSx, &dSy);
+            double dS = sqrt(dSx * dSx + dSy * dSy);
+            double r = (P0[0] + P1[0]) * 0.5, u = (P0[1] + P1[1]) * 0.5, v = (P0[2] + P1[2]) * 0.5;
+            double T = (P0[4] + P1[4]) * 0.5, c = gas_T2c(&o->gas, T);
+            double nx = dSx / dS, ny = dSy / dS;
+            Lc += (fabs(nx * u + ny * v) + c) * dS;
+            if (o->prm.laminar) {
+                double mi = sutherland(T);
+                Lv += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr) * dS * dS;
+            }
+        }
+        o->dtL[i] = Lc;
+        o->D[i] = 0.5 * o->prm.wimp * Lc;
+        if (visc) o->D[i] += Lv / o->omega[i];
+    }
+    for (int g = 0; g < o->nb; g++) {
+        int e1 = o->bface[3 * g];
+        const double* P1 = o->P + (size_t)e1 * ns;
+        double dSx, dSy, nx, ny, Lv = 0;
+        face_ds(o, o->bface[3 * g + 1], o->bface[3 * g + 2], &dSx, &dSy);
+        double dS = sqrt(dSx * dSx + dSy * dSy);
+        double r = P1[0], u = P1[1], v = P1[2], T = P1[4], c = gas_T2c(&o->gas, T);
+        if (dS > 0) { nx = dSx / dS; ny = dSy / dS; } else { nx = 0.0; ny = 1.0; }
+        double Lc = (fabs(nx * u + ny * v) + c) * dS;
+        if (o->prm.laminar || o->prm.sa || o->prm.sst) {
+            double mi = sutherland(T);
+            Lv = fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr) * dS * dS;
+        }
+        o->dtL[e1] += Lc;
+        o->D[e1] += 0.5 * o->prm.wimp * Lc;
+        if (visc) o->D[e1] += Lv / o->omega[e1];
+    }
+    for (int i = 0; i < nc; i++) o->dtL[i] = o->omega[i] / o->dtL[i];
+    double dt = o->dtL[0];
+    for (int i = 1; i < nc; i++) dt = fmin(dt, o->dtL[i]);
+    dt *= o->prm.cfl;
+    for (int i = 0; i < nc; i++) o->D[i] += o->omega[i] / dt;
+}
+
+/* implicitAuxCalcFlux2 / implicitCalcDeltaFlux, implicit.c:183-231 */
+static void aux_flux(const double* U, double p, double nx, double ny, double* F)
+{
+    double V = (nx * U[1] + ny * U[2]) / U[0];
+    F[0] = U[0] * V;
+    F[1] = U[1] * V + nx * p;
+    F[2] = U[2] * V + ny * p;
+    F[3] = (U[3] + p) * V;
+}
+
+static void delta_flux(const csu_oracle* o, const double* Pj, const double* d, double nx, double ny, double* dF)
+{
+    double rho = Pj[0], u = Pj[1], v = Pj[2], p = Pj[3], F0[4], F1[4], U[4];
+    double T = p / (o->gas.R * rho);
+    double e0 = gas_T2e(&o->gas, T);
+    double E = e0 + (u * u + v * v) / 2;
+    U[0] = rho; U[1] = rho * u; U[2] = rho * v; U[3] = rho * E;
+    aux_flux(U, p, nx, ny, F0);
+    for (int k = 0; k < 4; k++) U[k] = U[k] + d[k];
+    rho = U[0]; u = U[1] / rho; v = U[2] / rho; E = U[3] / rho;
+    double e1 = E - 0.5 * (u * u + v * v);
+    T = T + (e1 - e0) / gas_T2Cv(&o->gas, T);       /* gasprop_e2Taprox */
+    p = o->gas.R * T * rho;
+    aux_flux(U, p, nx, ny, F1);
+    for (int k = 0; k < 4; k++) dF[k] = F1[k] - F0[k];
+}
+
+/* face j of cell i seen from i: neighbour, outward normal (implicit.c:288-305) */
+static int face_from(const csu_oracle* o, int i, int slot, double* dSx, double* dSy)
+{
+    int f = o->cface[4 * i + slot];
+    if (f == 0) return -1;
+    const int* q = o->face + 4 * (f > 0 ? f - 1 : -f - 1);
+    if (f > 0) { face_ds(o, q[2], q[3], dSx, dSy); return q[1]; }
+    face_ds(o, q[3], q[2], dSx, dSy);
+    return q[0];
+}
+
+/* implicitFunc, implicit.c:233-272: dW[.][i] -= 0.5 (dF dS - ra dW[.][j]) */
+static void implicit_func(const csu_oracle* o, int i, int j, double dSx, double dSy, double* dW)
+{
+    const int nc = o->nc, ns = o->nv + 1;
+    double dS = sqrt(dSx * dSx + dSy * dSy), nx = dSx / dS, ny = dSy / dS;
+    const double* Pj = o->P + (size_t)j * ns;
+    double d[4] = {dW[j], dW[nc + j], dW[2 * nc + j], dW[3 * nc + j]}, dF[4];
+    delta_flux(o, Pj, d, nx, ny, dF);
+    double c = gas_T2c(&o->gas, Pj[4]);
+    double ra = o->prm.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
+    if (o->prm.laminar) {
+        double r = Pj[0], T = Pj[4], mi = sutherland(T), x0, y0, x1, y1;
+        cell_center(o, i, &x0, &y0);
+        cell_center(o, j, &x1, &y1);
+        double dd = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+        ra += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr) * dS / dd;
+    }
+    for (int k = 0; k < o->nv; k++) dW[(size_t)k * nc + i] -= 0.5 * (dF[k] * dS - ra * dW[(size_t)k * nc + j]);
+}
+
+/* implicitLUSGS_L / _U, implicit.c:274-482: serial sweeps in cell order */
+static void lusgs(csu_oracle* o)
+{
+    const int nc = o->nc, nv = o->nv;
+    for (int i = 0; i < nc; i++) {
+        for (int k = 0; k < nv; k++) o->dW0[(size_t)k * nc + i] = -o->R[(size_t)k * nc + i];
+        for (int s = 0; s < 4; s++) {
+            double dSx, dSy;
+            int j = face_from(o, i, s, &dSx, &dSy);
+            if (j >= 0 && j < i) implicit_func(o, i, j, dSx, dSy, o->dW0);
+        }
+        for (int k = 0; k < nv; k++) o->dW0[(size_t)k * nc + i] /= o->D[i];
+    }
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < nv; k++) o->dW1[(size_t)k * nc + i] = o->D[i] * o->dW0[(size_t)k * nc + i];
+    for (int i = nc - 1; i >= 0; i--) {
+        for (int s = 0; s < 4; s++) {
+            double dSx, dSy;
+            int j = face_from(o, i, s, &dSx, &dSy);
+            if (j > i) implicit_func(o, i, j, dSx, dSy, o->dW1);
+        }
+        for (int k = 0; k < nv; k++) o->dW1[(size_t)k * nc + i] /= o->D[i];
+    }
+}
+
+/* implicitUpdateA, implicit.c:689-813: finite-difference flux Jacobian
+   blocks of every interior neighbour, h = 1e-6 */
+static void dplur_blocks(csu_oracle* o)
+{
+    const int nc = o->nc, nv = o->nv, ns = nv + 1;
+    const double h = 1.0e-6;
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++) {
+        for (int s = 0; s < 4; s++) {
+            double dSx, dSy;
+            int j = face_from(o, i, s, &dSx, &dSy);
+            if (j < 0) continue;
+            double* A = o->A + ((size_t)i * 4 + s) * nv * nv;
+            double dS = sqrt(dSx * dSx + dSy * dSy), nx = dSx / dS, ny = dSy / dS;
+            const double* Pj = o->P + (size_t)j * ns;
+            double Uj[4] = {o->U[j], o->U[nc + j], o->U[2 * nc + j], o->U[3 * nc + j]}, F0[4], F[4], Up[4];
+            double rho = Uj[0], u = Uj[1] / rho, v = Uj[2] / rho, E = Uj[3] / rho;
+            aux_flux(Uj, Pj[3], nx, ny, F0);
+            double ee0 = E - 0.5 * (u * u + v * v);
+            for (int m = 0; m < 4; m++) {
+                for (int n = 0; n < 4; n++) Up[n] = m == n ? Uj[n] + h : Uj[n];
+                rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
+                double ee1 = E - 0.5 * (u * u + v * v);
+                double T = Pj[4] + (ee1 - ee0) / gas_T2Cv(&o->gas, Pj[4]);
+                double p = o->gas.R * rho * T;
+                aux_flux(Up, p, nx, ny, F);
+                for (int n = 0; n < 4; n++) A[n * nv + m] = 0.5 * (F[n] - F0[n]) * dS / h;
+            }
+            double c = gas_T2c(&o->gas, Pj[4]);
+            double ra = o->prm.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
+            if (o->prm.laminar) {
+                const double* Pi = o->P + (size_t)i * ns;
+                double r = (Pi[0] + Pj[0]) * 0.5, T = (Pi[4] + Pj[4]) * 0.5, mi = sutherland(T), x0, y0, x1, y1;
+                cell_center(o, i, &x0, &y0);
+                cell_center(o, j, &x1, &y1);
+                double dd = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+                ra += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr) * dS / dd;
+            }
+            for (int n = 0; n < 4; n++) A[n * nv + n] -= 0.5 * ra;
+        }
+    }
+}
+
+/* implicitCalcDPLUR, implicit.c:990-1034: damped Jacobi, w = 2/3 */
+static void dplur(csu_oracle* o)
+{
+    const int nc = o->nc, nv = o->nv;
+    implicit_D(o);
+    dplur_blocks(o);
+    const double w = 2. / 3.;
+    for (int mm = 0; mm < o->prm.nlinear; mm++) {
+        double* t = o->dW1; o->dW1 = o->dW0; o->dW0 = t;
+        const double* x = o->dW0;
+        double* y = o->dW1;
+#pragma omp parallel for
+        for (int i = 0; i < nc; i++) {
+            for (int k = 0; k < nv; k++) y[(size_t)k * nc + i] = o->D[i] * x[(size_t)k * nc + i];
+            for (int s = 0; s < 4; s++) {
+                double dSx, dSy;
+                int j = face_from(o, i, s, &dSx, &dSy);
+                if (j < 0) continue;
+                const double* A = o->A + ((size_t)i * 4 + s) * nv * nv;
+                for (int k = 0; k < nv; k++)
+                    for (int n = 0; n < nv; n++) y[(size_t)k * nc + i] += A[k * nv + n] * x[(size_t)n * nc + j];
+            }
+        }
+#pragma omp parallel for
+        for (int i = 0; i < nc; i++)
+            for (int k = 0; k < nv; k++)
+                y[(size_t)k * nc + i] = w * (-o->R[(size_t)k * nc + i] - y[(size_t)k * nc + i]) / o->D[i] + x[(size_t)k * nc + i];
+        double ans = fabs(y[0] - x[0]);
+        for (int i = 1; i < nc; i++) ans = fmax(ans, fabs(y[i] - x[i]));
+        if (ans < 1e-14) break;
+    }
+}
+
+static void update_implicit(csu_oracle* o)   /* solverUpdateUImplicit, solver.c:1945-1955 */
+{
+    size_t n = (size_t)o->nv * o->nc;
+    for (size_t t = 0; t < n; t++) o->U[t] += o->dW1[t];
+}
+
 /* ---------------- residual norm and forces ---------------- */
 static void calc_res(const csu_oracle* o, double* res)       /* solver.c:867-892 */
 {
@@ -992,7 +1217,7 @@ static void* dup(const void* p, size_t n) { void* q = malloc(n ? n : 1); if (n) 
 
 int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
 {
-    if (p->sa || p->sst || p->time_scheme != CSU_TIME_RK) return CSU_ERR_UNSUPPORTED;
+    if (p->sa || p->sst) return CSU_ERR_UNSUPPORTED;
     csu_oracle* o = calloc(1, sizeof(*o));
     o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;
     o->prm = *p; o->dt_local = p->dt_local;
@@ -1016,6 +1241,10 @@ int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
     o->dtL = calloc(nc, sizeof(double));
     o->P = calloc((size_t)(nv + 1) * nc, sizeof(double));
     o->Pb = calloc((size_t)(nv + 1) * (o->nb ? o->nb : 1), sizeof(double));
+    o->D = calloc(nc, sizeof(double));
+    o->dW0 = calloc(nvc, sizeof(double));
+    o->dW1 = calloc(nvc, sizeof(double));      /* implicitInitDPLUR zeroes dW1 */
+    if (p->time_scheme == CSU_TIME_DPLUR) o->A = calloc((size_t)nc * 4 * nv * nv, sizeof(double));
     for (int k = 0; k < nv; k++)
         for (int i = 0; i < nc; i++) o->U[(size_t)k * nc + i] = p->uin[k];
     *out = o;
@@ -1028,6 +1257,7 @@ void oracle_destroy(csu_oracle* o)
     free(o->xy); free(o->cv); free(o->face); free(o->bface); free(o->nbr); free(o->cface); free(o->btype);
     free(o->omega); free(o->U); free(o->Uaux); free(o->R); free(o->dPx); free(o->dPy); free(o->phi);
     free(o->flux); free(o->dtL); free(o->P); free(o->Pb);
+    free(o->D); free(o->dW0); free(o->dW1); free(o->A);
     free(o);
 }
 
@@ -1065,8 +1295,19 @@ int oracle_residual(csu_oracle* o, double* R, double* dPx, double* dPy, double* 
 int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
 {
     for (int it = 0; it < niter; it++) {
-        calc_dt(o);
-        step_rk(o);
+        if (o->prm.time_scheme == CSU_TIME_RK) {
+            calc_dt(o);
+            step_rk(o);
+        } else if (o->prm.time_scheme == CSU_TIME_LUSGS) {
+            calc_R(o, o->U);
+            implicit_D(o);
+            lusgs(o);
+            update_implicit(o);
+        } else {
+            calc_R(o, o->U);
+            dplur(o);
+            update_implicit(o);
+        }
         if (o->iter == o->prm.dt_local_n) o->dt_local = 0;   /* solver.c:1886-1889 */
         if (res) calc_res(o, res + (size_t)it * o->nv);
         if (coef) calc_coeff(o, coef + (size_t)it * 4);

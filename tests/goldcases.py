@@ -8,9 +8,10 @@ from csu_amd import cases, native
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 RAMP = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
-        "ramp_ausmpup2_sub", "ramp_quad_global_dt", "ramp_tp_dtswitch"]
+        "ramp_ausmpup2_sub", "ramp_quad_global_dt", "ramp_tp_dtswitch",
+        "ramp_lusgs_ausmdv", "ramp_dplur_ausm"]
 PLATE = ["plate_lam", "plate_lamT_ausmdv", "plate_lam_axi_roe", "plate_euler_axi",
-         "plate_lam_pup_lim1"]
+         "plate_lam_pup_lim1", "plate_lusgs_lam", "plate_lusgs_lamT", "plate_dplur_lam"]
 ALL = RAMP + PLATE
 
 

@@ -32,6 +32,8 @@ SMALL = [
     ("ramp_ausmpup2_sub", dict(nx=12, ny=6), {"flux": "AUSMpup2", "mach": 0.6, "stages": 4}, 20),
     ("ramp_quad_global_dt", dict(nx=12, ny=6, split=False), {"dtLocal": 0, "stages": 1, "CFL": 0.4}, 20),
     ("ramp_tp_dtswitch", dict(nx=12, ny=6), {"TP": 1, "dtLocalN": 5}, 20),
+    ("ramp_lusgs_ausmdv", dict(nx=12, ny=6), {"timeScheme": "LUSGS", "CFL": 5, "flux": "AUSMDV"}, 20),
+    ("ramp_dplur_ausm", dict(nx=12, ny=6), {"timeScheme": "DPLUR", "CFL": 5, "flux": "AUSM"}, 20),
 ]
 
 
@@ -43,6 +45,9 @@ PLATE = [
     ("plate_euler_axi", dict(_PL, y_shift=0.05), {"axisymmetric": "1", "laminar": "0"}, 20),
     ("plate_lam_pup_lim1", dict(_PL, jitter=0.1), {"flux": "AUSMpup", "limiter": "1", "limK": "0.3",
                                                    "BC:top": "symmetry"}, 20),
+    ("plate_lusgs_lam", dict(_PL, jitter=0.1), {"timeScheme": "LUSGS", "CFL": "10"}, 20),
+    ("plate_lusgs_lamT", _PL, {"timeScheme": "LUSGS", "CFL": "10", "BC:plate": "wallT", "Twall": "330"}, 20),
+    ("plate_dplur_lam", dict(_PL, jitter=0.1), {"timeScheme": "DPLUR", "CFL": "20", "Nlinear": "8"}, 20),
 ]
 
 

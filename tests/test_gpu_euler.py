@@ -130,3 +130,32 @@ def test_laminar_history_vs_oracle(tmpcase, inp):
     assert np.max(np.abs(rg / ro - 1)) < 1e-10
     assert np.max(np.abs(cg - co)) <= 1e-10 * np.abs(co).max()
     assert _rel(dev.get_state(), orc.get_state()) < 1e-10
+
+
+@pytest.mark.parametrize("kind,inp", [
+    ("ramp", {"timeScheme": "LUSGS", "CFL": 4, "flux": "AUSMDV"}),
+    ("ramp", {"timeScheme": "DPLUR", "CFL": 4, "flux": "AUSM", "Nlinear": 12}),
+    ("plate", {"timeScheme": "LUSGS", "CFL": 20}),
+    ("plate", {"timeScheme": "DPLUR", "CFL": 30, "Nlinear": 20}),
+])
+def test_implicit_history_vs_oracle(tmpcase, kind, inp):
+    """LU-SGS (level-scheduled sweeps in the reference's cell order) and
+    DPLUR (damped Jacobi with the early exit) over 60 iterations: GPU
+    against the oracle."""
+    d = tmpcase("i")
+    print("implicit", kind, inp, flush=True)
+    if kind == "ramp":
+        cases.ramp_case(d, nx=40, ny=20, threads=1, nmax=60, **inp)
+    else:
+        cases.write_case(d, cases.plate_mesh(nx=60, ny=24, y1=2e-3, growth=1.15, jitter=0.1),
+                         cases.plate_input(threads=1, nmax=60, **inp))
+    c = native.HostCase(d)
+    dev = native.Device(c)
+    orc = po.Oracle(c)
+    rg, cg = dev.iterate(60)
+    ro, co = orc.iterate(60)
+    rel = np.max(np.abs(rg / ro - 1))
+    print("max rel residual diff", rel, flush=True)
+    assert rel < 1e-10
+    assert np.max(np.abs(cg - co)) <= 1e-10 * np.abs(co).max()
+    assert _rel(dev.get_state(), orc.get_state()) < 1e-10
