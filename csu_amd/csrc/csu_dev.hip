@@ -149,8 +149,12 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
   if (p->order != 1 && p->order != 2) return set_err(CSU_ERR_ARG, "order %d", p->order);
   if (p->flux < 0 || p->flux > 4) return set_err(CSU_ERR_ARG, "flux %d", p->flux);
   if (p->limiter != 0 && p->limiter != 1) return set_err(CSU_ERR_ARG, "limiter %d", p->limiter);
-  if (p->sa || p->sst) return set_err(CSU_ERR_UNSUPPORTED, "turbulence models not built yet");
-  if (p->nvar != 4) return set_err(CSU_ERR_ARG, "nvar %d for a laminar/Euler run", p->nvar);
+  if (p->sst) return set_err(CSU_ERR_UNSUPPORTED, "SST not built yet");
+  if (p->sa && p->laminar) return set_err(CSU_ERR_UNSUPPORTED, "laminar and sa both set");
+  if (p->nvar != (p->sa ? 5 : 4)) return set_err(CSU_ERR_ARG, "nvar %d does not match the model", p->nvar);
+  if (p->sa && p->flux == CSU_FLUX_ROE)
+    return set_err(CSU_ERR_UNSUPPORTED, "Roe has no turbulence-scalar form in the reference (flux.c:1325-1345)");
+  if (p->sa && !m->wall_dist) return set_err(CSU_ERR_ARG, "SA needs the wall distance");
   if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);
   if (p->time_scheme == CSU_TIME_DPLUR && p->nlinear < 0) return set_err(CSU_ERR_ARG, "Nlinear %d", p->nlinear);
   if (p->precision != CSU_PREC_FP64) return set_err(CSU_ERR_UNSUPPORTED, "mixed precision not built yet");
@@ -357,8 +361,9 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   d->wall = const_cast<int*>(wall_d);
   s.pref2 = const_cast<double*>(pref_d);
   if (d->visc) {
-    if ((rc = dalloc(d, &s.dTx, nc)) || (rc = dalloc(d, &s.dTy, nc)) || (rc = dalloc(d, &s.vflux, 3 * (size_t)nf)) ||
-        (rc = dalloc(d, &s.bvflux, 3 * (size_t)nb)) || (rc = dalloc(d, &s.wfric, 2 * (size_t)nb))) {
+    if ((rc = dalloc(d, &s.dTx, nc)) || (rc = dalloc(d, &s.dTy, nc)) ||
+        (rc = dalloc(d, &s.vflux, (size_t)(nv - 1) * nf)) || (rc = dalloc(d, &s.bvflux, (size_t)(nv - 1) * nb)) ||
+        (rc = dalloc(d, &s.wfric, 2 * (size_t)nb)) || (rc = dalloc(d, &s.miT, nf)) || (rc = dalloc(d, &s.src, nc))) {
       csu_dev_destroy(d);
       return rc;
     }
@@ -567,7 +572,7 @@ static int residual_t(csu_dev* d) {
 extern "C" int csu_dev_residual(csu_dev* d, double* R, double* dPx, double* dPy, double* phi, double* P,
                                 double* Pb) {
   if (!d) return set_err(CSU_ERR_ARG, "null argument");
-  int rc = d->visc ? residual_t<4, true>(d) : residual_t<4, false>(d);
+  int rc = d->nv == 5 ? residual_t<5, true>(d) : d->visc ? residual_t<4, true>(d) : residual_t<4, false>(d);
   if (rc) return rc;
   const size_t nvc = (size_t)d->nv * d->nc;
   const size_t nc = d->nc;
@@ -730,9 +735,11 @@ extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* c
   const int start = d->iter;
   int rc;
   if (d->prm.time_scheme == CSU_TIME_RK)
-    rc = d->visc ? iterate_t<4, true>(d, niter) : iterate_t<4, false>(d, niter);
+    rc = d->nv == 5 ? iterate_t<5, true>(d, niter)
+                    : d->visc ? iterate_t<4, true>(d, niter) : iterate_t<4, false>(d, niter);
   else
-    rc = d->visc ? iterate_implicit<4, true>(d, niter) : iterate_implicit<4, false>(d, niter);
+    rc = d->nv == 5 ? iterate_implicit<5, true>(d, niter)
+                    : d->visc ? iterate_implicit<4, true>(d, niter) : iterate_implicit<4, false>(d, niter);
   if (rc) return rc;
   if (res || coef) {
     std::vector<double> rh((size_t)d->hist_cap * d->nv), ch((size_t)d->hist_cap * 4);

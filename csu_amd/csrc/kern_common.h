@@ -32,6 +32,7 @@ struct Geo {
 struct St {
   double *P, *Pb, *dPx, *dPy, *phi, *dTx, *dTy;
   double *fflux, *vflux, *bvflux, *wfric, *pref2, *miT;
+  double *src;          // SA: source term Qt*Omega per cell
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference

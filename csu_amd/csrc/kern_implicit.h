@@ -87,10 +87,51 @@ __device__ __forceinline__ double diag(const double* Dp, const double* omega, co
   return Dp[i] + omega[i] / dtg[0];
 }
 
+// implicitCalcDeltaFlux_sa (implicit.c:495-553): with the positivity clamps
+__device__ __forceinline__ void delta_flux_sa(const KP& kp, const double* Pj, const double* d, double nx, double ny,
+                                              double* dF) {
+  double rho = Pj[0], u = Pj[1], v = Pj[2], p = Pj[3], n = Pj[5];
+  double T = p / (kp.gas.R * rho);
+  double e0 = gas_T2e<double>(kp.gas, T);
+  double E = e0 + (u * u + v * v) / 2;
+  double U[5] = {rho, rho * u, rho * v, rho * E, rho * n};
+  double V = (nx * U[1] + ny * U[2]) / U[0];
+  double F0[5] = {U[0] * V, U[1] * V + nx * p, U[2] * V + ny * p, (U[3] + p) * V, U[4] * V};
+#pragma unroll
+  for (int k = 0; k < 5; k++) U[k] = U[k] + d[k];
+  rho = U[0];
+  rho = fmax(rho, kp.rlim);
+  U[0] = rho;
+  u = U[1] / rho; v = U[2] / rho; E = U[3] / rho;
+  double e1 = E - (u * u + v * v) * 0.5;
+  e1 = fmax(e1, 7.176325e+02);
+  T = T + (e1 - e0) / gas_T2Cv<double>(kp.gas, T);
+  if (T < 1) {
+    T = 1;
+    e1 = gas_T2e<double>(kp.gas, T);
+    E = e1 + (u * u + v * v) * 0.5;
+    U[3] = rho * E;
+  }
+  p = kp.gas.R * T * rho;
+  n = U[4] / rho;
+  n = fmax(n, 0.0);
+  U[4] = rho * n;
+  V = (nx * U[1] + ny * U[2]) / U[0];
+  dF[0] = U[0] * V - F0[0];
+  dF[1] = (U[1] * V + nx * p) - F0[1];
+  dF[2] = (U[2] * V + ny * p) - F0[2];
+  dF[3] = (U[3] + p) * V - F0[3];
+  dF[4] = U[4] * V - F0[4];
+}
+
 // implicitCalcDeltaFlux (implicit.c:194-231) at state Pj with increment d
 template <int NV>
 __device__ __forceinline__ void delta_flux(const KP& kp, const double* Pj, const double* d, double nx, double ny,
                                            double* dF) {
+  if (NV == 5) {
+    delta_flux_sa(kp, Pj, d, nx, ny, dF);
+    return;
+  }
   double rho = Pj[0], u = Pj[1], v = Pj[2], p = Pj[3];
   double T = p / (kp.gas.R * rho);
   double e0 = gas_T2e<double>(kp.gas, T);
@@ -114,10 +155,11 @@ __device__ __forceinline__ void delta_flux(const KP& kp, const double* Pj, const
 // Face slot j of cell i seen from i: neighbour id and outward face vector
 // (the reference flips p0/p1 when i is the face's e1, implicit.c:288-305;
 // the flipped vector is the exact negation).
-__device__ __forceinline__ int face_from(const Geo& g, int nc, int i, int j, double& dSx, double& dSy, double& dS) {
+__device__ __forceinline__ int face_from(const Geo& g, int nc, int i, int j, double& dSx, double& dSy, double& dS,
+                                         int& fi) {
   const int f = g.cface[j * nc + i];
   if (f == 0) return -1;
-  const int fi = f > 0 ? f - 1 : -f - 1;
+  fi = f > 0 ? f - 1 : -f - 1;
   dS = g.fds[fi];
   if (f > 0) { dSx = g.fdsx[fi]; dSy = g.fdsy[fi]; return g.fe1[fi]; }
   dSx = -g.fdsx[fi]; dSy = -g.fdsy[fi];
@@ -141,13 +183,14 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     double dSx, dSy, dS;
-    const int nb = face_from(g, nc, i, j, dSx, dSy, dS);
+    int fi;
+    const int nb = face_from(g, nc, i, j, dSx, dSy, dS, fi);
     if (nb < 0) continue;
     if (DIR == 0 ? !(nb < i) : !(nb > i)) continue;
     const double nx = dSx / dS, ny = dSy / dS;
-    double Pj[5];
+    double Pj[NV + 1];
 #pragma unroll
-    for (int m = 0; m < 5; m++) Pj[m] = s.P[m * nc + nb];
+    for (int m = 0; m < NV + 1; m++) Pj[m] = s.P[m * nc + nb];
     double d[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) d[k] = dW[k * nc + nb];
@@ -155,7 +198,13 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
     delta_flux<NV>(kp, Pj, d, nx, ny, dF);
     double c = gas_T2c<double>(kp.gas, Pj[4]);
     double ra = kp.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
-    if (VISC && kp.laminar) {
+    if (NV == 5) {
+      // implicitFunc_sa (implicit.c:555-595)
+      double r = Pj[0], T = Pj[4], mi = sutherland(T);
+      double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
+      double dd_ = sqrt(ddx * ddx + ddy * ddy);
+      ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr + s.miT[fi] / kp.pr_t) * dS / dd_;
+    } else if (VISC && kp.laminar) {
       double r = Pj[0], T = Pj[4], mi = sutherland(T);
       double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
       double dd_ = sqrt(ddx * ddx + ddy * ddy);
@@ -194,36 +243,49 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
 #pragma unroll 1
   for (int j = 0; j < 4; j++) {
     double dSx, dSy, dS;
-    const int nb = face_from(g, nc, i, j, dSx, dSy, dS);
+    int fi;
+    const int nb = face_from(g, nc, i, j, dSx, dSy, dS, fi);
     if (nb < 0) continue;
     const double nx = dSx / dS, ny = dSy / dS;
     const double P3 = s.P[3 * nc + nb], P4 = s.P[4 * nc + nb];
-    double Uj[4];
+    double Uj[NV];
 #pragma unroll
-    for (int k = 0; k < 4; k++) Uj[k] = U[k * nc + nb];
+    for (int k = 0; k < NV; k++) Uj[k] = U[k * nc + nb];
     double rho = Uj[0], u = Uj[1] / rho, v = Uj[2] / rho, E = Uj[3] / rho;
     double V = (nx * Uj[1] + ny * Uj[2]) / Uj[0];
-    const double F0[4] = {Uj[0] * V, Uj[1] * V + nx * P3, Uj[2] * V + ny * P3, (Uj[3] + P3) * V};
+    double F0[NV];
+    F0[0] = Uj[0] * V; F0[1] = Uj[1] * V + nx * P3; F0[2] = Uj[2] * V + ny * P3; F0[3] = (Uj[3] + P3) * V;
+    if (NV == 5) F0[4] = Uj[4] * V;
     const double ee0 = E - 0.5 * (u * u + v * v);
     const double cv = gas_T2Cv<double>(kp.gas, P4);
     double* Aj = A + (size_t)j * NV * NV * nc;
 #pragma unroll
-    for (int mm = 0; mm < 4; mm++) {
-      double Up[4];
+    for (int mm = 0; mm < NV; mm++) {
+      double Up[NV];
 #pragma unroll
-      for (int n = 0; n < 4; n++) Up[n] = mm == n ? Uj[n] + h : Uj[n];
+      for (int n = 0; n < NV; n++) Up[n] = mm == n ? Uj[n] + h : Uj[n];
       rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
       double ee1 = E - 0.5 * (u * u + v * v);
       double T = P4 + (ee1 - ee0) / cv;
       double p = kp.gas.R * rho * T;
       double Vp = (nx * Up[1] + ny * Up[2]) / Up[0];
-      double F[4] = {Up[0] * Vp, Up[1] * Vp + nx * p, Up[2] * Vp + ny * p, (Up[3] + p) * Vp};
+      double F[NV];
+      F[0] = Up[0] * Vp; F[1] = Up[1] * Vp + nx * p; F[2] = Up[2] * Vp + ny * p; F[3] = (Up[3] + p) * Vp;
+      if (NV == 5) F[4] = Up[4] * Vp;
 #pragma unroll
-      for (int n = 0; n < 4; n++) Aj[(size_t)(n * NV + mm) * nc + i] = 0.5 * (F[n] - F0[n]) * dS / h;
+      for (int n = 0; n < NV; n++) Aj[(size_t)(n * NV + mm) * nc + i] = 0.5 * (F[n] - F0[n]) * dS / h;
     }
     double c = gas_T2c<double>(kp.gas, P4);
     double ra = kp.wimp * (fabs(nx * s.P[nc + nb] + ny * s.P[2 * nc + nb]) + c) * dS;
-    if (VISC && kp.laminar) {
+    if (NV == 5) {
+      // implicitUpdateA_sa (implicit.c:815-936)
+      double r = (s.P[i] + s.P[nb]) * 0.5;
+      double T = (s.P[4 * nc + i] + P4) * 0.5;
+      double mi = sutherland(T);
+      double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
+      double dd_ = sqrt(ddx * ddx + ddy * ddy);
+      ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr + s.miT[fi] / kp.pr_t) * dS / dd_;
+    } else if (VISC && kp.laminar) {
       double r = (s.P[i] + s.P[nb]) * 0.5;
       double T = (s.P[4 * nc + i] + P4) * 0.5;
       double mi = sutherland(T);
@@ -232,7 +294,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
       ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr) * dS / dd_;
     }
 #pragma unroll
-    for (int n = 0; n < 4; n++) Aj[(size_t)(n * NV + n) * nc + i] -= 0.5 * ra;
+    for (int n = 0; n < NV; n++) Aj[(size_t)(n * NV + n) * nc + i] -= 0.5 * ra;
   }
 }
 
@@ -260,7 +322,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       double dSx, dSy, dS;
-      const int nb = face_from(g, nc, i, j, dSx, dSy, dS);
+      int fi;
+      const int nb = face_from(g, nc, i, j, dSx, dSy, dS, fi);
       if (nb < 0) continue;
       const double* Aj = A + (size_t)j * NV * NV * nc;
       double xj[NV];

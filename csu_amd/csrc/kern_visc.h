@@ -123,3 +123,150 @@ __device__ __forceinline__ void wall_friction(double dSx, double dSy, double dx,
 }
 
 }  // namespace csu
+
+namespace csu {
+
+// ---------------------------------------------------------------------------
+// Spalart-Allmaras (reference sa.c)
+// ---------------------------------------------------------------------------
+// saCalcFace (sa.c:273-290): fv1 and the nu~ diffusion flux density.
+__device__ __forceinline__ void sa_calc_face(double ni, double ni_L, double r, double dnix, double dniy,
+                                             double& fv1, double& tx, double& ty) {
+  const double Cv1 = 7.1;
+  double X = ni / ni_L;
+  double aux = X * X * X;
+  fv1 = aux / (aux + Cv1 * Cv1 * Cv1);
+  const double sig = 2. / 3.;
+  tx = r * (ni_L + ni) * dnix / sig;
+  ty = r * (ni_L + ni) * dniy / sig;
+}
+
+// saCalcSource (sa.c:292-349), Allmaras et al. 2012, without trip terms.
+__device__ __forceinline__ double sa_source(double ni, double ni_L, double S, double d, double rho, double drx,
+                                            double dry, double dnix, double dniy) {
+  const double Cv1 = 7.1;
+  double X = ni / ni_L;
+  double aux = X * X * X;
+  double fv1 = aux / (aux + Cv1 * Cv1 * Cv1);
+  const double k = 0.41;
+  double fv2 = 1. - X / (1. + X * fv1);
+  double Sbar = ni * fv2 / (k * k * d * d);
+  double Stil;
+  const double cv2 = 0.7, cv3 = 0.9;
+  if (Sbar >= -cv2 * S) Stil = S + Sbar;
+  else Stil = S + S * (cv2 * cv2 * S + cv3 * Sbar) / ((cv3 - 2 * cv2) * S - Sbar);
+  const double sig = 2. / 3., Cb1 = 0.1355, Cb2 = 0.622;
+  const double Cw1 = Cb1 / (k * k) + (1. + Cb2) / sig;
+  const double Cw2 = 0.3;
+  double r = ni / (Stil * k * k * d * d);
+  r = fmin(r, 10.0);
+  aux = r * r;
+  aux = aux * aux * aux;
+  double g = r + Cw2 * (aux - r);
+  const double Cw3 = 2.;
+  aux = Cw3 * Cw3;
+  aux = aux * aux * aux;
+  double fw = g * pow((1. + aux) / (g * g * g * g * g * g + aux), 1. / 6.);
+  return rho * (Cb1 * Stil * ni + Cb2 * (dnix * dnix + dniy * dniy) / sig - Cw1 * fw * (ni * ni / (d * d))) -
+         (ni_L + ni) * (dnix * drx + dniy * dry) / sig;
+}
+
+// Interior SA viscous face flux F[0..3] (= reference faceFlux[1..4]) and
+// the face eddy viscosity, saInterFace (sa.c:138-221) or, with
+// viscBlazek, saInterFaceB (sa.c:26-134).  Gradients: u v T nu~.
+__device__ __forceinline__ void sa_face(const KP& kp, double dSx, double dSy, double dx, double dy,
+                                        const double* P0, const double* P1, const double* g0x, const double* g0y,
+                                        const double* g1x, const double* g1y, double* F, double& mi_t) {
+  double dux = (g0x[0] + g1x[0]) * 0.5, dvx = (g0x[1] + g1x[1]) * 0.5;
+  double dTx = (g0x[2] + g1x[2]) * 0.5, dnx = (g0x[3] + g1x[3]) * 0.5;
+  double duy = (g0y[0] + g1y[0]) * 0.5, dvy = (g0y[1] + g1y[1]) * 0.5;
+  double dTy = (g0y[2] + g1y[2]) * 0.5, dny = (g0y[3] + g1y[3]) * 0.5;
+  if (kp.visc_blazek) {
+    double L = sqrt(dx * dx + dy * dy);
+    double dul = (P1[1] - P0[1]) / L, dvl = (P1[2] - P0[2]) / L, dTl = (P1[4] - P0[4]) / L, dnl = (P1[5] - P0[5]) / L;
+    double a, b;
+    edge_correct(dux, duy, dul, dx, dy, L, a, b); dux = a; duy = b;
+    edge_correct(dvx, dvy, dvl, dx, dy, L, a, b); dvx = a; dvy = b;
+    edge_correct(dTx, dTy, dTl, dx, dy, L, a, b); dTx = a; dTy = b;
+    edge_correct(dnx, dny, dnl, dx, dy, L, a, b); dnx = a; dny = b;
+  }
+  double r = (P1[0] + P0[0]) * 0.5, u = (P1[1] + P0[1]) * 0.5, v = (P1[2] + P0[2]) * 0.5;
+  double T = (P1[4] + P0[4]) * 0.5, n = (P1[5] + P0[5]) * 0.5;
+  double mi_L = sutherland(T), n_L = mi_L / r, fv1, tx, ty;
+  sa_calc_face(n, n_L, r, dnx, dny, fv1, tx, ty);
+  mi_t = fv1 * r * n;
+  double mi = mi_L + mi_t;
+  double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
+  Tau t = tau_of(mi, dux, duy, dvx, dvy);
+  F[0] = t.xx * dSx + t.xy * dSy;
+  F[1] = t.xy * dSx + t.yy * dSy;
+  F[2] = (t.xx * dSx + t.xy * dSy) * u + (t.xy * dSx + t.yy * dSy) * v + k * (dTx * dSx + dTy * dSy);
+  F[3] = tx * dSx + ty * dSy;
+}
+
+// SA boundary viscous flux f[0..3] (= reference f[1..4]),
+// saBoundaryFaceViscFlux (sa.c:370-692).  Gradients: u v T nu~.
+__device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, double dSy, double nx, double ny,
+                                           double dS2, double dx, double dy, const double* Pi, const double* gx,
+                                           const double* gy, double* f) {
+  double fv1, tx, ty;
+  if (type == CSU_BC_SYMMETRY) {
+    double dux = gx[0] - (gx[0] * nx + gy[0] * ny) * nx, duy = gy[0] - (gx[0] * nx + gy[0] * ny) * ny;
+    double dvx = gx[1] - (gx[1] * nx + gy[1] * ny) * nx, dvy = gy[1] - (gx[1] * nx + gy[1] * ny) * ny;
+    double dTx = gx[2] - (gx[2] * nx + gy[2] * ny) * nx, dTy = gy[2] - (gx[2] * nx + gy[2] * ny) * ny;
+    double rho = Pi[0], u = Pi[1], v = Pi[2], T = Pi[4], n = Pi[5];
+    double mi_L = sutherland(T), n_L = mi_L / rho;
+    sa_calc_face(n, n_L, rho, 0.0, 0.0, fv1, tx, ty);  // tx, ty unused: f[3] = 0
+    double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+    double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
+    Tau t = tau_of(mi, dux, duy, dvx, dvy);
+    f[0] = (t.xx * nx + t.xy * ny) * dS2;
+    f[1] = (t.xy * nx + t.yy * ny) * dS2;
+    f[2] = (u * (t.xx * nx + t.xy * ny) + v * (t.xy * nx + t.yy * ny) + k * (dTx * nx + dTy * ny)) * dS2;
+    f[3] = 0.0;
+    return;
+  }
+  if (type == CSU_BC_OUTLET) {
+    double rho = Pi[0], u = Pi[1], v = Pi[2], T = Pi[4], n = Pi[5];
+    double mi_L = sutherland(T), n_L = mi_L / rho;
+    sa_calc_face(n, n_L, rho, gx[3], gy[3], fv1, tx, ty);
+    double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+    double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
+    Tau t = tau_of(mi, gx[0], gy[0], gx[1], gy[1]);
+    f[0] = t.xx * dSx + t.xy * dSy;
+    f[1] = t.xy * dSx + t.yy * dSy;
+    f[2] = u * (t.xx * dSx + t.xy * dSy) + v * (t.xy * dSx + t.yy * dSy) + k * (gx[2] * dSx + gy[2] * dSy);
+    f[3] = tx * dSx + ty * dSy;
+    return;
+  }
+  const bool inl = type == CSU_BC_INLET;
+  double L = sqrt(dx * dx + dy * dy);
+  double dul = ((inl ? kp.pin[1] : 0.0) - Pi[1]) / L, dvl = ((inl ? kp.pin[2] : 0.0) - Pi[2]) / L;
+  double dnl = ((inl ? kp.pin[5] : 0.0) - Pi[5]) / L;
+  double dux, duy, dvx, dvy, dnx, dny, dTx = 0.0, dTy = 0.0;
+  edge_correct(gx[0], gy[0], dul, dx, dy, L, dux, duy);
+  edge_correct(gx[1], gy[1], dvl, dx, dy, L, dvx, dvy);
+  if (type != CSU_BC_WALL) {
+    double dTl = ((inl ? kp.pin[4] : kp.twall) - Pi[4]) / L;
+    edge_correct(gx[2], gy[2], dTl, dx, dy, L, dTx, dTy);
+  }
+  edge_correct(gx[3], gy[3], dnl, dx, dy, L, dnx, dny);
+  double rho = inl ? kp.pin[0] : Pi[0];
+  double T = inl ? kp.pin[4] : (type == CSU_BC_WALLT ? kp.twall : Pi[4]);
+  double n = inl ? kp.pin[5] : 0.0;
+  double mi_L = sutherland(T), n_L = mi_L / rho;
+  sa_calc_face(n, n_L, rho, dnx, dny, fv1, tx, ty);
+  double mi_t = fv1 * rho * n;
+  double mi = type == CSU_BC_WALL ? mi_L : mi_L + mi_t;
+  Tau t = tau_of(mi, dux, duy, dvx, dvy);
+  f[0] = t.xx * dSx + t.xy * dSy;
+  f[1] = t.xy * dSx + t.yy * dSy;
+  f[3] = tx * dSx + ty * dSy;
+  if (type == CSU_BC_WALL) { f[2] = 0.0; return; }
+  double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
+  if (type == CSU_BC_WALLT) { f[2] = k * (dTx * dSx + dTy * dSy); return; }
+  double u = kp.pin[1], v = kp.pin[2];
+  f[2] = u * (t.xx * dSx + t.xy * dSy) + v * (t.xy * dSx + t.yy * dSy) + k * (dTx * dSx + dTy * dSy);
+}
+
+}  // namespace csu

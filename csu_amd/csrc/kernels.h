@@ -110,7 +110,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     lim_aux = kp.limK * sqrt(g.omega[i] / kp.volmax);
     lim_aux = lim_aux * lim_aux * lim_aux;
   }
-  double gux = 0, guy = 0, gvx = 0, gvy = 0;
+  double gux = 0, guy = 0, gvx = 0, gvy = 0, grx = 0, gry = 0, gnx = 0, gny = 0;
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) {
     const int m = kk > 3 ? kk + 1 : kk;
@@ -122,8 +122,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gx, gy);
     s.dPx[kk * nc + i] = gx;
     s.dPy[kk * nc + i] = gy;
+    if (kk == 0) { grx = gx; gry = gy; }
     if (kk == 1) { gux = gx; guy = gy; }
     if (kk == 2) { gvx = gx; gvy = gy; }
+    if (kk == 4) { gnx = gx; gny = gy; }
     if (kp.order == 2) {
       double pref2 = kp.limiter == 0 ? s.pref2[kk] * lim_aux : s.pref2[kk];
       double Pmin = u0, Pmax = u0;
@@ -152,14 +154,26 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gTx, gTy);
     s.dTx[i] = gTx;
     s.dTy[i] = gTy;
-    const double gx3[3] = {gux, gvx, gTx}, gy3[3] = {guy, gvy, gTy};
+    const double gx3[4] = {gux, gvx, gTx, gnx}, gy3[4] = {guy, gvy, gTy, gny};
+    if (NV == 5) {
+      // saInterSource (sa.c:224-260): uses grad rho, du/dy, dv/dx, grad nu~
+      const double mi_L = sutherland(Pi[4]);
+      const double S = fabs(guy - gvx);
+      const double Qt = sa_source(Pi[5], mi_L / Pi[0], S, g.wdist[i], Pi[0], grx, gry, gnx, gny);
+      s.src[i] = Qt * g.omega[i];
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int q = gb[j];
       if (q < 0) continue;
       const int t = g.btype[q];
-      double f[3];
-      laminar_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, f);
+      double f[4];
+      if (NV == 5) {
+        sa_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, f);
+        s.bvflux[3 * nb + q] = f[3];
+      } else {
+        laminar_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, f);
+      }
       s.bvflux[0 * nb + q] = f[0];
       s.bvflux[1 * nb + q] = f[1];
       s.bvflux[2 * nb + q] = f[2];
@@ -282,6 +296,20 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
     s.vflux[1 * nf + f] = F[1];
     s.vflux[2 * nf + f] = F[2];
   }
+  if (VISC && NV == 5) {
+    double P0[6], P1[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) { P0[m] = s.P[m * nc + e0]; P1[m] = s.P[m * nc + e1]; }
+    const double g0x[4] = {s.dPx[nc + e0], s.dPx[2 * nc + e0], s.dTx[e0], s.dPx[4 * nc + e0]};
+    const double g0y[4] = {s.dPy[nc + e0], s.dPy[2 * nc + e0], s.dTy[e0], s.dPy[4 * nc + e0]};
+    const double g1x[4] = {s.dPx[nc + e1], s.dPx[2 * nc + e1], s.dTx[e1], s.dPx[4 * nc + e1]};
+    const double g1y[4] = {s.dPy[nc + e1], s.dPy[2 * nc + e1], s.dTy[e1], s.dPy[4 * nc + e1]};
+    double F[4], mi_t;
+    sa_face(kp, dSx, dSy, g.xc[e1] - g.xc[e0], g.yc[e1] - g.yc[e0], P0, P1, g0x, g0y, g1x, g1y, F, mi_t);
+#pragma unroll
+    for (int k = 0; k < 4; k++) s.vflux[k * nf + f] = F[k];
+    s.miT[f] = mi_t;
+  }
 }
 
 // Residual sums + RK update.  Order of the per-cell sums follows
@@ -369,6 +397,29 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
         } else {
           R[3] -= s.bvflux[2 * nb + q];
         }
+      }
+    }
+    if (VISC && NV == 5) {
+      // saInterFace gather (sa.c:199-220), saInterSource (sa.c:224-260),
+      // saBoundary (sa.c:351-368)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int f = cf[j];
+        if (f > 0) {
+#pragma unroll
+          for (int kk = 1; kk < 5; kk++) R[kk] -= s.vflux[(kk - 1) * nf + f - 1];
+        } else if (f < 0) {
+#pragma unroll
+          for (int kk = 1; kk < 5; kk++) R[kk] += s.vflux[(kk - 1) * nf - f - 1];
+        }
+      }
+      R[4] -= s.src[i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int q = gbq[j];
+        if (q < 0) continue;
+#pragma unroll
+        for (int kk = 1; kk < 5; kk++) R[kk] -= s.bvflux[(kk - 1) * nb + q];
       }
     }
     if (flags & F_STORE_R) {

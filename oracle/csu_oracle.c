@@ -37,6 +37,7 @@ struct csu_oracle {
     double dt, volMax, Pref20[6];
     /* implicit schemes */
     double *D, *dW0, *dW1, *A;    /* A: DPLUR blocks [nc][4][nv*nv] */
+    double *wd, *miT;             /* SA: wall distance [nc], face eddy viscosity [nf] */
 };
 
 /* ---------------- gas model: gasprop.c:44-274 ---------------- */
@@ -862,6 +863,202 @@ static void laminar_boundary(csu_oracle* o)
     }
 }
 
+
+/* ---------------- Spalart-Allmaras: sa.c ---------------- */
+static void sa_calc_face(double ni, double ni_L, double r, double dnix, double dniy, double* fv1, double* tx,
+                         double* ty)       /* saCalcFace, sa.c:273-290 */
+{
+    double Cv1 = 7.1, X = ni / ni_L, aux = X * X * X;
+    *fv1 = aux / (aux + Cv1 * Cv1 * Cv1);
+    double sig = 2. / 3.;
+    *tx = r * (ni_L + ni) * dnix / sig;
+    *ty = r * (ni_L + ni) * dniy / sig;
+}
+
+/* saCalcSource, sa.c:292-349 (Allmaras et al. 2012) */
+static double sa_source(double ni, double ni_L, double S, double d, double rho, double drx, double dry,
+                        double dnix, double dniy)
+{
+    double Cv1 = 7.1, X = ni / ni_L, aux = X * X * X;
+    double fv1 = aux / (aux + Cv1 * Cv1 * Cv1);
+    double k = 0.41;
+    double fv2 = 1. - X / (1. + X * fv1);
+    double Sbar = ni * fv2 / (k * k * d * d), Stil;
+    double cv2 = 0.7, cv3 = 0.9;
+    if (Sbar >= -cv2 * S) Stil = S + Sbar;
+    else Stil = S + S * (cv2 * cv2 * S + cv3 * Sbar) / ((cv3 - 2 * cv2) * S - Sbar);
+    double sig = 2. / 3., Cb1 = 0.1355, Cb2 = 0.622;
+    double Cw1 = Cb1 / (k * k) + (1. + Cb2) / sig;
+    double Cw2 = 0.3;
+    double r = ni / (Stil * k * k * d * d);
+    r = fmin(r, 10);
+    aux = r * r;
+    aux = aux * aux * aux;
+    double g = r + Cw2 * (aux - r);
+    double Cw3 = 2.;
+    aux = Cw3 * Cw3;
+    aux = aux * aux * aux;
+    double fw = g * pow((1. + aux) / (g * g * g * g * g * g + aux), 1. / 6.);
+    return rho * (Cb1 * Stil * ni + Cb2 * (dnix * dnix + dniy * dniy) / sig - Cw1 * fw * (ni * ni / (d * d)))
+           - (ni_L + ni) * (dnix * drx + dniy * dry) / sig;
+}
+
+/* saInterFace (sa.c:138-221) or saInterFaceB (sa.c:26-134), saInterSource
+   (sa.c:224-260) */
+static void sa_inter(csu_oracle* o)
+{
+    const int nc = o->nc, nf = o->nf, ns = o->nv + 1;
+    const double *gx = o->dPx, *gy = o->dPy;
+#pragma omp parallel for
+    for (int f = 0; f < nf; f++) {
+        const int* q = o->face + 4 * f;
+        int e0 = q[0], e1 = q[1];
+        const double *P0 = o->P + (size_t)e0 * ns, *P1 = o->P + (size_t)e1 * ns;
+        double dSx, dSy;
+        face_ds(o, q[2], q[3], &dSx, &dSy);
+        double dux = (gx[nc + e0] + gx[nc + e1]) * 0.5, dvx = (gx[2 * nc + e0] + gx[2 * nc + e1]) * 0.5;
+        double dTx = (gx[3 * nc + e0] + gx[3 * nc + e1]) * 0.5, dnx = (gx[4 * nc + e0] + gx[4 * nc + e1]) * 0.5;
+        double duy = (gy[nc + e0] + gy[nc + e1]) * 0.5, dvy = (gy[2 * nc + e0] + gy[2 * nc + e1]) * 0.5;
+        double dTy = (gy[3 * nc + e0] + gy[3 * nc + e1]) * 0.5, dny = (gy[4 * nc + e0] + gy[4 * nc + e1]) * 0.5;
+        if (o->prm.visc_blazek) {
+            double x0, y0, x1, y1;
+            cell_center(o, e0, &x0, &y0);
+            cell_center(o, e1, &x1, &y1);
+            double dx = x1 - x0, dy = y1 - y0, L = sqrt(dx * dx + dy * dy);
+            double dul = (P1[1] - P0[1]) / L, dvl = (P1[2] - P0[2]) / L, dTl = (P1[4] - P0[4]) / L;
+            double dnl = (P1[5] - P0[5]) / L;
+            double a, b;
+            a = dux + (dul - (dux * dx + duy * dy) / L) * dx / L; b = duy + (dul - (dux * dx + duy * dy) / L) * dy / L;
+            dux = a; duy = b;
+            a = dvx + (dvl - (dvx * dx + dvy * dy) / L) * dx / L; b = dvy + (dvl - (dvx * dx + dvy * dy) / L) * dy / L;
+            dvx = a; dvy = b;
+            a = dTx + (dTl - (dTx * dx + dTy * dy) / L) * dx / L; b = dTy + (dTl - (dTx * dx + dTy * dy) / L) * dy / L;
+            dTx = a; dTy = b;
+            a = dnx + (dnl - (dnx * dx + dny * dy) / L) * dx / L; b = dny + (dnl - (dnx * dx + dny * dy) / L) * dy / L;
+            dnx = a; dny = b;
+        }
+        double r = (P1[0] + P0[0]) * 0.5, u = (P1[1] + P0[1]) * 0.5, v = (P1[2] + P0[2]) * 0.5;
+        double T = (P1[4] + P0[4]) * 0.5, n = (P1[5] + P0[5]) * 0.5;
+        double mi_L = sutherland(T), n_L = mi_L / r, fv1, tx, ty;
+        sa_calc_face(n, n_L, r, dnx, dny, &fv1, &tx, &ty);
+        double mi_t = fv1 * r * n, mi = mi_L + mi_t;
+        double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
+        o->miT[f] = mi_t;
+        double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+        o->flux[(size_t)1 * nf + f] = txx * dSx + txy * dSy;
+        o->flux[(size_t)2 * nf + f] = txy * dSx + tyy * dSy;
+        o->flux[(size_t)3 * nf + f] = (txx * dSx + txy * dSy) * u + (txy * dSx + tyy * dSy) * v + k * (dTx * dSx + dTy * dSy);
+        o->flux[(size_t)4 * nf + f] = tx * dSx + ty * dSy;
+    }
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < 4; j++) {
+            int f = o->cface[4 * i + j];
+            if (f > 0) for (int k = 1; k < 5; k++) o->R[(size_t)k * nc + i] -= o->flux[(size_t)k * nf + f - 1];
+            else if (f < 0) for (int k = 1; k < 5; k++) o->R[(size_t)k * nc + i] += o->flux[(size_t)k * nf - f - 1];
+        }
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++) {
+        const double* P = o->P + (size_t)i * ns;
+        double rho = P[0], T = P[4], n = P[5], mi_L = sutherland(T), n_L = mi_L / rho;
+        double S = fabs(gy[nc + i] - gx[2 * nc + i]);
+        double Qt = sa_source(n, n_L, S, o->wd[i], rho, gx[i], gy[i], gx[4 * nc + i], gy[4 * nc + i]);
+        o->R[(size_t)4 * nc + i] -= Qt * o->omega[i];
+    }
+}
+
+/* saBoundaryFaceViscFlux, sa.c:370-692 */
+static void sa_bc_flux(const csu_oracle* o, int g, double* f)
+{
+    const int nc = o->nc, ns = o->nv + 1, e0 = o->bface[3 * g], t = o->btype[g];
+    const int p0 = o->bface[3 * g + 1], p1 = o->bface[3 * g + 2];
+    const double* P = o->P + (size_t)e0 * ns;
+    const double *gx = o->dPx, *gy = o->dPy;
+    double dux, duy, dvx, dvy, dTx = 0, dTy = 0, dnx = 0, dny = 0, dSx, dSy, fv1, tx, ty;
+    if (t == CSU_BC_SYMMETRY) {
+        double x0 = o->xy[2 * p0], y0 = o->xy[2 * p0 + 1], x1 = o->xy[2 * p1], y1 = o->xy[2 * p1 + 1];
+        double sx = y1 - y0, sy = -(x1 - x0), dS = sqrt(sx * sx + sy * sy);
+        double nx = sx / dS, ny = sy / dS;
+        if (o->axi == 1) dS = dS * (y0 + y1) * 0.5;
+        double duxm = gx[nc + e0], dvxm = gx[2 * nc + e0], dTxm = gx[3 * nc + e0];
+        double duym = gy[nc + e0], dvym = gy[2 * nc + e0], dTym = gy[3 * nc + e0];
+        dux = duxm - (duxm * nx + duym * ny) * nx; duy = duym - (duxm * nx + duym * ny) * ny;
+        dvx = dvxm - (dvxm * nx + dvym * ny) * nx; dvy = dvym - (dvxm * nx + dvym * ny) * ny;
+        dTx = dTxm - (dTxm * nx + dTym * ny) * nx; dTy = dTym - (dTxm * nx + dTym * ny) * ny;
+        double rho = P[0], u = P[1], v = P[2], T = P[4], n = P[5], mi_L = sutherland(T), n_L = mi_L / rho;
+        sa_calc_face(n, n_L, rho, 0.0, 0.0, &fv1, &tx, &ty);   /* tx, ty unused (f[4] = 0) */
+        double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+        double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
+        double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+        f[1] = (txx * nx + txy * ny) * dS;
+        f[2] = (txy * nx + tyy * ny) * dS;
+        f[3] = (u * (txx * nx + txy * ny) + v * (txy * nx + tyy * ny) + k * (dTx * nx + dTy * ny)) * dS;
+        f[4] = 0.0;
+        return;
+    }
+    face_ds(o, p0, p1, &dSx, &dSy);
+    if (t == CSU_BC_OUTLET) {
+        dux = gx[nc + e0]; dvx = gx[2 * nc + e0]; dTx = gx[3 * nc + e0]; dnx = gx[4 * nc + e0];
+        duy = gy[nc + e0]; dvy = gy[2 * nc + e0]; dTy = gy[3 * nc + e0]; dny = gy[4 * nc + e0];
+        double rho = P[0], u = P[1], v = P[2], T = P[4], n = P[5], mi_L = sutherland(T), n_L = mi_L / rho;
+        sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
+        double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+        double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
+        double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+        f[1] = txx * dSx + txy * dSy;
+        f[2] = txy * dSx + tyy * dSy;
+        f[3] = u * (txx * dSx + txy * dSy) + v * (txy * dSx + tyy * dSy) + k * (dTx * dSx + dTy * dSy);
+        f[4] = tx * dSx + ty * dSy;
+        return;
+    }
+    /* inlet, wall, wallT: one-sided differences to the boundary values */
+    const double* Pin = o->prm.pin;
+    double x0, y0, x1, y1;
+    cell_center(o, e0, &x0, &y0);
+    x1 = (o->xy[2 * p0] + o->xy[2 * p1]) * 0.5; y1 = (o->xy[2 * p0 + 1] + o->xy[2 * p1 + 1]) * 0.5;
+    double dx = x1 - x0, dy = y1 - y0, L = sqrt(dx * dx + dy * dy);
+    int inl = t == CSU_BC_INLET;
+    double dul = ((inl ? Pin[1] : 0) - P[1]) / L, dvl = ((inl ? Pin[2] : 0) - P[2]) / L;
+    double dnl = ((inl ? Pin[5] : 0) - P[5]) / L;
+    double duxm = gx[nc + e0], dvxm = gx[2 * nc + e0], dnxm = gx[4 * nc + e0];
+    double duym = gy[nc + e0], dvym = gy[2 * nc + e0], dnym = gy[4 * nc + e0];
+    dux = duxm + (dul - (duxm * dx + duym * dy) / L) * dx / L; duy = duym + (dul - (duxm * dx + duym * dy) / L) * dy / L;
+    dvx = dvxm + (dvl - (dvxm * dx + dvym * dy) / L) * dx / L; dvy = dvym + (dvl - (dvxm * dx + dvym * dy) / L) * dy / L;
+    if (t != CSU_BC_WALL) {
+        double dTl = ((inl ? Pin[4] : o->prm.twall) - P[4]) / L;
+        double dTxm = gx[3 * nc + e0], dTym = gy[3 * nc + e0];
+        dTx = dTxm + (dTl - (dTxm * dx + dTym * dy) / L) * dx / L;
+        dTy = dTym + (dTl - (dTxm * dx + dTym * dy) / L) * dy / L;
+    }
+    dnx = dnxm + (dnl - (dnxm * dx + dnym * dy) / L) * dx / L; dny = dnym + (dnl - (dnxm * dx + dnym * dy) / L) * dy / L;
+    double rho = inl ? Pin[0] : P[0];
+    double T = inl ? Pin[4] : (t == CSU_BC_WALLT ? o->prm.twall : P[4]);
+    double n = inl ? Pin[5] : 0.0;
+    double mi_L = sutherland(T), n_L = mi_L / rho;
+    sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
+    double mi_t = fv1 * rho * n, mi = t == CSU_BC_WALL ? mi_L : mi_L + mi_t;
+    double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+    f[1] = txx * dSx + txy * dSy;
+    f[2] = txy * dSx + tyy * dSy;
+    f[4] = tx * dSx + ty * dSy;
+    if (t == CSU_BC_WALL) { f[3] = 0.0; return; }
+    double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
+    if (t == CSU_BC_WALLT) { f[3] = k * (dTx * dSx + dTy * dSy); return; }
+    double u = Pin[1], v = Pin[2];
+    f[3] = u * (txx * dSx + txy * dSy) + v * (txy * dSx + tyy * dSy) + k * (dTx * dSx + dTy * dSy);
+}
+
+static void sa_boundary(csu_oracle* o)       /* saBoundary / saBoundaryFace, sa.c:351-368 */
+{
+    const int nc = o->nc;
+    for (int g = 0; g < o->nb; g++) {
+        double f[5];
+        sa_bc_flux(o, g, f);
+        int e0 = o->bface[3 * g];
+        for (int k = 1; k < 5; k++) o->R[(size_t)k * nc + e0] -= f[k];
+    }
+}
+
 /* solverCalcR, solver.c:685-739 */
 static void calc_R(csu_oracle* o, double* U)
 {
@@ -875,6 +1072,11 @@ static void calc_R(csu_oracle* o, double* U)
         grad_T(o);
         laminar_inter(o);
         laminar_boundary(o);
+    }
+    if (o->prm.sa == 1) {
+        grad_T(o);
+        sa_inter(o);
+        sa_boundary(o);
     }
 }
 
@@ -969,6 +1171,10 @@ static void implicit_D(csu_oracle* o)
                 double mi = sutherland(T);
                 Lv += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr) * dS * dS;
             }
+            if (o->prm.sa || o->prm.sst) {
+                double mi = sutherland(T);
+                Lv += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr + o->miT[f1] / o->prm.pr_t) * dS * dS;
+            }
         }
         o->dtL[i] = Lc;
         o->D[i] = 0.5 * o->prm.wimp * Lc;
@@ -1025,28 +1231,83 @@ static void delta_flux(const csu_oracle* o, const double* Pj, const double* d, d
     for (int k = 0; k < 4; k++) dF[k] = F1[k] - F0[k];
 }
 
+/* implicitAuxCalcFlux_sa2 / implicitCalcDeltaFlux_sa, implicit.c:484-553 */
+static void aux_flux_sa(const double* U, double p, double nx, double ny, double* F)
+{
+    double V = (nx * U[1] + ny * U[2]) / U[0];
+    F[0] = U[0] * V;
+    F[1] = U[1] * V + nx * p;
+    F[2] = U[2] * V + ny * p;
+    F[3] = (U[3] + p) * V;
+    F[4] = U[4] * V;
+}
+
+static void delta_flux_sa(const csu_oracle* o, const double* Pj, const double* d, double nx, double ny, double* dF)
+{
+    double rho = Pj[0], u = Pj[1], v = Pj[2], p = Pj[3], n = Pj[5], F0[5], F1[5], U[5];
+    double T = p / (o->gas.R * rho);
+    double e0 = gas_T2e(&o->gas, T);
+    double E = e0 + (u * u + v * v) / 2;
+    U[0] = rho; U[1] = rho * u; U[2] = rho * v; U[3] = rho * E; U[4] = rho * n;
+    aux_flux_sa(U, p, nx, ny, F0);
+    for (int k = 0; k < 5; k++) U[k] = U[k] + d[k];
+    rho = U[0];
+    rho = fmax(rho, o->prm.rlim);
+    U[0] = rho;
+    u = U[1] / rho; v = U[2] / rho; E = U[3] / rho;
+    double e1 = E - (u * u + v * v) * 0.5;
+    e1 = fmax(e1, 7.176325e+02);
+    T = T + (e1 - e0) / gas_T2Cv(&o->gas, T);
+    if (T < 1) {
+        T = 1;
+        e1 = gas_T2e(&o->gas, T);
+        E = e1 + (u * u + v * v) * 0.5;
+        U[3] = rho * E;
+    }
+    p = o->gas.R * T * rho;
+    n = U[4] / rho;
+    n = fmax(n, 0);
+    U[4] = rho * n;
+    aux_flux_sa(U, p, nx, ny, F1);
+    for (int k = 0; k < 5; k++) dF[k] = F1[k] - F0[k];
+}
+
 /* face j of cell i seen from i: neighbour, outward normal (implicit.c:288-305) */
-static int face_from(const csu_oracle* o, int i, int slot, double* dSx, double* dSy)
+static int face_from_id(const csu_oracle* o, int i, int slot, double* dSx, double* dSy, int* fid)
 {
     int f = o->cface[4 * i + slot];
     if (f == 0) return -1;
+    *fid = f > 0 ? f - 1 : -f - 1;
     const int* q = o->face + 4 * (f > 0 ? f - 1 : -f - 1);
     if (f > 0) { face_ds(o, q[2], q[3], dSx, dSy); return q[1]; }
     face_ds(o, q[3], q[2], dSx, dSy);
     return q[0];
 }
 
+static int face_from(const csu_oracle* o, int i, int slot, double* dSx, double* dSy)
+{
+    int fid;
+    return face_from_id(o, i, slot, dSx, dSy, &fid);
+}
+
 /* implicitFunc, implicit.c:233-272: dW[.][i] -= 0.5 (dF dS - ra dW[.][j]) */
-static void implicit_func(const csu_oracle* o, int i, int j, double dSx, double dSy, double* dW)
+static void implicit_func(const csu_oracle* o, int i, int j, int fid, double dSx, double dSy, double* dW)
 {
     const int nc = o->nc, ns = o->nv + 1;
     double dS = sqrt(dSx * dSx + dSy * dSy), nx = dSx / dS, ny = dSy / dS;
     const double* Pj = o->P + (size_t)j * ns;
-    double d[4] = {dW[j], dW[nc + j], dW[2 * nc + j], dW[3 * nc + j]}, dF[4];
-    delta_flux(o, Pj, d, nx, ny, dF);
+    double d[5] = {dW[j], dW[nc + j], dW[2 * nc + j], dW[3 * nc + j], o->nv == 5 ? dW[4 * nc + j] : 0.0}, dF[5];
+    if (o->nv == 5) delta_flux_sa(o, Pj, d, nx, ny, dF);
+    else delta_flux(o, Pj, d, nx, ny, dF);
     double c = gas_T2c(&o->gas, Pj[4]);
     double ra = o->prm.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
-    if (o->prm.laminar) {
+    if (o->prm.sa) {        /* implicitFunc_sa, implicit.c:555-595 */
+        double r = Pj[0], T = Pj[4], mi = sutherland(T), x0, y0, x1, y1;
+        cell_center(o, i, &x0, &y0);
+        cell_center(o, j, &x1, &y1);
+        double dd = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+        ra += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr + o->miT[fid] / o->prm.pr_t) * dS / dd;
+    } else if (o->prm.laminar) {
         double r = Pj[0], T = Pj[4], mi = sutherland(T), x0, y0, x1, y1;
         cell_center(o, i, &x0, &y0);
         cell_center(o, j, &x1, &y1);
@@ -1064,8 +1325,8 @@ static void lusgs(csu_oracle* o)
         for (int k = 0; k < nv; k++) o->dW0[(size_t)k * nc + i] = -o->R[(size_t)k * nc + i];
         for (int s = 0; s < 4; s++) {
             double dSx, dSy;
-            int j = face_from(o, i, s, &dSx, &dSy);
-            if (j >= 0 && j < i) implicit_func(o, i, j, dSx, dSy, o->dW0);
+            int fid, j = face_from_id(o, i, s, &dSx, &dSy, &fid);
+            if (j >= 0 && j < i) implicit_func(o, i, j, fid, dSx, dSy, o->dW0);
         }
         for (int k = 0; k < nv; k++) o->dW0[(size_t)k * nc + i] /= o->D[i];
     }
@@ -1074,8 +1335,8 @@ static void lusgs(csu_oracle* o)
     for (int i = nc - 1; i >= 0; i--) {
         for (int s = 0; s < 4; s++) {
             double dSx, dSy;
-            int j = face_from(o, i, s, &dSx, &dSy);
-            if (j > i) implicit_func(o, i, j, dSx, dSy, o->dW1);
+            int fid, j = face_from_id(o, i, s, &dSx, &dSy, &fid);
+            if (j > i) implicit_func(o, i, j, fid, dSx, dSy, o->dW1);
         }
         for (int k = 0; k < nv; k++) o->dW1[(size_t)k * nc + i] /= o->D[i];
     }
@@ -1091,27 +1352,35 @@ static void dplur_blocks(csu_oracle* o)
     for (int i = 0; i < nc; i++) {
         for (int s = 0; s < 4; s++) {
             double dSx, dSy;
-            int j = face_from(o, i, s, &dSx, &dSy);
+            int fid, j = face_from_id(o, i, s, &dSx, &dSy, &fid);
             if (j < 0) continue;
             double* A = o->A + ((size_t)i * 4 + s) * nv * nv;
             double dS = sqrt(dSx * dSx + dSy * dSy), nx = dSx / dS, ny = dSy / dS;
             const double* Pj = o->P + (size_t)j * ns;
-            double Uj[4] = {o->U[j], o->U[nc + j], o->U[2 * nc + j], o->U[3 * nc + j]}, F0[4], F[4], Up[4];
+            double Uj[5], F0[5], F[5], Up[5];
+            for (int k = 0; k < nv; k++) Uj[k] = o->U[(size_t)k * nc + j];
             double rho = Uj[0], u = Uj[1] / rho, v = Uj[2] / rho, E = Uj[3] / rho;
-            aux_flux(Uj, Pj[3], nx, ny, F0);
+            if (nv == 5) aux_flux_sa(Uj, Pj[3], nx, ny, F0); else aux_flux(Uj, Pj[3], nx, ny, F0);
             double ee0 = E - 0.5 * (u * u + v * v);
-            for (int m = 0; m < 4; m++) {
-                for (int n = 0; n < 4; n++) Up[n] = m == n ? Uj[n] + h : Uj[n];
+            for (int m = 0; m < nv; m++) {
+                for (int n = 0; n < nv; n++) Up[n] = m == n ? Uj[n] + h : Uj[n];
                 rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
                 double ee1 = E - 0.5 * (u * u + v * v);
                 double T = Pj[4] + (ee1 - ee0) / gas_T2Cv(&o->gas, Pj[4]);
                 double p = o->gas.R * rho * T;
-                aux_flux(Up, p, nx, ny, F);
-                for (int n = 0; n < 4; n++) A[n * nv + m] = 0.5 * (F[n] - F0[n]) * dS / h;
+                if (nv == 5) aux_flux_sa(Up, p, nx, ny, F); else aux_flux(Up, p, nx, ny, F);
+                for (int n = 0; n < nv; n++) A[n * nv + m] = 0.5 * (F[n] - F0[n]) * dS / h;
             }
             double c = gas_T2c(&o->gas, Pj[4]);
             double ra = o->prm.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
-            if (o->prm.laminar) {
+            if (o->prm.sa) {     /* implicitUpdateA_sa, implicit.c:815-936 */
+                const double* Pi = o->P + (size_t)i * ns;
+                double r = (Pi[0] + Pj[0]) * 0.5, T = (Pi[4] + Pj[4]) * 0.5, mi = sutherland(T), x0, y0, x1, y1;
+                cell_center(o, i, &x0, &y0);
+                cell_center(o, j, &x1, &y1);
+                double dd = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+                ra += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr + o->miT[fid] / o->prm.pr_t) * dS / dd;
+            } else if (o->prm.laminar) {
                 const double* Pi = o->P + (size_t)i * ns;
                 double r = (Pi[0] + Pj[0]) * 0.5, T = (Pi[4] + Pj[4]) * 0.5, mi = sutherland(T), x0, y0, x1, y1;
                 cell_center(o, i, &x0, &y0);
@@ -1119,7 +1388,7 @@ static void dplur_blocks(csu_oracle* o)
                 double dd = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
                 ra += fmax(4 / (3 * r), gas_T2gamma(&o->gas, T) / r) * (mi / o->prm.pr) * dS / dd;
             }
-            for (int n = 0; n < 4; n++) A[n * nv + n] -= 0.5 * ra;
+            for (int n = 0; n < nv; n++) A[n * nv + n] -= 0.5 * ra;
         }
     }
 }
@@ -1217,7 +1486,8 @@ static void* dup(const void* p, size_t n) { void* q = malloc(n ? n : 1); if (n) 
 
 int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
 {
-    if (p->sa || p->sst) return CSU_ERR_UNSUPPORTED;
+    if (p->sst) return CSU_ERR_UNSUPPORTED;
+    if (p->sa && (!m->wall_dist || p->flux == CSU_FLUX_ROE)) return CSU_ERR_UNSUPPORTED;
     csu_oracle* o = calloc(1, sizeof(*o));
     o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;
     o->prm = *p; o->dt_local = p->dt_local;
@@ -1242,6 +1512,9 @@ int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
     o->P = calloc((size_t)(nv + 1) * nc, sizeof(double));
     o->Pb = calloc((size_t)(nv + 1) * (o->nb ? o->nb : 1), sizeof(double));
     o->D = calloc(nc, sizeof(double));
+    o->miT = calloc((size_t)(o->nf ? o->nf : 1), sizeof(double));
+    o->wd = calloc(nc, sizeof(double));
+    if (m->wall_dist) memcpy(o->wd, m->wall_dist, sizeof(double) * nc);
     o->dW0 = calloc(nvc, sizeof(double));
     o->dW1 = calloc(nvc, sizeof(double));      /* implicitInitDPLUR zeroes dW1 */
     if (p->time_scheme == CSU_TIME_DPLUR) o->A = calloc((size_t)nc * 4 * nv * nv, sizeof(double));
@@ -1257,7 +1530,7 @@ void oracle_destroy(csu_oracle* o)
     free(o->xy); free(o->cv); free(o->face); free(o->bface); free(o->nbr); free(o->cface); free(o->btype);
     free(o->omega); free(o->U); free(o->Uaux); free(o->R); free(o->dPx); free(o->dPy); free(o->phi);
     free(o->flux); free(o->dtL); free(o->P); free(o->Pb);
-    free(o->D); free(o->dW0); free(o->dW1); free(o->A);
+    free(o->D); free(o->dW0); free(o->dW1); free(o->A); free(o->miT); free(o->wd);
     free(o);
 }
 

@@ -48,6 +48,15 @@ PLATE = [
     ("plate_lusgs_lam", dict(_PL, jitter=0.1), {"timeScheme": "LUSGS", "CFL": "10"}, 20),
     ("plate_lusgs_lamT", _PL, {"timeScheme": "LUSGS", "CFL": "10", "BC:plate": "wallT", "Twall": "330"}, 20),
     ("plate_dplur_lam", dict(_PL, jitter=0.1), {"timeScheme": "DPLUR", "CFL": "20", "Nlinear": "8"}, 20),
+    # Spalart-Allmaras (nvar 5); the wall distance is part of the mesh trace
+    ("plate_sa_ausmdv", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "flux": "AUSMDV"}, 20),
+    ("plate_sa_blazek_wallT", _PL, {"laminar": "0", "sa": "1", "viscBlazek": "1", "BC:plate": "wallT",
+                                    "Twall": "320"}, 20),
+    ("plate_sa_pup_lim1", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "flux": "AUSMpup", "limiter": "1",
+                                                  "limK": "0.3", "turb1order": "1"}, 20),
+    ("plate_sa_lusgs", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
+    ("plate_sa_dplur", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "timeScheme": "DPLUR", "CFL": "20",
+                                              "Nlinear": "6"}, 20),
 ]
 
 
@@ -55,15 +64,17 @@ def case_dir(tag):
     return tempfile.mkdtemp(prefix="gd", dir="/tmp") + "/"
 
 
-def make_small():
+def make_small(only=None):
     for name, mk, inp, nit in SMALL + PLATE:
+        if only and name not in only:
+            continue
         d = case_dir(name)
         if name.startswith("plate"):
             cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nit, **inp))
         else:
             cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nit, **inp))
         mesh = po.ref_trace(d, "mesh")
-        nv = 4
+        nv = 5 if inp.get("sa") == "1" else 4
         nc = mesh["cellv"].size // 4
         rng = np.random.default_rng(5)
         # perturbed free stream: the reference's own initial state, +-5 %
