@@ -13,6 +13,7 @@
 #include "csu.h"
 #include "kernels.h"
 #include "kern_implicit.h"
+#include "kern_setup.h"
 
 using namespace csu;
 
@@ -154,7 +155,6 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
   if (p->nvar != (p->sa ? 5 : 4)) return set_err(CSU_ERR_ARG, "nvar %d does not match the model", p->nvar);
   if (p->sa && p->flux == CSU_FLUX_ROE)
     return set_err(CSU_ERR_UNSUPPORTED, "Roe has no turbulence-scalar form in the reference (flux.c:1325-1345)");
-  if (p->sa && !m->wall_dist) return set_err(CSU_ERR_ARG, "SA needs the wall distance");
   if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);
   if (p->time_scheme == CSU_TIME_DPLUR && p->nlinear < 0) return set_err(CSU_ERR_ARG, "Nlinear %d", p->nlinear);
   if (p->precision != CSU_PREC_FP64) return set_err(CSU_ERR_UNSUPPORTED, "mixed precision not built yet");
@@ -193,6 +193,71 @@ static double tri_omega(const csu_mesh_desc* m, int p0, int p1, int p2) {
   const double* q = m->node_xy;
   double x1 = q[2 * p0], x2 = q[2 * p1], x3 = q[2 * p2], y1 = q[2 * p0 + 1], y2 = q[2 * p1 + 1], y3 = q[2 * p2 + 1];
   return 0.5 * ((x1 - x2) * (y1 + y2) + (x2 - x3) * (y2 + y3) + (x3 - x1) * (y3 + y1));
+}
+
+// Wall segments (p0x, p0y, p1x, p1y) SoA [4][nw] of the wall / wallT
+// boundary faces, in boundary-face order.
+static std::vector<double> wall_segments(const csu_mesh_desc* m, int* nw) {
+  std::vector<int> w;
+  for (int g = 0; g < m->nbface; g++) {
+    const int t = m->mark_type[m->bface_mark[g]];
+    if (t == CSU_BC_WALL || t == CSU_BC_WALLT) w.push_back(g);
+  }
+  const size_t n = w.size();
+  std::vector<double> seg(4 * (n ? n : 1));
+  for (size_t k = 0; k < n; k++) {
+    const int32_t* q = m->bface + 3 * w[k];
+    seg[k] = m->node_xy[2 * q[1]];
+    seg[n + k] = m->node_xy[2 * q[1] + 1];
+    seg[2 * n + k] = m->node_xy[2 * q[2]];
+    seg[3 * n + k] = m->node_xy[2 * q[2] + 1];
+  }
+  *nw = (int)n;
+  return seg;
+}
+
+// Wall distance on the device into dist[nc] (device memory), from device
+// cell centres; synchronous on `st`.
+static int wall_distance_launch(const csu_mesh_desc* m, hipStream_t st, const double* xc, const double* yc,
+                                double* dist) {
+  int nw = 0;
+  std::vector<double> seg = wall_segments(m, &nw);
+  double* dseg = nullptr;
+  if (hipMalloc(&dseg, seg.size() * sizeof(double)) != hipSuccess)
+    return set_err(CSU_ERR_NOMEM, "hipMalloc of %zu wall-segment bytes failed", seg.size() * sizeof(double));
+  int rc = CSU_OK;
+  if (hipMemcpyAsync(dseg, seg.data(), seg.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = set_err(CSU_ERR_HIP, "wall-segment upload failed");
+  if (!rc && m->ncell > 0) {
+    hipLaunchKernelGGL(k_wall_dist, dim3(nblk(m->ncell)), dim3(CSU_BLOCK), 0, st, m->ncell, xc, yc, nw, dseg, dist);
+    if (hipGetLastError() != hipSuccess) rc = set_err(CSU_ERR_HIP, "k_wall_dist launch failed");
+  }
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = set_err(CSU_ERR_HIP, "k_wall_dist failed");
+  (void)hipFree(dseg);
+  return rc;
+}
+
+extern "C" int csu_wall_distance_dev(const csu_mesh_desc* m, double* wd) {
+  if (!m || !wd || m->ncell <= 0 || !m->node_xy || !m->cell_vert) return set_err(CSU_ERR_ARG, "bad mesh or output");
+  if (m->nbface > 0 && (!m->bface || !m->bface_mark || !m->mark_type)) return set_err(CSU_ERR_ARG, "bad boundary");
+  const int nc = m->ncell;
+  std::vector<double> xy(2 * (size_t)nc);
+  for (int i = 0; i < nc; i++) elem_center(m, i, &xy[i], &xy[nc + i]);
+  double *dxy = nullptr, *dd = nullptr;
+  if (hipMalloc(&dxy, xy.size() * sizeof(double)) != hipSuccess) return set_err(CSU_ERR_NOMEM, "hipMalloc failed");
+  if (hipMalloc(&dd, (size_t)nc * sizeof(double)) != hipSuccess) {
+    (void)hipFree(dxy);
+    return set_err(CSU_ERR_NOMEM, "hipMalloc failed");
+  }
+  int rc = CSU_OK;
+  if (hipMemcpy(dxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_err(CSU_ERR_HIP, "centre upload failed");
+  if (!rc) rc = wall_distance_launch(m, 0, dxy, dxy + nc, dd);
+  if (!rc && hipMemcpy(wd, dd, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = set_err(CSU_ERR_HIP, "wall-distance download failed");
+  (void)hipFree(dxy);
+  (void)hipFree(dd);
+  return rc;
 }
 
 extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_dev** out) {
@@ -360,6 +425,15 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   }
   d->wall = const_cast<int*>(wall_d);
   s.pref2 = const_cast<double*>(pref_d);
+  if ((p->sa || p->sst) && !m->wall_dist) {
+    // saCalcD on the device when the caller brings no wall distance
+    double* wdd = nullptr;
+    if ((rc = dalloc(d, &wdd, nc)) || (rc = wall_distance_launch(m, d->stream, g.xc, g.yc, wdd))) {
+      csu_dev_destroy(d);
+      return rc;
+    }
+    g.wdist = wdd;
+  }
   if (d->visc) {
     if ((rc = dalloc(d, &s.dTx, nc)) || (rc = dalloc(d, &s.dTy, nc)) ||
         (rc = dalloc(d, &s.vflux, (size_t)(nv - 1) * nf)) || (rc = dalloc(d, &s.bvflux, (size_t)(nv - 1) * nb)) ||

@@ -175,10 +175,9 @@ int csu_case_load(const char* wd, csu_case** out)
     }
     p->precision = opt_i(in, "precision", CSU_PREC_FP64);
 
-    if (p->sa == 1 || p->sst == 1) {
-        rc = csu_wall_distance(c);
-        if (rc) { csu_case_free(c); return rc; }
-    }
+    /* saCalcD (sa.c:703) is not run here: csu_dev_create computes the wall
+       distance on the device when mesh.wall_dist is NULL; csu_wall_distance
+       is the host restatement for callers that want it on the host. */
 
     /* solverInitDomain, solver.c:1660-1697 (steady mode only) */
     c->tube = atoi(need(in, "tube"));

@@ -102,6 +102,7 @@ def lib():
     L.csu_read_restart.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_write_solution.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_wall_distance.argtypes = [C.POINTER(Case)]
+    L.csu_wall_distance_dev.argtypes = [C.POINTER(MeshDesc), f64p]
     _lib = L
     return L
 
@@ -135,7 +136,10 @@ def read_input(path: str) -> list:
 class HostCase:
     """A case directory loaded by the host C (csu_case_load)."""
 
-    def __init__(self, wd: str):
+    def __init__(self, wd: str, host_wall_distance: bool = True):
+        """host_wall_distance: for SA/SST cases, fill mesh.wall_dist with the
+        host restatement of saCalcD (csu_wall_distance).  With False the
+        mesh carries none and csu_dev_create computes it on the device."""
         if not wd.endswith("/"):
             wd += "/"
         self.wd = wd
@@ -152,6 +156,8 @@ class HostCase:
         self.nvar = c.prm.nvar
         self.nmax = c.nmax
         self.params = c.prm
+        if host_wall_distance and (c.prm.sa or c.prm.sst):
+            check(L.csu_wall_distance(self._c))
 
     def _arr(self, ptr, n, dtype):
         return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
@@ -203,6 +209,13 @@ class HostCase:
             self.close()
         except Exception:
             pass
+
+
+def wall_distance_dev(case: "HostCase") -> np.ndarray:
+    """saCalcD on the device (csu_wall_distance_dev) for a loaded case."""
+    out = np.empty(case.ncell, np.float64)
+    check(lib().csu_wall_distance_dev(C.byref(case.mesh), _p(out, f64p)))
+    return out
 
 
 class Device:

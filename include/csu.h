@@ -95,6 +95,13 @@ typedef struct csu_dev csu_dev;
 int  csu_dev_create(const csu_mesh_desc* mesh, const csu_params* prm, csu_dev** out);
 void csu_dev_destroy(csu_dev* dev);
 
+/* Wall distance of every cell, reference saCalcD (sa.c:703-763): the
+ * minimum over wall / wallT boundary faces of the distance from the cell's
+ * vertex average to the face segment, computed on the current device.
+ * wd: [ncell] host output.  csu_dev_create runs the same kernel itself when
+ * SA or SST is on and mesh->wall_dist is NULL. */
+int  csu_wall_distance_dev(const csu_mesh_desc* mesh, double* wd);
+
 /* Host <-> device state, SoA [nvar][ncell] FP64. */
 int  csu_dev_set_state(csu_dev* dev, const double* U);
 int  csu_dev_get_state(csu_dev* dev, double* U);
