@@ -50,6 +50,15 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int) -> float:
     return 0.0
 
 
+# HBM bytes per launch on the default 2000x1000x2 workload, from rocprofv3
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) passes, both scaled by what the
+# counters read for a plain 1 GiB copy (profiles/euler_ramp_4M_rk5_fp64.md).
+# A counter pass cannot run inside this timed process, so the figure is
+# reported for that workload only.
+MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 939.13e6, ("k_flux", 2000, 1000): 1689.89e6,
+                    ("k_update", 2000, 1000): 892.76e6}
+
+
 def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool):
     from csu_amd import cases, native
     d = tempfile.mkdtemp(prefix="bn", dir="/tmp") + "/"
@@ -105,7 +114,22 @@ def main():
     ap.add_argument("--ny", type=int, default=1000)
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 200x100x2 timing (profiling runs of the big mesh alone)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB ten times (the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.copy_calib:
+        import torch
+        n = a.copy_calib * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(10):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8}))
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
               "multi-GPU sharding is a design note, not a benchmark", file=sys.stderr)
@@ -115,7 +139,7 @@ def main():
     log(f"ramp {a.nx}x{a.ny}x2: generating, loading, {a.warmup}+{a.steps} iterations")
     big = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=True)
     log(f"big done: {big['seconds']:.3f} s; ramp 200x100x2")
-    small = gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False)
+    small = None if a.no_small else gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False)
     log("small done; cpu baseline" if not a.no_cpu else "small done")
     nc = big["nc"]
     ms = 1e3 * big["seconds"] / a.steps
@@ -128,7 +152,8 @@ def main():
     traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"])
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((dom, a.nx, a.ny)),
+            "kernel": dom,
             "kernel_avg_ms": avg_ms,
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
             "iteration_alg_bytes_per_cell": 2334.5,
@@ -141,7 +166,8 @@ def main():
         "config": {"workload": f"euler-ramp M2.5 Roe/LSQ/Venkat(K=5) RK5 CFL0.5 FP64, "
                                f"{a.nx}x{a.ny}x2 tri mesh",
                    "cells": nc, "faces": big["nf"], "boundary_faces": big["nb"],
-                   "config0_40k_cell_iter_per_s": small["nc"] * max(a.steps, 50) / small["seconds"]},
+                   "config0_40k_cell_iter_per_s":
+                       None if small is None else small["nc"] * max(a.steps, 50) / small["seconds"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

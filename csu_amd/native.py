@@ -214,7 +214,7 @@ class HostCase:
 def wall_distance_dev(case: "HostCase") -> np.ndarray:
     """saCalcD on the device (csu_wall_distance_dev) for a loaded case."""
     out = np.empty(case.ncell, np.float64)
-    check(lib().csu_wall_distance_dev(C.byref(case.mesh), _p(out, f64p)))
+    check(lib().csu_wall_distance_dev(C.byref(case.mesh), _p(out, C.c_double)))
     return out
 
 
