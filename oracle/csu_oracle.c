@@ -37,7 +37,10 @@ struct csu_oracle {
     double dt, volMax, Pref20[6];
     /* implicit schemes */
     double *D, *dW0, *dW1, *A;    /* A: DPLUR blocks [nc][4][nv*nv] */
-    double *wd, *miT;             /* SA: wall distance [nc], face eddy viscosity [nf] */
+    double *wd, *miT;             /* SA/SST: wall distance [nc], face eddy viscosity [nf] */
+    /* SST: source Jacobians of the LU-SGS 2x2 block (sst.c:664-671) */
+    double *dQkdr, *dQkdrk, *dQkdro, *dQodr, *dQodrk, *dQodro;
+    double sst_g1, sst_g2;        /* sstInit, sst.c:18-38 */
 };
 
 /* ---------------- gas model: gasprop.c:44-274 ---------------- */
@@ -101,6 +104,16 @@ static double gas_T2c(const ogas* g, double T)           /* gasprop.c:168-184 */
 }
 
 static double gas_T2Cp(const ogas* g, double T) { return gas_T2Cv(g, T) + g->R; }   /* gasprop.c:174-177 */
+
+/* SST model constants, sstInit (sst.c:18-38) */
+#define SST_SK1 0.85
+#define SST_SO1 0.5
+#define SST_B1 0.075
+#define SST_SK2 1.0
+#define SST_SO2 0.856
+#define SST_B2 0.0828
+#define SST_BS 0.09
+#define SST_A1 0.31
 
 static double sutherland(double T) { return 1.458e-6 * T * sqrt(T) / (T + 110.4); }  /* solver.c:1222-1225 */
 
@@ -220,19 +233,22 @@ static void flux_roe(const csu_oracle* o, const double* L, const double* Rt, dou
 }
 
 /* enthalpy and temperature from a primitive state (flux.c:138-144 pattern) */
-static void HT(const csu_oracle* o, double r, double u, double v, double p, double* H, double* T)
+/* total enthalpy; kt is the turbulent-energy term the _sst fluxes add to
+   rho*E (k*rho, or k alone in fluxAUSMDV_sst, flux.c:993); 0 otherwise,
+   which leaves the sum unchanged */
+static void HT(const csu_oracle* o, double r, double u, double v, double p, double kt, double* H, double* T)
 {
     *T = p / (o->gas.R * r);
-    double U3 = gas_T2e(&o->gas, *T) * r + (u * u + v * v) * r / 2;
+    double U3 = gas_T2e(&o->gas, *T) * r + (u * u + v * v) * r / 2 + kt;
     *H = (U3 + p) / r;
 }
 
-/* L/R: r u v p [s] (s = passive scalar, used when ns == 1) */
-static void flux_ausm(const csu_oracle* o, const double* L, const double* Rt, int ns, double* f)  /* flux.c:132-203, 512-585 */
+/* L/R: r u v p [s0 [s1]]: ns passive scalars (SA nu~; SST k, omega) */
+static void flux_ausm(const csu_oracle* o, const double* L, const double* Rt, int ns, double* f)  /* flux.c:132-203, 512-585, 901-972 */
 {
     double HL, TL, HR, TR;
-    HT(o, L[0], L[1], L[2], L[3], &HL, &TL);
-    HT(o, Rt[0], Rt[1], Rt[2], Rt[3], &HR, &TR);
+    HT(o, L[0], L[1], L[2], L[3], ns == 2 ? L[4] * L[0] : 0.0, &HL, &TL);
+    HT(o, Rt[0], Rt[1], Rt[2], Rt[3], ns == 2 ? Rt[4] * Rt[0] : 0.0, &HR, &TR);
     double cL = gas_T2c(&o->gas, TL), cR = gas_T2c(&o->gas, TR);
     double ML = L[1] / cL, MR = Rt[1] / cR, Mp, Mm_, Pp, Pm;
     if (fabs(ML) >= 1) { Mp = 0.5 * (ML + fabs(ML)); Pp = Mp / ML; }
@@ -247,15 +263,15 @@ static void flux_ausm(const csu_oracle* o, const double* L, const double* Rt, in
     f[1] = f[0] * S[1] + pm;
     f[2] = f[0] * S[2];
     f[3] = f[0] * H;
-    if (ns) f[4] = f[0] * S[4];
+    for (int q = 0; q < ns; q++) f[4 + q] = f[0] * S[4 + q];
 }
 
-static void flux_ausmdv(const csu_oracle* o, const double* L, const double* Rt, int ns, double* f)  /* flux.c:206-287, 588-670 */
+static void flux_ausmdv(const csu_oracle* o, const double* L, const double* Rt, int ns, double* f)  /* flux.c:206-287, 588-670, 975-1058 */
 {
     double rL = L[0], uL = L[1], vL = L[2], pL = L[3], rR = Rt[0], uR = Rt[1], vR = Rt[2], pR = Rt[3];
     double HL, TL, HR, TR, aux;
-    HT(o, rL, uL, vL, pL, &HL, &TL);
-    HT(o, rR, uR, vR, pR, &HR, &TR);
+    HT(o, rL, uL, vL, pL, ns == 2 ? L[4] : 0.0, &HL, &TL);
+    HT(o, rR, uR, vR, pR, ns == 2 ? Rt[4] : 0.0, &HR, &TR);
     double cL = gas_T2c(&o->gas, TL), cR = gas_T2c(&o->gas, TR);
     double cm = fmax(cL, cR);
     double aL = (2.0 * pL / rL) / (pL / rL + pR / rR);
@@ -277,14 +293,15 @@ static void flux_ausmdv(const csu_oracle* o, const double* L, const double* Rt, 
     f[1] = pLp + pRm;
     f[2] = 0.5 * (rU * (vR + vL) - fabs(rU) * (vR - vL));
     f[3] = 0.5 * (rU * (HR + HL) - fabs(rU) * (HR - HL));
-    if (ns) f[4] = 0.5 * (rU * (Rt[4] + L[4]) - fabs(rU) * (Rt[4] - L[4]));
+    for (int q = 0; q < ns; q++) f[4 + q] = 0.5 * (rU * (Rt[4 + q] + L[4 + q]) - fabs(rU) * (Rt[4 + q] - L[4 + q]));
     double fD = 0.5 * (rU * (uR + uL) - fabs(rU) * (uR - uL));
     double fV = uLp * rL * uL + uRm * rR * uR;
     double s = 0.5 * fmin(1, 10 * fabs(pR - pL) / fmin(pL, pR));
     f[1] += (0.5 + s) * fV + (0.5 - s) * fD;
     int A = (uL - cL < 0.0) && (uR - cR > 0.0);
     int B = (uL + cL < 0.0) && (uR + cR > 0.0);
-    double psL[5] = {1.0, uL, vL, HL, ns ? L[4] : 0.0}, psR[5] = {1.0, uR, vR, HR, ns ? Rt[4] : 0.0};
+    double psL[6] = {1.0, uL, vL, HL, ns ? L[4] : 0.0, ns > 1 ? L[5] : 0.0};
+    double psR[6] = {1.0, uR, vR, HR, ns ? Rt[4] : 0.0, ns > 1 ? Rt[5] : 0.0};
     if (A && !B) {
         aux = 0.125 * ((uR - cR) - (uL - cL));
         for (int k = 0; k < n; k++) f[k] -= aux * (rR * psR[k] - rL * psL[k]);
@@ -296,14 +313,16 @@ static void flux_ausmdv(const csu_oracle* o, const double* L, const double* Rt, 
 
 static void flux_ausmpup(const csu_oracle* o, const double* L, const double* Rt, int ns, int v2, double* f)
 {
-    /* flux.c:290-398 (v2 == 0) and 401-508 (v2 == 1); _sa 673-899 */
+    /* flux.c:290-398 (v2 == 0) and 401-508 (v2 == 1); _sa 673-899; _sst 1061-1297
+       (the _sst forms take the critical speed of sound from H - k) */
     double rL = L[0], uL = L[1], vL = L[2], pL = L[3], rR = Rt[0], uR = Rt[1], vR = Rt[2], pR = Rt[3];
     double HL, TL, HR, TR;
-    HT(o, rL, uL, vL, pL, &HL, &TL);
-    HT(o, rR, uR, vR, pR, &HR, &TR);
-    double as = gas_critic_H2c(&o->gas, HL);
+    const double kL = ns == 2 ? L[4] : 0.0, kR = ns == 2 ? Rt[4] : 0.0;
+    HT(o, rL, uL, vL, pL, kL * rL, &HL, &TL);
+    HT(o, rR, uR, vR, pR, kR * rR, &HR, &TR);
+    double as = gas_critic_H2c(&o->gas, HL - kL);
     double ahL = as * as / fmax(as, fabs(uL));
-    as = gas_critic_H2c(&o->gas, HR);
+    as = gas_critic_H2c(&o->gas, HR - kR);
     double ahR = as * as / fmax(as, fabs(uR));
     double am = fmin(ahL, ahR);
     double Kp = 0.25, Ku = 0.75, sig = 1.0, beta = 1.0 / 8.0;
@@ -337,12 +356,12 @@ static void flux_ausmpup(const csu_oracle* o, const double* L, const double* Rt,
     const double* S = mm > 0 ? L : Rt;
     double H = mm > 0 ? HL : HR;
     f[0] = mm; f[1] = mm * S[1] + pm; f[2] = mm * S[2]; f[3] = mm * H;
-    if (ns) f[4] = mm * S[4];
+    for (int q = 0; q < ns; q++) f[4 + q] = mm * S[4 + q];
 }
 
-static void flux(const csu_oracle* o, const double* L, const double* Rt, double* f)  /* flux.c:1300-1345 */
+static void flux(const csu_oracle* o, const double* L, const double* Rt, double* f)  /* flux.c:1300-1366 */
 {
-    int ns = o->nv == 5 ? 1 : 0;
+    int ns = o->nv - 4;
     switch (o->prm.flux) {
         case 0: flux_roe(o, L, Rt, f); break;
         case 1: flux_ausm(o, L, Rt, ns, f); break;
@@ -465,7 +484,7 @@ static void calc_primitive(csu_oracle* o, double* U)
             bc_outlet(o, PL, Q, dSx / dS, dSy / dS);
             for (int k = 0; k < 4; k++) B[k] = Q[k];
             B[4] = B[3] / (B[0] * o->gas.R);
-        } else if (o->prm.laminar == 1 || o->prm.sa == 1) {
+        } else if (o->prm.laminar == 1 || o->prm.sa == 1) {   /* SST walls take the inviscid branch */
             if (t == CSU_BC_WALLT) {
                 B[0] = PL[3] / (o->gas.R * o->prm.twall); B[1] = 0.0; B[2] = 0.0; B[3] = PL[3]; B[4] = o->prm.twall;
             } else {
@@ -480,6 +499,15 @@ static void calc_primitive(csu_oracle* o, double* U)
             if (t == CSU_BC_SYMMETRY || t == CSU_BC_OUTLET) B[5] = E[5];
             else if (t == CSU_BC_INLET) B[5] = o->prm.pin[5];
             else B[5] = 0.0;
+        }
+        if (o->prm.sst == 1) {        /* boundary.c:660-690 */
+            if (t == CSU_BC_SYMMETRY || t == CSU_BC_OUTLET) { B[5] = E[5]; B[6] = E[6]; }
+            else if (t == CSU_BC_INLET) { B[5] = o->prm.pin[5]; B[6] = o->prm.pin[6]; }
+            else {
+                double n = sutherland(B[4]) / B[0], d = o->wd[e0];
+                B[5] = 1e-14;
+                B[6] = o->prm.sst_owallfactor * 6 * n / (SST_B1 * d * d);
+            }
         }
     }
 }
@@ -654,24 +682,26 @@ static void boundary(csu_oracle* o)
     for (int g = 0; g < o->nb; g++) {
         const int e0 = o->bface[3 * g];
         const double* E = o->P + (size_t)e0 * ns;
-        double dSx, dSy, dS, L[5] = {E[0], E[1], E[2], E[3], nv == 5 ? E[5] : 0.0}, B[5], M[5], fl[5];
+        double dSx, dSy, dS, L[6] = {E[0], E[1], E[2], E[3], nv > 4 ? E[5] : 0.0, nv > 5 ? E[6] : 0.0};
+        double B[6], M[6], fl[6] = {0};
         face_ds(o, o->bface[3 * g + 1], o->bface[3 * g + 2], &dSx, &dSy);
         dS = sqrt(dSx * dSx + dSy * dSy);
         int t = o->btype[g];
         if (t == CSU_BC_SYMMETRY) {
             rotation(L, dSx, dSy, dS);
-            M[0] = L[0]; M[1] = -L[1]; M[2] = L[2]; M[3] = L[3]; M[4] = L[4];
+            M[0] = L[0]; M[1] = -L[1]; M[2] = L[2]; M[3] = L[3]; M[4] = L[4]; M[5] = L[5];
             flux(o, L, M, fl);
         } else if (t == CSU_BC_INLET || t == CSU_BC_OUTLET) {
             if (t == CSU_BC_INLET) bc_inlet(o, o->prm.pin, L, B, dSx / dS, dSy / dS);
             else bc_outlet(o, L, B, dSx / dS, dSy / dS);
             B[4] = t == CSU_BC_INLET ? o->prm.pin[5] : L[4];
+            B[5] = t == CSU_BC_INLET ? o->prm.pin[6] : L[5];
             rotation(L, dSx, dSy, dS);
             rotation(B, dSx, dSy, dS);
             flux(o, L, B, fl);
-        } else if (o->prm.laminar == 1 || nv == 5) {
+        } else if (o->prm.laminar == 1 || nv > 4) {
             rotation(L, dSx, dSy, dS);
-            M[0] = L[0]; M[1] = -L[1]; M[2] = -L[2]; M[3] = L[3]; M[4] = L[4];
+            M[0] = L[0]; M[1] = -L[1]; M[2] = -L[2]; M[3] = L[3]; M[4] = L[4]; M[5] = L[5];
             flux(o, L, M, fl);
         } else {
             bc_wall(o, L, B, dSx / dS, dSy / dS);
@@ -1059,6 +1089,297 @@ static void sa_boundary(csu_oracle* o)       /* saBoundary / saBoundaryFace, sa.
     }
 }
 
+/* ---------------- Menter SST: sst.c ---------------- */
+typedef struct {
+    double dux, duy, dvx, dvy, dkx, dky, dox, doy;
+    double r, T, k, om, d, mi_L, mi_t;
+    double tkx, tky, tox, toy, F1, F2, Qtk, Qto;
+} sstv;
+
+static double sst_blend(double x1, double x2, double F1) { return x1 * F1 + x2 * (1 - F1); }  /* sst.c:693 */
+
+/* sstF1 (sst.c:788-809).  Its long double temporaries only ever hold
+   values computed in double, so double restates them exactly. */
+static double sst_F1(const sstv* v, double n_L_term, double sqrtk_term)
+{
+    double aux1 = 2 * v->r * SST_SO2 * (v->dkx * v->dox + v->dky * v->doy) / v->om;
+    if (v->om == 0) aux1 = 0;
+    double CD = fmax(aux1, 1.0e-20);
+    aux1 = fmax(sqrtk_term, n_L_term);
+    double aux2 = 4 * v->r * SST_SO2 * v->k / (CD * v->d * v->d);
+    double arg1 = fmin(aux1, aux2);
+    return tanh(arg1 * arg1 * arg1 * arg1);
+}
+
+static double sst_F2(double n_L_term, double sqrtk_term)      /* sst.c:811-818 */
+{
+    double arg2 = fmax(2 * sqrtk_term, n_L_term);
+    return tanh(arg2 * arg2);
+}
+
+/* sstFlux (sst.c:749-786): eddy viscosity and the k, omega diffusion flux densities */
+static void sst_flux(sstv* v)
+{
+    double n_L = v->mi_L / v->r;
+    double omega = fabs(v->duy - v->dvx);
+    double sqrtk_term = sqrt(v->k) / (SST_BS * v->om * v->d);
+    double n_L_term = 500 * n_L / (v->d * v->d * v->om);
+    double F1 = sst_F1(v, n_L_term, sqrtk_term);
+    double F2 = sst_F2(n_L_term, sqrtk_term);
+    v->mi_t = v->r * SST_A1 * v->k / fmax(SST_A1 * v->om, omega * F2);
+    double sk = sst_blend(SST_SK1, SST_SK2, F1), so = sst_blend(SST_SO1, SST_SO2, F1);
+    double aux = v->mi_L + sk * v->mi_t;
+    v->tkx = aux * v->dkx;
+    v->tky = aux * v->dky;
+    aux = v->mi_L + so * v->mi_t;
+    v->tox = aux * v->dox;
+    v->toy = aux * v->doy;
+}
+
+/* sstSources (sst.c:698-747) */
+static void sst_sources(const csu_oracle* o, sstv* v)
+{
+    double n_L = v->mi_L / v->r;
+    double omega = fabs(v->duy - v->dvx);
+    double sqrtk_term = sqrt(v->k) / (SST_BS * v->om * v->d);
+    double n_L_term = 500 * n_L / (v->d * v->d * v->om);
+    double F1 = sst_F1(v, n_L_term, sqrtk_term);
+    double F2 = sst_F2(n_L_term, sqrtk_term);
+    v->mi_t = v->r * SST_A1 * v->k / fmax(SST_A1 * v->om, omega * F2);
+    double n_t = v->mi_t / v->r;
+    double g = sst_blend(o->sst_g1, o->sst_g2, F1), b = sst_blend(SST_B1, SST_B2, F1);
+    double txx = 2 * v->mi_t * (v->dux - (v->dux + v->dvy) / 3) - 2. * v->r * v->k / 3;
+    double tyy = 2 * v->mi_t * (v->dvy - (v->dux + v->dvy) / 3) - 2. * v->r * v->k / 3;
+    double txy = v->mi_t * (v->duy + v->dvx);
+    double P = v->dux * txx + v->dvy * tyy + (v->duy + v->dvx) * txy;
+    P = fmin(P, 10 * SST_BS * v->r * v->om * v->k);
+    v->F1 = F1;
+    v->F2 = F2;
+    double cd = 2 * (1 - F1) * v->r * SST_SO2 * (v->dkx * v->dox + v->dky * v->doy) / v->om;
+    v->Qtk = P - SST_BS * v->r * v->om * v->k;
+    v->Qto = g * P / n_t - b * v->r * v->om * v->om + cd;
+}
+
+/* stresses with the -2/3 rho k term (sst.c:163-165) */
+static void sst_tau(double mi, const sstv* v, double* txx, double* tyy, double* txy)
+{
+    *txx = 2 * mi * (v->dux - (v->dux + v->dvy) / 3) - 2. * v->r * v->k / 3.;
+    *tyy = 2 * mi * (v->dvy - (v->dux + v->dvy) / 3) - 2. * v->r * v->k / 3.;
+    *txy = mi * (v->duy + v->dvx);
+}
+
+/* sstInterFace (sst.c:56-194) and sstInterSource (sst.c:595-691) */
+static void sst_inter(csu_oracle* o)
+{
+    const int nc = o->nc, nf = o->nf, ns = o->nv + 1;
+    const double *gx = o->dPx, *gy = o->dPy;
+#pragma omp parallel for
+    for (int f = 0; f < nf; f++) {
+        const int* q = o->face + 4 * f;
+        int e0 = q[0], e1 = q[1];
+        const double *P0 = o->P + (size_t)e0 * ns, *P1 = o->P + (size_t)e1 * ns;
+        double dSx, dSy, x0, y0, x1, y1;
+        face_ds(o, q[2], q[3], &dSx, &dSy);
+        cell_center(o, e0, &x0, &y0);
+        cell_center(o, e1, &x1, &y1);
+        double dx = x1 - x0, dy = y1 - y0, L = sqrt(dx * dx + dy * dy);
+        double nx = dx / L, ny = dy / L;
+        double dul = (P1[1] - P0[1]) / L, dvl = (P1[2] - P0[2]) / L, dTl = (P1[4] - P0[4]) / L;
+        double dkl = (P1[5] - P0[5]) / L, dol = (P1[6] - P0[6]) / L;
+        double duxm = 0.5 * (gx[nc + e0] + gx[nc + e1]), dvxm = 0.5 * (gx[2 * nc + e0] + gx[2 * nc + e1]);
+        double dTxm = 0.5 * (gx[3 * nc + e0] + gx[3 * nc + e1]), dkxm = 0.5 * (gx[4 * nc + e0] + gx[4 * nc + e1]);
+        double doxm = 0.5 * (gx[5 * nc + e0] + gx[5 * nc + e1]);
+        double duym = 0.5 * (gy[nc + e0] + gy[nc + e1]), dvym = 0.5 * (gy[2 * nc + e0] + gy[2 * nc + e1]);
+        double dTym = 0.5 * (gy[3 * nc + e0] + gy[3 * nc + e1]), dkym = 0.5 * (gy[4 * nc + e0] + gy[4 * nc + e1]);
+        double doym = 0.5 * (gy[5 * nc + e0] + gy[5 * nc + e1]);
+        sstv v;
+        double aux;
+        aux = duxm * nx + duym * ny; v.dux = duxm + (dul - aux) * nx; v.duy = duym + (dul - aux) * ny;
+        aux = dvxm * nx + dvym * ny; v.dvx = dvxm + (dvl - aux) * nx; v.dvy = dvym + (dvl - aux) * ny;
+        aux = dTxm * nx + dTym * ny;
+        double dTx = dTxm + (dTl - aux) * nx, dTy = dTym + (dTl - aux) * ny;
+        aux = dkxm * nx + dkym * ny; v.dkx = dkxm + (dkl - aux) * nx; v.dky = dkym + (dkl - aux) * ny;
+        aux = doxm * nx + doym * ny; v.dox = doxm + (dol - aux) * nx; v.doy = doym + (dol - aux) * ny;
+        v.r = (P1[0] + P0[0]) * 0.5;
+        double u = (P1[1] + P0[1]) * 0.5, w = (P1[2] + P0[2]) * 0.5;
+        v.T = (P1[4] + P0[4]) * 0.5;
+        v.k = (P1[5] + P0[5]) * 0.5;
+        v.om = (P1[6] + P0[6]) * 0.5;
+        v.d = (o->wd[e1] + o->wd[e0]) * 0.5;
+        v.mi_L = sutherland(v.T);
+        sst_flux(&v);
+        double mi = v.mi_L + v.mi_t;
+        double kk = gas_T2Cp(&o->gas, v.T) * (v.mi_L / o->prm.pr + v.mi_t / o->prm.pr_t);
+        o->miT[f] = v.mi_t;
+        double txx, tyy, txy;
+        sst_tau(mi, &v, &txx, &tyy, &txy);
+        o->flux[(size_t)1 * nf + f] = txx * dSx + txy * dSy;
+        o->flux[(size_t)2 * nf + f] = txy * dSx + tyy * dSy;
+        o->flux[(size_t)3 * nf + f] = (txx * dSx + txy * dSy) * u + (txy * dSx + tyy * dSy) * w + kk * (dTx * dSx + dTy * dSy);
+        o->flux[(size_t)4 * nf + f] = v.tkx * dSx + v.tky * dSy;
+        o->flux[(size_t)5 * nf + f] = v.tox * dSx + v.toy * dSy;
+    }
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < 4; j++) {
+            int f = o->cface[4 * i + j];
+            if (f > 0) for (int k = 1; k < 6; k++) o->R[(size_t)k * nc + i] -= o->flux[(size_t)k * nf + f - 1];
+            else if (f < 0) for (int k = 1; k < 6; k++) o->R[(size_t)k * nc + i] += o->flux[(size_t)k * nf - f - 1];
+        }
+#pragma omp parallel for
+    for (int i = 0; i < nc; i++) {
+        const double* P = o->P + (size_t)i * ns;
+        const double om_i = o->omega[i];
+        sstv v;
+        v.dux = gx[nc + i]; v.dvx = gx[2 * nc + i]; v.dkx = gx[4 * nc + i]; v.dox = gx[5 * nc + i];
+        v.duy = gy[nc + i]; v.dvy = gy[2 * nc + i]; v.dky = gy[4 * nc + i]; v.doy = gy[5 * nc + i];
+        v.r = P[0]; v.T = P[4]; v.k = P[5]; v.om = P[6]; v.d = o->wd[i];
+        v.mi_L = sutherland(v.T);
+        if (v.om == 0) {             /* omLim, solver.c:1155 */
+            o->dQkdr[i] = 0; o->dQkdrk[i] = 0; o->dQkdro[i] = 0;
+            o->dQodr[i] = 0; o->dQodrk[i] = 0; o->dQodro[i] = 0;
+            continue;
+        }
+        sst_sources(o, &v);
+        o->R[(size_t)4 * nc + i] -= v.Qtk * om_i;
+        o->R[(size_t)5 * nc + i] -= v.Qto * om_i;
+        double dQkdr = v.Qtk, dQkdo = v.Qtk, dQkdk = v.Qtk;
+        double dQodr = v.Qto, dQodo = v.Qto, dQodk = v.Qto;
+        const double h = 1e-6, dh = 1e-6;
+        double hh;
+        hh = fabs(v.om * h); hh = fmax(hh, dh);
+        v.om += hh;
+        sst_sources(o, &v);
+        dQkdo = (v.Qtk - dQkdo) / hh; dQodo = (v.Qto - dQodo) / hh;
+        v.om = P[6];
+        hh = fabs(v.k * h); hh = fmax(hh, dh);
+        v.k += hh;
+        sst_sources(o, &v);
+        dQkdk = (v.Qtk - dQkdk) / hh; dQodk = (v.Qto - dQodk) / hh;
+        v.k = P[5];
+        hh = fabs(v.r * h); hh = fmax(hh, dh);
+        v.r += hh;
+        sst_sources(o, &v);
+        dQkdr = (v.Qtk - dQkdr) / hh; dQodr = (v.Qto - dQodr) / hh;
+        v.r = P[0];
+        o->dQkdr[i] = (dQkdr - dQkdk * v.k / v.r - dQkdo * v.om / v.r) * om_i;
+        o->dQkdrk[i] = dQkdk * om_i / v.r;
+        o->dQkdro[i] = dQkdo * om_i / v.r;
+        o->dQodr[i] = (dQodr - dQodk * v.k / v.r - dQodo * v.om / v.r) * om_i;
+        o->dQodrk[i] = dQodk * om_i / v.r;
+        o->dQodro[i] = dQodo * om_i / v.r;
+    }
+}
+
+/* sstBoundaryFaceViscFlux (sst.c:202-547): f[1..5] */
+static void sst_bc_flux(const csu_oracle* o, int g, double* f)
+{
+    const int nc = o->nc, ns = o->nv + 1, e0 = o->bface[3 * g], t = o->btype[g];
+    const int p0 = o->bface[3 * g + 1], p1 = o->bface[3 * g + 2];
+    const double* P = o->P + (size_t)e0 * ns;
+    const double *gx = o->dPx, *gy = o->dPy;
+    const double* Pin = o->prm.pin;
+    sstv v;
+    double dTx = 0, dTy = 0, txx, tyy, txy;
+    if (t == CSU_BC_SYMMETRY) {
+        double x0 = o->xy[2 * p0], y0 = o->xy[2 * p0 + 1], x1 = o->xy[2 * p1], y1 = o->xy[2 * p1 + 1];
+        double sx = y1 - y0, sy = -(x1 - x0), dS = sqrt(sx * sx + sy * sy);
+        double nx = sx / dS, ny = sy / dS;
+        if (o->axi == 1) dS = dS * (y0 + y1) * 0.5;
+        double duxm = gx[nc + e0], dvxm = gx[2 * nc + e0], dkxm = gx[4 * nc + e0], doxm = gx[5 * nc + e0];
+        double duym = gy[nc + e0], dvym = gy[2 * nc + e0], dkym = gy[4 * nc + e0], doym = gy[5 * nc + e0];
+        v.dux = duxm - (duxm * nx + duym * ny) * nx; v.duy = duym - (duxm * nx + duym * ny) * ny;
+        v.dvx = dvxm - (dvxm * nx + dvym * ny) * nx; v.dvy = dvym - (dvxm * nx + dvym * ny) * ny;
+        v.dkx = dkxm - (dkxm * nx + dkym * ny) * nx; v.dky = dkym - (dkxm * nx + dkym * ny) * ny;
+        v.dox = doxm - (doxm * nx + doym * ny) * nx; v.doy = doym - (doxm * nx + doym * ny) * ny;
+        v.r = P[0]; v.T = P[4]; v.k = P[5]; v.om = P[6]; v.d = o->wd[e0];
+        double u = P[1], w = P[2];
+        v.mi_L = sutherland(v.T);
+        sst_flux(&v);
+        sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
+        f[1] = (txx * nx + txy * ny) * dS;
+        f[2] = (txy * nx + tyy * ny) * dS;
+        f[3] = (u * (txx * nx + txy * ny) + w * (txy * nx + tyy * ny)) * dS;   /* no heat flux */
+        f[4] = 0.0;
+        f[5] = 0.0;
+        return;
+    }
+    double dSx, dSy;
+    face_ds(o, p0, p1, &dSx, &dSy);
+    if (t == CSU_BC_INLET || t == CSU_BC_WALL || t == CSU_BC_WALLT) {
+        double x0, y0;
+        cell_center(o, e0, &x0, &y0);
+        double x1 = (o->xy[2 * p0] + o->xy[2 * p1]) * 0.5, y1 = (o->xy[2 * p0 + 1] + o->xy[2 * p1 + 1]) * 0.5;
+        double dx = x1 - x0, dy = y1 - y0, L = sqrt(dx * dx + dy * dy);
+        double ub, vb, Tb = 0, kb, ob;
+        v.d = o->wd[e0];
+        if (t == CSU_BC_INLET) {
+            v.r = Pin[0]; v.T = Pin[4]; v.k = Pin[5]; v.om = Pin[6];
+            ub = Pin[1]; vb = Pin[2]; Tb = Pin[4]; kb = Pin[5]; ob = Pin[6];
+            v.mi_L = sutherland(v.T);
+        } else {
+            v.r = P[0]; v.T = P[4]; v.k = P[5]; v.om = P[6];
+            v.mi_L = sutherland(v.T);
+            double n_L = v.mi_L / v.r;
+            double owall = o->prm.sst_owallfactor * 6 * n_L / (SST_B1 * v.d * v.d);
+            ub = 0; vb = 0; kb = 0; ob = owall;
+            if (t == CSU_BC_WALLT) Tb = o->prm.twall;
+        }
+        double dul = (ub - P[1]) / L, dvl = (vb - P[2]) / L, dkl = (kb - P[5]) / L, dol = (ob - P[6]) / L;
+        double duxm = gx[nc + e0], dvxm = gx[2 * nc + e0], dkxm = gx[4 * nc + e0], doxm = gx[5 * nc + e0];
+        double duym = gy[nc + e0], dvym = gy[2 * nc + e0], dkym = gy[4 * nc + e0], doym = gy[5 * nc + e0];
+        v.dux = duxm + (dul - (duxm * dx + duym * dy) / L) * dx / L;
+        v.duy = duym + (dul - (duxm * dx + duym * dy) / L) * dy / L;
+        v.dvx = dvxm + (dvl - (dvxm * dx + dvym * dy) / L) * dx / L;
+        v.dvy = dvym + (dvl - (dvxm * dx + dvym * dy) / L) * dy / L;
+        if (t != CSU_BC_WALL) {
+            double dTl = (Tb - P[4]) / L, dTxm = gx[3 * nc + e0], dTym = gy[3 * nc + e0];
+            dTx = dTxm + (dTl - (dTxm * dx + dTym * dy) / L) * dx / L;
+            dTy = dTym + (dTl - (dTxm * dx + dTym * dy) / L) * dy / L;
+        }
+        v.dkx = dkxm + (dkl - (dkxm * dx + dkym * dy) / L) * dx / L;
+        v.dky = dkym + (dkl - (dkxm * dx + dkym * dy) / L) * dy / L;
+        v.dox = doxm + (dol - (doxm * dx + doym * dy) / L) * dx / L;
+        v.doy = doym + (dol - (doxm * dx + doym * dy) / L) * dy / L;
+        sst_flux(&v);
+        sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
+        f[1] = txx * dSx + txy * dSy;
+        f[2] = txy * dSx + tyy * dSy;
+        f[4] = v.tkx * dSx + v.tky * dSy;
+        f[5] = v.tox * dSx + v.toy * dSy;
+        if (t == CSU_BC_WALL) { f[3] = 0.0; return; }
+        double kk = gas_T2Cp(&o->gas, v.T) * (v.mi_L / o->prm.pr + v.mi_t / o->prm.pr_t);
+        if (t == CSU_BC_WALLT) { f[3] = kk * (dTx * dSx + dTy * dSy); return; }
+        f[3] = ub * (txx * dSx + txy * dSy) + vb * (txy * dSx + tyy * dSy) + kk * (dTx * dSx + dTy * dSy);
+        return;
+    }
+    /* outlet */
+    v.dux = gx[nc + e0]; v.dvx = gx[2 * nc + e0]; dTx = gx[3 * nc + e0]; v.dkx = gx[4 * nc + e0]; v.dox = gx[5 * nc + e0];
+    v.duy = gy[nc + e0]; v.dvy = gy[2 * nc + e0]; dTy = gy[3 * nc + e0]; v.dky = gy[4 * nc + e0]; v.doy = gy[5 * nc + e0];
+    v.r = P[0]; v.T = P[4]; v.k = P[5]; v.om = P[6]; v.d = o->wd[e0];
+    double u = P[1], w = P[2];
+    v.mi_L = sutherland(v.T);
+    sst_flux(&v);
+    double kk = gas_T2Cp(&o->gas, v.T) * (v.mi_L / o->prm.pr + v.mi_t / o->prm.pr_t);
+    sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
+    f[1] = txx * dSx + txy * dSy;
+    f[2] = txy * dSx + tyy * dSy;
+    f[3] = u * (txx * dSx + txy * dSy) + w * (txy * dSx + tyy * dSy) + kk * (dTx * dSx + dTy * dSy);
+    f[4] = v.tkx * dSx + v.tky * dSy;
+    f[5] = v.tox * dSx + v.toy * dSy;
+}
+
+static void sst_boundary(csu_oracle* o)      /* sstBoundary / sstBoundaryFace, sst.c:549-575 */
+{
+    const int nc = o->nc;
+    for (int g = 0; g < o->nb; g++) {
+        double f[6];
+        sst_bc_flux(o, g, f);
+        int e0 = o->bface[3 * g];
+        for (int k = 1; k < 6; k++) o->R[(size_t)k * nc + e0] -= f[k];
+    }
+}
+
 /* solverCalcR, solver.c:685-739 */
 static void calc_R(csu_oracle* o, double* U)
 {
@@ -1077,6 +1398,11 @@ static void calc_R(csu_oracle* o, double* U)
         grad_T(o);
         sa_inter(o);
         sa_boundary(o);
+    }
+    if (o->prm.sst == 1) {
+        grad_T(o);
+        sst_inter(o);
+        sst_boundary(o);
     }
 }
 
@@ -1232,6 +1558,17 @@ static void delta_flux(const csu_oracle* o, const double* Pj, const double* d, d
 }
 
 /* implicitAuxCalcFlux_sa2 / implicitCalcDeltaFlux_sa, implicit.c:484-553 */
+static void aux_flux_sst(const double* U, double p, double nx, double ny, double* F)   /* implicit.c:1036-1046 */
+{
+    double V = (nx * U[1] + ny * U[2]) / U[0];
+    F[0] = U[0] * V;
+    F[1] = U[1] * V + nx * p;
+    F[2] = U[2] * V + ny * p;
+    F[3] = (U[3] + p) * V;
+    F[4] = U[4] * V;
+    F[5] = U[5] * V;
+}
+
 static void aux_flux_sa(const double* U, double p, double nx, double ny, double* F)
 {
     double V = (nx * U[1] + ny * U[2]) / U[0];
@@ -1272,6 +1609,39 @@ static void delta_flux_sa(const csu_oracle* o, const double* Pj, const double* d
     for (int k = 0; k < 5; k++) dF[k] = F1[k] - F0[k];
 }
 
+/* implicitCalcDeltaFlux_sst (implicit.c:1048-1117), with its clamps */
+static void delta_flux_sst(const csu_oracle* o, const double* Pj, const double* d, double nx, double ny, double* dF)
+{
+    double rho = Pj[0], u = Pj[1], v = Pj[2], p = Pj[3], k = Pj[5], om = Pj[6], F0[6], F1[6], U[6];
+    double T = p / (o->gas.R * rho);
+    double e0 = gas_T2e(&o->gas, T);
+    double E = e0 + (u * u + v * v) * 0.5 + k;
+    U[0] = rho; U[1] = rho * u; U[2] = rho * v; U[3] = rho * E; U[4] = rho * k; U[5] = rho * om;
+    aux_flux_sst(U, p, nx, ny, F0);
+    for (int q = 0; q < 6; q++) U[q] = U[q] + d[q];
+    rho = U[0];
+    rho = fmax(rho, o->prm.rlim);
+    U[0] = rho;
+    u = U[1] / rho; v = U[2] / rho;
+    k = U[4] / rho;
+    if (k < 1e-14) { k = 1e-14; U[4] = rho * k; }
+    om = U[5] / rho;
+    if (om < 0) { om = 0; U[5] = rho * om; }
+    E = U[3] / rho;
+    double e1 = E - (u * u + v * v) * 0.5 - k;
+    e1 = fmax(e1, 7.176325e+02);
+    T = T + (e1 - e0) / gas_T2Cv(&o->gas, T);        /* gasprop_e2Taprox, gasprop.c:156-159 */
+    if (T < 1) {
+        T = 1;
+        e1 = gas_T2e(&o->gas, T);
+        E = e1 + (u * u + v * v) * 0.5 + k;
+        U[3] = rho * E;
+    }
+    p = o->gas.R * T * rho;
+    aux_flux_sst(U, p, nx, ny, F1);
+    for (int q = 0; q < 6; q++) dF[q] = F1[q] - F0[q];
+}
+
 /* face j of cell i seen from i: neighbour, outward normal (implicit.c:288-305) */
 static int face_from_id(const csu_oracle* o, int i, int slot, double* dSx, double* dSy, int* fid)
 {
@@ -1296,12 +1666,14 @@ static void implicit_func(const csu_oracle* o, int i, int j, int fid, double dSx
     const int nc = o->nc, ns = o->nv + 1;
     double dS = sqrt(dSx * dSx + dSy * dSy), nx = dSx / dS, ny = dSy / dS;
     const double* Pj = o->P + (size_t)j * ns;
-    double d[5] = {dW[j], dW[nc + j], dW[2 * nc + j], dW[3 * nc + j], o->nv == 5 ? dW[4 * nc + j] : 0.0}, dF[5];
-    if (o->nv == 5) delta_flux_sa(o, Pj, d, nx, ny, dF);
+    double d[6] = {dW[j], dW[nc + j], dW[2 * nc + j], dW[3 * nc + j], o->nv > 4 ? dW[4 * nc + j] : 0.0,
+                   o->nv > 5 ? dW[5 * nc + j] : 0.0}, dF[6];
+    if (o->nv == 6) delta_flux_sst(o, Pj, d, nx, ny, dF);
+    else if (o->nv == 5) delta_flux_sa(o, Pj, d, nx, ny, dF);
     else delta_flux(o, Pj, d, nx, ny, dF);
     double c = gas_T2c(&o->gas, Pj[4]);
     double ra = o->prm.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
-    if (o->prm.sa) {        /* implicitFunc_sa, implicit.c:555-595 */
+    if (o->prm.sa || o->prm.sst) {   /* implicitFunc_sa / _sst, implicit.c:555-595, 1120-1161 */
         double r = Pj[0], T = Pj[4], mi = sutherland(T), x0, y0, x1, y1;
         cell_center(o, i, &x0, &y0);
         cell_center(o, j, &x1, &y1);
@@ -1318,6 +1690,31 @@ static void implicit_func(const csu_oracle* o, int i, int j, int fid, double dSx
 }
 
 /* implicitLUSGS_L / _U, implicit.c:274-482: serial sweeps in cell order */
+/* diagonal solve of cell i: SST couples k and omega through the source
+   Jacobians, a 2x2 block solved by Cramer's rule with a long double
+   determinant (implicit.c:326-351, 447-472) */
+static void lusgs_diag(const csu_oracle* o, int i, double* dW)
+{
+    const int nc = o->nc;
+    const double D = o->D[i];
+    if (o->nv != 6) {
+        for (int k = 0; k < o->nv; k++) dW[(size_t)k * nc + i] /= D;
+        return;
+    }
+    for (int k = 0; k < 4; k++) dW[(size_t)k * nc + i] /= D;
+    double A = dW[4 * (size_t)nc + i] + o->dQkdr[i] * dW[i];
+    double B = dW[5 * (size_t)nc + i] + o->dQodr[i] * dW[i];
+    double a = D - o->dQkdrk[i], b = -o->dQkdro[i], c = -o->dQodrk[i], d = D - o->dQodro[i];
+    long double det = a * d - b * c;
+    if (fabs(det) < 1e-14) {
+        dW[4 * (size_t)nc + i] /= D;
+        dW[5 * (size_t)nc + i] /= D;
+    } else {
+        dW[4 * (size_t)nc + i] = (A * d - B * b) / det;
+        dW[5 * (size_t)nc + i] = (-A * c + B * a) / det;
+    }
+}
+
 static void lusgs(csu_oracle* o)
 {
     const int nc = o->nc, nv = o->nv;
@@ -1328,17 +1725,27 @@ static void lusgs(csu_oracle* o)
             int fid, j = face_from_id(o, i, s, &dSx, &dSy, &fid);
             if (j >= 0 && j < i) implicit_func(o, i, j, fid, dSx, dSy, o->dW0);
         }
-        for (int k = 0; k < nv; k++) o->dW0[(size_t)k * nc + i] /= o->D[i];
+        lusgs_diag(o, i, o->dW0);
     }
-    for (int i = 0; i < nc; i++)
+    for (int i = 0; i < nc; i++) {
         for (int k = 0; k < nv; k++) o->dW1[(size_t)k * nc + i] = o->D[i] * o->dW0[(size_t)k * nc + i];
+        if (nv == 6) {       /* implicit.c:372-392 */
+            const double D = o->D[i], *x = o->dW0;
+            double a = D - o->dQkdrk[i], b = -o->dQkdro[i], c = -o->dQodrk[i], d = D - o->dQodro[i];
+            long double det = a * d - b * c;
+            if (!(fabs(det) < 1e-14)) {
+                o->dW1[4 * (size_t)nc + i] = a * x[4 * (size_t)nc + i] + b * x[5 * (size_t)nc + i] - o->dQkdr[i] * x[i];
+                o->dW1[5 * (size_t)nc + i] = c * x[4 * (size_t)nc + i] + d * x[5 * (size_t)nc + i] - o->dQodr[i] * x[i];
+            }
+        }
+    }
     for (int i = nc - 1; i >= 0; i--) {
         for (int s = 0; s < 4; s++) {
             double dSx, dSy;
             int fid, j = face_from_id(o, i, s, &dSx, &dSy, &fid);
             if (j > i) implicit_func(o, i, j, fid, dSx, dSy, o->dW1);
         }
-        for (int k = 0; k < nv; k++) o->dW1[(size_t)k * nc + i] /= o->D[i];
+        lusgs_diag(o, i, o->dW1);
     }
 }
 
@@ -1486,8 +1893,12 @@ static void* dup(const void* p, size_t n) { void* q = malloc(n ? n : 1); if (n) 
 
 int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
 {
-    if (p->sst) return CSU_ERR_UNSUPPORTED;
-    if (p->sa && (!m->wall_dist || p->flux == CSU_FLUX_ROE)) return CSU_ERR_UNSUPPORTED;
+    if ((p->sa || p->sst) && (!m->wall_dist || p->flux == CSU_FLUX_ROE)) return CSU_ERR_UNSUPPORTED;
+    /* SST: no transition model, no DPLUR (implicit.c:993-1000 has 4x4 and
+       5x5 blocks only), no global-range limiter (limiter.c:37-38 sizes its
+       range arrays for 5 variables) */
+    if (p->sst && (p->sst_trans || p->time_scheme == CSU_TIME_DPLUR || (p->order == 2 && p->limiter == 1)))
+        return CSU_ERR_UNSUPPORTED;
     csu_oracle* o = calloc(1, sizeof(*o));
     o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;
     o->prm = *p; o->dt_local = p->dt_local;
@@ -1515,6 +1926,14 @@ int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
     o->miT = calloc((size_t)(o->nf ? o->nf : 1), sizeof(double));
     o->wd = calloc(nc, sizeof(double));
     if (m->wall_dist) memcpy(o->wd, m->wall_dist, sizeof(double) * nc);
+    double* sq[6];
+    for (int q = 0; q < 6; q++) sq[q] = calloc(nc, sizeof(double));
+    o->dQkdr = sq[0]; o->dQkdrk = sq[1]; o->dQkdro = sq[2]; o->dQodr = sq[3]; o->dQodrk = sq[4]; o->dQodro = sq[5];
+    {
+        const double kap = 0.41;
+        o->sst_g1 = SST_B1 / SST_BS - SST_SO1 * kap * kap / sqrt(SST_BS);
+        o->sst_g2 = SST_B2 / SST_BS - SST_SO2 * kap * kap / sqrt(SST_BS);
+    }
     o->dW0 = calloc(nvc, sizeof(double));
     o->dW1 = calloc(nvc, sizeof(double));      /* implicitInitDPLUR zeroes dW1 */
     if (p->time_scheme == CSU_TIME_DPLUR) o->A = calloc((size_t)nc * 4 * nv * nv, sizeof(double));
@@ -1531,6 +1950,7 @@ void oracle_destroy(csu_oracle* o)
     free(o->omega); free(o->U); free(o->Uaux); free(o->R); free(o->dPx); free(o->dPy); free(o->phi);
     free(o->flux); free(o->dtL); free(o->P); free(o->Pb);
     free(o->D); free(o->dW0); free(o->dW1); free(o->A); free(o->miT); free(o->wd);
+    free(o->dQkdr); free(o->dQkdrk); free(o->dQkdro); free(o->dQodr); free(o->dQodrk); free(o->dQodro);
     free(o);
 }
 

@@ -13,7 +13,8 @@ RAMP = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
 PLATE = ["plate_lam", "plate_lamT_ausmdv", "plate_lam_axi_roe", "plate_euler_axi",
          "plate_lam_pup_lim1", "plate_lusgs_lam", "plate_lusgs_lamT", "plate_dplur_lam"]
 SA = ["plate_sa_ausmdv", "plate_sa_blazek_wallT", "plate_sa_pup_lim1", "plate_sa_lusgs", "plate_sa_dplur"]
-PLATE = PLATE + SA
+SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_sst_pup2_t1", "plate_sst_lusgs"]
+PLATE = PLATE + SA + SST
 ALL = RAMP + PLATE
 
 

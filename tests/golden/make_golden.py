@@ -57,6 +57,15 @@ PLATE = [
     ("plate_sa_lusgs", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
     ("plate_sa_dplur", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "timeScheme": "DPLUR", "CFL": "20",
                                               "Nlinear": "6"}, 20),
+    # Menter SST (nvar 6)
+    ("plate_sst_ausm", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1"}, 20),
+    ("plate_sst_ausmdv_wallT", _PL, {"laminar": "0", "sst": "1", "flux": "AUSMDV", "BC:plate": "wallT",
+                                     "Twall": "320"}, 20),
+    ("plate_sst_pup_axi", dict(_PL, y_shift=0.05), {"laminar": "0", "sst": "1", "flux": "AUSMpup",
+                                                    "axisymmetric": "1", "BC:top": "symmetry"}, 20),
+    ("plate_sst_pup2_t1", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "flux": "AUSMpup2",
+                                                  "turb1order": "1", "oWallFactor": "5"}, 20),
+    ("plate_sst_lusgs", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
 ]
 
 
@@ -74,7 +83,7 @@ def make_small(only=None):
         else:
             cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nit, **inp))
         mesh = po.ref_trace(d, "mesh")
-        nv = 5 if inp.get("sa") == "1" else 4
+        nv = 5 if inp.get("sa") == "1" else 6 if inp.get("sst") == "1" else 4
         nc = mesh["cellv"].size // 4
         rng = np.random.default_rng(5)
         # perturbed free stream: the reference's own initial state, +-5 %

@@ -44,7 +44,7 @@ def test_history_bitwise_vs_reference(tmp_path, name):
     # the fixture holds solverCalcCoeff's totals (pressure and friction
     # accumulated in one interleaved sum, solver.c:1253-1301)
     tot = np.stack([coef[:, 0] + coef[:, 1], coef[:, 2] + coef[:, 3]], axis=1)
-    if c.params.laminar or c.params.sa:
+    if c.params.laminar or c.params.sa or c.params.sst:
         assert np.allclose(tot, g["hist_coef"], rtol=1e-13, atol=1e-16 * np.abs(g["hist_coef"]).max())
     else:
         assert np.array_equal(tot, g["hist_coef"])
