@@ -150,11 +150,16 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
   if (p->order != 1 && p->order != 2) return set_err(CSU_ERR_ARG, "order %d", p->order);
   if (p->flux < 0 || p->flux > 4) return set_err(CSU_ERR_ARG, "flux %d", p->flux);
   if (p->limiter != 0 && p->limiter != 1) return set_err(CSU_ERR_ARG, "limiter %d", p->limiter);
-  if (p->sst) return set_err(CSU_ERR_UNSUPPORTED, "SST not built yet");
-  if (p->sa && p->laminar) return set_err(CSU_ERR_UNSUPPORTED, "laminar and sa both set");
-  if (p->nvar != (p->sa ? 5 : 4)) return set_err(CSU_ERR_ARG, "nvar %d does not match the model", p->nvar);
-  if (p->sa && p->flux == CSU_FLUX_ROE)
-    return set_err(CSU_ERR_UNSUPPORTED, "Roe has no turbulence-scalar form in the reference (flux.c:1325-1345)");
+  if ((p->sa && p->laminar) || (p->sst && (p->laminar || p->sa)))
+    return set_err(CSU_ERR_UNSUPPORTED, "more than one of laminar / sa / sst set");
+  if (p->nvar != (p->sa ? 5 : p->sst ? 6 : 4)) return set_err(CSU_ERR_ARG, "nvar %d does not match the model", p->nvar);
+  if ((p->sa || p->sst) && p->flux == CSU_FLUX_ROE)
+    return set_err(CSU_ERR_UNSUPPORTED, "Roe has no turbulence-scalar form in the reference (flux.c:1325-1366)");
+  if (p->sst && p->sst_trans) return set_err(CSU_ERR_UNSUPPORTED, "sstTrans (transition model) is not built");
+  if (p->sst && p->time_scheme == CSU_TIME_DPLUR)
+    return set_err(CSU_ERR_UNSUPPORTED, "DPLUR with SST: the reference has 4x4 and 5x5 blocks only (implicit.c:993-1000)");
+  if (p->sst && p->order == 2 && p->limiter == 1)
+    return set_err(CSU_ERR_UNSUPPORTED, "limiter 1 with SST: the reference's range arrays hold 5 variables (limiter.c:37-38)");
   if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);
   if (p->time_scheme == CSU_TIME_DPLUR && p->nlinear < 0) return set_err(CSU_ERR_ARG, "Nlinear %d", p->nlinear);
   if (p->precision != CSU_PREC_FP64) return set_err(CSU_ERR_UNSUPPORTED, "mixed precision not built yet");
@@ -373,6 +378,12 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   kp.laminar = p->laminar; kp.sa = p->sa; kp.sst = p->sst; kp.limiter = p->limiter;
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
   kp.rlim = p->rlim; kp.pout = p->pout; kp.twall = p->twall; kp.efix = p->efix; kp.minf = p->mach_inf;
+  kp.sst_owall = p->sst_owallfactor;
+  {
+    const double kap = 0.41;  // sstInit (sst.c:31-35)
+    kp.sst_g1 = 0.075 / 0.09 - 0.5 * kap * kap / sqrt(0.09);
+    kp.sst_g2 = 0.0828 / 0.09 - 0.856 * kap * kap / sqrt(0.09);
+  }
   kp.limK = p->lim_k; kp.volmax = volmax; kp.pr = p->pr; kp.pr_t = p->pr_t; kp.wimp = p->wimp;
   for (int k = 0; k < 7; k++) kp.pin[k] = p->pin[k];
   kp.gas.gamma = p->gamma; kp.gas.R = p->gas_r;
@@ -437,7 +448,8 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   if (d->visc) {
     if ((rc = dalloc(d, &s.dTx, nc)) || (rc = dalloc(d, &s.dTy, nc)) ||
         (rc = dalloc(d, &s.vflux, (size_t)(nv - 1) * nf)) || (rc = dalloc(d, &s.bvflux, (size_t)(nv - 1) * nb)) ||
-        (rc = dalloc(d, &s.wfric, 2 * (size_t)nb)) || (rc = dalloc(d, &s.miT, nf)) || (rc = dalloc(d, &s.src, nc))) {
+        (rc = dalloc(d, &s.wfric, 2 * (size_t)nb)) || (rc = dalloc(d, &s.miT, nf)) ||
+        (rc = dalloc(d, &s.src, 2 * (size_t)nc)) || (rc = dalloc(d, &s.dq, 6 * (size_t)nc))) {
       csu_dev_destroy(d);
       return rc;
     }
@@ -646,7 +658,8 @@ static int residual_t(csu_dev* d) {
 extern "C" int csu_dev_residual(csu_dev* d, double* R, double* dPx, double* dPy, double* phi, double* P,
                                 double* Pb) {
   if (!d) return set_err(CSU_ERR_ARG, "null argument");
-  int rc = d->nv == 5 ? residual_t<5, true>(d) : d->visc ? residual_t<4, true>(d) : residual_t<4, false>(d);
+  int rc = d->nv == 6 ? residual_t<6, true>(d)
+           : d->nv == 5 ? residual_t<5, true>(d) : d->visc ? residual_t<4, true>(d) : residual_t<4, false>(d);
   if (rc) return rc;
   const size_t nvc = (size_t)d->nv * d->nc;
   const size_t nc = d->nc;
@@ -764,7 +777,7 @@ static int iterate_implicit(csu_dev* d, int niter) {
                            d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0);
       }
       hipLaunchKernelGGL(k_lusgs_uinit<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->Dp, d->dtg, d->dW0,
-                         d->dW1);
+                         d->dW1, d->st.dq);
       for (size_t l = 0; l + 1 < d->lvlU.size(); l++) {
         int off = d->lvlU[l], cnt = d->lvlU[l + 1] - off;
         hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
@@ -809,11 +822,15 @@ extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* c
   const int start = d->iter;
   int rc;
   if (d->prm.time_scheme == CSU_TIME_RK)
-    rc = d->nv == 5 ? iterate_t<5, true>(d, niter)
-                    : d->visc ? iterate_t<4, true>(d, niter) : iterate_t<4, false>(d, niter);
+    rc = d->nv == 6   ? iterate_t<6, true>(d, niter)
+         : d->nv == 5 ? iterate_t<5, true>(d, niter)
+         : d->visc    ? iterate_t<4, true>(d, niter)
+                      : iterate_t<4, false>(d, niter);
   else
-    rc = d->nv == 5 ? iterate_implicit<5, true>(d, niter)
-                    : d->visc ? iterate_implicit<4, true>(d, niter) : iterate_implicit<4, false>(d, niter);
+    rc = d->nv == 6   ? iterate_implicit<6, true>(d, niter)
+         : d->nv == 5 ? iterate_implicit<5, true>(d, niter)
+         : d->visc    ? iterate_implicit<4, true>(d, niter)
+                      : iterate_implicit<4, false>(d, niter);
   if (rc) return rc;
   if (res || coef) {
     std::vector<double> rh((size_t)d->hist_cap * d->nv), ch((size_t)d->hist_cap * 4);

@@ -241,20 +241,23 @@ __device__ __forceinline__ void flux_roe(const GasD& g, T efix, T rL, T uL, T vL
 }
 
 // Total enthalpy and sound speed from (r, u, v, p[, k]) via the gas model
+// Total enthalpy.  kt is the turbulent-energy term the _sst fluxes add to
+// rho*E: k*rho in fluxAUSM_sst / fluxAUSMpup(2)_sst, k alone in
+// fluxAUSMDV_sst (flux.c:993); 0 without SST, which leaves the sum unchanged.
 template <typename T>
-__device__ __forceinline__ void state_HTc(const GasD& g, T r, T u, T v, T p, T k, T& H, T& Tk) {
+__device__ __forceinline__ void state_HTc(const GasD& g, T r, T u, T v, T p, T kt, T& H, T& Tk) {
   Tk = p / ((T)g.R * r);
-  T U3 = gas_T2e<T>(g, Tk) * r + (u * u + v * v) * r / (T)2 + k * r;
+  T U3 = gas_T2e<T>(g, Tk) * r + (u * u + v * v) * r / (T)2 + kt;
   H = (U3 + p) / r;
 }
 
 template <typename T, int NS>
 __device__ __forceinline__ void flux_ausm(const GasD& g, T rL, T uL, T vL, T pL, const T* sL,
                                           T rR, T uR, T vR, T pR, const T* sR, T kL, T kR, T* f) {
-  // flux.c:132-203 (Liou-Steffen 1993); _sa: 512-585; _sst: 901-
+  // flux.c:132-203 (Liou-Steffen 1993); _sa: 512-585; _sst: 901-972
   T TL, HL, TR, HR;
-  state_HTc<T>(g, rL, uL, vL, pL, kL, HL, TL);
-  state_HTc<T>(g, rR, uR, vR, pR, kR, HR, TR);
+  state_HTc<T>(g, rL, uL, vL, pL, kL * rL, HL, TL);
+  state_HTc<T>(g, rR, uR, vR, pR, kR * rR, HR, TR);
   T cL = gas_T2c<T>(g, TL), cR = gas_T2c<T>(g, TR);
   T ML = uL / cL, MR = uR / cR;
   T Mplus, Mminus, Pplus, Pminus;
@@ -278,7 +281,7 @@ __device__ __forceinline__ void flux_ausm(const GasD& g, T rL, T uL, T vL, T pL,
 template <typename T, int NS>
 __device__ __forceinline__ void flux_ausmdv(const GasD& g, T rL, T uL, T vL, T pL, const T* sL,
                                             T rR, T uR, T vR, T pR, const T* sR, T kL, T kR, T* f) {
-  // flux.c:206-287 (Wada-Liou 1994); _sa: 588-670
+  // flux.c:206-287 (Wada-Liou 1994); _sa: 588-670; _sst: 975-1058
   T TL, HL, TR, HR, aux;
   state_HTc<T>(g, rL, uL, vL, pL, kL, HL, TL);
   state_HTc<T>(g, rR, uR, vR, pR, kR, HR, TR);
@@ -339,12 +342,13 @@ __device__ __forceinline__ void flux_ausmpup(const GasD& g, T minf, int variant,
                                              T* f) {
   // flux.c:290-398 (AUSM+up, Liou 2006) when variant == 0;
   // flux.c:401-508 (pressure flux of the "2" variant) when variant == 1
+  // _sst (flux.c:1061-1297): k*rho in rho*E, critical speed from H - k
   T TL, HL, TR, HR;
-  state_HTc<T>(g, rL, uL, vL, pL, kL, HL, TL);
-  state_HTc<T>(g, rR, uR, vR, pR, kR, HR, TR);
-  T astar = gas_critic_H2c<T>(g, HL);
+  state_HTc<T>(g, rL, uL, vL, pL, kL * rL, HL, TL);
+  state_HTc<T>(g, rR, uR, vR, pR, kR * rR, HR, TR);
+  T astar = gas_critic_H2c<T>(g, HL - kL);
   T ahL = astar * astar / fmax(astar, fabs(uL));
-  astar = gas_critic_H2c<T>(g, HR);
+  astar = gas_critic_H2c<T>(g, HR - kR);
   T ahR = astar * astar / fmax(astar, fabs(uR));
   T am = fmin(ahL, ahR);
   const T Kp = (T)0.25, Ku = (T)0.75, sig = (T)1.0, beta = (T)1.0 / (T)8.0;

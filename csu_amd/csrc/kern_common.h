@@ -12,6 +12,7 @@ namespace csu {
 struct KP {
   int nc, nf, nb, nv, order, flux, laminar, sa, sst, limiter, turb1order, axi, visc_blazek;
   double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t, wimp;
+  double sst_owall, sst_g1, sst_g2;  // SST: oWallFactor, blended gamma constants (sst.c:18-38)
   double pin[7];
   GasD gas;
 };
@@ -32,7 +33,8 @@ struct Geo {
 struct St {
   double *P, *Pb, *dPx, *dPy, *phi, *dTx, *dTy;
   double *fflux, *vflux, *bvflux, *wfric, *pref2, *miT;
-  double *src;          // SA: source term Qt*Omega per cell
+  double *src;          // SA: Qt*Omega [nc]; SST: Qtk*Omega, Qto*Omega [2][nc]
+  double *dq;           // SST source Jacobians dQk/d(r, rk, ro), dQo/d(r, rk, ro) [6][nc]
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference
@@ -123,6 +125,15 @@ __device__ __forceinline__ void ghost_state(const KP& kp, int type, double dSx, 
     else if (type == CSU_BC_INLET) Pg[5] = kp.pin[5];
     else Pg[5] = 0.0;
   }
+  if (NV == 6) {  // SST k, omega (boundary.c:660-690); walls: omega from the wall distance
+    if (type == CSU_BC_SYMMETRY || type == CSU_BC_OUTLET) { Pg[5] = Pi[5]; Pg[6] = Pi[6]; }
+    else if (type == CSU_BC_INLET) { Pg[5] = kp.pin[5]; Pg[6] = kp.pin[6]; }
+    else {
+      const double n = sutherland(Pg[4]) / Pg[0];
+      Pg[5] = 1e-14;
+      Pg[6] = kp.sst_owall * 6 * n / (0.075 * wall_d * wall_d);
+    }
+  }
   (void)wall_d;
 }
 
@@ -133,31 +144,36 @@ __device__ __forceinline__ void bc_flux(const KP& kp, int type, double dSx, doub
                                         const double* Pi, double* R) {
   double PL[NV], Pb[NV], fl[NV];
   PL[0] = Pi[0]; PL[1] = Pi[1]; PL[2] = Pi[2]; PL[3] = Pi[3];
-  if (NV == 5) PL[4] = Pi[5];
-  constexpr int NSC = NV == 5 ? 1 : 0;
+#pragma unroll
+  for (int q = 4; q < NV; q++) PL[q] = Pi[q + 1];
+  constexpr int NSC = NV - 4;
+  const double kL = NV == 6 ? PL[4] : 0.0;  // SST: the k of both states enters the energy
   if (type == CSU_BC_SYMMETRY) {
     rotate<double>(PL[1], PL[2], dSx, dSy, dS);
     double Pm[NV];
     Pm[0] = PL[0]; Pm[1] = -PL[1]; Pm[2] = PL[2]; Pm[3] = PL[3];
-    if (NV == 5) Pm[4] = PL[4];
+#pragma unroll
+    for (int q = 4; q < NV; q++) Pm[q] = PL[q];
     flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
-                               Pm[0], Pm[1], Pm[2], Pm[3], Pm + 4, 0.0, 0.0, fl);
+                               Pm[0], Pm[1], Pm[2], Pm[3], Pm + 4, kL, kL, fl);
   } else if (type == CSU_BC_INLET || type == CSU_BC_OUTLET) {
     if (type == CSU_BC_INLET) bc_inlet(kp.gas, kp.pin, PL, Pb, dSx / dS, dSy / dS);
     else bc_outlet(kp.gas, kp.pout, PL, Pb, dSx / dS, dSy / dS);
-    if (NV == 5) Pb[4] = (type == CSU_BC_INLET) ? kp.pin[5] : PL[4];
+#pragma unroll
+    for (int q = 4; q < NV; q++) Pb[q] = (type == CSU_BC_INLET) ? kp.pin[q + 1] : PL[q];
     rotate<double>(PL[1], PL[2], dSx, dSy, dS);
     rotate<double>(Pb[1], Pb[2], dSx, dSy, dS);
     flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
-                               Pb[0], Pb[1], Pb[2], Pb[3], Pb + 4, 0.0, 0.0, fl);
+                               Pb[0], Pb[1], Pb[2], Pb[3], Pb + 4, kL, NV == 6 ? Pb[4] : 0.0, fl);
   } else {
-    if (kp.laminar == 1 || NV == 5) {
+    if (kp.laminar == 1 || NV >= 5) {
       rotate<double>(PL[1], PL[2], dSx, dSy, dS);
       double Pm[NV];
       Pm[0] = PL[0]; Pm[1] = -PL[1]; Pm[2] = -PL[2]; Pm[3] = PL[3];
-      if (NV == 5) Pm[4] = PL[4];
+#pragma unroll
+      for (int q = 4; q < NV; q++) Pm[q] = PL[q];
       flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
-                                 Pm[0], Pm[1], Pm[2], Pm[3], Pm + 4, 0.0, 0.0, fl);
+                                 Pm[0], Pm[1], Pm[2], Pm[3], Pm + 4, kL, kL, fl);
     } else {
       bc_wall(kp.gas, PL, Pb, dSx / dS, dSy / dS);
       rotate<double>(Pb[1], Pb[2], dSx, dSy, dS);

@@ -124,10 +124,53 @@ __device__ __forceinline__ void delta_flux_sa(const KP& kp, const double* Pj, co
   dF[4] = U[4] * V - F0[4];
 }
 
+// implicitCalcDeltaFlux_sst (implicit.c:1048-1117): k and omega clamps
+__device__ __forceinline__ void delta_flux_sst(const KP& kp, const double* Pj, const double* d, double nx, double ny,
+                                               double* dF) {
+  double rho = Pj[0], u = Pj[1], v = Pj[2], p = Pj[3], k = Pj[5], om = Pj[6];
+  double T = p / (kp.gas.R * rho);
+  double e0 = gas_T2e<double>(kp.gas, T);
+  double E = e0 + (u * u + v * v) * 0.5 + k;
+  double U[6] = {rho, rho * u, rho * v, rho * E, rho * k, rho * om};
+  double V = (nx * U[1] + ny * U[2]) / U[0];
+  double F0[6] = {U[0] * V, U[1] * V + nx * p, U[2] * V + ny * p, (U[3] + p) * V, U[4] * V, U[5] * V};
+#pragma unroll
+  for (int q = 0; q < 6; q++) U[q] = U[q] + d[q];
+  rho = fmax(U[0], kp.rlim);
+  U[0] = rho;
+  u = U[1] / rho; v = U[2] / rho;
+  k = U[4] / rho;
+  if (k < 1e-14) { k = 1e-14; U[4] = rho * k; }
+  om = U[5] / rho;
+  if (om < 0.0) { om = 0.0; U[5] = rho * om; }
+  E = U[3] / rho;
+  double e1 = E - (u * u + v * v) * 0.5 - k;
+  e1 = fmax(e1, 7.176325e+02);
+  T = T + (e1 - e0) / gas_T2Cv<double>(kp.gas, T);
+  if (T < 1) {
+    T = 1;
+    e1 = gas_T2e<double>(kp.gas, T);
+    E = e1 + (u * u + v * v) * 0.5 + k;
+    U[3] = rho * E;
+  }
+  p = kp.gas.R * T * rho;
+  V = (nx * U[1] + ny * U[2]) / U[0];
+  dF[0] = U[0] * V - F0[0];
+  dF[1] = (U[1] * V + nx * p) - F0[1];
+  dF[2] = (U[2] * V + ny * p) - F0[2];
+  dF[3] = (U[3] + p) * V - F0[3];
+  dF[4] = U[4] * V - F0[4];
+  dF[5] = U[5] * V - F0[5];
+}
+
 // implicitCalcDeltaFlux (implicit.c:194-231) at state Pj with increment d
 template <int NV>
 __device__ __forceinline__ void delta_flux(const KP& kp, const double* Pj, const double* d, double nx, double ny,
                                            double* dF) {
+  if (NV == 6) {
+    delta_flux_sst(kp, Pj, d, nx, ny, dF);
+    return;
+  }
   if (NV == 5) {
     delta_flux_sa(kp, Pj, d, nx, ny, dF);
     return;
@@ -198,8 +241,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
     delta_flux<NV>(kp, Pj, d, nx, ny, dF);
     double c = gas_T2c<double>(kp.gas, Pj[4]);
     double ra = kp.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
-    if (NV == 5) {
-      // implicitFunc_sa (implicit.c:555-595)
+    if (NV >= 5) {
+      // implicitFunc_sa / _sst (implicit.c:555-595, 1120-1161)
       double r = Pj[0], T = Pj[4], mi = sutherland(T);
       double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
       double dd_ = sqrt(ddx * ddx + ddy * ddy);
@@ -214,6 +257,27 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
     for (int k = 0; k < NV; k++) acc[k] -= 0.5 * (dF[k] * dS - ra * d[k]);
   }
   const double D = diag(Dp, g.omega, dtg, i);
+  if (NV == 6) {
+    // SST: k-omega block with the source Jacobians, Cramer's rule
+    // (implicit.c:326-351, 447-472).  The reference divides by a long double
+    // determinant that holds a double value; the quotient here is the
+    // correctly rounded double one.
+#pragma unroll
+    for (int k = 0; k < 4; k++) dW[k * nc + i] = acc[k] / D;
+    const double x0 = acc[0] / D;
+    const double A = acc[4] + s.dq[i] * x0;
+    const double B = acc[5] + s.dq[3 * nc + i] * x0;
+    const double a = D - s.dq[nc + i], b = -s.dq[2 * nc + i], c = -s.dq[4 * nc + i], dd = D - s.dq[5 * nc + i];
+    const double det = a * dd - b * c;
+    if (fabs(det) < 1e-14) {
+      dW[4 * nc + i] = acc[4] / D;
+      dW[5 * nc + i] = acc[5] / D;
+    } else {
+      dW[4 * nc + i] = (A * dd - B * b) / det;
+      dW[5 * nc + i] = (-A * c + B * a) / det;
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < NV; k++) dW[k * nc + i] = acc[k] / D;
 }
@@ -222,13 +286,21 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs_uinit(KP kp, Geo g, const double* __restrict__ Dp,
                                                            const double* __restrict__ dtg, const double* __restrict__ dW0,
-                                                           double* __restrict__ dW1) {
+                                                           double* __restrict__ dW1, const double* __restrict__ dq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = kp.nc;
   if (i >= nc) return;
   const double D = diag(Dp, g.omega, dtg, i);
 #pragma unroll
   for (int k = 0; k < NV; k++) dW1[k * nc + i] = D * dW0[k * nc + i];
+  if (NV == 6) {  // implicit.c:372-392: the k-omega block times dW0
+    const double a = D - dq[nc + i], b = -dq[2 * nc + i], c = -dq[4 * nc + i], dd = D - dq[5 * nc + i];
+    const double det = a * dd - b * c;
+    if (!(fabs(det) < 1e-14)) {
+      dW1[4 * nc + i] = a * dW0[4 * nc + i] + b * dW0[5 * nc + i] - dq[i] * dW0[i];
+      dW1[5 * nc + i] = c * dW0[4 * nc + i] + dd * dW0[5 * nc + i] - dq[3 * nc + i] * dW0[i];
+    }
+  }
 }
 
 // DPLUR finite-difference blocks of every interior neighbour

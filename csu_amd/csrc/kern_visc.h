@@ -270,3 +270,243 @@ __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, d
 }
 
 }  // namespace csu
+
+namespace csu {
+
+// ---------------------------------------------------------------------------
+// Menter SST (reference sst.c)
+// ---------------------------------------------------------------------------
+struct SstV {
+  double dux, duy, dvx, dvy, dkx, dky, dox, doy;
+  double r, T, k, om, d, mi_L, mi_t;
+  double tkx, tky, tox, toy, Qtk, Qto;
+};
+constexpr double SST_SK1 = 0.85, SST_SO1 = 0.5, SST_B1 = 0.075, SST_SK2 = 1.0, SST_SO2 = 0.856,
+                 SST_B2 = 0.0828, SST_BS = 0.09, SST_A1 = 0.31;
+
+__device__ __forceinline__ double sst_blend(double x1, double x2, double F1) { return x1 * F1 + x2 * (1 - F1); }
+
+// sstF1 / sstF2 (sst.c:788-818); F1's long double temporaries only hold
+// values computed in double.
+__device__ __forceinline__ double sst_F1(const SstV& v, double n_L_term, double sqrtk_term) {
+  double aux1 = 2 * v.r * SST_SO2 * (v.dkx * v.dox + v.dky * v.doy) / v.om;
+  if (v.om == 0) aux1 = 0;
+  double CD = fmax(aux1, 1.0e-20);
+  aux1 = fmax(sqrtk_term, n_L_term);
+  double aux2 = 4 * v.r * SST_SO2 * v.k / (CD * v.d * v.d);
+  double arg1 = fmin(aux1, aux2);
+  return tanh(arg1 * arg1 * arg1 * arg1);
+}
+__device__ __forceinline__ double sst_F2(double n_L_term, double sqrtk_term) {
+  double arg2 = fmax(2 * sqrtk_term, n_L_term);
+  return tanh(arg2 * arg2);
+}
+
+// sstFlux (sst.c:749-786): eddy viscosity, k and omega diffusion fluxes.
+__device__ __forceinline__ void sst_flux(SstV& v) {
+  double n_L = v.mi_L / v.r;
+  double omega = fabs(v.duy - v.dvx);
+  double sqrtk_term = sqrt(v.k) / (SST_BS * v.om * v.d);
+  double n_L_term = 500 * n_L / (v.d * v.d * v.om);
+  double F1 = sst_F1(v, n_L_term, sqrtk_term);
+  double F2 = sst_F2(n_L_term, sqrtk_term);
+  v.mi_t = v.r * SST_A1 * v.k / fmax(SST_A1 * v.om, omega * F2);
+  double sk = sst_blend(SST_SK1, SST_SK2, F1), so = sst_blend(SST_SO1, SST_SO2, F1);
+  double aux = v.mi_L + sk * v.mi_t;
+  v.tkx = aux * v.dkx;
+  v.tky = aux * v.dky;
+  aux = v.mi_L + so * v.mi_t;
+  v.tox = aux * v.dox;
+  v.toy = aux * v.doy;
+}
+
+// sstSources (sst.c:698-747): Qtk, Qto.
+__device__ __forceinline__ void sst_sources(const KP& kp, SstV& v) {
+  double n_L = v.mi_L / v.r;
+  double omega = fabs(v.duy - v.dvx);
+  double sqrtk_term = sqrt(v.k) / (SST_BS * v.om * v.d);
+  double n_L_term = 500 * n_L / (v.d * v.d * v.om);
+  double F1 = sst_F1(v, n_L_term, sqrtk_term);
+  double F2 = sst_F2(n_L_term, sqrtk_term);
+  v.mi_t = v.r * SST_A1 * v.k / fmax(SST_A1 * v.om, omega * F2);
+  double n_t = v.mi_t / v.r;
+  double gm = sst_blend(kp.sst_g1, kp.sst_g2, F1), b = sst_blend(SST_B1, SST_B2, F1);
+  double txx = 2 * v.mi_t * (v.dux - (v.dux + v.dvy) / 3) - 2. * v.r * v.k / 3;
+  double tyy = 2 * v.mi_t * (v.dvy - (v.dux + v.dvy) / 3) - 2. * v.r * v.k / 3;
+  double txy = v.mi_t * (v.duy + v.dvx);
+  double P = v.dux * txx + v.dvy * tyy + (v.duy + v.dvx) * txy;
+  P = fmin(P, 10 * SST_BS * v.r * v.om * v.k);
+  double cd = 2 * (1 - F1) * v.r * SST_SO2 * (v.dkx * v.dox + v.dky * v.doy) / v.om;
+  v.Qtk = P - SST_BS * v.r * v.om * v.k;
+  v.Qto = gm * P / n_t - b * v.r * v.om * v.om + cd;
+}
+
+// Stresses with the -2/3 rho k term (sst.c:163-165).
+__device__ __forceinline__ Tau sst_tau(double mi, const SstV& v) {
+  Tau t;
+  t.xx = 2 * mi * (v.dux - (v.dux + v.dvy) / 3) - 2. * v.r * v.k / 3.;
+  t.yy = 2 * mi * (v.dvy - (v.dux + v.dvy) / 3) - 2. * v.r * v.k / 3.;
+  t.xy = mi * (v.duy + v.dvx);
+  return t;
+}
+
+// sstInterSource (sst.c:595-691) for one cell: the two source terms times
+// Omega and the six finite-difference source Jacobians of the LU-SGS block.
+// g*: cell gradients of u v T k om.
+__device__ __forceinline__ void sst_cell_source(const KP& kp, const double* Pi, double d, double om_i,
+                                                const double* gx, const double* gy, double* src, double* dq) {
+  SstV v;
+  v.dux = gx[0]; v.dvx = gx[1]; v.dkx = gx[3]; v.dox = gx[4];
+  v.duy = gy[0]; v.dvy = gy[1]; v.dky = gy[3]; v.doy = gy[4];
+  v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6]; v.d = d;
+  v.mi_L = sutherland(v.T);
+  if (v.om == 0) {  // omLim (solver.c:1155): no source, no Jacobians
+    src[0] = 0.0; src[1] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) dq[q] = 0.0;
+    return;
+  }
+  sst_sources(kp, v);
+  src[0] = v.Qtk * om_i;
+  src[1] = v.Qto * om_i;
+  const double Qk0 = v.Qtk, Qo0 = v.Qto;
+  const double h = 1e-6, dh = 1e-6;
+  double hh = fmax(fabs(v.om * h), dh);
+  v.om += hh;
+  sst_sources(kp, v);
+  const double dQkdo = (v.Qtk - Qk0) / hh, dQodo = (v.Qto - Qo0) / hh;
+  v.om = Pi[6];
+  hh = fmax(fabs(v.k * h), dh);
+  v.k += hh;
+  sst_sources(kp, v);
+  const double dQkdk = (v.Qtk - Qk0) / hh, dQodk = (v.Qto - Qo0) / hh;
+  v.k = Pi[5];
+  hh = fmax(fabs(v.r * h), dh);
+  v.r += hh;
+  sst_sources(kp, v);
+  const double dQkdr = (v.Qtk - Qk0) / hh, dQodr = (v.Qto - Qo0) / hh;
+  v.r = Pi[0];
+  dq[0] = (dQkdr - dQkdk * v.k / v.r - dQkdo * v.om / v.r) * om_i;
+  dq[1] = dQkdk * om_i / v.r;
+  dq[2] = dQkdo * om_i / v.r;
+  dq[3] = (dQodr - dQodk * v.k / v.r - dQodo * v.om / v.r) * om_i;
+  dq[4] = dQodk * om_i / v.r;
+  dq[5] = dQodo * om_i / v.r;
+}
+
+// Interior SST face flux F[0..4] (= reference faceFlux[1..5]) and the face
+// eddy viscosity, sstInterFace (sst.c:56-166).  Gradients: u v T k om.
+// Edge correction along (dx, dy)/L in the form sst.c:121-143 uses.
+__device__ __forceinline__ void sst_face(const KP& kp, double dSx, double dSy, double dx, double dy,
+                                         const double* P0, const double* P1, double d0, double d1,
+                                         const double* g0x, const double* g0y, const double* g1x,
+                                         const double* g1y, double* F, double& mi_t) {
+  const double L = sqrt(dx * dx + dy * dy);
+  const double nx = dx / L, ny = dy / L;
+  double dl[5] = {(P1[1] - P0[1]) / L, (P1[2] - P0[2]) / L, (P1[4] - P0[4]) / L, (P1[5] - P0[5]) / L,
+                  (P1[6] - P0[6]) / L};
+  double gxc[5], gyc[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const double gxm = 0.5 * (g0x[q] + g1x[q]), gym = 0.5 * (g0y[q] + g1y[q]);
+    const double aux = gxm * nx + gym * ny;
+    gxc[q] = gxm + (dl[q] - aux) * nx;
+    gyc[q] = gym + (dl[q] - aux) * ny;
+  }
+  SstV v;
+  v.dux = gxc[0]; v.duy = gyc[0]; v.dvx = gxc[1]; v.dvy = gyc[1];
+  v.dkx = gxc[3]; v.dky = gyc[3]; v.dox = gxc[4]; v.doy = gyc[4];
+  const double dTx = gxc[2], dTy = gyc[2];
+  v.r = (P1[0] + P0[0]) * 0.5;
+  const double u = (P1[1] + P0[1]) * 0.5, w = (P1[2] + P0[2]) * 0.5;
+  v.T = (P1[4] + P0[4]) * 0.5;
+  v.k = (P1[5] + P0[5]) * 0.5;
+  v.om = (P1[6] + P0[6]) * 0.5;
+  v.d = (d1 + d0) * 0.5;
+  v.mi_L = sutherland(v.T);
+  sst_flux(v);
+  const double mi = v.mi_L + v.mi_t;
+  const double kk = gas_T2Cp<double>(kp.gas, v.T) * (v.mi_L / kp.pr + v.mi_t / kp.pr_t);
+  mi_t = v.mi_t;
+  Tau t = sst_tau(mi, v);
+  F[0] = t.xx * dSx + t.xy * dSy;
+  F[1] = t.xy * dSx + t.yy * dSy;
+  F[2] = (t.xx * dSx + t.xy * dSy) * u + (t.xy * dSx + t.yy * dSy) * w + kk * (dTx * dSx + dTy * dSy);
+  F[3] = v.tkx * dSx + v.tky * dSy;
+  F[4] = v.tox * dSx + v.toy * dSy;
+}
+
+// SST boundary viscous flux f[0..4] (= reference f[1..5]),
+// sstBoundaryFaceViscFlux (sst.c:202-547).  Gradients: u v T k om.
+__device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, double dSy, double nx, double ny,
+                                            double dS2, double dx, double dy, const double* Pi, double d,
+                                            const double* gx, const double* gy, double* f) {
+  SstV v;
+  v.d = d;
+  if (type == CSU_BC_SYMMETRY) {
+    v.dux = gx[0] - (gx[0] * nx + gy[0] * ny) * nx; v.duy = gy[0] - (gx[0] * nx + gy[0] * ny) * ny;
+    v.dvx = gx[1] - (gx[1] * nx + gy[1] * ny) * nx; v.dvy = gy[1] - (gx[1] * nx + gy[1] * ny) * ny;
+    v.dkx = gx[3] - (gx[3] * nx + gy[3] * ny) * nx; v.dky = gy[3] - (gx[3] * nx + gy[3] * ny) * ny;
+    v.dox = gx[4] - (gx[4] * nx + gy[4] * ny) * nx; v.doy = gy[4] - (gx[4] * nx + gy[4] * ny) * ny;
+    v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6];
+    v.mi_L = sutherland(v.T);
+    sst_flux(v);
+    Tau t = sst_tau(v.mi_L + v.mi_t, v);
+    const double u = Pi[1], w = Pi[2];
+    f[0] = (t.xx * nx + t.xy * ny) * dS2;
+    f[1] = (t.xy * nx + t.yy * ny) * dS2;
+    f[2] = (u * (t.xx * nx + t.xy * ny) + w * (t.xy * nx + t.yy * ny)) * dS2;  // no heat flux
+    f[3] = 0.0;
+    f[4] = 0.0;
+    return;
+  }
+  if (type == CSU_BC_INLET || type == CSU_BC_WALL || type == CSU_BC_WALLT) {
+    const double L = sqrt(dx * dx + dy * dy);
+    double ub, vb, Tb = 0.0, kb, ob;
+    if (type == CSU_BC_INLET) {
+      v.r = kp.pin[0]; v.T = kp.pin[4]; v.k = kp.pin[5]; v.om = kp.pin[6];
+      ub = kp.pin[1]; vb = kp.pin[2]; Tb = kp.pin[4]; kb = kp.pin[5]; ob = kp.pin[6];
+      v.mi_L = sutherland(v.T);
+    } else {
+      v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6];
+      v.mi_L = sutherland(v.T);
+      const double n_L = v.mi_L / v.r;
+      ub = 0.0; vb = 0.0; kb = 0.0;
+      ob = kp.sst_owall * 6 * n_L / (SST_B1 * v.d * v.d);
+      if (type == CSU_BC_WALLT) Tb = kp.twall;
+    }
+    edge_correct(gx[0], gy[0], (ub - Pi[1]) / L, dx, dy, L, v.dux, v.duy);
+    edge_correct(gx[1], gy[1], (vb - Pi[2]) / L, dx, dy, L, v.dvx, v.dvy);
+    double dTx = 0.0, dTy = 0.0;
+    if (type != CSU_BC_WALL) edge_correct(gx[2], gy[2], (Tb - Pi[4]) / L, dx, dy, L, dTx, dTy);
+    edge_correct(gx[3], gy[3], (kb - Pi[5]) / L, dx, dy, L, v.dkx, v.dky);
+    edge_correct(gx[4], gy[4], (ob - Pi[6]) / L, dx, dy, L, v.dox, v.doy);
+    sst_flux(v);
+    Tau t = sst_tau(v.mi_L + v.mi_t, v);
+    f[0] = t.xx * dSx + t.xy * dSy;
+    f[1] = t.xy * dSx + t.yy * dSy;
+    f[3] = v.tkx * dSx + v.tky * dSy;
+    f[4] = v.tox * dSx + v.toy * dSy;
+    if (type == CSU_BC_WALL) { f[2] = 0.0; return; }
+    const double kk = gas_T2Cp<double>(kp.gas, v.T) * (v.mi_L / kp.pr + v.mi_t / kp.pr_t);
+    if (type == CSU_BC_WALLT) { f[2] = kk * (dTx * dSx + dTy * dSy); return; }
+    f[2] = ub * (t.xx * dSx + t.xy * dSy) + vb * (t.xy * dSx + t.yy * dSy) + kk * (dTx * dSx + dTy * dSy);
+    return;
+  }
+  // outlet
+  v.dux = gx[0]; v.dvx = gx[1]; v.dkx = gx[3]; v.dox = gx[4];
+  v.duy = gy[0]; v.dvy = gy[1]; v.dky = gy[3]; v.doy = gy[4];
+  v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6];
+  v.mi_L = sutherland(v.T);
+  sst_flux(v);
+  const double kk = gas_T2Cp<double>(kp.gas, v.T) * (v.mi_L / kp.pr + v.mi_t / kp.pr_t);
+  Tau t = sst_tau(v.mi_L + v.mi_t, v);
+  const double u = Pi[1], w = Pi[2];
+  f[0] = t.xx * dSx + t.xy * dSy;
+  f[1] = t.xy * dSx + t.yy * dSy;
+  f[2] = u * (t.xx * dSx + t.xy * dSy) + w * (t.xy * dSx + t.yy * dSy) + kk * (gx[2] * dSx + gy[2] * dSy);
+  f[3] = v.tkx * dSx + v.tky * dSy;
+  f[4] = v.tox * dSx + v.toy * dSy;
+}
+
+}  // namespace csu

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       gb[j] = q;
       dx[j] = g.bxm[q] - x0;
       dy[j] = g.bym[q] - y0;
-      ghost_state<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, 0.0, Pn[j]);
+      ghost_state<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, NV == 6 ? g.wdist[i] : 0.0, Pn[j]);
 #pragma unroll
       for (int m = 0; m < NS; m++) s.Pb[m * nb + q] = Pn[j][m];
     }
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     lim_aux = kp.limK * sqrt(g.omega[i] / kp.volmax);
     lim_aux = lim_aux * lim_aux * lim_aux;
   }
-  double gux = 0, guy = 0, gvx = 0, gvy = 0, grx = 0, gry = 0, gnx = 0, gny = 0;
+  double gux = 0, guy = 0, gvx = 0, gvy = 0, grx = 0, gry = 0, gnx = 0, gny = 0, gox = 0, goy = 0;
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) {
     const int m = kk > 3 ? kk + 1 : kk;
@@ -125,7 +125,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     if (kk == 0) { grx = gx; gry = gy; }
     if (kk == 1) { gux = gx; guy = gy; }
     if (kk == 2) { gvx = gx; gvy = gy; }
-    if (kk == 4) { gnx = gx; gny = gy; }
+    if (kk == 4) { gnx = gx; gny = gy; }  // SA nu~ or SST k
+    if (kk == 5) { gox = gx; goy = gy; }  // SST omega
     if (kp.order == 2) {
       double pref2 = kp.limiter == 0 ? s.pref2[kk] * lim_aux : s.pref2[kk];
       double Pmin = u0, Pmax = u0;
@@ -154,7 +155,15 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gTx, gTy);
     s.dTx[i] = gTx;
     s.dTy[i] = gTy;
-    const double gx3[4] = {gux, gvx, gTx, gnx}, gy3[4] = {guy, gvy, gTy, gny};
+    const double gx3[5] = {gux, gvx, gTx, gnx, gox}, gy3[5] = {guy, gvy, gTy, gny, goy};
+    if (NV == 6) {
+      double src[2], dq[6];
+      sst_cell_source(kp, Pi, g.wdist[i], g.omega[i], gx3, gy3, src, dq);
+      s.src[i] = src[0];
+      s.src[nc + i] = src[1];
+#pragma unroll
+      for (int q = 0; q < 6; q++) s.dq[q * nc + i] = dq[q];
+    }
     if (NV == 5) {
       // saInterSource (sa.c:224-260): uses grad rho, du/dy, dv/dx, grad nu~
       const double mi_L = sutherland(Pi[4]);
@@ -167,8 +176,13 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       const int q = gb[j];
       if (q < 0) continue;
       const int t = g.btype[q];
-      double f[4];
-      if (NV == 5) {
+      double f[5];
+      if (NV == 6) {
+        sst_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, g.wdist[i], gx3,
+                    gy3, f);
+        s.bvflux[3 * nb + q] = f[3];
+        s.bvflux[4 * nb + q] = f[4];
+      } else if (NV == 5) {
         sa_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, f);
         s.bvflux[3 * nb + q] = f[3];
       } else {
@@ -276,9 +290,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
   rotate<double>(PL[1], PL[2], dSx, dSy, dS);
   rotate<double>(PR[1], PR[2], dSx, dSy, dS);
   double fl[NV];
-  constexpr int NSC = NV == 5 ? 1 : 0;
+  constexpr int NSC = NV - 4;
   flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
-                             PR[0], PR[1], PR[2], PR[3], PR + 4, 0.0, 0.0, fl);
+                             PR[0], PR[1], PR[2], PR[3], PR + 4, NV == 6 ? PL[4] : 0.0, NV == 6 ? PR[4] : 0.0,
+                             fl);
   rotate<double>(fl[1], fl[2], dSx, -dSy, dS);
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) s.fflux[kk * nf + f] = fl[kk] * dS;
@@ -308,6 +323,21 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
     sa_face(kp, dSx, dSy, g.xc[e1] - g.xc[e0], g.yc[e1] - g.yc[e0], P0, P1, g0x, g0y, g1x, g1y, F, mi_t);
 #pragma unroll
     for (int k = 0; k < 4; k++) s.vflux[k * nf + f] = F[k];
+    s.miT[f] = mi_t;
+  }
+  if (VISC && NV == 6) {
+    double P0[7], P1[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) { P0[m] = s.P[m * nc + e0]; P1[m] = s.P[m * nc + e1]; }
+    const double g0x[5] = {s.dPx[nc + e0], s.dPx[2 * nc + e0], s.dTx[e0], s.dPx[4 * nc + e0], s.dPx[5 * nc + e0]};
+    const double g0y[5] = {s.dPy[nc + e0], s.dPy[2 * nc + e0], s.dTy[e0], s.dPy[4 * nc + e0], s.dPy[5 * nc + e0]};
+    const double g1x[5] = {s.dPx[nc + e1], s.dPx[2 * nc + e1], s.dTx[e1], s.dPx[4 * nc + e1], s.dPx[5 * nc + e1]};
+    const double g1y[5] = {s.dPy[nc + e1], s.dPy[2 * nc + e1], s.dTy[e1], s.dPy[4 * nc + e1], s.dPy[5 * nc + e1]};
+    double F[5], mi_t;
+    sst_face(kp, dSx, dSy, g.xc[e1] - g.xc[e0], g.yc[e1] - g.yc[e0], P0, P1, g.wdist[e0], g.wdist[e1], g0x, g0y,
+             g1x, g1y, F, mi_t);
+#pragma unroll
+    for (int k = 0; k < 5; k++) s.vflux[k * nf + f] = F[k];
     s.miT[f] = mi_t;
   }
 }
@@ -420,6 +450,30 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
         if (q < 0) continue;
 #pragma unroll
         for (int kk = 1; kk < 5; kk++) R[kk] -= s.bvflux[(kk - 1) * nb + q];
+      }
+    }
+    if (VISC && NV == 6) {
+      // sstInterFace gather (sst.c:168-192), sstInterSource (sst.c:638-639),
+      // sstBoundaryFace (sst.c:549-567)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int f = cf[j];
+        if (f > 0) {
+#pragma unroll
+          for (int kk = 1; kk < 6; kk++) R[kk] -= s.vflux[(kk - 1) * nf + f - 1];
+        } else if (f < 0) {
+#pragma unroll
+          for (int kk = 1; kk < 6; kk++) R[kk] += s.vflux[(kk - 1) * nf - f - 1];
+        }
+      }
+      R[4] -= s.src[i];
+      R[5] -= s.src[nc + i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int q = gbq[j];
+        if (q < 0) continue;
+#pragma unroll
+        for (int kk = 1; kk < 6; kk++) R[kk] -= s.bvflux[(kk - 1) * nb + q];
       }
     }
     if (flags & F_STORE_R) {

@@ -249,3 +249,40 @@ def test_sa_device_wall_distance_history(tmpcase):
     rg, _ = dev.iterate(30)
     ro, _ = orc.iterate(30)
     assert np.max(np.abs(rg / ro - 1)) < 1e-10
+
+
+@pytest.mark.parametrize("inp", [
+    {},
+    {"flux": "AUSMDV", "BC:plate": "wallT", "Twall": "320"},
+    {"flux": "AUSMpup2", "turb1order": "1", "oWallFactor": "5"},
+    {"timeScheme": "LUSGS", "CFL": "10"},
+])
+def test_sst_history_vs_oracle(tmpcase, inp):
+    """Menter SST (k and omega transport, blended constants, sources with
+    their finite-difference Jacobians, SST boundary viscous fluxes; LU-SGS
+    with the k-omega source block) over 60 iterations on a 60x24 hybrid
+    plate mesh: GPU against the oracle."""
+    d = tmpcase("k")
+    print("sst plate", inp, flush=True)
+    cases.write_case(d, cases.plate_mesh(nx=60, ny=24, y1=2e-3, growth=1.15, jitter=0.1),
+                     cases.plate_input(threads=1, nmax=60, laminar="0", sst="1", **inp))
+    c = native.HostCase(d)
+    assert c.nvar == 6
+    dev = native.Device(c)
+    orc = po.Oracle(c)
+    rg, cg = dev.iterate(60)
+    ro, co = orc.iterate(60)
+    _check_history(c, rg, ro, False)
+    assert np.max(np.abs(cg - co)) <= 1e-10 * np.abs(co).max()
+    assert _rel(dev.get_state(), orc.get_state()) < 1e-10
+
+
+def test_sst_rejects_what_the_reference_cannot_run(tmpcase):
+    """SST with DPLUR or with the global-range limiter has no reference
+    behaviour to match: creating such a device context fails loudly."""
+    for inp in ({"timeScheme": "DPLUR"}, {"limiter": "1"}):
+        d = tmpcase("r")
+        cases.write_case(d, cases.plate_mesh(nx=8, ny=4, y1=2e-3, growth=1.15),
+                         cases.plate_input(threads=1, nmax=1, laminar="0", sst="1", **inp))
+        with pytest.raises(native.CsuError):
+            native.Device(native.HostCase(d))
