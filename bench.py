@@ -27,6 +27,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -81,6 +83,81 @@ def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool):
     return info
 
 
+def serving_requests(n: int, seed: int = 4):
+    """Seeded request mix for the serving measurement: supersonic Euler
+    ramps, laminar plates and SA plates, LU-SGS on colour/RCM-renumbered
+    meshes, cell counts log-uniform in [10k, 100k], Mach per kind."""
+    from csu_amd import cases, native, renumber
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        kind = ("ramp", "plate", "sa")[k % 3]
+        ncell = float(np.exp(rng.uniform(np.log(1e4), np.log(1e5))))
+        d = tempfile.mkdtemp(prefix=f"sv{k}_", dir="/tmp") + "/"
+        if kind == "ramp":
+            ny = max(8, int(round(np.sqrt(ncell / 4))))
+            nx = max(8, int(round(ncell / (2 * ny))))
+            cases.ramp_case(d, nx=nx, ny=ny, seed=k, threads=1, nmax=1, timeScheme="LUSGS", CFL=20,
+                            flux="AUSMDV", mach=f"{rng.uniform(1.8, 3.0):.4f}")
+        else:
+            ny = max(8, int(round(np.sqrt(ncell / 5))))
+            nx = max(8, int(round(ncell / ny)))
+            inp = dict(timeScheme="LUSGS", CFL="20", mach=f"{rng.uniform(0.2, 0.5):.4f}")
+            if kind == "sa":
+                inp.update(laminar="0", sa="1")
+            cases.write_case(d, cases.plate_mesh(nx=nx, ny=ny, y1=2e-3, growth=1.05, jitter=0.05, seed=k),
+                             cases.plate_input(threads=1, nmax=1, **inp))
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+        out.append((kind, native.HostCase(d)))
+    return out
+
+
+def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 3000, chunk: int = 50):
+    """Feed the requests to a csu_server at the given arrival times (s);
+    returns the job results in submit order."""
+    from csu_amd import native
+    srv = native.Server(max_slots=slots, hbm_budget=250e9, chunk=chunk)
+    res, sub = {}, 0
+    while len(res) < len(reqs):
+        now = srv.clock()
+        while sub < len(reqs) and arrivals[sub] <= now:
+            srv.submit(reqs[sub][1], drop=drop, max_iter=max_iter)
+            sub += 1
+        srv.step(0.002)
+        for r in srv.results():
+            res[r["id"]] = r
+    srv.close()
+    return [res[i] for i in range(len(reqs))]
+
+
+def serving(n: int, rate: float, slots: int, log):
+    """Solves/hour and time-to-converge (1e-8 density-residual drop) of the
+    served form (`slots` solves in flight) against the plain one-at-a-time
+    path (one slot) on the same requests: a Poisson stream at `rate`
+    requests/s (p50/p99 time-to-converge), and all requests queued at once
+    (saturated throughput)."""
+    reqs = serving_requests(n)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = {"requests": n, "rate_per_s": rate, "slots": slots, "drop": 1e-8,
+           "cells": [c.ncell for _, c in reqs], "kinds": [k for k, _ in reqs]}
+    for name, sl in (("served", slots), ("one_at_a_time", 1)):
+        sat = serve_run(reqs, sl, np.zeros(n))
+        span = max(r["t_done"] for r in sat) - min(r["t_submit"] for r in sat)
+        log(f"serving {name}: saturated {n} solves in {span:.2f} s")
+        st = serve_run(reqs, sl, arrivals)
+        ttc = np.array([r["t_done"] - r["t_submit"] for r in st])
+        out[name] = {"solves_per_hr_saturated": 3600.0 * n / span,
+                     "p50_ttc_s": float(np.percentile(ttc, 50)), "p99_ttc_s": float(np.percentile(ttc, 99)),
+                     "converged": int(sum(r["converged"] for r in st)),
+                     "iters_p50": float(np.median([r["iters"] for r in st])),
+                     "cell_iters_per_s_saturated": sum(r["iters"] * r["ncell"] for r in sat) / span}
+        log(f"serving {name}: stream p50 {out[name]['p50_ttc_s']:.3f} s p99 {out[name]['p99_ttc_s']:.3f} s")
+    out["speedup_solves_per_hr"] = (out["served"]["solves_per_hr_saturated"]
+                                    / out["one_at_a_time"]["solves_per_hr_saturated"])
+    return out
+
+
 def cpu_baseline(nx: int, ny: int, iters: int):
     """Reference OpenMP build (oracle/_ref/csu_ref, compiled from the
     reference sources) on the host cores: Nc*N / (T(N) - T(0))."""
@@ -114,6 +191,10 @@ def main():
     ap.add_argument("--ny", type=int, default=1000)
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-n", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 200x100x2 timing (profiling runs of the big mesh alone)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -158,6 +239,7 @@ def main():
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
             "iteration_alg_bytes_per_cell": 2334.5,
             "iteration_frac": value * 2334.5 / (HBM_PEAK_GBS * 1e9)}
+    serve = None if a.no_serve else serving(a.serve_n, a.serve_rate, a.serve_slots, log)
     cpu = None if a.no_cpu else cpu_baseline(200, 100, a.cpu_iters)
     line = {
         "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1,
@@ -169,6 +251,7 @@ def main():
                    "config0_40k_cell_iter_per_s":
                        None if small is None else small["nc"] * max(a.steps, 50) / small["seconds"]},
         "roofline": roof,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -20,6 +20,8 @@ CC = os.environ.get("CC", "gcc")
 
 HOST_SRC = ["input.c", "mesh.c", "case.c", "output.c"]
 HIP_SRC = ["csu_dev.hip"]
+CXX_SRC = ["csu_serve.cpp"]   # host C++ over the C ABI (no device code)
+CXX = os.environ.get("CXX", "g++")
 
 # -ffp-contract=off: the FP64 path must round every product as the
 # reference's x86-64 build does (no fused multiply-add).
@@ -53,6 +55,11 @@ def build(verbose: bool = False) -> str:
     for s in HOST_SRC:
         o = os.path.join(objdir, s.replace(".c", ".o"))
         _run([CC, *HOST_FLAGS, "-c", os.path.join(PKG, "host", s), "-o", o], verbose)
+        objs.append(o)
+    for s in CXX_SRC:
+        o = os.path.join(objdir, s.replace(".cpp", ".o"))
+        _run([CXX, "-O2", "-fPIC", "-std=c++17", "-Wall", "-I" + INC, "-c", os.path.join(PKG, "csrc", s), "-o", o],
+             verbose)
         objs.append(o)
     for s in HIP_SRC:
         o = os.path.join(objdir, s.replace(".hip", ".o"))

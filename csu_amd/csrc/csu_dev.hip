@@ -73,6 +73,17 @@ struct csu_dev {
   int* jstate;                 // DPLUR: [converged, current buffer]
   std::vector<int> lvlL, lvlU; // LU-SGS level offsets (forward / backward)
   int *cellsL, *cellsU;        // cells sorted by level
+  // iteration graphs (captured once, replayed): RK keyed by local time
+  // step x which buffer holds U; implicit keyed by nothing
+  int* hctr;                   // device iteration counter: slot of res/coef history
+  hipGraphExec_t gexec[4];
+  double* gU[4];               // d->U at capture
+  double* Ubase;               // the buffer U starts in (graph parity key)
+  size_t bytes;                // device bytes allocated by this context
+  // asynchronous iteration: history rows copied to pinned host memory
+  double *hres, *hcoef;
+  int hcap, async_start, async_n;
+  int use_graphs;
   // profiling
   int prof;
   std::vector<std::string> kname;
@@ -93,6 +104,7 @@ static int dalloc(csu_dev* d, T** p, size_t n) {
     return set_err(CSU_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
   }
   d->buf.owned.push_back((void*)*p);
+  d->bytes += n * sizeof(T);
   return CSU_OK;
 }
 
@@ -162,12 +174,25 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
     return set_err(CSU_ERR_UNSUPPORTED, "limiter 1 with SST: the reference's range arrays hold 5 variables (limiter.c:37-38)");
   if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);
   if (p->time_scheme == CSU_TIME_DPLUR && p->nlinear < 0) return set_err(CSU_ERR_ARG, "Nlinear %d", p->nlinear);
-  if (p->precision != CSU_PREC_FP64) return set_err(CSU_ERR_UNSUPPORTED, "mixed precision not built yet");
+  if (p->precision != CSU_PREC_FP64 && p->precision != CSU_PREC_FP64_LD)
+    return set_err(CSU_ERR_UNSUPPORTED, "mixed precision not built yet");
   return CSU_OK;
+}
+
+static void graphs_drop(csu_dev* d) {
+  for (int k = 0; k < 4; k++) {
+    if (d->gexec[k]) (void)hipGraphExecDestroy(d->gexec[k]);
+    d->gexec[k] = nullptr;
+    d->gU[k] = nullptr;
+  }
 }
 
 extern "C" void csu_dev_destroy(csu_dev* d) {
   if (!d) return;
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  graphs_drop(d);
+  if (d->hres) (void)hipHostFree(d->hres);
+  if (d->hcoef) (void)hipHostFree(d->hcoef);
   for (hipEvent_t e : d->evpool) (void)hipEventDestroy(e);
   for (auto& pe : d->pending) {
     (void)hipEventDestroy(pe.second.first);
@@ -276,6 +301,9 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   d->visc = (p->laminar || p->sa || p->sst) ? 1 : 0;
   d->hist_cap = 0; d->res_hist = d->coef_hist = nullptr; d->fresh = 1;
   d->Dp = d->dW0 = d->dW1 = d->A = nullptr; d->jstate = nullptr; d->cellsL = d->cellsU = nullptr;
+  d->hctr = nullptr; d->use_graphs = 1; d->bytes = 0;
+  d->hres = d->hcoef = nullptr; d->hcap = 0; d->async_start = 0; d->async_n = 0;
+  for (int k = 0; k < 4; k++) { d->gexec[k] = nullptr; d->gU[k] = nullptr; }
   memset((void*)&d->kp, 0, sizeof(KP));
   memset((void*)&d->geo, 0, sizeof(Geo));
   memset((void*)&d->st, 0, sizeof(St));
@@ -332,11 +360,59 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
     if (f > 0) cface[t] = finv[f - 1] + 1;
     else if (f < 0) cface[t] = -(finv[-f - 1] + 1);
   }
+  // Least-squares weights per neighbour slot (solverCalcGrad3, solver.c:1031-1074):
+  // gx = sum_j wx_j du_j with wx_j = (c dx_j - b dy_j)/det, wy_j = (a dy_j - b dx_j)/det,
+  // a, b, c, det in long double from the products the reference forms in double.
+  std::vector<double> lwx(4 * (size_t)nc, 0.0), lwy(4 * (size_t)nc, 0.0);
+  for (int i = 0; i < nc; i++) {
+    double dx[4], dy[4];
+    long double a = 0, b = 0, c = 0;
+    for (int j = 0; j < cnp[i]; j++) {
+      const int n = m->cell_nbr[4 * i + j];
+      double xn, yn;
+      if (n >= 0) { xn = xc[n]; yn = yc[n]; }
+      else {
+        const int32_t* q = m->bface + 3 * (-1 - n);
+        xn = (m->node_xy[2 * q[1]] + m->node_xy[2 * q[2]]) / 2;
+        yn = (m->node_xy[2 * q[1] + 1] + m->node_xy[2 * q[2] + 1]) / 2;
+      }
+      dx[j] = xn - xc[i];
+      dy[j] = yn - yc[i];
+      a += dx[j] * dx[j];
+      b += dx[j] * dy[j];
+      c += dy[j] * dy[j];
+    }
+    const long double det = a * c - b * b;
+    for (int j = 0; j < cnp[i]; j++) {
+      lwx[(size_t)j * nc + i] = (double)((c * dx[j] - b * dy[j]) / det);
+      lwy[(size_t)j * nc + i] = (double)((a * dy[j] - b * dx[j]) / det);
+    }
+  }
+  // edge index of a face in a cell's vertex loop (the limiter and the
+  // reconstruction offsets are per edge)
+  auto edge_of = [m](int cell, int p0, int p1) -> int {
+    const int32_t* v = m->cell_vert + 4 * cell;
+    const int np = v[3] >= 0 ? 4 : 3;
+    for (int j = 0; j < np; j++) {
+      const int a = v[j], b = v[(j + 1) % np];
+      if ((a == p0 && b == p1) || (a == p1 && b == p0)) return j;
+    }
+    return -1;
+  };
+  std::vector<unsigned char> fsl(nf > 0 ? nf : 1, 0);
   std::vector<int> fe0(nf), fe1(nf);
   std::vector<double> fdsx(nf), fdsy(nf), fds(nf), fdx0(nf), fdy0(nf), fdx1(nf), fdy1(nf);
   for (int f = 0; f < nf; f++) {
     const int32_t* q = m->face + 4 * ford[f];
     fe0[f] = q[0]; fe1[f] = q[1];
+    {
+      const int s0 = edge_of(q[0], q[2], q[3]), s1 = edge_of(q[1], q[2], q[3]);
+      if (s0 < 0 || s1 < 0) {
+        delete d;
+        return set_err(CSU_ERR_ARG, "face %d (%d-%d) is not an edge of its cells %d, %d", f, q[2], q[3], q[0], q[1]);
+      }
+      fsl[f] = (unsigned char)(s0 | (s1 << 2));
+    }
     double dSx, dSy;
     calc_ds(m, q[2], q[3], &dSx, &dSy);
     fdsx[f] = dSx; fdsy[f] = dSy; fds[f] = sqrt(dSx * dSx + dSy * dSy);
@@ -377,6 +453,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   kp.nc = nc; kp.nf = nf; kp.nb = nb; kp.nv = nv; kp.order = p->order; kp.flux = p->flux;
   kp.laminar = p->laminar; kp.sa = p->sa; kp.sst = p->sst; kp.limiter = p->limiter;
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
+  kp.lsq_ld = p->precision == CSU_PREC_FP64_LD; kp.store_grad = 0;
   kp.rlim = p->rlim; kp.pout = p->pout; kp.twall = p->twall; kp.efix = p->efix; kp.minf = p->mach_inf;
   kp.sst_owall = p->sst_owallfactor;
   {
@@ -417,6 +494,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
       (rc = upload(d, &g.xc, xc)) || (rc = upload(d, &g.yc, yc)) || (rc = upload(d, &g.omega, om)) ||
       (rc = upload(d, &g.dsxm, dsxm)) || (rc = upload(d, &g.dsym, dsym)) || (rc = upload(d, &g.dslat, dslat)) ||
       (rc = upload(d, &g.edx, edx)) || (rc = upload(d, &g.edy, edy)) || (rc = upload(d, &g.wdist, wd)) ||
+      (rc = upload(d, &g.lwx, lwx)) || (rc = upload(d, &g.lwy, lwy)) || (rc = upload(d, &g.fsl, fsl)) ||
       (rc = upload(d, &g.fe0, fe0)) || (rc = upload(d, &g.fe1, fe1)) || (rc = upload(d, &g.fdsx, fdsx)) ||
       (rc = upload(d, &g.fdsy, fdsy)) || (rc = upload(d, &g.fds, fds)) || (rc = upload(d, &g.fdx0, fdx0)) ||
       (rc = upload(d, &g.fdy0, fdy0)) || (rc = upload(d, &g.fdx1, fdx1)) || (rc = upload(d, &g.fdy1, fdy1)) ||
@@ -436,6 +514,10 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   }
   d->wall = const_cast<int*>(wall_d);
   s.pref2 = const_cast<double*>(pref_d);
+  if (p->order == 2 && (rc = dalloc(d, &s.rec, 4 * nvc))) {
+    csu_dev_destroy(d);
+    return rc;
+  }
   if ((p->sa || p->sst) && !m->wall_dist) {
     // saCalcD on the device when the caller brings no wall distance
     double* wdd = nullptr;
@@ -501,13 +583,21 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
       return rc;
     }
   }
+  d->Ubase = d->U;
+  if ((rc = dalloc(d, &d->hctr, 1))) {
+    csu_dev_destroy(d);
+    return rc;
+  }
+  HIPCHK(hipMemset(d->hctr, 0, sizeof(int)));
   // free-stream initial state (solverInitU)
   std::vector<double> U0(nvc);
   for (int k = 0; k < nv; k++)
     for (int i = 0; i < nc; i++) U0[(size_t)k * nc + i] = p->uin[k];
   HIPCHK(hipMemcpy(d->U, U0.data(), nvc * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(s.Pb, 0, sizeof(double) * (size_t)(nv + 1) * (nb ? nb : 1)));
-  HIPCHK(hipDeviceSynchronize());
+  // this context's work only: other contexts' streams keep running
+  HIPCHK(hipStreamSynchronize(nullptr));
+  HIPCHK(hipStreamSynchronize(d->stream));
   *out = d;
   return CSU_OK;
 }
@@ -587,6 +677,14 @@ static int prof_collect(csu_dev* d) {
   return CSU_OK;
 }
 
+extern "C" int csu_dev_set_graphs(csu_dev* d, int32_t on) {
+  if (!d) return set_err(CSU_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  if (!on) graphs_drop(d);
+  d->use_graphs = on ? 1 : 0;
+  return CSU_OK;
+}
+
 extern "C" int csu_dev_set_profiling(csu_dev* d, int32_t on) {
   if (!d) return set_err(CSU_ERR_ARG, "null argument");
   int rc = prof_collect(d);
@@ -624,7 +722,10 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
   }
   {
     Prof p(d, "k_grad");
-    hipLaunchKernelGGL((k_grad<NV, VISC>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
+    if (kp.lsq_ld)
+      hipLaunchKernelGGL((k_grad<NV, VISC, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
+    else
+      hipLaunchKernelGGL((k_grad<NV, VISC, false>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
   }
   if (d->nf > 0) {
     Prof p(d, "k_flux");
@@ -650,7 +751,9 @@ static int prim_pass(csu_dev* d) {
 template <int NV, bool VISC>
 static int residual_t(csu_dev* d) {
   int rc = prim_pass<NV>(d);
+  d->kp.store_grad = 1;  // the caller may ask for dPx, dPy, phi
   if (!rc) rc = stage<NV, VISC>(d, F_STORE_R, 0.0, 0.0, nullptr, nullptr);
+  d->kp.store_grad = 0;
   d->fresh = 1;
   return rc;
 }
@@ -700,54 +803,111 @@ static const double RK3[] = {0.1481, 0.4, 1.0};
 static const double RK4[] = {0.0833, 0.2069, 0.4265, 1.0};
 static const double RK5[] = {0.0533, 0.1263, 0.2375, 0.4414, 1.0};
 
+// Residual norms and force coefficients of the iteration just done, into
+// the history slot the device counter names (k_forces advances it).
+template <int NV>
+static void iteration_tail(csu_dev* d, int nblocks_cells) {
+  {
+    Prof p(d, "k_res");
+    hipLaunchKernelGGL(k_res_final<NV>, dim3(1), dim3(RES_THREADS), 0, d->stream, d->nc, nblocks_cells, d->red,
+                       d->res_hist, d->hist_cap, (const int*)d->hctr);
+  }
+  {
+    Prof p(d, "k_forces");
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->nwall, d->wall, d->st.Pb,
+                       d->st.wfric, d->geo.bdsx, d->geo.bdsy, d->prm.sref, d->visc, d->coef_hist, d->hist_cap,
+                       d->hctr);
+  }
+}
+
+static int is_local(const csu_dev* d, int it) {
+  return d->prm.dt_local == 1 && !(d->prm.dt_local_n >= 0 && it > d->prm.dt_local_n);
+}
+
+// Capture `body` (which enqueues work on d->stream) into graph slot k, or
+// replay slot k when it was captured with the same U buffer.
+template <typename F>
+static int run_graph(csu_dev* d, int k, F body) {
+  if (!d->gexec[k] || d->gU[k] != d->U) {
+    if (d->gexec[k]) (void)hipGraphExecDestroy(d->gexec[k]);
+    d->gexec[k] = nullptr;
+    double* u0 = d->U;
+    HIPCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    int rc = body();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(d->stream, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return set_err(CSU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&d->gexec[k], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) { d->gexec[k] = nullptr; return set_err(CSU_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+    d->gU[k] = u0;
+    d->U = u0;  // the body's host-side pointer swaps are replayed by the graph
+    // a capture records work without running it: replay once now
+  }
+  HIPCHK(hipGraphLaunch(d->gexec[k], d->stream));
+  return CSU_OK;
+}
+
+// One RK iteration: stages, buffer swap, next time step, norms, forces.
 template <int NV, bool VISC>
-static int iterate_t(csu_dev* d, int niter) {
-  const KP& kp = d->kp;
+static int rk_iteration(csu_dev* d, int it_index) {
   const int nc = d->nc;
   const double* a = d->prm.stages == 1 ? RK1 : d->prm.stages == 3 ? RK3 : d->prm.stages == 4 ? RK4 : RK5;
   const double sigma = 0.5 * d->prm.stages;
   double* red_dt = d->red + (size_t)NV * d->nred;
-  auto is_local = [d](int it) {
-    return d->prm.dt_local == 1 && !(d->prm.dt_local_n >= 0 && it > d->prm.dt_local_n);
-  };
-  for (int it = 0; it < niter; it++) {
-    const int cur_local = is_local(d->iter), next_local = is_local(d->iter + 1);
-    int first_flags = F_CLAMP_UN;
-    if (d->fresh) {
-      // time step from the unclamped state (solverCalcDt), then primitives
-      Prof p(d, "k_dt");
-      hipLaunchKernelGGL(k_dt<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, cur_local, d->prm.cfl,
-                         d->U, d->dtl, red_dt);
-      if (!cur_local)
-        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nblk(nc), d->prm.cfl, red_dt, d->dtg);
-      int rc = prim_pass<NV>(d);
-      if (rc) return rc;
-      first_flags = 0;
-      d->fresh = 0;
-    }
-    for (int s = 0; s < d->prm.stages; s++) {
-      const bool last = s == d->prm.stages - 1;
-      int flags = F_RK | (cur_local ? F_CUR_LOCAL : 0) | (s == 0 ? first_flags : 0);
-      if (last) flags |= F_LAST | F_STORE_R | (next_local ? F_NEXT_LOCAL : 0);
-      int rc = stage<NV, VISC>(d, flags, sigma, a[s], d->U, d->Uaux);
-      if (rc) return rc;
-    }
-    std::swap(d->U, d->Uaux);
-    if (!next_local)
+  const int cur_local = is_local(d, it_index), next_local = is_local(d, it_index + 1);
+  int first_flags = F_CLAMP_UN;
+  if (d->fresh) {
+    // time step from the unclamped state (solverCalcDt), then primitives
+    Prof p(d, "k_dt");
+    hipLaunchKernelGGL(k_dt<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->geo, cur_local, d->prm.cfl,
+                       d->U, d->dtl, red_dt);
+    if (!cur_local)
       hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nblk(nc), d->prm.cfl, red_dt, d->dtg);
-    const int slot = d->iter % d->hist_cap;
-    {
-      Prof p(d, "k_res");
-      hipLaunchKernelGGL(k_res_final<NV>, dim3(1), dim3(RES_THREADS), 0, d->stream, nc, nblk(nc), d->red,
-                         d->res_hist + (size_t)slot * NV);
+    int rc = prim_pass<NV>(d);
+    if (rc) return rc;
+    first_flags = 0;
+    d->fresh = 0;
+  }
+  for (int s = 0; s < d->prm.stages; s++) {
+    const bool last = s == d->prm.stages - 1;
+    int flags = F_RK | (cur_local ? F_CUR_LOCAL : 0) | (s == 0 ? first_flags : 0);
+    if (last) flags |= F_LAST | F_STORE_R | (next_local ? F_NEXT_LOCAL : 0);
+    int rc = stage<NV, VISC>(d, flags, sigma, a[s], d->U, d->Uaux);
+    if (rc) return rc;
+  }
+  std::swap(d->U, d->Uaux);
+  if (!next_local)
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nblk(nc), d->prm.cfl, red_dt, d->dtg);
+  iteration_tail<NV>(d, nblk(nc));
+  HIPCHK(hipGetLastError());
+  return CSU_OK;
+}
+
+template <int NV, bool VISC>
+static int iterate_t(csu_dev* d, int niter) {
+  int it = 0;
+  while (it < niter) {
+    const int i0 = d->iter;
+    const bool steady = is_local(d, i0) == is_local(d, i0 + 1) && is_local(d, i0 + 1) == is_local(d, i0 + 2);
+    if (d->use_graphs && !d->prof && !d->fresh && niter - it >= 2 && steady) {
+      // two iterations per graph, so U ends in the buffer it started in
+      const int k = (is_local(d, i0) ? 2 : 0) + (d->U == d->Ubase ? 0 : 1);
+      int rc = run_graph(d, k, [&]() -> int {
+        int r = rk_iteration<NV, VISC>(d, i0);
+        if (!r) r = rk_iteration<NV, VISC>(d, i0 + 1);
+        return r;
+      });
+      if (rc) return rc;
+      d->iter += 2;
+      it += 2;
+      continue;
     }
-    {
-      Prof p(d, "k_forces");
-      hipLaunchKernelGGL(k_forces, dim3(1), dim3(CSU_BLOCK), 0, d->stream, kp, d->nwall, d->wall, d->st.Pb,
-                         d->st.wfric, d->geo.bdsx, d->geo.bdsy, d->prm.sref, d->visc, d->coef_hist + (size_t)slot * 4);
-    }
-    HIPCHK(hipGetLastError());
+    int rc = rk_iteration<NV, VISC>(d, d->iter);
+    if (rc) return rc;
     d->iter++;
+    it++;
   }
   return CSU_OK;
 }
@@ -755,71 +915,75 @@ static int iterate_t(csu_dev* d, int niter) {
 // implicit iteration: solverCalcR, then LU-SGS (implicitCalcD + sweeps) or
 // DPLUR (implicitCalcDPLUR), then U += dW (solver.c:1871-1884)
 template <int NV, bool VISC>
-static int iterate_implicit(csu_dev* d, int niter) {
+static int implicit_iteration(csu_dev* d) {
   const KP& kp = d->kp;
   const int nc = d->nc, nb = nblk(nc);
   double* red_min = d->red + (size_t)NV * d->nred;
+  int rc = prim_pass<NV>(d);
+  if (!rc) rc = stage<NV, VISC>(d, F_STORE_R | F_RES, 0.0, 0.0, nullptr, nullptr);
+  if (rc) return rc;
+  {
+    Prof p(d, "k_impD");
+    hipLaunchKernelGGL((k_impD<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->prm.wimp,
+                       d->Dp, red_min);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, d->prm.cfl, red_min, d->dtg);
+  }
+  if (d->prm.time_scheme == CSU_TIME_LUSGS) {
+    Prof p(d, "k_lusgs");
+    for (size_t l = 0; l + 1 < d->lvlL.size(); l++) {
+      int off = d->lvlL[l], cnt = d->lvlL[l + 1] - off;
+      hipLaunchKernelGGL((k_lusgs<NV, VISC, 0>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+                         d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0);
+    }
+    hipLaunchKernelGGL(k_lusgs_uinit<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->Dp, d->dtg, d->dW0,
+                       d->dW1, d->st.dq);
+    for (size_t l = 0; l + 1 < d->lvlU.size(); l++) {
+      int off = d->lvlU[l], cnt = d->lvlU[l + 1] - off;
+      hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+                         d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1);
+    }
+    hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW1, d->dW1,
+                       (const int*)nullptr);
+  } else {
+    Prof p(d, "k_dplur");
+    hipLaunchKernelGGL((k_blocks<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->U, d->A);
+    hipLaunchKernelGGL(k_jac_reset, dim3(1), dim3(1), 0, d->stream, d->jstate);
+    for (int mm = 0; mm < d->prm.nlinear; mm++) {
+      hipLaunchKernelGGL(k_jacobi<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->A, d->Dp, d->dtg,
+                         d->R, d->dW0, d->dW1, d->jstate, red_min);
+      hipLaunchKernelGGL(k_jac_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, red_min, d->jstate);
+    }
+    hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW0, d->dW1,
+                       (const int*)d->jstate);
+  }
+  iteration_tail<NV>(d, nb);
+  HIPCHK(hipGetLastError());
+  return CSU_OK;
+}
+
+template <int NV, bool VISC>
+static int iterate_implicit(csu_dev* d, int niter) {
   for (int it = 0; it < niter; it++) {
-    int rc = prim_pass<NV>(d);
-    if (!rc) rc = stage<NV, VISC>(d, F_STORE_R | F_RES, 0.0, 0.0, nullptr, nullptr);
+    int rc = d->use_graphs && !d->prof ? run_graph(d, 0, [&]() { return implicit_iteration<NV, VISC>(d); })
+                                       : implicit_iteration<NV, VISC>(d);
     if (rc) return rc;
-    {
-      Prof p(d, "k_impD");
-      hipLaunchKernelGGL((k_impD<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->prm.wimp,
-                         d->Dp, red_min);
-      hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, d->prm.cfl, red_min, d->dtg);
-    }
-    if (d->prm.time_scheme == CSU_TIME_LUSGS) {
-      Prof p(d, "k_lusgs");
-      for (size_t l = 0; l + 1 < d->lvlL.size(); l++) {
-        int off = d->lvlL[l], cnt = d->lvlL[l + 1] - off;
-        hipLaunchKernelGGL((k_lusgs<NV, VISC, 0>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
-                           d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0);
-      }
-      hipLaunchKernelGGL(k_lusgs_uinit<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->Dp, d->dtg, d->dW0,
-                         d->dW1, d->st.dq);
-      for (size_t l = 0; l + 1 < d->lvlU.size(); l++) {
-        int off = d->lvlU[l], cnt = d->lvlU[l + 1] - off;
-        hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
-                           d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1);
-      }
-      hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW1, d->dW1,
-                         (const int*)nullptr);
-    } else {
-      Prof p(d, "k_dplur");
-      hipLaunchKernelGGL((k_blocks<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->U, d->A);
-      hipLaunchKernelGGL(k_jac_reset, dim3(1), dim3(1), 0, d->stream, d->jstate);
-      for (int mm = 0; mm < d->prm.nlinear; mm++) {
-        hipLaunchKernelGGL(k_jacobi<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->A, d->Dp, d->dtg,
-                           d->R, d->dW0, d->dW1, d->jstate, red_min);
-        hipLaunchKernelGGL(k_jac_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, red_min, d->jstate);
-      }
-      hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW0, d->dW1,
-                         (const int*)d->jstate);
-    }
-    const int slot = d->iter % d->hist_cap;
-    hipLaunchKernelGGL(k_res_final<NV>, dim3(1), dim3(RES_THREADS), 0, d->stream, nc, nb, d->red,
-                       d->res_hist + (size_t)slot * NV);
-    hipLaunchKernelGGL(k_forces, dim3(1), dim3(CSU_BLOCK), 0, d->stream, kp, d->nwall, d->wall, d->st.Pb, d->st.wfric,
-                       d->geo.bdsx, d->geo.bdsy, d->prm.sref, d->visc, d->coef_hist + (size_t)slot * 4);
-    HIPCHK(hipGetLastError());
     d->iter++;
   }
   d->fresh = 1;
   return CSU_OK;
 }
 
-extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* coef) {
-  if (!d) return set_err(CSU_ERR_ARG, "null argument");
-  if (niter < 0) return set_err(CSU_ERR_ARG, "niter %d", niter);
-  if (niter == 0) return CSU_OK;
+// Enqueue niter iterations on the context's stream (no host wait).
+static int enqueue_iterations(csu_dev* d, int niter) {
   if (d->hist_cap < niter) {
+    // a new history ring: device slots continue from the iteration count
+    HIPCHK(hipStreamSynchronize(d->stream));
+    graphs_drop(d);
     int rc;
     if ((rc = dalloc(d, &d->res_hist, (size_t)niter * d->nv)) || (rc = dalloc(d, &d->coef_hist, (size_t)niter * 4)))
       return rc;
     d->hist_cap = niter;
   }
-  const int start = d->iter;
   int rc;
   if (d->prm.time_scheme == CSU_TIME_RK)
     rc = d->nv == 6   ? iterate_t<6, true>(d, niter)
@@ -831,6 +995,15 @@ extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* c
          : d->nv == 5 ? iterate_implicit<5, true>(d, niter)
          : d->visc    ? iterate_implicit<4, true>(d, niter)
                       : iterate_implicit<4, false>(d, niter);
+  return rc;
+}
+
+extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* coef) {
+  if (!d) return set_err(CSU_ERR_ARG, "null argument");
+  if (niter < 0) return set_err(CSU_ERR_ARG, "niter %d", niter);
+  if (niter == 0) return CSU_OK;
+  const int start = d->iter;
+  int rc = enqueue_iterations(d, niter);
   if (rc) return rc;
   if (res || coef) {
     std::vector<double> rh((size_t)d->hist_cap * d->nv), ch((size_t)d->hist_cap * 4);
@@ -844,4 +1017,71 @@ extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* c
     }
   }
   return prof_collect(d);
+}
+
+extern "C" int csu_dev_iterate_async(csu_dev* d, int32_t niter) {
+  if (!d) return set_err(CSU_ERR_ARG, "null argument");
+  if (niter <= 0) return set_err(CSU_ERR_ARG, "niter %d", niter);
+  if (d->prof) return set_err(CSU_ERR_ARG, "asynchronous iterations with profiling on");
+  const int start = d->iter;
+  int rc = enqueue_iterations(d, niter);
+  if (rc) return rc;
+  if (d->hcap < d->hist_cap) {
+    HIPCHK(hipStreamSynchronize(d->stream));
+    if (d->hres) (void)hipHostFree(d->hres);
+    if (d->hcoef) (void)hipHostFree(d->hcoef);
+    d->hres = d->hcoef = nullptr;
+    d->hcap = 0;
+    if (hipHostMalloc((void**)&d->hres, sizeof(double) * (size_t)d->hist_cap * d->nv, 0) != hipSuccess ||
+        hipHostMalloc((void**)&d->hcoef, sizeof(double) * (size_t)d->hist_cap * 4, 0) != hipSuccess)
+      return set_err(CSU_ERR_NOMEM, "pinned history buffers");
+    d->hcap = d->hist_cap;
+  }
+  HIPCHK(hipMemcpyAsync(d->hres, d->res_hist, sizeof(double) * (size_t)d->hist_cap * d->nv, hipMemcpyDeviceToHost,
+                        d->stream));
+  HIPCHK(hipMemcpyAsync(d->hcoef, d->coef_hist, sizeof(double) * (size_t)d->hist_cap * 4, hipMemcpyDeviceToHost,
+                        d->stream));
+  d->async_start = start;
+  d->async_n = niter;
+  return CSU_OK;
+}
+
+extern "C" int csu_dev_ready(csu_dev* d) {
+  if (!d) return -set_err(CSU_ERR_ARG, "null argument");
+  hipError_t e = hipStreamQuery(d->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return -set_err(CSU_ERR_HIP, "stream: %s", hipGetErrorString(e));
+}
+
+extern "C" int csu_dev_async_result(csu_dev* d, double* res, double* coef) {
+  if (!d) return set_err(CSU_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  for (int it = 0; it < d->async_n; it++) {
+    int slot = (d->async_start + it) % d->hist_cap;
+    if (res) memcpy(res + (size_t)it * d->nv, d->hres + (size_t)slot * d->nv, sizeof(double) * d->nv);
+    if (coef) memcpy(coef + (size_t)it * 4, d->hcoef + (size_t)slot * 4, sizeof(double) * 4);
+  }
+  return CSU_OK;
+}
+
+extern "C" int64_t csu_dev_bytes(const csu_dev* d) { return d ? (int64_t)d->bytes : -1; }
+
+// Device bytes csu_dev_create will allocate (plus a history ring of
+// hist iterations), from the allocation list of csu_dev_create.
+extern "C" int64_t csu_dev_footprint(const csu_mesh_desc* m, const csu_params* p, int32_t hist) {
+  if (!m || !p) return -1;
+  const double nc = m->ncell, nf = m->nface, nb = m->nbface, nv = p->nvar;
+  const bool visc = p->laminar || p->sa || p->sst;
+  double b = 0;
+  b += nc * (4 + 16 + 16 + 48 + 64 + 64 + 8) + nf * (8 + 1 + 56) + nb * (12 + 64 + 4);   // geometry
+  b += nc * (24 * nv + 8) + 2 * nv * 8 * (nc / 256 + 1) + 128;                           // U, Uaux, R, dt, reductions
+  b += (nv + 1) * 8 * (nc + nb) + 24 * nv * nc + 8 * nv * nf;                             // P, Pb, grad, phi, fflux
+  if (p->order == 2) b += 32 * nv * nc;                                                   // edge states
+  if (visc) b += nc * (16 + 16 + 48) + nf * (8 * (nv - 1) + 8) + nb * (8 * (nv - 1) + 16);
+  if (p->time_scheme != CSU_TIME_RK) b += nc * (8 + 16 * nv + 8);
+  if (p->time_scheme == CSU_TIME_DPLUR) b += 32 * nv * nv * nc;
+  b += (nv + 4) * 8 * (double)(hist > 0 ? hist : 1);
+  // allocation granularity: every array rounds up (at most 2 MiB slack per large array)
+  return (int64_t)(b * 1.02) + (int64_t)64 * 4096;
 }

@@ -1,0 +1,207 @@
+// csu_serve.cpp -- continuous-batching solve server (host code).
+//
+// Each admitted solve owns a csu_dev context and its stream.  A step admits
+// queued solves while slots and the HBM budget allow (first come, first
+// served), keeps a chunk of iterations enqueued on every active context,
+// and retires solves whose density residual has dropped by the requested
+// factor.  Contexts are independent, so the GPU overlaps their kernels and
+// each solve's numbers do not depend on what else is running.
+#include <chrono>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+#include <string.h>
+#include "csu.h"
+#include "csu_serve.h"
+
+namespace {
+
+struct Req {
+  int id;
+  const csu_mesh_desc* mesh;
+  const csu_params* prm;
+  const double* U0;
+  double drop;
+  int max_iter;
+  int keep;
+  double t_submit;
+};
+
+struct Job {
+  Req r;
+  csu_dev* dev = nullptr;
+  csu_job_result res{};
+  int chunk_n = 0;  // iterations of the chunk in flight
+  bool res0_set = false;
+};
+
+}  // namespace
+
+struct csu_server {
+  int max_slots, chunk;
+  int64_t budget, used = 0;
+  int next_id = 0;
+  std::chrono::steady_clock::time_point t0;
+  std::deque<Req> queue;
+  std::vector<Job*> active;
+  std::deque<csu_job_result> done;
+  std::map<int, std::vector<double>> states;
+  std::vector<double> rbuf, cbuf;
+};
+
+static double now(const csu_server* s) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+extern "C" double csu_server_clock(const csu_server* s) { return s ? now(s) : -1.0; }
+
+extern "C" int csu_server_create(int32_t max_slots, int64_t hbm_budget, int32_t chunk, csu_server** out) {
+  if (!out || max_slots <= 0 || hbm_budget <= 0 || chunk <= 0) return CSU_ERR_ARG;
+  csu_server* s = new csu_server();
+  s->max_slots = max_slots;
+  s->budget = hbm_budget;
+  s->chunk = chunk;
+  s->t0 = std::chrono::steady_clock::now();
+  *out = s;
+  return CSU_OK;
+}
+
+extern "C" void csu_server_destroy(csu_server* s) {
+  if (!s) return;
+  for (Job* j : s->active) {
+    csu_dev_destroy(j->dev);
+    delete j;
+  }
+  delete s;
+}
+
+extern "C" int csu_server_submit(csu_server* s, const csu_mesh_desc* mesh, const csu_params* prm, const double* U0,
+                                 double drop, int32_t max_iter, int32_t keep_state, int32_t* id) {
+  if (!s || !mesh || !prm || !(drop > 0) || max_iter <= 0) return CSU_ERR_ARG;
+  Req r{s->next_id++, mesh, prm, U0, drop, max_iter, keep_state ? 1 : 0, now(s)};
+  s->queue.push_back(r);
+  if (id) *id = r.id;
+  return CSU_OK;
+}
+
+extern "C" int csu_server_pending(const csu_server* s, int32_t* queued, int32_t* active) {
+  if (!s) return CSU_ERR_ARG;
+  if (queued) *queued = (int32_t)s->queue.size();
+  if (active) *active = (int32_t)s->active.size();
+  return CSU_OK;
+}
+
+static void retire(csu_server* s, size_t k, int status) {
+  Job* j = s->active[k];
+  j->res.status = status;
+  j->res.t_done = now(s);
+  if (j->dev) {
+    j->res.bytes = csu_dev_bytes(j->dev);
+    if (j->r.keep && status == CSU_OK) {
+      std::vector<double> U((size_t)j->r.prm->nvar * j->r.mesh->ncell);
+      if (csu_dev_get_state(j->dev, U.data()) == CSU_OK) s->states[j->r.id] = std::move(U);
+    }
+    s->used -= csu_dev_bytes(j->dev);
+    csu_dev_destroy(j->dev);
+  }
+  s->done.push_back(j->res);
+  delete j;
+  s->active.erase(s->active.begin() + (long)k);
+}
+
+static int enqueue_chunk(csu_server* s, Job* j) {
+  int n = j->r.max_iter - j->res.iters;
+  if (n > s->chunk) n = s->chunk;
+  j->chunk_n = n;
+  return csu_dev_iterate_async(j->dev, n);
+}
+
+static void admit(csu_server* s) {
+  while (!s->queue.empty() && (int)s->active.size() < s->max_slots) {
+    const Req& r = s->queue.front();
+    const int64_t est = csu_dev_footprint(r.mesh, r.prm, s->chunk);
+    // a solve larger than the whole budget still runs, alone
+    if (!s->active.empty() && s->used + est > s->budget) break;
+    Job* j = new Job();
+    j->r = r;
+    s->queue.pop_front();
+    j->res.id = r.id;
+    j->res.ncell = r.mesh->ncell;
+    j->res.t_submit = r.t_submit;
+    j->res.t_admit = now(s);
+    j->res.iter_conv = -1;
+    s->active.push_back(j);
+    int rc = csu_dev_create(r.mesh, r.prm, &j->dev);
+    if (rc) { j->dev = nullptr; retire(s, s->active.size() - 1, rc); continue; }
+    s->used += csu_dev_bytes(j->dev);
+    if (r.U0 && (rc = csu_dev_set_state(j->dev, r.U0))) { retire(s, s->active.size() - 1, rc); continue; }
+    if ((rc = enqueue_chunk(s, j))) { retire(s, s->active.size() - 1, rc); continue; }
+  }
+}
+
+// A finished chunk: read its history, decide, and feed the next chunk.
+static bool harvest(csu_server* s, size_t k) {
+  Job* j = s->active[k];
+  const int nv = j->r.prm->nvar;
+  s->rbuf.resize((size_t)j->chunk_n * nv);
+  s->cbuf.resize((size_t)j->chunk_n * 4);
+  int rc = csu_dev_async_result(j->dev, s->rbuf.data(), s->cbuf.data());
+  if (rc) { retire(s, k, rc); return true; }
+  for (int it = 0; it < j->chunk_n; it++) {
+    const double r0 = s->rbuf[(size_t)it * nv];
+    j->res.iters++;
+    if (!j->res0_set) { j->res.res0 = r0; j->res0_set = true; }
+    j->res.res_last = r0;
+    memcpy(j->res.coef_last, &s->cbuf[(size_t)it * 4], sizeof(double) * 4);
+    if (j->res.iter_conv < 0 && r0 <= j->r.drop * j->res.res0) {
+      j->res.iter_conv = j->res.iters;
+      j->res.converged = 1;
+      break;
+    }
+  }
+  if (j->res.converged || j->res.iters >= j->r.max_iter) { retire(s, k, CSU_OK); return true; }
+  if ((rc = enqueue_chunk(s, j))) { retire(s, k, rc); return true; }
+  return false;
+}
+
+extern "C" int csu_server_step(csu_server* s, double max_wait) {
+  if (!s) return -CSU_ERR_ARG;
+  admit(s);
+  const double t_end = now(s) + (max_wait > 0 ? max_wait : 0);
+  int retired = 0;
+  bool any = false;
+  while (!s->active.empty()) {
+    for (size_t k = 0; k < s->active.size();) {
+      int st = csu_dev_ready(s->active[k]->dev);
+      if (st < 0) { retire(s, k, -st); retired++; any = true; continue; }
+      if (st == 1) {
+        any = true;
+        if (harvest(s, k)) { retired++; continue; }
+      }
+      k++;
+    }
+    if (retired) admit(s);
+    if (any || now(s) >= t_end) break;
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+  return retired;
+}
+
+extern "C" int csu_server_results(csu_server* s, csu_job_result* out, int32_t max) {
+  if (!s || (!out && max > 0)) return -CSU_ERR_ARG;
+  int n = 0;
+  while (n < max && !s->done.empty()) {
+    out[n++] = s->done.front();
+    s->done.pop_front();
+  }
+  return n;
+}
+
+extern "C" int csu_server_state(const csu_server* s, int32_t id, double* U) {
+  if (!s || !U) return CSU_ERR_ARG;
+  auto it = s->states.find(id);
+  if (it == s->states.end()) return CSU_ERR_ARG;
+  memcpy(U, it->second.data(), it->second.size() * sizeof(double));
+  return CSU_OK;
+}

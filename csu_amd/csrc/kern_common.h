@@ -11,17 +11,32 @@ namespace csu {
 // Scalar options and free-stream data, passed by value.
 struct KP {
   int nc, nf, nb, nv, order, flux, laminar, sa, sst, limiter, turb1order, axi, visc_blazek;
+  int lsq_ld;     // 1: long double LSQ sums emulated per call; 0: precomputed weights
+  int store_grad; // write dPx/dPy/phi even when no viscous term reads them
   double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t, wimp;
   double sst_owall, sst_g1, sst_g2;  // SST: oWallFactor, blended gamma constants (sst.c:18-38)
   double pin[7];
   GasD gas;
 };
 
+// Workgroup -> work-tile mapping that gives each of the 8 XCDs (one L2 each)
+// a contiguous range of tiles.  Workgroups are dispatched round-robin over
+// the XCDs, so physical block b runs on XCD b % 8; with this map XCD x
+// processes tiles [x*per, (x+1)*per) and neighbour gathers stay in its L2.
+// The tail (nblocks % 8 blocks) maps to itself.  A bijection on [0, nblocks).
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+  const int per = nblocks >> 3;
+  if (b >= (per << 3)) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
 // Geometry (device pointers, read-only), passed by value.
 struct Geo {
   const int *cnp, *cnbr, *cface;                        // [nc], [4][nc], [4][nc]
   const double *xc, *yc, *omega, *dsxm, *dsym, *dslat;  // [nc]
   const double *edx, *edy;                              // [4][nc] own-edge midpoint offsets
+  const double *lwx, *lwy;                              // [4][nc] LSQ weights per neighbour slot
+  const unsigned char* fsl;                             // [nf] edge of the face in e0 (bits 0-1), e1 (2-3)
   const double *wdist;                                  // [nc] wall distance (SA/SST)
   const int *fe0, *fe1;                                 // [nf]
   const double *fdsx, *fdsy, *fds, *fdx0, *fdy0, *fdx1, *fdy1;
@@ -35,6 +50,7 @@ struct St {
   double *fflux, *vflux, *bvflux, *wfric, *pref2, *miT;
   double *src;          // SA: Qt*Omega [nc]; SST: Qtk*Omega, Qto*Omega [2][nc]
   double *dq;           // SST source Jacobians dQk/d(r, rk, ro), dQo/d(r, rk, ro) [6][nc]
+  double *rec;          // second order: reconstructed state at each cell edge [4][NV][nc]
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference

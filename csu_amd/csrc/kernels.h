@@ -61,10 +61,10 @@ __device__ __forceinline__ void lsq_grad(int np, const double* dx, const double*
 // limiter.c:98-147), boundaryCalcPrimitive (boundary.c:514-693),
 // boundaryFaceViscFlux (laminar.c:128-346), boundaryCalcFrictionWall
 // (boundary.c:695-742).
-template <int NV, bool VISC>
+template <int NV, bool VISC, bool LD>
 __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   constexpr int NS = NV + 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int nc = kp.nc, nb = kp.nb;
   if (i >= nc) return;
   const int np = g.cnp[i];
@@ -72,39 +72,73 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   double Pi[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) Pi[m] = s.P[m * nc + i];
-  double Pn[4][NS];
+  // neighbours: cells, or ghost states of boundary faces (written to Pb
+  // here and read back per variable below by this same thread)
   double dx[4], dy[4];
-  int gb[4];
+  int gb[4], nbr[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     gb[j] = -1;
+    nbr[j] = 0;
     if (j >= np) break;
     int n = g.cnbr[j * nc + i];
+    nbr[j] = n;
     if (n >= 0) {
       dx[j] = g.xc[n] - x0;
       dy[j] = g.yc[n] - y0;
-#pragma unroll
-      for (int m = 0; m < NS; m++) Pn[j][m] = s.P[m * nc + n];
     } else {
       int q = -1 - n;
       gb[j] = q;
       dx[j] = g.bxm[q] - x0;
       dy[j] = g.bym[q] - y0;
-      ghost_state<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, NV == 6 ? g.wdist[i] : 0.0, Pn[j]);
+      double Pg[NS];
+      ghost_state<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, NV == 6 ? g.wdist[i] : 0.0, Pg);
 #pragma unroll
-      for (int m = 0; m < NS; m++) s.Pb[m * nb + q] = Pn[j][m];
+      for (int m = 0; m < NS; m++) s.Pb[m * nb + q] = Pg[m];
     }
   }
-  dd a = {0, 0}, b = {0, 0}, c = {0, 0};
+  auto nval = [&](int j, int m) -> double {
+    const int n = nbr[j];
+    return n >= 0 ? s.P[m * nc + n] : s.Pb[m * nb + (-1 - n)];
+  };
+  // Least squares (solver.c:1031-1074).  Default: gx = sum_j wx_j du_j with
+  // the weights (c dx_j - b dy_j)/det, (a dy_j - b dx_j)/det precomputed per
+  // cell in long double.  LD: the normal-equation sums of every call in
+  // double-double, as the reference's x87 long double accumulators do.
+  dd a = {0, 0}, b = {0, 0}, c = {0, 0}, det = {0, 0}, nb_ = {0, 0};
+  double wx[4], wy[4];
+  if (LD) {
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    if (j >= np) break;
-    a = dd_add_d(a, dx[j] * dx[j]);
-    b = dd_add_d(b, dx[j] * dy[j]);
-    c = dd_add_d(c, dy[j] * dy[j]);
+    for (int j = 0; j < 4; j++) {
+      if (j >= np) break;
+      a = dd_add_d(a, dx[j] * dx[j]);
+      b = dd_add_d(b, dx[j] * dy[j]);
+      c = dd_add_d(c, dy[j] * dy[j]);
+    }
+    det = dd_sub(dd_mul(a, c), dd_mul(b, b));
+    nb_ = dd_neg(b);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      wx[j] = j < np ? g.lwx[j * nc + i] : 0.0;
+      wy[j] = j < np ? g.lwy[j * nc + i] : 0.0;
+    }
   }
-  const dd det = dd_sub(dd_mul(a, c), dd_mul(b, b));
-  const dd nb_ = dd_neg(b);
+  auto grad = [&](const double* du, double& gx, double& gy) {
+    if (LD) {
+      lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gx, gy);
+    } else {
+      gx = wx[0] * du[0];
+      gy = wy[0] * du[0];
+#pragma unroll
+      for (int j = 1; j < 4; j++) {
+        if (j >= np) break;
+        gx += wx[j] * du[j];
+        gy += wy[j] * du[j];
+      }
+    }
+  };
+  const bool store = VISC || kp.store_grad;
   double lim_aux = 0.0;
   if (kp.limiter == 0) {
     lim_aux = kp.limK * sqrt(g.omega[i] / kp.volmax);
@@ -115,13 +149,18 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   for (int kk = 0; kk < NV; kk++) {
     const int m = kk > 3 ? kk + 1 : kk;
     const double u0 = Pi[m];
-    double du[4];
+    double vn[4], du[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) du[j] = j < np ? Pn[j][m] - u0 : 0.0;
+    for (int j = 0; j < 4; j++) {
+      vn[j] = j < np ? nval(j, m) : u0;
+      du[j] = vn[j] - u0;
+    }
     double gx, gy;
-    lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gx, gy);
-    s.dPx[kk * nc + i] = gx;
-    s.dPy[kk * nc + i] = gy;
+    grad(du, gx, gy);
+    if (store) {
+      s.dPx[kk * nc + i] = gx;
+      s.dPy[kk * nc + i] = gy;
+    }
     if (kk == 0) { grx = gx; gry = gy; }
     if (kk == 1) { gux = gx; guy = gy; }
     if (kk == 2) { gvx = gx; gvy = gy; }
@@ -133,26 +172,33 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
-        Pmax = fmax(Pmax, Pn[j][m]);
-        Pmin = fmin(Pmin, Pn[j][m]);
+        Pmax = fmax(Pmax, vn[j]);
+        Pmin = fmin(Pmin, vn[j]);
       }
-      double ph = 0.0;
+      double ph = 0.0, d2[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
-        double d2 = gx * g.edx[j * nc + i] + gy * g.edy[j * nc + i];
-        double p0 = limiter_v2<double>(u0, Pmin, Pmax, d2, pref2);
+        d2[j] = gx * g.edx[j * nc + i] + gy * g.edy[j * nc + i];
+        double p0 = limiter_v2<double>(u0, Pmin, Pmax, d2[j], pref2);
         ph = (j == 0) ? p0 : fmin(ph, p0);
       }
-      s.phi[kk * nc + i] = ph;
+      if (store) s.phi[kk * nc + i] = ph;
+      // the state each face of this cell sees on this side (solver.c:598-603)
+      const bool first = kk > 3 && kp.turb1order;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (j >= np) break;
+        s.rec[(j * NV + kk) * nc + i] = first ? u0 : u0 + d2[j] * ph;
+      }
     }
   }
   if (VISC) {
     double du[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) du[j] = j < np ? Pn[j][4] - Pi[4] : 0.0;
+    for (int j = 0; j < 4; j++) du[j] = j < np ? nval(j, 4) - Pi[4] : 0.0;
     double gTx, gTy;
-    lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gTx, gTy);
+    grad(du, gTx, gTy);
     s.dTx[i] = gTx;
     s.dTy[i] = gTy;
     const double gx3[5] = {gux, gvx, gTx, gnx, gox}, gy3[5] = {guy, gvy, gTy, gny, goy};
@@ -260,24 +306,19 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, co
 // for viscous runs the laminar face flux (laminar.c:17-75).
 template <int NV, bool VISC>
 __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int nc = kp.nc, nf = kp.nf;
   if (f >= nf) return;
   const int e0 = g.fe0[f], e1 = g.fe1[f];
   const double dSx = g.fdsx[f], dSy = g.fdsy[f], dS = g.fds[f];
   double PL[NV], PR[NV];
   if (kp.order == 2) {
-    const double dx0 = g.fdx0[f], dy0 = g.fdy0[f], dx1 = g.fdx1[f], dy1 = g.fdy1[f];
+    // both sides' states were reconstructed at this edge by k_grad
+    const int sl = g.fsl[f], s0 = sl & 3, s1 = sl >> 2;
 #pragma unroll
     for (int kk = 0; kk < NV; kk++) {
-      const int m = kk > 3 ? kk + 1 : kk;
-      if (kk > 3 && kp.turb1order) {
-        PL[kk] = s.P[m * nc + e0];
-        PR[kk] = s.P[m * nc + e1];
-      } else {
-        PL[kk] = s.P[m * nc + e0] + (s.dPx[kk * nc + e0] * dx0 + s.dPy[kk * nc + e0] * dy0) * s.phi[kk * nc + e0];
-        PR[kk] = s.P[m * nc + e1] + (s.dPx[kk * nc + e1] * dx1 + s.dPy[kk * nc + e1] * dy1) * s.phi[kk * nc + e1];
-      }
+      PL[kk] = s.rec[(s0 * NV + kk) * nc + e0];
+      PR[kk] = s.rec[(s1 * NV + kk) * nc + e1];
     }
   } else {
 #pragma unroll
@@ -356,7 +397,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
                                                       double* __restrict__ Uout, double* __restrict__ Rout,
                                                       double* __restrict__ red_res, double* __restrict__ red_dt) {
   __shared__ double sred[NV + 1][CSU_BLOCK];
-  const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int t = threadIdx.x, i = tile * blockDim.x + t;
   const int nc = kp.nc, nf = kp.nf, nb = kp.nb;
   const bool active = i < nc;
   double R[NV];
@@ -528,8 +570,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
   }
   if (t == 0) {
 #pragma unroll
-    for (int kk = 0; kk < NV; kk++) red_res[kk * gridDim.x + blockIdx.x] = sred[kk][0];
-    red_dt[blockIdx.x] = sred[NV][0];
+    for (int kk = 0; kk < NV; kk++) red_res[kk * gridDim.x + tile] = sred[kk][0];
+    red_dt[tile] = sred[NV][0];
   }
 }
 
@@ -578,8 +620,12 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_dt_final(int nblocks, double cfl,
 // slice of the block partials for all variables at once, then a fixed tree.
 #define RES_THREADS 1024
 template <int NV>
+// res_hist slot = ctr % cap: the iteration counter lives on the device so
+// that a captured iteration graph writes each replay's norms to its own slot.
 __global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, const double* __restrict__ red,
-                                                           double* __restrict__ res) {
+                                                           double* __restrict__ res_hist, int cap,
+                                                           const int* __restrict__ ctr) {
+  double* res = res_hist + (size_t)(ctr[0] % cap) * NV;
   __shared__ double s[NV][RES_THREADS];
   const int t = threadIdx.x;
   double a[NV];
@@ -607,7 +653,9 @@ __global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, 
 __global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const int* __restrict__ wall,
                                                       const double* __restrict__ Pb, const double* __restrict__ wfric,
                                                       const double* __restrict__ bdsx, const double* __restrict__ bdsy,
-                                                      double sref, int visc, double* __restrict__ coef) {
+                                                      double sref, int visc, double* __restrict__ coef_hist, int cap,
+                                                      int* __restrict__ ctr) {
+  double* coef = coef_hist + (size_t)(ctr[0] % cap) * 4;
   __shared__ double sh[4][CSU_BLOCK];
   int t = threadIdx.x;
   const double r = kp.pin[0], u = kp.pin[1], v = kp.pin[2], Pinf = kp.pin[3];
@@ -634,6 +682,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const in
     double c[4] = {sh[0][0] / sref, sh[1][0] / sref, sh[2][0] / sref, sh[3][0] / sref};
     if (kp.axi) { c[0] *= 2 * M_PI; c[1] *= 2 * M_PI; }
     for (int k = 0; k < 4; k++) coef[k] = c[k];
+    ctr[0] = ctr[0] + 1;  // last kernel of an iteration: advance the history slot
   }
 }
 

@@ -62,6 +62,13 @@ class Case(C.Structure):
 _lib = None
 
 
+class JobResult(C.Structure):
+    _fields_ = [("id", C.c_int32), ("status", C.c_int32), ("converged", C.c_int32), ("iters", C.c_int32),
+                ("iter_conv", C.c_int32), ("ncell", C.c_int32), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("res0", C.c_double), ("res_last", C.c_double),
+                ("coef_last", C.c_double * 4), ("bytes", C.c_int64)]
+
+
 def lib():
     """Load libcsu.so (build it first with csu_amd.build)."""
     global _lib
@@ -83,6 +90,7 @@ def lib():
     L.csu_dev_iteration.argtypes = [C.c_void_p]
     L.csu_dev_sync.argtypes = [C.c_void_p]
     L.csu_dev_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    L.csu_dev_set_graphs.argtypes = [C.c_void_p, C.c_int32]
     L.csu_dev_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_char_p), f64p, i32p, C.c_int32]
     L.csu_case_load.argtypes = [C.c_char_p, C.POINTER(C.POINTER(Case))]
     L.csu_case_free.argtypes = [C.POINTER(Case)]
@@ -103,6 +111,24 @@ def lib():
     L.csu_write_solution.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_wall_distance.argtypes = [C.POINTER(Case)]
     L.csu_wall_distance_dev.argtypes = [C.POINTER(MeshDesc), f64p]
+    L.csu_dev_iterate_async.argtypes = [C.c_void_p, C.c_int32]
+    L.csu_dev_ready.argtypes = [C.c_void_p]
+    L.csu_dev_async_result.argtypes = [C.c_void_p, f64p, f64p]
+    L.csu_dev_bytes.argtypes = [C.c_void_p]
+    L.csu_dev_bytes.restype = C.c_int64
+    L.csu_dev_footprint.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.c_int32]
+    L.csu_dev_footprint.restype = C.c_int64
+    L.csu_server_create.argtypes = [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
+    L.csu_server_destroy.argtypes = [C.c_void_p]
+    L.csu_server_destroy.restype = None
+    L.csu_server_submit.argtypes = [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(Params), f64p, C.c_double,
+                                    C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    L.csu_server_step.argtypes = [C.c_void_p, C.c_double]
+    L.csu_server_results.argtypes = [C.c_void_p, C.POINTER(JobResult), C.c_int32]
+    L.csu_server_pending.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.csu_server_state.argtypes = [C.c_void_p, C.c_int32, f64p]
+    L.csu_server_clock.argtypes = [C.c_void_p]
+    L.csu_server_clock.restype = C.c_double
     _lib = L
     return L
 
@@ -265,6 +291,9 @@ class Device:
     def set_profiling(self, on: bool):
         check(lib().csu_dev_set_profiling(self._h, 1 if on else 0))
 
+    def set_graphs(self, on: bool):
+        check(lib().csu_dev_set_graphs(self._h, 1 if on else 0))
+
     def kernel_times(self) -> dict:
         names = (C.c_char_p * 64)()
         ms = np.zeros(64)
@@ -282,6 +311,69 @@ class Device:
         if self._h:
             lib().csu_dev_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching solve server (include/csu_serve.h).  Cases passed
+    to submit() are kept alive by the server object until it is closed."""
+
+    def __init__(self, max_slots: int = 4, hbm_budget: float = 250e9, chunk: int = 25):
+        self._h = C.c_void_p()
+        check(lib().csu_server_create(max_slots, int(hbm_budget), chunk, C.byref(self._h)))
+        self._keep = {}
+
+    def submit(self, case: "HostCase", U0=None, drop: float = 1e-8, max_iter: int = 5000,
+               keep_state: bool = False) -> int:
+        jid = C.c_int32()
+        u = None
+        if U0 is not None:
+            u = np.ascontiguousarray(U0, dtype=np.float64)
+        check(lib().csu_server_submit(self._h, C.byref(case.mesh), C.byref(case._c.contents.prm),
+                                      _p(u, C.c_double) if u is not None else None, drop, max_iter,
+                                      1 if keep_state else 0, C.byref(jid)))
+        self._keep[jid.value] = (case, u)
+        return jid.value
+
+    def step(self, max_wait: float = 0.01) -> int:
+        n = lib().csu_server_step(self._h, max_wait)
+        if n < 0:
+            raise CsuError(-n, lib().csu_last_error().decode(errors="replace"))
+        return n
+
+    def results(self) -> list:
+        buf = (JobResult * 256)()
+        out = []
+        while True:
+            n = lib().csu_server_results(self._h, buf, 256)
+            out += [{k: (list(getattr(buf[i], k)) if k == "coef_last" else getattr(buf[i], k))
+                     for k, _ in JobResult._fields_} for i in range(n)]
+            if n < 256:
+                return out
+
+    def pending(self):
+        q, a = C.c_int32(), C.c_int32()
+        check(lib().csu_server_pending(self._h, C.byref(q), C.byref(a)))
+        return q.value, a.value
+
+    def state(self, jid: int, nvar: int, ncell: int) -> np.ndarray:
+        U = np.empty((nvar, ncell), np.float64)
+        check(lib().csu_server_state(self._h, jid, _p(U, C.c_double)))
+        return U
+
+    def clock(self) -> float:
+        return lib().csu_server_clock(self._h)
+
+    def close(self):
+        if self._h:
+            lib().csu_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep.clear()
 
     def __del__(self):
         try:

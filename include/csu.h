@@ -44,9 +44,14 @@ enum { CSU_FLUX_ROE = 0, CSU_FLUX_AUSM = 1, CSU_FLUX_AUSMDV = 2,
 /* time schemes, reference solver.c:1957-1981 */
 enum { CSU_TIME_RK = 0, CSU_TIME_LUSGS = 1, CSU_TIME_DPLUR = 2 };
 /* arithmetic of the residual path */
-enum { CSU_PREC_FP64 = 0,   /* everything FP64 (parity path)            */
-       CSU_PREC_MIXED = 1 };/* FP32 reconstruction/gradient/limiter/flux,
+enum { CSU_PREC_FP64 = 0,   /* everything FP64; least-squares gradients
+                               from per-cell weights precomputed in long
+                               double (parity path)                      */
+       CSU_PREC_MIXED = 1,  /* FP32 reconstruction/gradient/limiter/flux,
                                FP64 state, accumulation, update, norms   */
+       CSU_PREC_FP64_LD = 2 };/* FP64 with the reference's long double
+                               least-squares sums emulated per call
+                               (double-double), solver.c:1041-1073       */
 
 /* Mesh after the reference's preprocessing (mesh.c:115-266): cells CCW,
  * interior faces in the reference's enumeration order, boundary faces
@@ -120,9 +125,30 @@ int  csu_dev_residual(csu_dev* dev, double* R, double* dPx, double* dPy,
  * Iterations are counted across calls (the dt_local_n switch uses it). */
 int  csu_dev_iterate(csu_dev* dev, int32_t niter, double* res, double* coef);
 
+/* Asynchronous form of csu_dev_iterate for callers that keep several
+ * contexts busy at once: enqueue niter iterations on the context's own
+ * stream and return; csu_dev_ready reports 1 once they (and the copy of
+ * their history) are done, 0 while running; csu_dev_async_result waits and
+ * returns the histories of that call ([niter][nvar], [niter][4]; NULL
+ * allowed).  No counterpart in the reference (one solve per process). */
+int  csu_dev_iterate_async(csu_dev* dev, int32_t niter);
+int  csu_dev_ready(csu_dev* dev);
+int  csu_dev_async_result(csu_dev* dev, double* res, double* coef);
+
+/* Device bytes held by a context, and an upper estimate of what
+ * csu_dev_create would allocate for a mesh and options with a history ring
+ * of hist iterations (used to admit solves by HBM footprint). */
+int64_t csu_dev_bytes(const csu_dev* dev);
+int64_t csu_dev_footprint(const csu_mesh_desc* mesh, const csu_params* prm, int32_t hist);
+
 /* Iterations done so far; local time step [ncell] of the last iteration. */
 int  csu_dev_iteration(const csu_dev* dev);
 int  csu_dev_get_dt(csu_dev* dev, double* dtl);
+
+/* Iterations are captured into HIP graphs and replayed (two RK iterations
+ * or one implicit iteration per graph) unless profiling is on or this is
+ * switched off; results are the same either way.  Default: on. */
+int  csu_dev_set_graphs(csu_dev* dev, int32_t on);
 
 /* Block until all queued device work of this context is done. */
 int  csu_dev_sync(csu_dev* dev);

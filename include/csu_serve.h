@@ -1,0 +1,70 @@
+/* csu_serve.h -- continuous-batching solve server over csu_dev contexts.
+ *
+ * No counterpart in the reference, which runs one solve per process
+ * (solver.c:1836-1943).  Each admitted solve is an independent device
+ * context with its own stream; the server keeps several in flight so their
+ * kernels overlap on the GPU, admits queued solves as slots and HBM budget
+ * free up, and retires a solve when its density residual has dropped by the
+ * requested factor from its first iteration (or its iteration cap is hit).
+ * Because contexts share nothing and every reduction is a fixed tree, a
+ * solve's results are bit-identical whether it runs alone or in the mix.
+ *
+ * Thread model: one host thread drives a server.
+ */
+#ifndef CSU_SERVE_H
+#define CSU_SERVE_H
+#include "csu.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct csu_server csu_server;
+
+typedef struct {
+    int32_t id;            /* submit order, from 0                                */
+    int32_t status;        /* CSU_OK, or the error code of its create / iterate   */
+    int32_t converged;     /* 1: drop reached; 0: iteration cap reached           */
+    int32_t iters;         /* iterations run                                       */
+    int32_t iter_conv;     /* first iteration (1-based) at the drop, -1 if none    */
+    int32_t ncell;
+    double t_submit, t_admit, t_done;   /* seconds on the server clock             */
+    double res0, res_last; /* density residual of the first and last iteration     */
+    double coef_last[4];   /* Cx_p Cx_v Cy_p Cy_v of the last iteration            */
+    int64_t bytes;         /* device bytes the solve held                          */
+} csu_job_result;
+
+/* max_slots: solves in flight at once; hbm_budget: device bytes the
+ * server's contexts may hold together; chunk: iterations enqueued per
+ * solve between convergence checks. */
+int  csu_server_create(int32_t max_slots, int64_t hbm_budget, int32_t chunk, csu_server** out);
+void csu_server_destroy(csu_server* s);
+
+/* Queue a solve.  mesh and prm (and U0 if not NULL: initial state
+ * [nvar][ncell], otherwise the free stream) must stay valid until the solve
+ * is admitted.  drop: residual drop that counts as converged (1e-8);
+ * max_iter: iteration cap; keep_state: keep the final state for
+ * csu_server_state.  *id receives the job id. */
+int  csu_server_submit(csu_server* s, const csu_mesh_desc* mesh, const csu_params* prm, const double* U0,
+                       double drop, int32_t max_iter, int32_t keep_state, int32_t* id);
+
+/* Admit what fits, keep every admitted solve's stream fed, wait up to
+ * max_wait seconds for at least one chunk to finish and retire finished
+ * solves.  Returns the number of solves retired by this call, or < 0. */
+int  csu_server_step(csu_server* s, double max_wait);
+
+/* Move up to max finished results into out; returns the count. */
+int  csu_server_results(csu_server* s, csu_job_result* out, int32_t max);
+
+/* Queued (not yet admitted) and active solves. */
+int  csu_server_pending(const csu_server* s, int32_t* queued, int32_t* active);
+
+/* Final state of a finished job submitted with keep_state. */
+int  csu_server_state(const csu_server* s, int32_t id, double* U);
+
+/* Seconds since csu_server_create. */
+double csu_server_clock(const csu_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CSU_SERVE_H */

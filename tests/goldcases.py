@@ -1,19 +1,21 @@
 """Golden fixture catalogue shared by the tests: which fixtures exist and how
 to rebuild the case directory each one was made from."""
+import ast
 import os
 
 import numpy as np
 
-from csu_amd import cases, native
+from csu_amd import cases, native, renumber
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 RAMP = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
         "ramp_ausmpup2_sub", "ramp_quad_global_dt", "ramp_tp_dtswitch",
-        "ramp_lusgs_ausmdv", "ramp_dplur_ausm"]
+        "ramp_lusgs_ausmdv", "ramp_dplur_ausm", "ramp_lusgs_colour"]
 PLATE = ["plate_lam", "plate_lamT_ausmdv", "plate_lam_axi_roe", "plate_euler_axi",
          "plate_lam_pup_lim1", "plate_lusgs_lam", "plate_lusgs_lamT", "plate_dplur_lam"]
 SA = ["plate_sa_ausmdv", "plate_sa_blazek_wallT", "plate_sa_pup_lim1", "plate_sa_lusgs", "plate_sa_dplur"]
-SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_sst_pup2_t1", "plate_sst_lusgs"]
+SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_sst_pup2_t1", "plate_sst_lusgs",
+       "plate_sa_lusgs_colour"]
 PLATE = PLATE + SA + SST
 ALL = RAMP + PLATE
 
@@ -23,12 +25,15 @@ def load(name):
 
 
 def write(name, g, d, nmax=1):
-    mk = eval(str(g["mesh_kwargs"]))
-    inp = eval(str(g["inputs"]))
+    mk = ast.literal_eval(str(g["mesh_kwargs"]))
+    inp = ast.literal_eval(str(g["inputs"]))
+    ren = mk.pop("renumber", False)
     if name.startswith("plate"):
         cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nmax, **inp))
     else:
         cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nmax, **inp))
+    if ren:
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
     return d
 
 

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from csu_amd import cases  # noqa: E402
+from csu_amd import cases, renumber  # noqa: E402
 from oracle import pyoracle as po  # noqa: E402
 
 # (name, mesh kwargs, input overrides, history iterations)
@@ -34,6 +34,8 @@ SMALL = [
     ("ramp_tp_dtswitch", dict(nx=12, ny=6), {"TP": 1, "dtLocalN": 5}, 20),
     ("ramp_lusgs_ausmdv", dict(nx=12, ny=6), {"timeScheme": "LUSGS", "CFL": 5, "flux": "AUSMDV"}, 20),
     ("ramp_dplur_ausm", dict(nx=12, ny=6), {"timeScheme": "DPLUR", "CFL": 5, "flux": "AUSM"}, 20),
+    # LU-SGS on the colour/RCM-renumbered mesh (csu_amd/renumber.py)
+    ("ramp_lusgs_colour", dict(nx=16, ny=8, renumber=True), {"timeScheme": "LUSGS", "CFL": 8, "flux": "AUSMDV"}, 20),
 ]
 
 
@@ -66,6 +68,8 @@ PLATE = [
     ("plate_sst_pup2_t1", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "flux": "AUSMpup2",
                                                   "turb1order": "1", "oWallFactor": "5"}, 20),
     ("plate_sst_lusgs", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
+    ("plate_sa_lusgs_colour", dict(_PL, jitter=0.1, renumber=True),
+     {"laminar": "0", "sa": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
 ]
 
 
@@ -78,10 +82,13 @@ def make_small(only=None):
         if only and name not in only:
             continue
         d = case_dir(name)
+        mk_ = {k: v for k, v in mk.items() if k != "renumber"}
         if name.startswith("plate"):
-            cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nit, **inp))
+            cases.write_case(d, cases.plate_mesh(**mk_), cases.plate_input(threads=1, nmax=nit, **inp))
         else:
-            cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nit, **inp))
+            cases.write_case(d, cases.ramp_mesh(seed=0, **mk_), cases.ramp_input(threads=1, nmax=nit, **inp))
+        if mk.get("renumber"):
+            renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
         mesh = po.ref_trace(d, "mesh")
         nv = 5 if inp.get("sa") == "1" else 6 if inp.get("sst") == "1" else 4
         nc = mesh["cellv"].size // 4

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("csu.h", "csu_host.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")):
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(csu_\w+)\s*\(", src, flags=re.M):
@@ -36,3 +36,20 @@ def test_executable_missing_key_exits_zero(tmp_path):
     r = subprocess.run([native.EXE, d], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0
     assert "Error: Value not found in the input file: flux." in r.stdout
+
+
+def test_footprint_estimate_scales(tmp_path):
+    """csu_dev_footprint (host arithmetic only) grows with the mesh and the
+    model, and sits in the range the allocation list implies."""
+    import ctypes as C
+    L = native.lib()
+    out = []
+    for nx, ny, inp in ((20, 10, {}), (40, 20, {}), (40, 20, {"timeScheme": "DPLUR"})):
+        d = str(tmp_path / f"f{nx}{len(inp)}") + "/"
+        cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=1, **inp)
+        c = native.HostCase(d)
+        b = L.csu_dev_footprint(C.byref(c.mesh), C.byref(c._c.contents.prm), 100)
+        out.append((b, c.ncell))
+    (b1, n1), (b2, n2), (b3, _) = out
+    assert b2 > 3 * b1 / 2 and b3 > b2
+    assert 400 < b2 / n2 < 4000

@@ -121,14 +121,16 @@ def test_vs_reference_fixture(tmpcase, name):
     assert _rel(dev2.get_state(), g["hist_U"]) < 1e-10
 
 
-def test_config0_residual_history_1e10(tmpcase):
+@pytest.mark.parametrize("precision", [0, 2])
+def test_config0_residual_history_1e10(tmpcase, precision):
     """The parity bar: config-0 plain FP64 path (200x100x2 ramp, Roe, LSQ,
     Venkatakrishnan K=5, RK5, CFL 0.5, local dt) reproduces the reference's
     per-iteration L2 density residual to 1e-10 relative over the first 100
-    iterations, and its state at 1000 seeded cells."""
+    iterations, and its state at 1000 seeded cells.  precision 0: LSQ from
+    precomputed per-cell weights; 2: long double sums emulated per call."""
     g = np.load(f"{GOLD}/config0_hist100.npz")
     d = tmpcase("c")
-    cases.ramp_case(d, nx=200, ny=100, seed=0, threads=1, nmax=100)
+    cases.ramp_case(d, nx=200, ny=100, seed=0, threads=1, nmax=100, precision=precision)
     import hashlib
     assert hashlib.sha256(open(d + "mesh.su2", "rb").read()).hexdigest() == str(g["mesh_sha256"])
     dev = native.Device(native.HostCase(d))
@@ -286,3 +288,42 @@ def test_sst_rejects_what_the_reference_cannot_run(tmpcase):
                          cases.plate_input(threads=1, nmax=1, laminar="0", sst="1", **inp))
         with pytest.raises(native.CsuError):
             native.Device(native.HostCase(d))
+
+
+@pytest.mark.parametrize("kind,inp", [
+    ("ramp", {}),
+    ("ramp", {"dtLocalN": 7, "stages": 3}),
+    ("ramp", {"timeScheme": "LUSGS", "CFL": 4}),
+    ("ramp", {"timeScheme": "DPLUR", "CFL": 4, "flux": "AUSM", "Nlinear": 6}),
+    ("plate", {"laminar": "0", "sa": "1"}),
+])
+def test_graph_replay_equals_eager(tmpcase, kind, inp):
+    """Iterations replayed from captured HIP graphs give the same bits as
+    eager launches, across calls of uneven length with host reads and
+    state changes between the replays."""
+    d = tmpcase("gr")
+    print("graphs", kind, inp, flush=True)
+    if kind == "ramp":
+        cases.ramp_case(d, nx=30, ny=15, threads=1, nmax=10, **inp)
+    else:
+        cases.write_case(d, cases.plate_mesh(nx=30, ny=12, y1=2e-3, growth=1.15, jitter=0.1),
+                         cases.plate_input(threads=1, nmax=10, **inp))
+    c = native.HostCase(d)
+    runs = []
+    for graphs in (False, True):
+        dev = native.Device(c)
+        dev.set_graphs(graphs)
+        hist, states = [], []
+        for n in (3, 4, 1, 6, 2):
+            r, f = dev.iterate(n)
+            hist.append(np.concatenate([r, f], axis=1))
+            states.append(dev.get_state())
+        U = dev.get_state()
+        dev.set_state(U)            # host round trip, then more replays
+        r, f = dev.iterate(5)
+        hist.append(np.concatenate([r, f], axis=1))
+        states.append(dev.get_state())
+        runs.append((np.concatenate(hist), states))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(a, b)
