@@ -61,10 +61,10 @@ MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 939.13e6, ("k_flux", 2000, 1000): 16
                     ("k_update", 2000, 1000): 892.76e6}
 
 
-def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool):
+def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision: int = 0, keep: bool = False):
     from csu_amd import cases, native
     d = tempfile.mkdtemp(prefix="bn", dir="/tmp") + "/"
-    cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=steps)
+    cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=steps, precision=precision)
     case = native.HostCase(d)
     dev = native.Device(case)
     dev.iterate(warmup, want=False)
@@ -75,9 +75,11 @@ def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool):
     dev.sync()
     t1 = time.perf_counter()
     kt = dev.kernel_times() if profile else {}
-    res, _ = dev.iterate(1)
+    res, coef = dev.iterate(1)
     info = dict(nc=case.ncell, nf=case.nface, nb=case.nbface, nv=case.nvar,
-                seconds=t1 - t0, kernels=kt, res=res[0].tolist())
+                seconds=t1 - t0, kernels=kt, res=res[0].tolist(), coef=coef[0].tolist())
+    if keep:
+        info["U"] = dev.get_state()
     dev.close()
     case.close()
     return info
@@ -192,6 +194,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
     ap.add_argument("--serve-n", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=8)
@@ -218,8 +221,24 @@ def main():
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     log(f"ramp {a.nx}x{a.ny}x2: generating, loading, {a.warmup}+{a.steps} iterations")
-    big = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=True)
-    log(f"big done: {big['seconds']:.3f} s; ramp 200x100x2")
+    big = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=True, keep=not a.no_mixed)
+    log(f"big done: {big['seconds']:.3f} s")
+    mixed = None
+    if not a.no_mixed:
+        mx = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=False, precision=1, keep=True)
+        U64, U32 = big.pop("U"), mx.pop("U")
+        c64, c32 = big["coef"], mx["coef"]
+        cx64, cy64, cx32, cy32 = c64[0] + c64[1], c64[2] + c64[3], c32[0] + c32[1], c32[2] + c32[3]
+        mixed = {"value": mx["nc"] * a.steps / mx["seconds"], "unit": "cell-iter/s",
+                 "ms_per_step": 1e3 * mx["seconds"] / a.steps,
+                 "speedup_vs_fp64": big["seconds"] / mx["seconds"],
+                 "after_iterations": a.warmup + a.steps + 1,
+                 "field_linf_rel": (np.abs(U32 - U64).max(axis=1) / np.abs(U64).max(axis=1)).tolist(),
+                 "Cx_rel": abs(cx32 / cx64 - 1) if cx64 else None, "Cy_rel": abs(cy32 / cy64 - 1) if cy64 else None,
+                 "residual_rel": abs(mx["res"][0] / big["res"][0] - 1)}
+        del U64, U32
+        log(f"mixed done: {mx['seconds']:.3f} s")
+    log("ramp 200x100x2")
     small = None if a.no_small else gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False)
     log("small done; cpu baseline" if not a.no_cpu else "small done")
     nc = big["nc"]
@@ -251,6 +270,7 @@ def main():
                    "config0_40k_cell_iter_per_s":
                        None if small is None else small["nc"] * max(a.steps, 50) / small["seconds"]},
         "roofline": roof,
+        "mixed_precision": mixed,
         "serving": serve,
         "cpu_baseline": cpu,
     }

@@ -201,3 +201,84 @@ def plate_case(dirpath: str, nx: int = 500, ny: int = 100, seed: int = 1, thread
                nmax: int = 100, mesh_kw: dict | None = None, **inputs) -> str:
     return write_case(dirpath, plate_mesh(nx, ny, seed=seed, **(mesh_kw or {})),
                       plate_input(threads=threads, nmax=nmax, **inputs))
+
+
+# --------------------------------------------------------------------------
+# NACA 0012 (4-digit thickness formula, closed trailing edge) in a hybrid
+# O-mesh: `nwall` quad layers extruded along the surface normals from a first
+# spacing `dy1` (chords) with geometric growth `growth`, then layers that
+# blend to a circle of radius `rfar` chords whose quads are split into
+# triangles (a seeded fraction `tri_frac` of them, all by default).
+# Markers: airfoil (wall), inlet (far field, x < 0.5), outlet (far field,
+# x >= 0.5).  AoA is an input (nx = cos a, ny = sin a), not a mesh rotation.
+# --------------------------------------------------------------------------
+def naca0012_y(x: np.ndarray) -> np.ndarray:
+    t = 0.12
+    return 5 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 - 0.1036 * x ** 4)
+
+
+def naca_mesh(ni: int = 400, nwall: int = 40, nfar: int = 60, dy1: float = 1e-6, growth: float = 1.15,
+              rfar: float = 50.0, tri_frac: float = 1.0, seed: int = 2) -> Mesh2D:
+    rng = np.random.default_rng(seed)
+    # surface: ni points, cosine spacing, TE -> lower -> LE -> upper -> TE (closed loop, CCW interior side out)
+    half = ni // 2
+    beta = np.linspace(0.0, np.pi, half + 1)
+    xc = 0.5 * (1 + np.cos(beta))            # 1 -> 0
+    yl = -naca0012_y(xc)
+    xu = xc[::-1][1:-1]                       # 0 -> 1, without LE and TE duplicates
+    yu = naca0012_y(xu)
+    xs = np.concatenate([xc, xu])
+    ys = np.concatenate([yl, yu])
+    ni = len(xs)
+    # outward normals from central differences along the loop
+    tx = np.roll(xs, -1) - np.roll(xs, 1)
+    ty = np.roll(ys, -1) - np.roll(ys, 1)
+    nrm = np.hypot(tx, ty)
+    nxv, nyv = -ty / nrm, tx / nrm       # the loop runs clockwise: (-ty, tx) points out
+    # wall layers
+    d = dy1 * (growth ** np.arange(nwall + 1) - 1) / (growth - 1)
+    X = [xs + nxv * dj for dj in d]
+    Y = [ys + nyv * dj for dj in d]
+    # blend from the last wall layer to the far circle
+    th = np.arctan2(Y[-1] - 0.0, X[-1] - 0.5)
+    fx, fy = 0.5 + rfar * np.cos(th), rfar * np.sin(th)
+    s = np.geomspace(1.0, 1.0 + (rfar / max(d[-1], 1e-12)), nfar + 1)
+    s = (s - 1.0) / (s[-1] - 1.0)
+    for k in range(1, nfar + 1):
+        X.append((1 - s[k]) * X[nwall] + s[k] * fx)
+        Y.append((1 - s[k]) * Y[nwall] + s[k] * fy)
+    X, Y = np.array(X), np.array(Y)          # [layers][ni]
+    nl = X.shape[0]
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+    nid = lambda j, i: j * ni + (i % ni)     # noqa: E731
+    cells = []
+    for j in range(nl - 1):
+        for i in range(ni):
+            a, b, c, e = nid(j, i), nid(j, i + 1), nid(j + 1, i + 1), nid(j + 1, i)
+            if j >= nwall and rng.random() < tri_frac:
+                cells.append((a, b, c))
+                cells.append((a, c, e))
+            else:
+                cells.append((a, b, c, e))
+    wall = [(nid(0, i + 1), nid(0, i)) for i in range(ni)]
+    far = [(nid(nl - 1, i), nid(nl - 1, i + 1)) for i in range(ni)]
+    xm = np.array([0.5 * (X[nl - 1, i] + X[nl - 1, (i + 1) % ni]) for i in range(ni)])
+    inlet = [e for e, x in zip(far, xm) if x < 0.5]
+    outlet = [e for e, x in zip(far, xm) if x >= 0.5]
+    return Mesh2D(nodes=nodes, cells=cells, markers=[("airfoil", wall), ("inlet", inlet), ("outlet", outlet)])
+
+
+def naca_input(threads: int = 8, nmax: int = 100, aoa: float = 0.0, **over) -> dict:
+    """Mach 0.15, Re = 6e6 per chord at T = 300 K (p = 183,134.3 Pa with
+    Sutherland mu(300 K)); AoA through the free-stream direction."""
+    a = math.radians(aoa)
+    dct = {
+        "pressure": "183134.3", "temperature": "300.", "mach": "0.15",
+        "nx": repr(math.cos(a)), "ny": repr(math.sin(a)), "pout": "183134.3",
+        "BC:airfoil": "wall", "BC:inlet": "inlet", "BC:outlet": "outlet",
+        "Nmax": str(nmax), "order": "2", "stages": "5", "threads": str(threads),
+        "flux": "AUSMDV", "tube": "0", "axisymmetric": "0", "CFL": "20", "laminar": "0", "sa": "1",
+        "limiter": "0", "limK": "5", "dtLocal": "1", "timeScheme": "LUSGS",
+    }
+    dct.update({k: str(v) for k, v in over.items()})
+    return dct

@@ -174,8 +174,8 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
     return set_err(CSU_ERR_UNSUPPORTED, "limiter 1 with SST: the reference's range arrays hold 5 variables (limiter.c:37-38)");
   if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);
   if (p->time_scheme == CSU_TIME_DPLUR && p->nlinear < 0) return set_err(CSU_ERR_ARG, "Nlinear %d", p->nlinear);
-  if (p->precision != CSU_PREC_FP64 && p->precision != CSU_PREC_FP64_LD)
-    return set_err(CSU_ERR_UNSUPPORTED, "mixed precision not built yet");
+  if (p->precision != CSU_PREC_FP64 && p->precision != CSU_PREC_FP64_LD && p->precision != CSU_PREC_MIXED)
+    return set_err(CSU_ERR_ARG, "precision %d", p->precision);
   return CSU_OK;
 }
 
@@ -454,6 +454,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   kp.laminar = p->laminar; kp.sa = p->sa; kp.sst = p->sst; kp.limiter = p->limiter;
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
   kp.lsq_ld = p->precision == CSU_PREC_FP64_LD; kp.store_grad = 0;
+  kp.mixed = p->precision == CSU_PREC_MIXED;
   kp.rlim = p->rlim; kp.pout = p->pout; kp.twall = p->twall; kp.efix = p->efix; kp.minf = p->mach_inf;
   kp.sst_owall = p->sst_owallfactor;
   {
@@ -514,9 +515,20 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   }
   d->wall = const_cast<int*>(wall_d);
   s.pref2 = const_cast<double*>(pref_d);
-  if (p->order == 2 && (rc = dalloc(d, &s.rec, 4 * nvc))) {
+  if (p->order == 2 && !kp.mixed && (rc = dalloc(d, &s.rec, 4 * nvc))) {
     csu_dev_destroy(d);
     return rc;
+  }
+  if (kp.mixed) {
+    auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
+    if ((rc = upload(d, &g.lwx32, f32(lwx))) || (rc = upload(d, &g.lwy32, f32(lwy))) ||
+        (rc = upload(d, &g.edx32, f32(edx))) || (rc = upload(d, &g.edy32, f32(edy))) ||
+        (rc = upload(d, &g.fdsx32, f32(fdsx))) || (rc = upload(d, &g.fdsy32, f32(fdsy))) ||
+        (rc = upload(d, &g.fds32, f32(fds))) || (rc = dalloc(d, &s.fflux32, (size_t)nv * nf)) ||
+        (p->order == 2 && (rc = dalloc(d, &s.rec32, 4 * nvc)))) {
+      csu_dev_destroy(d);
+      return rc;
+    }
   }
   if ((p->sa || p->sst) && !m->wall_dist) {
     // saCalcD on the device when the caller brings no wall distance
@@ -722,19 +734,32 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
   }
   {
     Prof p(d, "k_grad");
-    if (kp.lsq_ld)
+    if (kp.mixed)
+      hipLaunchKernelGGL((k_grad<NV, VISC, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo,
+                         d->st);
+    else if (kp.lsq_ld)
       hipLaunchKernelGGL((k_grad<NV, VISC, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
     else
       hipLaunchKernelGGL((k_grad<NV, VISC, false>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
   }
   if (d->nf > 0) {
     Prof p(d, "k_flux");
-    hipLaunchKernelGGL((k_flux<NV, VISC>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
+    if (kp.mixed)
+      hipLaunchKernelGGL((k_flux<NV, VISC, true>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo,
+                         d->st);
+    else
+      hipLaunchKernelGGL((k_flux<NV, VISC>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
   }
   {
     Prof p(d, "k_update");
-    hipLaunchKernelGGL((k_update<NV, VISC>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, flags,
-                       sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, d->red + (size_t)NV * d->nred);
+    if (kp.mixed)
+      hipLaunchKernelGGL((k_update<NV, VISC, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+                         flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
+                         d->red + (size_t)NV * d->nred);
+    else
+      hipLaunchKernelGGL((k_update<NV, VISC>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+                         flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
+                         d->red + (size_t)NV * d->nred);
   }
   HIPCHK(hipGetLastError());
   return CSU_OK;

@@ -473,10 +473,17 @@ __device__ __forceinline__ void bc_wall(const GasD& g, const double* Pd, double*
 
 // Venkatakrishnan-type limiter function, reference limiter.c:98-147
 template <typename T>
+__device__ __forceinline__ T limiter_v2d(T d1max, T d1min, T d2, T ee);
+
+template <typename T>
 __device__ __forceinline__ T limiter_v2(T Ui, T Umin, T Umax, T d2, T ee) {
+  return limiter_v2d<T>(Umax - Ui, Umin - Ui, d2, ee);
+}
+
+// limiterV2 from the differences to the neighbourhood max / min
+template <typename T>
+__device__ __forceinline__ T limiter_v2d(T d1max, T d1min, T d2, T ee) {
   T ans;
-  T d1max = Umax - Ui;
-  T d1min = Umin - Ui;
   if (d2 == (T)0) {
     ans = (T)1;
   } else if (d2 > (T)0) {

@@ -12,6 +12,7 @@ namespace csu {
 struct KP {
   int nc, nf, nb, nv, order, flux, laminar, sa, sst, limiter, turb1order, axi, visc_blazek;
   int lsq_ld;     // 1: long double LSQ sums emulated per call; 0: precomputed weights
+  int mixed;      // 1: FP32 gradients, limiter, reconstruction and convective flux
   int store_grad; // write dPx/dPy/phi even when no viscous term reads them
   double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t, wimp;
   double sst_owall, sst_g1, sst_g2;  // SST: oWallFactor, blended gamma constants (sst.c:18-38)
@@ -37,6 +38,8 @@ struct Geo {
   const double *edx, *edy;                              // [4][nc] own-edge midpoint offsets
   const double *lwx, *lwy;                              // [4][nc] LSQ weights per neighbour slot
   const unsigned char* fsl;                             // [nf] edge of the face in e0 (bits 0-1), e1 (2-3)
+  const float *lwx32, *lwy32, *edx32, *edy32;           // mixed precision copies of lwx, lwy, edx, edy
+  const float *fdsx32, *fdsy32, *fds32;                 // mixed precision face vectors
   const double *wdist;                                  // [nc] wall distance (SA/SST)
   const int *fe0, *fe1;                                 // [nf]
   const double *fdsx, *fdsy, *fds, *fdx0, *fdy0, *fdx1, *fdy1;
@@ -51,6 +54,8 @@ struct St {
   double *src;          // SA: Qt*Omega [nc]; SST: Qtk*Omega, Qto*Omega [2][nc]
   double *dq;           // SST source Jacobians dQk/d(r, rk, ro), dQo/d(r, rk, ro) [6][nc]
   double *rec;          // second order: reconstructed state at each cell edge [4][NV][nc]
+  float *rec32;         // mixed precision: the same in FP32
+  float *fflux32;       // mixed precision: convective face fluxes in FP32 [NV][nf]
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference

@@ -2,16 +2,18 @@
 //
 // Per explicit stage (reference solverCalcR + solverRK, solver.c:685-766):
 //   k_grad    cells  ghost states of the cell's boundary faces, least-squares
-//                    gradients (double-double sums), Venkatakrishnan phi;
-//                    viscous: T gradient, boundary viscous fluxes, wall
+//                    gradients (precomputed weights, or double-double sums),
+//                    Venkatakrishnan phi, the reconstructed state at every
+//                    edge; viscous: T gradient, boundary viscous fluxes, wall
 //                    friction (all need only the cell's own fresh data)
-//   k_flux    faces  reconstruction + Riemann flux (+ viscous face flux)
+//   k_flux    faces  Riemann flux of the two edge states (+ viscous face flux)
 //   k_update  cells  per-cell sums in the reference's order, then the RK
 //                    update fused with the next stage's primitive recovery;
 //                    on the last stage also the next time step and the
 //                    residual-norm block partials
 // Every cell gathers its own faces: no atomics, fixed summation order.
 #pragma once
+#include <type_traits>
 #include "kern_common.h"
 #include "kern_visc.h"
 
@@ -61,8 +63,11 @@ __device__ __forceinline__ void lsq_grad(int np, const double* dx, const double*
 // limiter.c:98-147), boundaryCalcPrimitive (boundary.c:514-693),
 // boundaryFaceViscFlux (laminar.c:128-346), boundaryCalcFrictionWall
 // (boundary.c:695-742).
-template <int NV, bool VISC, bool LD>
+// MIX (mixed precision): gradients, limiter and reconstruction in FP32 from
+// FP64 differences; edge states stored in FP32.
+template <int NV, bool VISC, bool LD, bool MIX = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
+  using TF = typename std::conditional<MIX, float, double>::type;
   constexpr int NS = NV + 1;
   const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int nc = kp.nc, nb = kp.nb;
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   // cell in long double.  LD: the normal-equation sums of every call in
   // double-double, as the reference's x87 long double accumulators do.
   dd a = {0, 0}, b = {0, 0}, c = {0, 0}, det = {0, 0}, nb_ = {0, 0};
-  double wx[4], wy[4];
+  TF wx[4], wy[4];
   if (LD) {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -120,22 +125,28 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   } else {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      wx[j] = j < np ? g.lwx[j * nc + i] : 0.0;
-      wy[j] = j < np ? g.lwy[j * nc + i] : 0.0;
+      if (MIX) {
+        wx[j] = j < np ? g.lwx32[j * nc + i] : 0.0f;
+        wy[j] = j < np ? g.lwy32[j * nc + i] : 0.0f;
+      } else {
+        wx[j] = j < np ? g.lwx[j * nc + i] : 0.0;
+        wy[j] = j < np ? g.lwy[j * nc + i] : 0.0;
+      }
     }
   }
   auto grad = [&](const double* du, double& gx, double& gy) {
     if (LD) {
       lsq_grad<4>(np, dx, dy, du, a, b, nb_, c, det, gx, gy);
     } else {
-      gx = wx[0] * du[0];
-      gy = wy[0] * du[0];
+      TF ax = wx[0] * (TF)du[0], ay = wy[0] * (TF)du[0];
 #pragma unroll
       for (int j = 1; j < 4; j++) {
         if (j >= np) break;
-        gx += wx[j] * du[j];
-        gy += wy[j] * du[j];
+        ax += wx[j] * (TF)du[j];
+        ay += wy[j] * (TF)du[j];
       }
+      gx = ax;
+      gy = ay;
     }
   };
   const bool store = VISC || kp.store_grad;
@@ -175,12 +186,14 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
         Pmax = fmax(Pmax, vn[j]);
         Pmin = fmin(Pmin, vn[j]);
       }
-      double ph = 0.0, d2[4];
+      TF ph = 0, d2[4];
+      const TF d1max = (TF)(Pmax - u0), d1min = (TF)(Pmin - u0), tgx = (TF)gx, tgy = (TF)gy;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
-        d2[j] = gx * g.edx[j * nc + i] + gy * g.edy[j * nc + i];
-        double p0 = limiter_v2<double>(u0, Pmin, Pmax, d2[j], pref2);
+        if (MIX) d2[j] = tgx * g.edx32[j * nc + i] + tgy * g.edy32[j * nc + i];
+        else d2[j] = tgx * g.edx[j * nc + i] + tgy * g.edy[j * nc + i];
+        TF p0 = limiter_v2d<TF>(d1max, d1min, d2[j], (TF)pref2);
         ph = (j == 0) ? p0 : fmin(ph, p0);
       }
       if (store) s.phi[kk * nc + i] = ph;
@@ -189,7 +202,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
-        s.rec[(j * NV + kk) * nc + i] = first ? u0 : u0 + d2[j] * ph;
+        if (MIX) s.rec32[(j * NV + kk) * nc + i] = (float)(first ? u0 : u0 + (double)(d2[j] * ph));
+        else s.rec[(j * NV + kk) * nc + i] = first ? u0 : u0 + d2[j] * ph;
       }
     }
   }
@@ -304,40 +318,72 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, co
 
 // Interior face flux, reference inter() face loop (solver.c:550-644), and
 // for viscous runs the laminar face flux (laminar.c:17-75).
-template <int NV, bool VISC>
+// MIX: the convective flux in FP32 from FP32 edge states, stored in FP32.
+template <int NV, bool VISC, bool MIX = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
+  using TF = typename std::conditional<MIX, float, double>::type;
   const int f = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int nc = kp.nc, nf = kp.nf;
   if (f >= nf) return;
   const int e0 = g.fe0[f], e1 = g.fe1[f];
-  const double dSx = g.fdsx[f], dSy = g.fdsy[f], dS = g.fds[f];
-  double PL[NV], PR[NV];
-  if (kp.order == 2) {
-    // both sides' states were reconstructed at this edge by k_grad
-    const int sl = g.fsl[f], s0 = sl & 3, s1 = sl >> 2;
+  if (MIX) {
+    const TF dSx = g.fdsx32[f], dSy = g.fdsy32[f], dS = g.fds32[f];
+    TF PL[NV], PR[NV];
+    if (kp.order == 2) {
+      const int sl = g.fsl[f], s0 = sl & 3, s1 = sl >> 2;
 #pragma unroll
-    for (int kk = 0; kk < NV; kk++) {
-      PL[kk] = s.rec[(s0 * NV + kk) * nc + e0];
-      PR[kk] = s.rec[(s1 * NV + kk) * nc + e1];
-    }
-  } else {
+      for (int kk = 0; kk < NV; kk++) {
+        PL[kk] = s.rec32[(s0 * NV + kk) * nc + e0];
+        PR[kk] = s.rec32[(s1 * NV + kk) * nc + e1];
+      }
+    } else {
 #pragma unroll
-    for (int kk = 0; kk < NV; kk++) {
-      const int m = kk > 3 ? kk + 1 : kk;
-      PL[kk] = s.P[m * nc + e0];
-      PR[kk] = s.P[m * nc + e1];
+      for (int kk = 0; kk < NV; kk++) {
+        const int m = kk > 3 ? kk + 1 : kk;
+        PL[kk] = (TF)s.P[m * nc + e0];
+        PR[kk] = (TF)s.P[m * nc + e1];
+      }
     }
+    rotate<TF>(PL[1], PL[2], dSx, dSy, dS);
+    rotate<TF>(PR[1], PR[2], dSx, dSy, dS);
+    TF fl[NV];
+    constexpr int NSC = NV - 4;
+    flux_dispatch<TF, NSC>(kp.gas, kp.flux, (TF)kp.efix, (TF)kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4, PR[0],
+                           PR[1], PR[2], PR[3], PR + 4, NV == 6 ? PL[4] : (TF)0, NV == 6 ? PR[4] : (TF)0, fl);
+    rotate<TF>(fl[1], fl[2], dSx, -dSy, dS);
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) s.fflux32[kk * nf + f] = fl[kk] * dS;
   }
-  rotate<double>(PL[1], PL[2], dSx, dSy, dS);
-  rotate<double>(PR[1], PR[2], dSx, dSy, dS);
-  double fl[NV];
-  constexpr int NSC = NV - 4;
-  flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
-                             PR[0], PR[1], PR[2], PR[3], PR + 4, NV == 6 ? PL[4] : 0.0, NV == 6 ? PR[4] : 0.0,
-                             fl);
-  rotate<double>(fl[1], fl[2], dSx, -dSy, dS);
+  const double dSx = g.fdsx[f], dSy = g.fdsy[f], dS = g.fds[f];
+  if (!MIX) {
+    double PL[NV], PR[NV];
+    if (kp.order == 2) {
+      // both sides' states were reconstructed at this edge by k_grad
+      const int sl = g.fsl[f], s0 = sl & 3, s1 = sl >> 2;
 #pragma unroll
-  for (int kk = 0; kk < NV; kk++) s.fflux[kk * nf + f] = fl[kk] * dS;
+      for (int kk = 0; kk < NV; kk++) {
+        PL[kk] = s.rec[(s0 * NV + kk) * nc + e0];
+        PR[kk] = s.rec[(s1 * NV + kk) * nc + e1];
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) {
+        const int m = kk > 3 ? kk + 1 : kk;
+        PL[kk] = s.P[m * nc + e0];
+        PR[kk] = s.P[m * nc + e1];
+      }
+    }
+    rotate<double>(PL[1], PL[2], dSx, dSy, dS);
+    rotate<double>(PR[1], PR[2], dSx, dSy, dS);
+    double fl[NV];
+    constexpr int NSC = NV - 4;
+    flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
+                               PR[0], PR[1], PR[2], PR[3], PR + 4, NV == 6 ? PL[4] : 0.0, NV == 6 ? PR[4] : 0.0,
+                               fl);
+    rotate<double>(fl[1], fl[2], dSx, -dSy, dS);
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) s.fflux[kk * nf + f] = fl[kk] * dS;
+  }
   if (VISC && NV == 4) {
     double P0[5], P1[5];
 #pragma unroll
@@ -391,7 +437,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
 // fused with the next stage's solverCalcPrimitive, and on the last stage
 // the next time step (solver.c:895-950) and solverCalcRes partials
 // (solver.c:867-892).
-template <int NV, bool VISC>
+template <int NV, bool VISC, bool MIX = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int flags, double sigma, double rk_a,
                                                       double cfl, double* dtl, const double* dtg, double* Un,
                                                       double* __restrict__ Uout, double* __restrict__ Rout,
@@ -414,10 +460,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
       const int f = cf[j];
       if (f > 0) {
 #pragma unroll
-        for (int kk = 0; kk < NV; kk++) R[kk] += s.fflux[kk * nf + f - 1];
+        for (int kk = 0; kk < NV; kk++) R[kk] += MIX ? (double)s.fflux32[kk * nf + f - 1] : s.fflux[kk * nf + f - 1];
       } else if (f < 0) {
 #pragma unroll
-        for (int kk = 0; kk < NV; kk++) R[kk] -= s.fflux[kk * nf - f - 1];
+        for (int kk = 0; kk < NV; kk++) R[kk] -= MIX ? (double)s.fflux32[kk * nf - f - 1] : s.fflux[kk * nf - f - 1];
       }
     }
     double Pi[NV + 1];

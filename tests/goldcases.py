@@ -14,9 +14,10 @@ RAMP = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
 PLATE = ["plate_lam", "plate_lamT_ausmdv", "plate_lam_axi_roe", "plate_euler_axi",
          "plate_lam_pup_lim1", "plate_lusgs_lam", "plate_lusgs_lamT", "plate_dplur_lam"]
 SA = ["plate_sa_ausmdv", "plate_sa_blazek_wallT", "plate_sa_pup_lim1", "plate_sa_lusgs", "plate_sa_dplur"]
+NACA = ["naca_sa_lusgs", "naca_sst_lusgs"]
 SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_sst_pup2_t1", "plate_sst_lusgs",
        "plate_sa_lusgs_colour"]
-PLATE = PLATE + SA + SST
+PLATE = PLATE + SA + SST + NACA
 ALL = RAMP + PLATE
 
 
@@ -30,6 +31,8 @@ def write(name, g, d, nmax=1):
     ren = mk.pop("renumber", False)
     if name.startswith("plate"):
         cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nmax, **inp))
+    elif name.startswith("naca"):
+        cases.write_case(d, cases.naca_mesh(**mk), cases.naca_input(threads=1, nmax=nmax, **inp))
     else:
         cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nmax, **inp))
     if ren:

@@ -70,6 +70,9 @@ PLATE = [
     ("plate_sst_lusgs", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
     ("plate_sa_lusgs_colour", dict(_PL, jitter=0.1, renumber=True),
      {"laminar": "0", "sa": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
+    # NACA 0012 hybrid O-mesh, Mach 0.15, Re 6e6, AoA 4 deg
+    ("naca_sa_lusgs", dict(ni=48, nwall=8, nfar=8), {"aoa": 4.0}, 20),
+    ("naca_sst_lusgs", dict(ni=48, nwall=8, nfar=8), {"aoa": 4.0, "sa": "0", "sst": "1", "CFL": "5"}, 20),
 ]
 
 
@@ -85,12 +88,14 @@ def make_small(only=None):
         mk_ = {k: v for k, v in mk.items() if k != "renumber"}
         if name.startswith("plate"):
             cases.write_case(d, cases.plate_mesh(**mk_), cases.plate_input(threads=1, nmax=nit, **inp))
+        elif name.startswith("naca"):
+            cases.write_case(d, cases.naca_mesh(**mk_), cases.naca_input(threads=1, nmax=nit, **inp))
         else:
             cases.write_case(d, cases.ramp_mesh(seed=0, **mk_), cases.ramp_input(threads=1, nmax=nit, **inp))
         if mk.get("renumber"):
             renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
         mesh = po.ref_trace(d, "mesh")
-        nv = 5 if inp.get("sa") == "1" else 6 if inp.get("sst") == "1" else 4
+        nv = 6 if inp.get("sst") == "1" else 5 if inp.get("sa") == "1" or name.startswith("naca") else 4
         nc = mesh["cellv"].size // 4
         rng = np.random.default_rng(5)
         # perturbed free stream: the reference's own initial state, +-5 %
