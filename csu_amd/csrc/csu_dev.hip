@@ -543,7 +543,8 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
     if ((rc = dalloc(d, &s.dTx, nc)) || (rc = dalloc(d, &s.dTy, nc)) ||
         (rc = dalloc(d, &s.vflux, (size_t)(nv - 1) * nf)) || (rc = dalloc(d, &s.bvflux, (size_t)(nv - 1) * nb)) ||
         (rc = dalloc(d, &s.wfric, 2 * (size_t)nb)) || (rc = dalloc(d, &s.miT, nf)) ||
-        (rc = dalloc(d, &s.src, 2 * (size_t)nc)) || (rc = dalloc(d, &s.dq, 6 * (size_t)nc))) {
+        (rc = dalloc(d, &s.src, 2 * (size_t)nc)) || (rc = dalloc(d, &s.dq, 6 * (size_t)nc)) ||
+        (rc = dalloc(d, &s.wten, 4 * (size_t)nb))) {
       csu_dev_destroy(d);
       return rc;
     }
@@ -1087,6 +1088,17 @@ extern "C" int csu_dev_async_result(csu_dev* d, double* res, double* coef) {
     if (res) memcpy(res + (size_t)it * d->nv, d->hres + (size_t)slot * d->nv, sizeof(double) * d->nv);
     if (coef) memcpy(coef + (size_t)it * 4, d->hcoef + (size_t)slot * 4, sizeof(double) * 4);
   }
+  return CSU_OK;
+}
+
+extern "C" int csu_dev_wall_tensor(csu_dev* d, double* out) {
+  if (!d || !out) return set_err(CSU_ERR_ARG, "null argument");
+  if (!d->visc) return set_err(CSU_ERR_ARG, "no viscous terms: no wall shear");
+  std::vector<double> w(4 * (size_t)d->nb);
+  HIPCHK(hipMemcpyAsync(w.data(), d->st.wten, w.size() * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  for (int q = 0; q < d->nb; q++)
+    for (int k = 0; k < 4; k++) out[4 * (size_t)q + k] = w[(size_t)k * d->nb + q];
   return CSU_OK;
 }
 

@@ -56,6 +56,7 @@ struct St {
   double *rec;          // second order: reconstructed state at each cell edge [4][NV][nc]
   float *rec32;         // mixed precision: the same in FP32
   float *fflux32;       // mixed precision: convective face fluxes in FP32 [NV][nf]
+  double *wten;         // viscous: wall shear tensor txx txy tyy yp per boundary face [4][nb]
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference

@@ -510,3 +510,37 @@ __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, 
 }
 
 }  // namespace csu
+
+namespace csu {
+
+// Wall shear tensor written to friction.csv (solverCalcCoeff2,
+// solver.c:1303-1340): boundaryCalcTensorWall (boundary.c:744-793) for
+// laminar runs, saCalcTensorWall (sa.c:795-852) for SA and SST.  Pi: cell
+// primitives; g*: cell gradients of u, v; (dx, dy): face centre minus cell
+// centre; xm: face centre x; d: the cell's wall distance.  out: txx txy tyy yp.
+__device__ __forceinline__ void wall_tensor(bool turb, double dSx, double dSy, double dx, double dy, double xm,
+                                            const double* Pi, double d, const double* gx, const double* gy,
+                                            double* out) {
+  const double L = sqrt(dx * dx + dy * dy);
+  const double dul = (0 - Pi[1]) / L, dvl = (0 - Pi[2]) / L;
+  double dux, duy, dvx, dvy;
+  edge_correct(gx[0], gy[0], dul, dx, dy, L, dux, duy);
+  edge_correct(gx[1], gy[1], dvl, dx, dy, L, dvx, dvy);
+  const double mi = sutherland(Pi[4]);
+  const double txx = 2 * mi * (dux - (dux + dvy) / 3);
+  const double tyy = 2 * mi * (dvy - (dux + dvy) / 3);
+  const double txy = mi * (duy + dvx);
+  if (!turb) {
+    const double aux = 5.29 * xm / sqrt(Pi[0] * Pi[1] * xm / mi);
+    out[0] = txx; out[1] = txy; out[2] = tyy; out[3] = L / aux;
+    return;
+  }
+  const double dS = sqrt(dSx * dSx + dSy * dSy);
+  const double fx = (txx * dSx + txy * dSy) / dS;
+  const double fy = (txy * dSx + tyy * dSy) / dS;
+  const double ft = (fx * dSy - fy * dSx) / dS;
+  const double ut = sqrt(fabs(ft) / Pi[0]);
+  out[0] = 0.0; out[1] = ft; out[2] = 0.0; out[3] = Pi[0] * d * ut / mi;
+}
+
+}  // namespace csu

@@ -256,6 +256,11 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
         wall_friction(g.bdsx[q], g.bdsy[q], dx[j], dy[j], Pi, gx3, gy3, fx, fy);
         s.wfric[q] = fx;
         s.wfric[nb + q] = fy;
+        double wt[4];
+        wall_tensor(NV > 4, g.bdsx[q], g.bdsy[q], dx[j], dy[j], g.bxm[q], Pi, NV > 4 ? g.wdist[i] : 0.0, gx3, gy3,
+                    wt);
+#pragma unroll
+        for (int k = 0; k < 4; k++) s.wten[k * nb + q] = wt[k];
       }
     }
   }

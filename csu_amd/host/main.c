@@ -83,10 +83,28 @@ int main(int argc, char** argv)
     fclose(conv);
     free(res); free(coef);
 
-    /* friction.csv is created (empty for inviscid runs), solverCalcCoeff2 */
+    /* friction.csv (solverCalcCoeff2, solver.c:1303-1340): per wall / wallT
+       face in marker order, x, txx, txy, tyy, yp; empty for inviscid runs */
     snprintf(path, sizeof path, "%sfriction.csv", wd);
     FILE* fr = fopen(path, "w");
-    if (fr) fclose(fr);
+    if (fr) {
+        const csu_mesh_desc* m = &c->mesh;
+        if (c->prm.laminar || c->prm.sa || c->prm.sst) {
+            double* wt = malloc(sizeof(double) * 4 * (size_t)(m->nbface ? m->nbface : 1));
+            if (wt && csu_dev_wall_tensor(d, wt) == CSU_OK) {
+                for (int g = 0; g < m->nbface; g++) {
+                    int t = m->mark_type[m->bface_mark[g]];
+                    if (t != CSU_BC_WALL && t != CSU_BC_WALLT) continue;
+                    const int32_t* q = m->bface + 3 * g;
+                    double x = (m->node_xy[2 * q[1]] + m->node_xy[2 * q[2]]) / 2;
+                    const double* w = wt + 4 * (size_t)g;
+                    fprintf(fr, "%e, %e, %e, %e, %e,\n", x, w[0], w[1], w[2], w[3]);
+                }
+            }
+            free(wt);
+        }
+        fclose(fr);
+    }
 
     rc = csu_dev_get_state(d, c->U);
     if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }

@@ -112,6 +112,7 @@ def lib():
     L.csu_wall_distance.argtypes = [C.POINTER(Case)]
     L.csu_wall_distance_dev.argtypes = [C.POINTER(MeshDesc), f64p]
     L.csu_dev_iterate_async.argtypes = [C.c_void_p, C.c_int32]
+    L.csu_dev_wall_tensor.argtypes = [C.c_void_p, f64p]
     L.csu_dev_ready.argtypes = [C.c_void_p]
     L.csu_dev_async_result.argtypes = [C.c_void_p, f64p, f64p]
     L.csu_dev_bytes.argtypes = [C.c_void_p]
@@ -290,6 +291,12 @@ class Device:
 
     def set_profiling(self, on: bool):
         check(lib().csu_dev_set_profiling(self._h, 1 if on else 0))
+
+    def wall_tensor(self) -> np.ndarray:
+        """[nbface][4] txx txy tyy yp of the last residual (wall rows set)."""
+        out = np.full((max(self.case.nbface, 1), 4), np.nan)
+        check(lib().csu_dev_wall_tensor(self._h, _p(out, C.c_double)))
+        return out[:self.case.nbface]
 
     def set_graphs(self, on: bool):
         check(lib().csu_dev_set_graphs(self._h, 1 if on else 0))

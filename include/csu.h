@@ -141,6 +141,13 @@ int  csu_dev_async_result(csu_dev* dev, double* res, double* coef);
 int64_t csu_dev_bytes(const csu_dev* dev);
 int64_t csu_dev_footprint(const csu_mesh_desc* mesh, const csu_params* prm, int32_t hist);
 
+/* Wall shear of the last residual evaluation, per boundary face
+ * ([nbface][4] = txx txy tyy yp; rows of wall / wallT faces are set), as
+ * solverCalcCoeff2 writes it to friction.csv (solver.c:1303-1340):
+ * boundaryCalcTensorWall (boundary.c:744-793) for laminar runs,
+ * saCalcTensorWall (sa.c:795-852) for SA and SST.  Viscous runs only. */
+int  csu_dev_wall_tensor(csu_dev* dev, double* out);
+
 /* Iterations done so far; local time step [ncell] of the last iteration. */
 int  csu_dev_iteration(const csu_dev* dev);
 int  csu_dev_get_dt(csu_dev* dev, double* dtl);

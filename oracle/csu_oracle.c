@@ -2008,3 +2008,45 @@ int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
     }
     return CSU_OK;
 }
+
+/* Wall shear of the last residual evaluation, solverCalcCoeff2
+   (solver.c:1303-1340): boundaryCalcTensorWall (boundary.c:744-793) for
+   laminar runs, saCalcTensorWall (sa.c:795-852) for SA and SST.
+   out: [nb][4] = txx txy tyy yp, rows of wall / wallT faces only. */
+int oracle_wall_tensor(const csu_oracle* o, double* out)
+{
+    const int nc = o->nc, ns = o->nv + 1;
+    if (!(o->prm.laminar || o->prm.sa || o->prm.sst)) return CSU_ERR_ARG;
+    for (int g = 0; g < o->nb; g++) {
+        int t = o->btype[g];
+        if (t != CSU_BC_WALL && t != CSU_BC_WALLT) continue;
+        const int e0 = o->bface[3 * g], p0 = o->bface[3 * g + 1], p1 = o->bface[3 * g + 2];
+        const double* P = o->P + (size_t)e0 * ns;
+        double dSx, dSy, x0, y0;
+        face_ds(o, p0, p1, &dSx, &dSy);
+        cell_center(o, e0, &x0, &y0);
+        double x1 = (o->xy[2 * p0] + o->xy[2 * p1]) / 2, y1 = (o->xy[2 * p0 + 1] + o->xy[2 * p1 + 1]) / 2;
+        double dx = x1 - x0, dy = y1 - y0, L = sqrt(dx * dx + dy * dy);
+        double dul = (0 - P[1]) / L, dvl = (0 - P[2]) / L;
+        double duxm = o->dPx[nc + e0], dvxm = o->dPx[2 * nc + e0];
+        double duym = o->dPy[nc + e0], dvym = o->dPy[2 * nc + e0];
+        double dux = duxm + (dul - (duxm * dx + duym * dy) / L) * dx / L;
+        double duy = duym + (dul - (duxm * dx + duym * dy) / L) * dy / L;
+        double dvx = dvxm + (dvl - (dvxm * dx + dvym * dy) / L) * dx / L;
+        double dvy = dvym + (dvl - (dvxm * dx + dvym * dy) / L) * dy / L;
+        double mi = sutherland(P[4]);
+        double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
+        double* w = out + 4 * (size_t)g;
+        if (o->prm.laminar) {
+            double aux = 5.29 * x1 / sqrt(P[0] * P[1] * x1 / mi);
+            w[0] = txx; w[1] = txy; w[2] = tyy; w[3] = L / aux;
+        } else {
+            double dS = sqrt(dSx * dSx + dSy * dSy);
+            double fx = (txx * dSx + txy * dSy) / dS, fy = (txy * dSx + tyy * dSy) / dS;
+            double ft = (fx * dSy - fy * dSx) / dS;
+            double ut = sqrt(fabs(ft) / P[0]);
+            w[0] = 0.0; w[1] = ft; w[2] = 0.0; w[3] = P[0] * o->wd[e0] * ut / mi;
+        }
+    }
+    return CSU_OK;
+}

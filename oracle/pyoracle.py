@@ -43,6 +43,7 @@ def lib():
         L.oracle_residual.argtypes = [vp, f64, f64, f64, f64, f64, f64]
         L.oracle_iterate.argtypes = [vp, C.c_int, f64, f64]
         L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_wall_tensor.argtypes = [vp, f64]
         _lib = L
     return _lib
 
@@ -94,6 +95,14 @@ class Oracle:
         coef = np.empty((n, 4))
         lib().oracle_iterate(self._h, n, _p(res), _p(coef))
         return res, coef
+
+    def wall_tensor(self):
+        """[nbface][4] txx txy tyy yp of the last residual (wall rows set)."""
+        out = np.full((max(self.nbface, 1), 4), np.nan)
+        rc = lib().oracle_wall_tensor(self._h, _p(out))
+        if rc != 0:
+            raise RuntimeError(f"oracle_wall_tensor failed ({rc})")
+        return out[:self.nbface]
 
     def close(self):
         if self._h:

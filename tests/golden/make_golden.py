@@ -138,8 +138,30 @@ def make_config0(n_iter=100, n_sample=1000):
     print("wrote config0_hist100", digest)
 
 
+# friction.csv of the reference executable (solverCalcCoeff2) after n iterations
+FRICTION = [
+    ("friction_plate_lam", dict(_PL, jitter=0.1), {}, 60),
+    ("friction_plate_sa", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1"}, 60),
+]
+
+
+def make_friction():
+    import subprocess
+    for name, mk, inp, nit in FRICTION:
+        d = case_dir(name)
+        cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nit, **inp))
+        # the reference can crash on exit (solverFree); friction.csv is written before
+        subprocess.run([po.REF_EXE, d], capture_output=True, check=False)
+        rows = [ln.strip().rstrip(",").split(",") for ln in open(d + "friction.csv") if ln.strip()]
+        F = np.array([[float(v) for v in r] for r in rows])
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), mesh_kwargs=np.array(repr(mk)),
+                            inputs=np.array(repr(inp)), niter=nit, friction=F)
+        print("wrote", name, F.shape)
+
+
 if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref is not built (needs the reference sources): make -C oracle ref")
     make_small()
     make_config0()
+    make_friction()

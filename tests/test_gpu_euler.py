@@ -327,3 +327,91 @@ def test_graph_replay_equals_eager(tmpcase, kind, inp):
     assert np.array_equal(runs[0][0], runs[1][0])
     for a, b in zip(runs[0][1], runs[1][1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("inp", [{}, {"laminar": "0", "sa": "1"}, {"laminar": "0", "sst": "1"}])
+def test_wall_tensor_vs_oracle(tmpcase, inp):
+    """Wall shear (friction.csv rows) after 40 iterations: device vs oracle."""
+    d = tmpcase("wt")
+    cases.write_case(d, cases.plate_mesh(nx=40, ny=16, y1=2e-3, growth=1.15, jitter=0.1),
+                     cases.plate_input(threads=1, nmax=40, **inp))
+    c = native.HostCase(d)
+    dev, orc = native.Device(c), po.Oracle(c)
+    dev.iterate(40)
+    orc.iterate(40)
+    wd, wo = dev.wall_tensor(), orc.wall_tensor()
+    rows = ~np.isnan(wo[:, 1])
+    assert rows.sum() > 0
+    scale = np.abs(wo[rows]).max(axis=0) + 1e-300
+    rel = np.abs(wd[rows] - wo[rows]) / scale
+    print("wall tensor rel diff", rel.max(axis=0), flush=True)
+    assert rel.max() < 1e-10
+
+
+@pytest.mark.parametrize("name", ["friction_plate_lam", "friction_plate_sa"])
+def test_executable_friction_csv_vs_reference(tmpcase, name):
+    """The drop-in executable's friction.csv against the reference
+    executable's on the same case (7 printed digits)."""
+    import ast
+    import subprocess
+    g = np.load(f"{GOLD}/{name}.npz")
+    mk, inp, n = ast.literal_eval(str(g["mesh_kwargs"])), ast.literal_eval(str(g["inputs"])), int(g["niter"])
+    d = tmpcase("fx")
+    cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=n, **inp))
+    r = subprocess.run([native.EXE, d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    rows = [ln.strip().rstrip(",").split(",") for ln in open(d + "friction.csv") if ln.strip()]
+    F = np.array([[float(v) for v in row] for row in rows])
+    assert F.shape == g["friction"].shape
+    assert np.allclose(F, g["friction"], rtol=2e-6, atol=1e-300)
+
+
+def test_converged_fields_1e8(tmpcase):
+    """Converged state (density residual down 1e-10; Euler ramp, LU-SGS on
+    the colour/RCM-renumbered mesh): device vs oracle to 1e-8 relative."""
+    from csu_amd import renumber
+    d = tmpcase("cv")
+    cases.ramp_case(d, nx=60, ny=30, threads=1, nmax=10, timeScheme="LUSGS", CFL=10, flux="AUSMDV")
+    renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+    c = native.HostCase(d)
+    dev, orc = native.Device(c), po.Oracle(c, threads=8)
+    n = 700
+    rg, cg = dev.iterate(n)
+    ro, co = orc.iterate(n)
+    drop = ro[-1, 0] / ro[0, 0]
+    print("oracle drop", drop, "device drop", rg[-1, 0] / rg[0, 0], flush=True)
+    assert drop < 1e-10
+    Ug, Uo = dev.get_state(), orc.get_state()
+    rel = np.abs(Ug - Uo).max(axis=1) / np.abs(Uo).max(axis=1)
+    print("fields rel", rel, flush=True)
+    assert rel.max() < 1e-8
+    assert np.max(np.abs(cg[-1] - co[-1])) <= 1e-8 * np.abs(co[-1]).max()
+
+
+@pytest.mark.parametrize("kind", ["lam", "sa"])
+def test_long_run_fields_and_cf_1e8(tmpcase, kind):
+    """1500 LU-SGS iterations on a renumbered plate (the reference's scheme
+    levels off near a 1e-3 drop on this case): fields and wall shear,
+    device vs oracle, to 1e-8 relative."""
+    from csu_amd import renumber
+    d = tmpcase("lr")
+    inp = {"timeScheme": "LUSGS", "CFL": "20"}
+    if kind == "sa":
+        inp.update(laminar="0", sa="1")
+    cases.write_case(d, cases.plate_mesh(nx=40, ny=16, y1=2e-3, growth=1.15, jitter=0.1),
+                     cases.plate_input(threads=1, nmax=10, **inp))
+    renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+    c = native.HostCase(d)
+    dev, orc = native.Device(c), po.Oracle(c, threads=8)
+    n = 1500
+    dev.iterate(n)
+    orc.iterate(n)
+    Ug, Uo = dev.get_state(), orc.get_state()
+    rel = np.abs(Ug - Uo).max(axis=1) / np.abs(Uo).max(axis=1)
+    print(kind, "fields rel", rel, flush=True)
+    assert rel.max() < 1e-8
+    wd, wo = dev.wall_tensor(), orc.wall_tensor()
+    rows = ~np.isnan(wo[:, 1])
+    cf = np.abs(wd[rows, 1] - wo[rows, 1]) / np.abs(wo[rows, 1]).max()
+    print("Cf rel", cf.max(), flush=True)
+    assert cf.max() < 1e-8

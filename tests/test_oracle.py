@@ -150,3 +150,26 @@ def test_interior_flux_conservation(tmp_path):
     assert inner.size > 0
     assert np.all(np.abs(tot) < 0.5 * mag)
     assert np.all(np.isfinite(scale))
+
+
+@pytest.mark.parametrize("name", ["friction_plate_lam", "friction_plate_sa"])
+def test_wall_tensor_vs_reference_friction_csv(tmp_path, name):
+    """The oracle's wall shear after n iterations against the reference
+    executable's friction.csv (printed with %e: 7 significant digits)."""
+    import ast
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    mk, inp, n = ast.literal_eval(str(g["mesh_kwargs"])), ast.literal_eval(str(g["inputs"])), int(g["niter"])
+    d = str(tmp_path) + "/"
+    cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=n, **inp))
+    c = native.HostCase(d)
+    o = po.Oracle(c)
+    o.iterate(n)
+    w = o.wall_tensor()
+    a = c.arrays()
+    wall = [q for q in range(c.nbface) if a["mark_type"][a["bface_mark"][q]] in (3, 4)]
+    xy = a["node_xy"]
+    x = np.array([(xy[a["bface"][q, 1], 0] + xy[a["bface"][q, 2], 0]) / 2 for q in wall])
+    ours = np.column_stack([x, w[wall]])
+    ref = g["friction"]
+    assert ours.shape == ref.shape
+    assert np.allclose(ours, ref, rtol=6e-7, atol=1e-300)
