@@ -61,10 +61,18 @@ MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 939.13e6, ("k_flux", 2000, 1000): 16
                     ("k_update", 2000, 1000): 892.76e6}
 
 
+_MESHES = {}
+
+
 def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision: int = 0, keep: bool = False):
     from csu_amd import cases, native
     d = tempfile.mkdtemp(prefix="bn", dir="/tmp") + "/"
-    cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=steps, precision=precision)
+    if (nx, ny) in _MESHES:       # same mesh, other options: link it, write input.dat
+        os.symlink(_MESHES[(nx, ny)], d + "mesh.su2")
+        cases.write_input(d + "input.dat", cases.ramp_input(threads=1, nmax=steps, precision=precision))
+    else:
+        cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=steps, precision=precision)
+        _MESHES[(nx, ny)] = d + "mesh.su2"
     case = native.HostCase(d)
     dev = native.Device(case)
     dev.iterate(warmup, want=False)

@@ -399,7 +399,10 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
     }
     return -1;
   };
-  std::vector<unsigned char> fsl(nf > 0 ? nf : 1, 0);
+  std::vector<unsigned char> fsl(nf > 0 ? nf : 1, 0), bmask(nc, 0);
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < cnp[i]; j++)
+      if (m->cell_nbr[4 * i + j] < 0) bmask[i] |= (unsigned char)(1 << j);
   std::vector<int> fe0(nf), fe1(nf);
   std::vector<double> fdsx(nf), fdsy(nf), fds(nf), fdx0(nf), fdy0(nf), fdx1(nf), fdy1(nf);
   for (int f = 0; f < nf; f++) {
@@ -496,6 +499,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
       (rc = upload(d, &g.dsxm, dsxm)) || (rc = upload(d, &g.dsym, dsym)) || (rc = upload(d, &g.dslat, dslat)) ||
       (rc = upload(d, &g.edx, edx)) || (rc = upload(d, &g.edy, edy)) || (rc = upload(d, &g.wdist, wd)) ||
       (rc = upload(d, &g.lwx, lwx)) || (rc = upload(d, &g.lwy, lwy)) || (rc = upload(d, &g.fsl, fsl)) ||
+      (rc = upload(d, &g.bmask, bmask)) ||
       (rc = upload(d, &g.fe0, fe0)) || (rc = upload(d, &g.fe1, fe1)) || (rc = upload(d, &g.fdsx, fdsx)) ||
       (rc = upload(d, &g.fdsy, fdsy)) || (rc = upload(d, &g.fds, fds)) || (rc = upload(d, &g.fdx0, fdx0)) ||
       (rc = upload(d, &g.fdy0, fdy0)) || (rc = upload(d, &g.fdx1, fdx1)) || (rc = upload(d, &g.fdy1, fdy1)) ||

@@ -38,6 +38,7 @@ struct Geo {
   const double *edx, *edy;                              // [4][nc] own-edge midpoint offsets
   const double *lwx, *lwy;                              // [4][nc] LSQ weights per neighbour slot
   const unsigned char* fsl;                             // [nf] edge of the face in e0 (bits 0-1), e1 (2-3)
+  const unsigned char* bmask;                           // [nc] bit j: neighbour slot j is a boundary face
   const float *lwx32, *lwy32, *edx32, *edy32;           // mixed precision copies of lwx, lwy, edx, edy
   const float *fdsx32, *fdsy32, *fds32;                 // mixed precision face vectors
   const double *wdist;                                  // [nc] wall distance (SA/SST)

@@ -457,7 +457,6 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
   for (int kk = 0; kk < NV; kk++) R[kk] = 0.0;
   double dt_next = INFINITY;
   if (active) {
-    const int np = g.cnp[i];
     int cf[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -474,10 +473,11 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
     double Pi[NV + 1];
     bool loaded = false;
     int gbq[4];
+    const int bm = g.bmask[i];  // most cells have no boundary face: skip their neighbour list
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       gbq[j] = -1;
-      if (j >= np) break;
+      if (!((bm >> j) & 1)) continue;
       const int n = g.cnbr[j * nc + i];
       if (n < 0) {
         const int q = -1 - n;

@@ -14,7 +14,7 @@ def _jobs(tmpcase):
         ("ramp", dict(nx=40, ny=20, timeScheme="LUSGS", CFL=20, flux="AUSMDV")),
         ("ramp", dict(nx=24, ny=12, timeScheme="LUSGS", CFL=10, flux="ROE")),
         ("plate", dict(laminar="1", timeScheme="LUSGS", CFL="20")),
-        ("ramp", dict(nx=30, ny=30, timeScheme="LUSGS", CFL=30, flux="AUSM", mach=1.8)),
+        ("ramp", dict(nx=30, ny=30, timeScheme="LUSGS", CFL=10, flux="AUSMDV", mach=2.0)),
         ("plate", dict(laminar="0", sa="1", timeScheme="LUSGS", CFL="10")),
         ("ramp", dict(nx=50, ny=10)),
     ]
@@ -53,8 +53,8 @@ def test_mix_equals_alone(tmpcase):
         print(i, a["status"], a["converged"], a["iters"], a["iter_conv"], a["res0"], a["res_last"], flush=True)
         assert a["status"] == 0 and b["status"] == 0
         assert (a["iters"], a["iter_conv"], a["converged"]) == (b["iters"], b["iter_conv"], b["converged"])
-        assert a["res_last"] == b["res_last"] and a["coef_last"] == b["coef_last"]
-        assert np.array_equal(smix[i], salone[i])
+        assert np.array_equal([a["res_last"]] + a["coef_last"], [b["res_last"]] + b["coef_last"], equal_nan=True)
+        assert np.array_equal(smix[i], salone[i], equal_nan=True)
     # the mix overlapped: some solve was admitted before another finished
     spans = sorted((r["t_admit"], r["t_done"]) for r in mix.values())
     assert any(spans[k + 1][0] < spans[k][1] for k in range(len(spans) - 1))
