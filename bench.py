@@ -94,35 +94,28 @@ def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision:
 
 
 def serving_requests(n: int, seed: int = 4):
-    """Seeded request mix for the serving measurement: supersonic Euler
-    ramps, laminar plates and SA plates, LU-SGS on colour/RCM-renumbered
-    meshes, cell counts log-uniform in [10k, 100k], Mach per kind."""
+    """Seeded requests for the serving measurement: supersonic Euler ramps
+    (15 deg, seeded node jitter), Mach U[2.4, 3.2], cell counts log-uniform
+    in [10k, 40k], LU-SGS at CFL 5 with Roe flux on colour/RCM-renumbered
+    meshes -- settings under which the reference's scheme reaches a 1e-8
+    residual drop (its laminar/SA plates level off near 1e-3, and LU-SGS at
+    higher CFL or Mach < 2.4 diverges on these meshes)."""
     from csu_amd import cases, native, renumber
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n):
-        kind = ("ramp", "plate", "sa")[k % 3]
-        ncell = float(np.exp(rng.uniform(np.log(1e4), np.log(1e5))))
+        ncell = float(np.exp(rng.uniform(np.log(1e4), np.log(4e4))))
         d = tempfile.mkdtemp(prefix=f"sv{k}_", dir="/tmp") + "/"
-        if kind == "ramp":
-            ny = max(8, int(round(np.sqrt(ncell / 4))))
-            nx = max(8, int(round(ncell / (2 * ny))))
-            cases.ramp_case(d, nx=nx, ny=ny, seed=k, threads=1, nmax=1, timeScheme="LUSGS", CFL=20,
-                            flux="AUSMDV", mach=f"{rng.uniform(1.8, 3.0):.4f}")
-        else:
-            ny = max(8, int(round(np.sqrt(ncell / 5))))
-            nx = max(8, int(round(ncell / ny)))
-            inp = dict(timeScheme="LUSGS", CFL="20", mach=f"{rng.uniform(0.2, 0.5):.4f}")
-            if kind == "sa":
-                inp.update(laminar="0", sa="1")
-            cases.write_case(d, cases.plate_mesh(nx=nx, ny=ny, y1=2e-3, growth=1.05, jitter=0.05, seed=k),
-                             cases.plate_input(threads=1, nmax=1, **inp))
+        ny = max(8, int(round(np.sqrt(ncell / 4))))
+        nx = max(8, int(round(ncell / (2 * ny))))
+        cases.ramp_case(d, nx=nx, ny=ny, seed=k, threads=1, nmax=1, timeScheme="LUSGS", CFL=5, flux="ROE",
+                        mach=f"{rng.uniform(2.4, 3.2):.4f}")
         renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
-        out.append((kind, native.HostCase(d)))
+        out.append(("ramp", native.HostCase(d)))
     return out
 
 
-def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 3000, chunk: int = 50):
+def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 4000, chunk: int = 50):
     """Feed the requests to a csu_server at the given arrival times (s);
     returns the job results in submit order."""
     from csu_amd import native
@@ -154,10 +147,11 @@ def serving(n: int, rate: float, slots: int, log):
     for name, sl in (("served", slots), ("one_at_a_time", 1)):
         sat = serve_run(reqs, sl, np.zeros(n))
         span = max(r["t_done"] for r in sat) - min(r["t_submit"] for r in sat)
-        log(f"serving {name}: saturated {n} solves in {span:.2f} s")
+        nconv = sum(r["converged"] for r in sat)
+        log(f"serving {name}: saturated {n} solves ({nconv} converged) in {span:.2f} s")
         st = serve_run(reqs, sl, arrivals)
-        ttc = np.array([r["t_done"] - r["t_submit"] for r in st])
-        out[name] = {"solves_per_hr_saturated": 3600.0 * n / span,
+        ttc = np.array([r["t_done"] - r["t_submit"] for r in st if r["converged"]] or [np.nan])
+        out[name] = {"solves_per_hr_saturated": 3600.0 * nconv / span,
                      "p50_ttc_s": float(np.percentile(ttc, 50)), "p99_ttc_s": float(np.percentile(ttc, 99)),
                      "converged": int(sum(r["converged"] for r in st)),
                      "iters_p50": float(np.median([r["iters"] for r in st])),

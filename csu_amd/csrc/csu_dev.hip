@@ -519,6 +519,10 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   }
   d->wall = const_cast<int*>(wall_d);
   s.pref2 = const_cast<double*>(pref_d);
+  if ((rc = dalloc(d, &s.bflux, (size_t)nv * nb))) {
+    csu_dev_destroy(d);
+    return rc;
+  }
   if (p->order == 2 && !kp.mixed && (rc = dalloc(d, &s.rec, 4 * nvc))) {
     csu_dev_destroy(d);
     return rc;
@@ -736,6 +740,11 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
     Prof p(d, "k_lim1");
     hipLaunchKernelGGL(k_lim1_partial<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->st.P, d->red);
     hipLaunchKernelGGL(k_lim1_final<NV>, dim3(1), dim3(CSU_BLOCK), 0, d->stream, kp, nblk(nc), d->red, d->st.pref2);
+  }
+  if (d->nb > 0) {
+    // ghost states first: the gradients read them
+    Prof p(d, "k_bcflux");
+    hipLaunchKernelGGL(k_bcflux<NV>, dim3(nblk(d->nb)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
   }
   {
     Prof p(d, "k_grad");

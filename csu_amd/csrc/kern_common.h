@@ -58,6 +58,7 @@ struct St {
   float *rec32;         // mixed precision: the same in FP32
   float *fflux32;       // mixed precision: convective face fluxes in FP32 [NV][nf]
   double *wten;         // viscous: wall shear tensor txx txy tyy yp per boundary face [4][nb]
+  double *bflux;        // convective boundary-face flux times dS [NV][nb]
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference

@@ -77,8 +77,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   double Pi[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) Pi[m] = s.P[m * nc + i];
-  // neighbours: cells, or ghost states of boundary faces (written to Pb
-  // here and read back per variable below by this same thread)
+  // neighbours: cells, or the ghost states of boundary faces (k_bcflux)
   double dx[4], dy[4];
   int gb[4], nbr[4];
 #pragma unroll
@@ -96,10 +95,6 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       gb[j] = q;
       dx[j] = g.bxm[q] - x0;
       dy[j] = g.bym[q] - y0;
-      double Pg[NS];
-      ghost_state<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, NV == 6 ? g.wdist[i] : 0.0, Pg);
-#pragma unroll
-      for (int m = 0; m < NS; m++) s.Pb[m * nb + q] = Pg[m];
     }
   }
   auto nval = [&](int j, int m) -> double {
@@ -434,6 +429,29 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
   }
 }
 
+// Per boundary face, from its cell's primitives: the ghost state
+// (boundaryCalcPrimitive, boundary.c:514-693) into Pb, and the convective
+// flux from the first-order state, boundaryCalc / _sa / _sst
+// (boundary.c:143-446): f*dS, or 0 when dS = 0.
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_bcflux(KP kp, Geo g, St s) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc, nb = kp.nb;
+  if (q >= nb) return;
+  const int i = g.be0[q];
+  double Pi[NV + 1], R[NV], Pg[NV + 1];
+#pragma unroll
+  for (int m = 0; m < NV + 1; m++) Pi[m] = s.P[m * nc + i];
+  ghost_state<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, NV == 6 ? g.wdist[i] : 0.0, Pg);
+#pragma unroll
+  for (int m = 0; m < NV + 1; m++) s.Pb[m * nb + q] = Pg[m];
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) R[kk] = 0.0;
+  bc_flux<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, R);
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) s.bflux[kk * nb + q] = R[kk];
+}
+
 // Residual sums + RK update.  Order of the per-cell sums follows
 // solverCalcR: interior convective faces (solver.c:646-667), boundary
 // convective fluxes (boundary.c:447-464), axisymmetric pressure
@@ -470,8 +488,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
         for (int kk = 0; kk < NV; kk++) R[kk] -= MIX ? (double)s.fflux32[kk * nf - f - 1] : s.fflux[kk * nf - f - 1];
       }
     }
-    double Pi[NV + 1];
-    bool loaded = false;
+    // boundary fluxes (k_bcflux), in the cell's slot order = marker order
     int gbq[4];
     const int bm = g.bmask[i];  // most cells have no boundary face: skip their neighbour list
 #pragma unroll
@@ -482,12 +499,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
       if (n < 0) {
         const int q = -1 - n;
         gbq[j] = q;
-        if (!loaded) {
 #pragma unroll
-          for (int m = 0; m < NV + 1; m++) Pi[m] = s.P[m * nc + i];
-          loaded = true;
-        }
-        bc_flux<NV>(kp, g.btype[q], g.bdsx[q], g.bdsy[q], g.bds[q], Pi, R);
+        for (int kk = 0; kk < NV; kk++) R[kk] += s.bflux[kk * nb + q];
       }
     }
     if (kp.axi == 1) {
