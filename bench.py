@@ -42,13 +42,18 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int) -> float:
     ns = nv + 1
     if kernel == "k_prim":
         return nc * (8 * nv + 8 * ns)
+    edges = 2 * nf + nb                 # cell-edge slots (interior faces count twice)
     if kernel == "k_grad":
-        return (nc * (8 * ns + 4 + 16 + 16 + 8 + 64 + 16 * nv + 8 * nv)
-                + nb * (8 * ns + 44))
+        # own P, cnp, cnbr, omega; per slot LSQ weights and edge offsets; the
+        # edge states written; ghost states read
+        return nc * (8 * ns + 4 + 16 + 8) + edges * (16 + 16 + 8 * nv) + nb * 8 * ns
     if kernel == "k_flux":
-        return nf * (8 + 56 + 8 * nv) + nc * (8 * ns + 16 * nv + 8 * nv)
+        # e0, e1, edge slots, face vector; both edge states; the flux
+        return nf * (8 + 1 + 24 + 2 * 8 * nv + 8 * nv)
     if kernel == "k_update":
-        return nc * (16 + 16 + 4 + 8 * nv * 3 + 16) + nf * 8 * nv + nb * 44
+        # face list, boundary mask, U^n, U out, P out, omega, dt; each face
+        # flux once
+        return nc * (16 + 1 + 8 * nv * 2 + 8 * ns + 8 + 8) + nf * 8 * nv + nb * 8 * nv
     return 0.0
 
 
@@ -57,8 +62,8 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int) -> float:
 # counters read for a plain 1 GiB copy (profiles/euler_ramp_4M_rk5_fp64.md).
 # A counter pass cannot run inside this timed process, so the figure is
 # reported for that workload only.
-MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 939.13e6, ("k_flux", 2000, 1000): 1689.89e6,
-                    ("k_update", 2000, 1000): 892.76e6}
+MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 1182.2e6, ("k_flux", 2000, 1000): 775.5e6,
+                    ("k_update", 2000, 1000): 786.6e6}
 
 
 _MESHES = {}
