@@ -119,6 +119,54 @@ def ramp_input(threads: int = 8, nmax: int = 2000, **over) -> dict:
     return d
 
 
+# --------------------------------------------------------------------------
+# Shock tube (the reference's tube mode): channel [0,length]x[0,height],
+# nx x ny quads (split into triangles unless split=False), interior nodes
+# jittered as in ramp_mesh; markers wallDown, wallRight, wallUp, wallLeft as
+# in the reference's Sod geometry (caseSod/geometry.geo).
+# --------------------------------------------------------------------------
+def tube_mesh(nx: int = 125, ny: int = 12, length: float = 50.0, height: float = 5.0, jitter: float = 0.1,
+              seed: int = 0, split: bool = True) -> Mesh2D:
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.linspace(0.0, length, nx + 1), np.linspace(0.0, height, ny + 1), indexing="ij")
+    dx, dy = length / nx, height / ny
+    jx = rng.uniform(-jitter, jitter, size=(nx + 1, ny + 1))
+    jy = rng.uniform(-jitter, jitter, size=(nx + 1, ny + 1))
+    X[1:nx, 1:ny] += jx[1:nx, 1:ny] * dx
+    Y[1:nx, 1:ny] += jy[1:nx, 1:ny] * dy
+    nid = lambda i, j: i * (ny + 1) + j          # noqa: E731
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+    cells = []
+    for i in range(nx):
+        for j in range(ny):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            if split:
+                cells.append((a, b, c))
+                cells.append((a, c, d))
+            else:
+                cells.append((a, b, c, d))
+    down = [(nid(i, 0), nid(i + 1, 0)) for i in range(nx)]
+    right = [(nid(nx, j), nid(nx, j + 1)) for j in range(ny)]
+    up = [(nid(i + 1, ny), nid(i, ny)) for i in range(nx)]
+    left = [(nid(0, j + 1), nid(0, j)) for j in range(ny)]
+    markers = [("wallDown", down), ("wallRight", right), ("wallUp", up), ("wallLeft", left)]
+    return Mesh2D(nodes=nodes, cells=cells, markers=markers)
+
+
+def tube_input(threads: int = 8, tmax: float = 0.02, **over) -> dict:
+    """Sod problem of the reference's caseSod/input.dat: 1e5 Pa, 300 K left
+    of xm = 25, 1e4 Pa, 240 K right of it, gas at rest, walls all round."""
+    d = {
+        "pressure1": "1e5", "mach1": "0.", "temperature1": "300.", "nx1": "1.", "ny1": "0.",
+        "pressure2": "1e4", "mach2": "0.", "temperature2": "240.", "nx2": "1.", "ny2": "0.",
+        "BC:wallUp": "wall", "BC:wallDown": "wall", "BC:wallRight": "wall", "BC:wallLeft": "wall",
+        "xm": "25.", "tmax": repr(float(tmax)), "order": "1", "stages": "5", "threads": str(threads),
+        "flux": "ROE", "tube": "1", "axisymmetric": "0", "CFL": "1.0",
+    }
+    d.update({k: str(v) for k, v in over.items()})
+    return d
+
+
 def write_case(dirpath: str, mesh: Mesh2D, inputs: dict) -> str:
     os.makedirs(dirpath, exist_ok=True)
     if not dirpath.endswith("/"):

@@ -84,6 +84,7 @@ struct csu_dev {
   double *hres, *hcoef;
   int hcap, async_start, async_n;
   int use_graphs;
+  double* tstate;              // time-accurate run: t, tmax, state, steps (k_time_clamp)
   // profiling
   int prof;
   std::vector<std::string> kname;
@@ -301,7 +302,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   d->visc = (p->laminar || p->sa || p->sst) ? 1 : 0;
   d->hist_cap = 0; d->res_hist = d->coef_hist = nullptr; d->fresh = 1;
   d->Dp = d->dW0 = d->dW1 = d->A = nullptr; d->jstate = nullptr; d->cellsL = d->cellsU = nullptr;
-  d->hctr = nullptr; d->use_graphs = 1; d->bytes = 0;
+  d->hctr = nullptr; d->use_graphs = 1; d->bytes = 0; d->tstate = nullptr;
   d->hres = d->hcoef = nullptr; d->hcap = 0; d->async_start = 0; d->async_n = 0;
   for (int k = 0; k < 4; k++) { d->gexec[k] = nullptr; d->gU[k] = nullptr; }
   memset((void*)&d->kp, 0, sizeof(KP));
@@ -605,7 +606,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
     }
   }
   d->Ubase = d->U;
-  if ((rc = dalloc(d, &d->hctr, 1))) {
+  if ((rc = dalloc(d, &d->hctr, 1)) || (rc = dalloc(d, &d->tstate, 4))) {
     csu_dev_destroy(d);
     return rc;
   }
@@ -836,6 +837,25 @@ extern "C" int csu_dev_residual(csu_dev* d, double* R, double* dPx, double* dPy,
   return CSU_OK;
 }
 
+// solverCalcPrimitive(solver, U) (solver.c:1153-1213) on the device state:
+// U clamped in place, P [nc][nv+1] to the host (NULL allowed).
+extern "C" int csu_dev_primitives(csu_dev* d, double* P) {
+  if (!d) return set_err(CSU_ERR_ARG, "null argument");
+  int rc = d->nv == 6 ? prim_pass<6>(d) : d->nv == 5 ? prim_pass<5>(d) : prim_pass<4>(d);
+  if (rc) return rc;
+  d->fresh = 1;
+  if (P) {
+    const int ns = d->nv + 1;
+    std::vector<double> tmp((size_t)ns * d->nc);
+    HIPCHK(hipMemcpyAsync(tmp.data(), d->st.P, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    for (int i = 0; i < d->nc; i++)
+      for (int m = 0; m < ns; m++) P[(size_t)i * ns + m] = tmp[(size_t)m * d->nc + i];
+  }
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return CSU_OK;
+}
+
 // RK coefficients, reference solverStepRK (solver.c:788-840)
 static const double RK1[] = {1.0};
 static const double RK3[] = {0.1481, 0.4, 1.0};
@@ -909,6 +929,8 @@ static int rk_iteration(csu_dev* d, int it_index) {
     first_flags = 0;
     d->fresh = 0;
   }
+  if (d->st.tube)
+    hipLaunchKernelGGL(k_time_clamp, dim3(1), dim3(64), 0, d->stream, d->tstate, d->dtg, d->prm.stages);
   for (int s = 0; s < d->prm.stages; s++) {
     const bool last = s == d->prm.stages - 1;
     int flags = F_RK | (cur_local ? F_CUR_LOCAL : 0) | (s == 0 ? first_flags : 0);
@@ -1055,6 +1077,50 @@ extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* c
       if (coef) memcpy(coef + (size_t)it * 4, ch.data() + (size_t)slot * 4, sizeof(double) * 4);
     }
   }
+  return prof_collect(d);
+}
+
+// Time-accurate run to tmax, the tube branch of solverSolve
+// (solver.c:1912-1941).  The step control runs on the device
+// (k_time_clamp), so iterations are enqueued in chunks; the chunk length
+// follows the remaining time over the current step, and iterations a chunk
+// holds past the last step leave U unchanged.
+extern "C" int csu_dev_advance_time(csu_dev* d, double tmax, int32_t max_iter, double* t, int32_t* niter,
+                                    int32_t* finished) {
+  if (!d || !t) return set_err(CSU_ERR_ARG, "null argument");
+  if (d->prm.time_scheme != CSU_TIME_RK || d->prm.dt_local == 1)
+    return set_err(CSU_ERR_UNSUPPORTED, "a time-accurate run needs the RK scheme with a global time step");
+  if (!(tmax >= *t)) return set_err(CSU_ERR_ARG, "tmax %g before t %g", tmax, *t);
+  double ts[4] = {*t, tmax, 0.0, 0.0}, dt = 0.0;
+  HIPCHK(hipMemcpyAsync(d->tstate, ts, sizeof ts, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  graphs_drop(d);
+  d->st.tube = d->tstate;
+  int rc = CSU_OK, chunk = 4;
+  for (;;) {
+    const int done = (int)ts[3];
+    if (max_iter > 0 && chunk > max_iter - done) chunk = max_iter - done;
+    if ((rc = enqueue_iterations(d, chunk))) break;
+    HIPCHK(hipMemcpyAsync(ts, d->tstate, sizeof ts, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(&dt, d->dtg, sizeof dt, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    if (ts[2] != 0.0) break;
+    if (!(dt > 0.0) || !std::isfinite(dt)) {
+      rc = set_err(CSU_ERR_ARG, "time step %g at t = %g after %d steps: the state is no longer physical", dt, ts[0],
+                   (int)ts[3]);
+      break;
+    }
+    if (max_iter > 0 && (int)ts[3] >= max_iter) break;
+    const double left = (tmax - ts[0]) / (dt * d->prm.stages / 2.0);
+    chunk = left > 256 ? 256 : left < 1 ? 1 : (int)left + 1;
+  }
+  graphs_drop(d);
+  d->st.tube = nullptr;
+  d->fresh = 1;
+  *t = ts[0];
+  if (niter) *niter = (int32_t)ts[3];
+  if (finished) *finished = ts[2] != 0.0;
+  if (rc) return rc;
   return prof_collect(d);
 }
 

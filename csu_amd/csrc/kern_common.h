@@ -59,6 +59,7 @@ struct St {
   float *fflux32;       // mixed precision: convective face fluxes in FP32 [NV][nf]
   double *wten;         // viscous: wall shear tensor txx txy tyy yp per boundary face [4][nb]
   double *bflux;        // convective boundary-face flux times dS [NV][nb]
+  const double *tube;   // time-accurate run: t, tmax, state (0 running, 1 last step, 2 past it), steps
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference

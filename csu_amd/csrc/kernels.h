@@ -590,7 +590,9 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
       double un[NV], un0[NV], p[NV + 1];
 #pragma unroll
       for (int kk = 0; kk < NV; kk++) un0[kk] = un[kk] = Un[kk * nc + i];
-      if (flags & F_CLAMP_UN) {
+      // a time-accurate run that has taken its last step holds U bitwise
+      const bool hold = s.tube && s.tube[2] > 1.5;
+      if ((flags & F_CLAMP_UN) && !hold) {
         // first stage: the reference clamps U^n in place (solverCalcR(U))
         prim_point<NV>(kp, un, p);
 #pragma unroll
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
       const double om = g.omega[i];
       double unew[NV];
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) unew[kk] = un[kk] - dt * sigma * rk_a * R[kk] / om;
+      for (int kk = 0; kk < NV; kk++) unew[kk] = hold ? un0[kk] : un[kk] - dt * sigma * rk_a * R[kk] / om;
       if (flags & F_LAST) {
         // U^{n+1} stays unclamped until the next residual, as in the reference
         dt_next = cell_dt<NV>(kp, unew, om, g.dsxm[i], g.dsym[i]);
@@ -678,6 +680,29 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_dt_final(int nblocks, double cfl,
     __syncthreads();
   }
   if (t == 0) dtglob[0] = smin[0] * cfl;
+}
+
+// Time-accurate step control, the tube branch of solverSolve
+// (solver.c:1912-1936): the global step is cut to reach tmax and the run
+// stops after that step; t advances by dt*stages/2.  ts = t, tmax, state,
+// steps; state 0 running, 1 this is the last step, 2 past the last step
+// (dt 0 and the update kernel holds U).
+__global__ void k_time_clamp(double* __restrict__ ts, double* __restrict__ dtg, int stages) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ts[2] != 0.0) {
+    ts[2] = 2.0;
+    dtg[0] = 0.0;
+    return;
+  }
+  const double t = ts[0];
+  double dt = dtg[0];
+  if (t + dt > ts[1]) {
+    dt = ts[1] - t;
+    dtg[0] = dt;
+    ts[2] = 1.0;
+  }
+  ts[0] = t + dt * stages / 2.0;
+  ts[3] += 1.0;
 }
 
 // Final level of the residual norm: one block, each thread sums a strided

@@ -85,6 +85,36 @@ static int choice(const char* s, const char* const* names, int n)
 
 const char* csu_host_missing_key(void) { return g_missing; }
 
+/* conditionState (solver.c:40-75) plus the SA/SST inlet values
+   (saInitU, sstInitU): primitive pin[7] and conservative uin[6] */
+static void condition_state(const hgas* g, const csu_params* p, double pinf, double Tinf, double mach, double nx,
+                            double ny, double* pin, double* uin)
+{
+    double r = pinf / (g->R * Tinf);
+    double cs = hgas_T2c(g, Tinf);
+    double u = nx * mach * cs;
+    double v = ny * mach * cs;
+    double k = 0.0;
+    for (int i = 0; i < 7; i++) pin[i] = 0.0;
+    for (int i = 0; i < 6; i++) uin[i] = 0.0;
+    if (p->sst == 1) {
+        double L = p->sst_l, Uv = cs * mach;
+        double ReL = (r * Uv * L) / sutherland(Tinf);
+        k = p->sst_kfactor * Uv * Uv / ReL;
+        double om = p->sst_ofactor * Uv / L;
+        uin[4] = r * k; uin[5] = r * om;
+        pin[5] = k; pin[6] = om;
+    }
+    double E = hgas_T2e(g, Tinf) + (u * u + v * v) / 2 + k;
+    uin[0] = r; uin[1] = r * u; uin[2] = r * v; uin[3] = r * E;
+    pin[0] = r; pin[1] = u; pin[2] = v; pin[3] = g->R * Tinf * r; pin[4] = Tinf;
+    if (p->sa == 1) {
+        double n = p->turb_ratio * sutherland(Tinf) / r;
+        uin[4] = r * n;
+        pin[5] = n;
+    }
+}
+
 int csu_case_load(const char* wd, csu_case** out)
 {
     csu_case* c = calloc(1, sizeof(csu_case));
@@ -179,44 +209,43 @@ int csu_case_load(const char* wd, csu_case** out)
        distance on the device when mesh.wall_dist is NULL; csu_wall_distance
        is the host restatement for callers that want it on the host. */
 
-    /* solverInitDomain, solver.c:1660-1697 (steady mode only) */
+    /* solverInitDomain, solver.c:1660-1725 */
     c->tube = atoi(need(in, "tube"));
-    if (c->tube != 0) { csu_case_free(c); return CSU_ERR_UNSUPPORTED; }
-    c->nmax = atoi(need(in, "Nmax"));
-    double pinf = strtod(need(in, "pressure"), NULL);
-    double Tinf = strtod(need(in, "temperature"), NULL);
-    double mach = strtod(need(in, "mach"), NULL);
-    double nx = strtod(need(in, "nx"), NULL);
-    double ny = strtod(need(in, "ny"), NULL);
-    if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
-
     hgas g;
     hgas_init(&g, p->gamma, p->gas_r, p->tp);
-    /* conditionState, solver.c:40-89 */
-    double r = pinf / (g.R * Tinf);
-    double cs = hgas_T2c(&g, Tinf);
-    double u = nx * mach * cs;
-    double v = ny * mach * cs;
-    double k = 0.0;
-    for (int i = 0; i < 7; i++) p->pin[i] = 0.0;
-    for (int i = 0; i < 6; i++) p->uin[i] = 0.0;
-    if (p->sst == 1) {
-        double L = p->sst_l, Uv = cs * mach;
-        double ReL = (r * Uv * L) / sutherland(Tinf);
-        k = p->sst_kfactor * Uv * Uv / ReL;
-        double om = p->sst_ofactor * Uv / L;
-        p->uin[4] = r * k; p->uin[5] = r * om;
-        p->pin[5] = k; p->pin[6] = om;
+    double pin2[7], uin2[6], xm = 0.0;
+    if (c->tube == 0) {
+        c->nmax = atoi(need(in, "Nmax"));
+        double pinf = strtod(need(in, "pressure"), NULL);
+        double Tinf = strtod(need(in, "temperature"), NULL);
+        double mach = strtod(need(in, "mach"), NULL);
+        double nx = strtod(need(in, "nx"), NULL);
+        double ny = strtod(need(in, "ny"), NULL);
+        if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
+        condition_state(&g, p, pinf, Tinf, mach, nx, ny, p->pin, p->uin);
+        p->mach_inf = mach;
+    } else {
+        /* two states split at x = xm (solverInitUTube, solver.c:952-975);
+           the first one is the "inlet" of the boundary conditions */
+        double p1 = strtod(need(in, "pressure1"), NULL), T1 = strtod(need(in, "temperature1"), NULL);
+        double m1 = strtod(need(in, "mach1"), NULL), nx1 = strtod(need(in, "nx1"), NULL);
+        double ny1 = strtod(need(in, "ny1"), NULL);
+        double p2 = strtod(need(in, "pressure2"), NULL), T2 = strtod(need(in, "temperature2"), NULL);
+        double m2 = strtod(need(in, "mach2"), NULL), nx2 = strtod(need(in, "nx2"), NULL);
+        double ny2 = strtod(need(in, "ny2"), NULL);
+        xm = strtod(need(in, "xm"), NULL);
+        c->tmax = strtod(need(in, "tmax"), NULL);
+        if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
+        /* solverInitUTube sets 4 variables only: no turbulence model here;
+           the tube loop calls solverStepRK whatever timeScheme says, and
+           with dtLocal 1 it would advance t by a step it never computes */
+        if (p->sa || p->sst || p->dt_local == 1) { csu_case_free(c); return CSU_ERR_UNSUPPORTED; }
+        p->time_scheme = CSU_TIME_RK;
+        condition_state(&g, p, p1, T1, m1, nx1, ny1, p->pin, p->uin);
+        condition_state(&g, p, p2, T2, m2, nx2, ny2, pin2, uin2);
+        p->mach_inf = m1;
+        c->nmax = 0;
     }
-    double E = hgas_T2e(&g, Tinf) + (u * u + v * v) / 2 + k;
-    p->uin[0] = r; p->uin[1] = r * u; p->uin[2] = r * v; p->uin[3] = r * E;
-    p->pin[0] = r; p->pin[1] = u; p->pin[2] = v; p->pin[3] = g.R * Tinf * r; p->pin[4] = Tinf;
-    if (p->sa == 1) {
-        double n = p->turb_ratio * sutherland(Tinf) / r;
-        p->uin[4] = r * n;
-        p->pin[5] = n;
-    }
-    p->mach_inf = mach;
 
     int nc = c->mesh.ncell, nv = p->nvar;
     c->U = malloc(sizeof(double) * (size_t)nv * (size_t)nc);
@@ -225,10 +254,19 @@ int csu_case_load(const char* wd, csu_case** out)
         snprintf(path, sizeof path, "%srestart.csv", wd);
         rc = csu_read_restart(c, c->U, path);
         if (rc) { csu_case_free(c); return rc; }
-    } else {
+    } else if (c->tube == 0) {
         /* solverInitU (4 variables) + saInitU / sstInitU */
         for (int kk = 0; kk < nv; kk++)
             for (int i = 0; i < nc; i++) c->U[(size_t)kk * nc + i] = p->uin[kk];
+    } else {
+        for (int i = 0; i < nc; i++) {
+            const int32_t* cv = c->mesh.cell_vert + 4 * i;
+            int np = cv[3] >= 0 ? 4 : 3;
+            double x = 0.0;
+            for (int k = 0; k < np; k++) x += c->mesh.node_xy[2 * cv[k]];
+            x /= np;
+            for (int kk = 0; kk < 4; kk++) c->U[(size_t)kk * nc + i] = x < xm ? p->uin[kk] : uin2[kk];
+        }
     }
     *out = c;
     return CSU_OK;

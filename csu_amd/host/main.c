@@ -5,7 +5,8 @@
  * (solver.c:1836-1911): reads <case>/input.dat, mesh.su2 (and restart.csv
  * when restart=1), runs Nmax iterations, prints one line per iteration and
  * appends it to convergence.csv, then writes solution.csv, restart.csv and
- * friction.csv.  The iterations run on the GPU through csu.h; the host only
+ * friction.csv.  With tube=1 it runs the time-accurate branch to tmax
+ * instead (solver.c:1912-1941) and writes solution.csv and restart.csv.  The iterations run on the GPU through csu.h; the host only
  * formats the per-iteration records, which come back in blocks.
  *
  * As the reference, input errors print a message and exit with status 0
@@ -55,6 +56,18 @@ int main(int argc, char** argv)
     }
 
     char path[1200];
+    if (c->tube) {
+        /* tube branch of solverSolve (solver.c:1912-1941): no convergence
+           file, no wall shear; the run ends at tmax */
+        printf("\nmain: running solution:\n");
+        double t = 0.0;
+        int32_t n = 0, fin = 0;
+        rc = csu_dev_advance_time(d, c->tmax, 0, &t, &n, &fin);
+        if (rc) { printf("Error: time-accurate run failed: %s\n", csu_last_error()); return 1; }
+        printf("%i steps\n", n);
+        printf("time %f s\n", t);
+        goto write;
+    }
     snprintf(path, sizeof path, "%sconvergence.csv", wd);
     FILE* conv = fopen(path, c->restart ? "a" : "w");
     if (!conv) { printf("Error: cannot open %s\n", path); return 1; }
@@ -106,11 +119,17 @@ int main(int argc, char** argv)
         fclose(fr);
     }
 
+write:
     rc = csu_dev_get_state(d, c->U);
     if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }
     printf("main: saving the solution.\n");
     snprintf(path, sizeof path, "%ssolution.csv", wd);
     csu_write_solution(c, c->U, path);
+    /* solverWriteSolution2 clamps U in place (solverCalcPrimitive) before
+       the reference writes restart.csv */
+    rc = csu_dev_primitives(d, NULL);
+    if (!rc) rc = csu_dev_get_state(d, c->U);
+    if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }
     printf("main: saving the restart file.\n");
     snprintf(path, sizeof path, "%srestart.csv", wd);
     csu_write_restart(c, c->U, path);

@@ -52,6 +52,7 @@ class Params(C.Structure):
 class Case(C.Structure):
     _fields_ = [("mesh", MeshDesc), ("prm", Params), ("U", f64p), ("nmax", C.c_int32),
                 ("threads", C.c_int32), ("restart", C.c_int32), ("tube", C.c_int32),
+                ("tmax", C.c_double),
                 ("write_surf", C.c_char * 64), ("wd", C.c_char * 1024),
                 ("mark_name", C.c_void_p),
                 ("node_xy", f64p), ("cell_vert", i32p), ("face", i32p), ("bface", i32p),
@@ -87,6 +88,8 @@ def lib():
     L.csu_dev_get_dt.argtypes = [C.c_void_p, f64p]
     L.csu_dev_residual.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p]
     L.csu_dev_iterate.argtypes = [C.c_void_p, C.c_int32, f64p, f64p]
+    L.csu_dev_primitives.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_advance_time.argtypes = [C.c_void_p, C.c_double, C.c_int32, f64p, i32p, i32p]
     L.csu_dev_iteration.argtypes = [C.c_void_p]
     L.csu_dev_sync.argtypes = [C.c_void_p]
     L.csu_dev_set_profiling.argtypes = [C.c_void_p, C.c_int32]
@@ -182,6 +185,7 @@ class HostCase:
             m.ncell, m.nnode, m.nface, m.nbface, m.nmark)
         self.nvar = c.prm.nvar
         self.nmax = c.nmax
+        self.tube, self.tmax = c.tube, c.tmax
         self.params = c.prm
         if host_wall_distance and (c.prm.sa or c.prm.sst):
             check(L.csu_wall_distance(self._c))
@@ -285,6 +289,19 @@ class Device:
         check(lib().csu_dev_iterate(self._h, n, _p(res, C.c_double) if want else None,
                                     _p(coef, C.c_double) if want else None))
         return res, coef
+
+    def primitives(self) -> np.ndarray:
+        """Clamp the state in place (solverCalcPrimitive); P [ncell][nvar+1]."""
+        P = np.empty((self.case.ncell, self.nvar + 1))
+        check(lib().csu_dev_primitives(self._h, _p(P, C.c_double)))
+        return P
+
+    def advance_time(self, tmax: float, max_iter: int = 0, t0: float = 0.0):
+        """Time-accurate run to tmax (tube mode).  Returns (t, steps, finished)."""
+        t = C.c_double(t0)
+        n, fin = C.c_int32(0), C.c_int32(0)
+        check(lib().csu_dev_advance_time(self._h, tmax, max_iter, C.byref(t), C.byref(n), C.byref(fin)))
+        return t.value, n.value, bool(fin.value)
 
     def sync(self):
         check(lib().csu_dev_sync(self._h))

@@ -125,6 +125,23 @@ int  csu_dev_residual(csu_dev* dev, double* R, double* dPx, double* dPy,
  * Iterations are counted across calls (the dt_local_n switch uses it). */
 int  csu_dev_iterate(csu_dev* dev, int32_t niter, double* res, double* coef);
 
+/* solverCalcPrimitive(solver, solver->U) (solver.c:1153-1213): the
+ * positivity clamps applied to the device state in place, as the
+ * reference's solverWriteSolution2 does before solverWriteReestart writes
+ * restart.csv (main.c:29-31); P: [ncell][nvar+1] = r u v p T [nu~ | k om],
+ * or NULL. */
+int  csu_dev_primitives(csu_dev* dev, double* P);
+
+/* Time-accurate run: the tube branch of solverSolve (solver.c:1912-1941).
+ * Each step takes the global time step (CFL x the smallest local one), cut
+ * so that t + dt does not pass tmax; t advances by dt*stages/2 per step and
+ * the run ends after the cut step.  *t: start time in, end time out (the
+ * reference starts at 0).  max_iter > 0 caps the number of steps (0: no
+ * cap); *niter receives the steps taken and *finished 1 when the cut step
+ * was taken (either may be NULL).  Needs the RK scheme and dt_local 0. */
+int  csu_dev_advance_time(csu_dev* dev, double tmax, int32_t max_iter, double* t,
+                          int32_t* niter, int32_t* finished);
+
 /* Asynchronous form of csu_dev_iterate for callers that keep several
  * contexts busy at once: enqueue niter iterations on the context's own
  * stream and return; csu_dev_ready reports 1 once they (and the copy of

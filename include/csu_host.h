@@ -35,7 +35,8 @@ typedef struct csu_case {
     int32_t  nmax;             /* Nmax                                     */
     int32_t  threads;          /* threads (host CPU paths only)            */
     int32_t  restart;
-    int32_t  tube;
+    int32_t  tube;             /* 1: unsteady shock-tube mode to t = tmax  */
+    double   tmax;             /* tube: final time                         */
     char     write_surf[64];
     char     wd[1024];
     char   (*mark_name)[64];   /* [nmark]                                  */

@@ -2009,6 +2009,34 @@ int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
     return CSU_OK;
 }
 
+/* Time-accurate run, the tube branch of solverSolve (solver.c:1912-1941):
+   global step from solverCalcDt, cut to tmax - t when t + dt passes tmax
+   (that step is the last), solverStepRK, t += dt*stages/2.  The reference
+   loops without a cap and with no check of dt; here a non-positive or
+   non-finite step is an error and max_iter > 0 caps the steps. */
+int oracle_advance_time(csu_oracle* o, double tmax, int max_iter, double* t, int* niter, int* finished)
+{
+    if (o->prm.time_scheme != CSU_TIME_RK || o->dt_local == 1) return CSU_ERR_UNSUPPORTED;
+    double tt = *t;
+    int n = 0, stop = 0, rc = CSU_OK;
+    while (!stop && !(max_iter > 0 && n >= max_iter)) {
+        calc_dt(o);
+        if (!(o->dt > 0.0) || !isfinite(o->dt)) { rc = CSU_ERR_ARG; break; }
+        if (tt + o->dt > tmax) {
+            o->dt = tmax - tt;
+            stop = 1;
+        }
+        step_rk(o);
+        tt += o->dt * o->prm.stages / 2.0;
+        n++;
+        o->iter++;
+    }
+    *t = tt;
+    if (niter) *niter = n;
+    if (finished) *finished = stop;
+    return rc;
+}
+
 /* Wall shear of the last residual evaluation, solverCalcCoeff2
    (solver.c:1303-1340): boundaryCalcTensorWall (boundary.c:744-793) for
    laminar runs, saCalcTensorWall (sa.c:795-852) for SA and SST.

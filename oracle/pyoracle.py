@@ -44,6 +44,7 @@ def lib():
         L.oracle_iterate.argtypes = [vp, C.c_int, f64, f64]
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_wall_tensor.argtypes = [vp, f64]
+        L.oracle_advance_time.argtypes = [vp, C.c_double, C.c_int, f64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -95,6 +96,15 @@ class Oracle:
         coef = np.empty((n, 4))
         lib().oracle_iterate(self._h, n, _p(res), _p(coef))
         return res, coef
+
+    def advance_time(self, tmax: float, max_iter: int = 0, t0: float = 0.0):
+        """Time-accurate run to tmax (tube mode).  Returns (t, steps, finished)."""
+        t = C.c_double(t0)
+        n, fin = C.c_int(0), C.c_int(0)
+        rc = lib().oracle_advance_time(self._h, tmax, max_iter, C.byref(t), C.byref(n), C.byref(fin))
+        if rc != 0:
+            raise RuntimeError(f"oracle_advance_time failed ({rc})")
+        return t.value, n.value, bool(fin.value)
 
     def wall_tensor(self):
         """[nbface][4] txx txy tyy yp of the last residual (wall rows set)."""

@@ -25,6 +25,10 @@
  *       N iterations of the solverSolve loop body (solver.c:1864-1890);
  *       writes res.f64 [N][Nvar], coef.f64 [N][2] (solverCalcCoeff Cx, Cy),
  *       U.f64 (final state), dtl.f64 (last local time step, when present)
+ *   csu_ref_trace <case/> <out/> tube [N]
+ *       the tube branch of solverSolve (solver.c:1912-1936) to tmax (or N
+ *       steps at most) with the reference's solverCalcDt and solverStepRK;
+ *       writes U.f64 and t.f64 = [t, steps]
  *
  * Everything the reference prints goes to stdout; callers discard it.
  */
@@ -190,7 +194,7 @@ static void iterate_once(SOLVER* s, int ii)
 int main(int argc, char** argv)
 {
     if (argc < 4) {
-        fprintf(stderr, "usage: %s <case/> <out/> mesh|resid [U.f64]|hist N [U.f64]\n", argv[0]);
+        fprintf(stderr, "usage: %s <case/> <out/> mesh|resid [U.f64]|hist N [U.f64]|tube [N]\n", argv[0]);
         return 2;
     }
     snprintf(outdir, sizeof(outdir), "%s", argv[2]);
@@ -232,6 +236,23 @@ int main(int argc, char** argv)
         if (s->dtLocal == 1 || s->timeScheme != 0)
             write_bin("dtl.f64", s->dtL, sizeof(double) * (size_t)nc);
         free(res); free(coef);
+    } else if (strcmp(mode, "tube") == 0) {
+        double tmax = strtod(inputGetValue(s->input, "tmax"), NULL);
+        double t = 0.0;
+        int stop = 0, n = 0, cap = argc > 4 ? atoi(argv[4]) : 0;
+        while (!stop && !(cap > 0 && n >= cap)) {
+            solverCalcDt(s);
+            if (t + s->dt > tmax) {
+                s->dt = tmax - t;
+                stop = 1;
+            }
+            solverStepRK(s);
+            t += s->dt * s->stages / 2.0;
+            n++;
+        }
+        double tn[2] = {t, (double)n};
+        write_bin("t.f64", tn, sizeof tn);
+        write_soa("U.f64", s->U, s->Nvar, nc);
     } else {
         fprintf(stderr, "unknown mode %s\n", mode);
         return 2;

@@ -19,6 +19,9 @@ SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_s
        "plate_sa_lusgs_colour"]
 PLATE = PLATE + SA + SST + NACA
 ALL = RAMP + PLATE
+# tube mode (time-accurate runs to tmax)
+TUBE = ["tube_roe_o1", "tube_ausmdv_o2", "tube_ausmdv_o2_barth", "tube_ausm_quad", "tube_pup_moving",
+        "tube_roe_tp"]
 
 
 def load(name):
@@ -33,6 +36,8 @@ def write(name, g, d, nmax=1):
         cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=nmax, **inp))
     elif name.startswith("naca"):
         cases.write_case(d, cases.naca_mesh(**mk), cases.naca_input(threads=1, nmax=nmax, **inp))
+    elif name.startswith("tube"):
+        cases.write_case(d, cases.tube_mesh(**mk), cases.tube_input(threads=1, **inp))
     else:
         cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=nmax, **inp))
     if ren:

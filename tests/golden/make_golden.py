@@ -76,6 +76,45 @@ PLATE = [
 ]
 
 
+# tube mode: (name, tube_mesh kwargs, input overrides).  The reference's
+# second-order tube runs stay bounded only on cells elongated across the
+# tube (as in its own caseSodRef mesh, 0.2 x 1.0): on near-isotropic cells
+# its first step already gives negative densities, so the second-order
+# fixtures use 250 x 5 cells.
+TUBE = [
+    ("tube_roe_o1", dict(nx=100, ny=8), {}),
+    ("tube_ausmdv_o2", dict(nx=250, ny=5), {"flux": "AUSMDV", "order": 2}),
+    ("tube_ausmdv_o2_barth", dict(nx=250, ny=5), {"flux": "AUSMDV", "order": 2, "limiter": 1}),
+    ("tube_ausm_quad", dict(nx=80, ny=4, split=False), {"flux": "AUSM", "stages": 3}),
+    ("tube_pup_moving", dict(nx=100, ny=6), {"flux": "AUSMpup", "stages": 4, "mach1": 0.3}),
+    ("tube_roe_tp", dict(nx=100, ny=4), {"TP": 1, "tmax": 0.01}),
+]
+
+
+def make_tube():
+    """Tube-mode runs: the initial two-state field (solverInitUTube), the
+    trace driver's restatement of the tube loop with the reference's
+    solverCalcDt / solverStepRK (full precision: U, t, steps), and the
+    reference executable's own restart.csv of the same case (solverSolve's
+    tube branch, %.10e; written after solverWriteSolution2 has clamped U in
+    place) that pins the restatement to it."""
+    import subprocess
+    for name, mk, inp in TUBE:
+        d = case_dir(name)
+        cases.write_case(d, cases.tube_mesh(**mk), cases.tube_input(threads=1, **inp))
+        init = po.ref_trace(d, "resid")
+        nc = init["U"].size // 4
+        tr = po.ref_trace(d, "tube")
+        # the executable clamps U (solverCalcPrimitive) before writing restart.csv
+        clamped = po.ref_trace(d, "resid", U=tr["U"].reshape(4, nc))["U"].reshape(4, nc)
+        subprocess.run([po.REF_EXE, d], capture_output=True, check=False, timeout=600)
+        U_exe = np.loadtxt(d + "restart.csv", delimiter=",", skiprows=2, usecols=range(4)).T
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), mesh_kwargs=np.array(repr(mk)),
+                            inputs=np.array(repr(inp)), U_init=init["U"].reshape(4, nc), t=tr["t"][0],
+                            steps=int(tr["t"][1]), U=tr["U"].reshape(4, nc), U_clamped=clamped, U_exe=U_exe)
+        print("wrote", name, nc, "cells", int(tr["t"][1]), "steps")
+
+
 def case_dir(tag):
     return tempfile.mkdtemp(prefix="gd", dir="/tmp") + "/"
 
@@ -165,3 +204,4 @@ if __name__ == "__main__":
     make_small()
     make_config0()
     make_friction()
+    make_tube()
