@@ -167,6 +167,87 @@ def serving(n: int, rate: float, slots: int, log):
     return out
 
 
+def _config_solve(d: str, n: int, precision: int):
+    """n iterations of the case in d on the device at the given precision:
+    timing, histories, wall shear and final state."""
+    from csu_amd import native
+    case = native.HostCase(d, host_wall_distance=False)
+    case.params.precision = precision
+    dev = native.Device(case)
+    dev.iterate(2, want=False)            # first residual, graph capture
+    dev.sync()
+    t0 = time.perf_counter()
+    res, coef = dev.iterate(n)
+    dev.sync()
+    t1 = time.perf_counter()
+    out = dict(nc=case.ncell, seconds=t1 - t0, res=res, coef=coef, U=dev.get_state(),
+               wt=dev.wall_tensor() if (case.params.laminar or case.params.sa or case.params.sst) else None)
+    dev.close()
+    case.close()
+    return out
+
+
+def other_configs(log, n_plate: int = 200, n_naca: int = 100):
+    """The viscous configurations beside the headline one, each in plain
+    FP64 and in mixed precision (FP32 gradients, limiter, edge states and
+    convective flux; FP64 accumulation and update) on the same mesh and
+    iteration count: throughput, residual drop reached, and the mixed
+    error against FP64 in the force coefficients, the wall shear and the
+    fields.
+
+    laminar plate: 500x100 cells, wall stretching 1.08, 20 % of the
+      far-field quads split (56,030 cells), Mach 0.3, Re_L 1e5, AUSM,
+      Venkatakrishnan (K=5), DPLUR at CFL 50.
+    NACA 0012: hybrid O-mesh, 40 quad layers from 1e-6 chord (growth 1.15)
+      and triangles to 50 chords (1000 x (40 + 2x105) = 250,000 cells),
+      Mach 0.15, Re 6e6, AoA drawn from seed 2 in [0, 10] deg, SA and SST,
+      AUSMDV (the reference's SA/SST fluxes have no Roe branch), LU-SGS at
+      CFL 20 on the colour/RCM-renumbered mesh."""
+    from csu_amd import cases, renumber
+    rng = np.random.default_rng(2)
+    aoa = float(rng.uniform(0.0, 10.0))
+    specs = []
+    d = tempfile.mkdtemp(prefix="c1", dir="/tmp") + "/"
+    cases.write_case(d, cases.plate_mesh(nx=500, ny=100, growth=1.08, tri_frac=0.2, seed=1),
+                     cases.plate_input(threads=1, nmax=n_plate, timeScheme="DPLUR", CFL=50))
+    specs.append(("laminar_plate_dplur", d, n_plate, 0.0,
+                  "laminar flat plate M0.3 Re_L 1e5, 500x100 hybrid (20% tris), AUSM/Venkat K=5, DPLUR CFL 50"))
+    mesh = cases.naca_mesh(ni=1000, nwall=40, nfar=105, seed=2)
+    for model in ("sa", "sst"):
+        d = tempfile.mkdtemp(prefix="c2" + model, dir="/tmp") + "/"
+        over = {"sa": "1"} if model == "sa" else {"sa": "0", "sst": "1"}
+        cases.write_case(d, mesh, cases.naca_input(threads=1, nmax=n_naca, aoa=aoa, **over))
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+        specs.append((f"naca_{model}_lusgs", d, n_naca, aoa,
+                      f"NACA 0012 M0.15 Re 6e6 AoA {aoa:.3f} deg, {model.upper()}, 250k hybrid O-mesh, "
+                      "AUSMDV, LU-SGS CFL 20 (colour/RCM order)"))
+    out = {}
+    for name, d, n, a_deg, label in specs:
+        r64 = _config_solve(d, n, 0)
+        r32 = _config_solve(d, n, 1)
+        a = np.radians(a_deg)
+
+        def lift_drag(c):
+            cx, cy = c[-1, 0] + c[-1, 1], c[-1, 2] + c[-1, 3]
+            return cy * np.cos(a) - cx * np.sin(a), cx * np.cos(a) + cy * np.sin(a)
+        L64, D64 = lift_drag(r64["coef"])
+        L32, D32 = lift_drag(r32["coef"])
+        wt64, wt32 = r64["wt"][:, 1], r32["wt"][:, 1]
+        m = np.abs(wt64) > 0
+        entry = {"workload": label, "cells": r64["nc"], "iterations": n}
+        for tag, r in (("fp64", r64), ("mixed", r32)):
+            entry[tag] = {"cell_iter_per_s": r["nc"] * n / r["seconds"], "ms_per_iter": 1e3 * r["seconds"] / n,
+                          "density_residual_drop": float(r["res"][-1, 0] / r["res"][0, 0])}
+        entry["mixed_vs_fp64"] = {
+            "CL_rel": abs(L32 / L64 - 1) if L64 else None, "CD_rel": abs(D32 / D64 - 1) if D64 else None,
+            "Cf_linf_rel": float(np.abs(wt32[m] - wt64[m]).max() / np.abs(wt64[m]).max()) if m.any() else None,
+            "field_linf_rel": (np.abs(r32["U"] - r64["U"]).max(axis=1) / np.abs(r64["U"]).max(axis=1)).tolist()}
+        entry["fp64"]["CL"], entry["fp64"]["CD"] = float(L64), float(D64)
+        out[name] = entry
+        log(f"{name}: fp64 {entry['fp64']['ms_per_iter']:.3f} ms/iter, mixed {entry['mixed']['ms_per_iter']:.3f}")
+    return out
+
+
 def cpu_baseline(nx: int, ny: int, iters: int):
     """Reference OpenMP build (oracle/_ref/csu_ref, compiled from the
     reference sources) on the host cores: Nc*N / (T(N) - T(0))."""
@@ -202,6 +283,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the laminar-plate and NACA 0012 SA/SST measurements")
     ap.add_argument("--serve-n", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=8)
@@ -265,6 +348,7 @@ def main():
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
             "iteration_alg_bytes_per_cell": 2334.5,
             "iteration_frac": value * 2334.5 / (HBM_PEAK_GBS * 1e9)}
+    others = None if a.no_configs else other_configs(log)
     serve = None if a.no_serve else serving(a.serve_n, a.serve_rate, a.serve_slots, log)
     cpu = None if a.no_cpu else cpu_baseline(200, 100, a.cpu_iters)
     line = {
@@ -278,6 +362,7 @@ def main():
                        None if small is None else small["nc"] * max(a.steps, 50) / small["seconds"]},
         "roofline": roof,
         "mixed_precision": mixed,
+        "other_configs": others,
         "serving": serve,
         "cpu_baseline": cpu,
     }

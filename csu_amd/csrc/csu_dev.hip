@@ -168,7 +168,6 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
   if (p->nvar != (p->sa ? 5 : p->sst ? 6 : 4)) return set_err(CSU_ERR_ARG, "nvar %d does not match the model", p->nvar);
   if ((p->sa || p->sst) && p->flux == CSU_FLUX_ROE)
     return set_err(CSU_ERR_UNSUPPORTED, "Roe has no turbulence-scalar form in the reference (flux.c:1325-1366)");
-  if (p->sst && p->sst_trans) return set_err(CSU_ERR_UNSUPPORTED, "sstTrans (transition model) is not built");
   if (p->sst && p->time_scheme == CSU_TIME_DPLUR)
     return set_err(CSU_ERR_UNSUPPORTED, "DPLUR with SST: the reference has 4x4 and 5x5 blocks only (implicit.c:993-1000)");
   if (p->sst && p->order == 2 && p->limiter == 1)
@@ -456,6 +455,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   KP& kp = d->kp;
   kp.nc = nc; kp.nf = nf; kp.nb = nb; kp.nv = nv; kp.order = p->order; kp.flux = p->flux;
   kp.laminar = p->laminar; kp.sa = p->sa; kp.sst = p->sst; kp.limiter = p->limiter;
+  kp.sst_trans = p->sst ? p->sst_trans : 0;
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
   kp.lsq_ld = p->precision == CSU_PREC_FP64_LD; kp.store_grad = 0;
   kp.mixed = p->precision == CSU_PREC_MIXED;

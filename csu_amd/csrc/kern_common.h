@@ -14,6 +14,7 @@ struct KP {
   int lsq_ld;     // 1: long double LSQ sums emulated per call; 0: precomputed weights
   int mixed;      // 1: FP32 gradients, limiter, reconstruction and convective flux
   int store_grad; // write dPx/dPy/phi even when no viscous term reads them
+  int sst_trans;  // SST: the transition variant's eddy viscosity and sources (sstTrans.c)
   double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t, wimp;
   double sst_owall, sst_g1, sst_g2;  // SST: oWallFactor, blended gamma constants (sst.c:18-38)
   double pin[7];

@@ -302,8 +302,80 @@ __device__ __forceinline__ double sst_F2(double n_L_term, double sqrtk_term) {
   return tanh(arg2 * arg2);
 }
 
+// The transition variant (sstTrans.c): its own constants, Ry = Rt = 1e6
+// fixed (sstTrans.c:48,58,140,150), F1 without the omega = 0 guard.
+constexpr double STR_SK1 = 0.85, STR_SO1 = 0.5, STR_B1 = 0.075, STR_SK2 = 1.0, STR_SO2 = 0.856, STR_B2 = 0.0828,
+                 STR_BS2 = 0.09, STR_ALPHAS2 = 1.0, STR_ALPHA2 = 0.4403, STR_A1 = 0.31, STR_RY = 1e6, STR_RT = 1e6;
+
+// sstTransF1 (sstTrans.c:96-115): long double temporaries holding doubles.
+__device__ __forceinline__ double str_F1s(const SstV& v, double n_L_term, double sqrtk_term) {
+  double aux1 = 2 * v.r * STR_SO2 * (v.dkx * v.dox + v.dky * v.doy) / v.om;
+  double CD = fmax(aux1, 1.0e-20);
+  aux1 = fmax(sqrtk_term, n_L_term);
+  double aux2 = 4 * v.r * STR_SO2 * v.k / (CD * v.d * v.d);
+  double arg1 = fmin(aux1, aux2);
+  return tanh(arg1 * arg1 * arg1 * arg1);
+}
+
+// Blending functions of the transition variant, sstTransFlux /
+// sstTransSources preamble (sstTrans.c:34-59, 126-151).
+__device__ __forceinline__ void str_blend_fns(const SstV& v, double& omega, double& F1, double& F2) {
+  double n_L = v.mi_L / v.r;
+  omega = fabs(v.duy - v.dvx);
+  double sqrtk = sqrt(v.k);
+  double sqrtk_term = sqrtk / (STR_BS2 * v.om * v.d);
+  double n_L_term = 500 * n_L / (v.d * v.d * v.om);
+  double Fs1 = str_F1s(v, n_L_term, sqrtk_term);
+  double aux = STR_RY / 120;
+  aux = aux * aux;
+  aux = aux * aux;
+  aux = aux * aux;
+  F1 = fmax(Fs1, exp(-aux));
+  double arg2 = fmax(2 * sqrtk_term, n_L_term);
+  F2 = tanh(arg2 * arg2);
+}
+
+// sstTransFlux (sstTrans.c:34-74).
+__device__ __forceinline__ void str_flux(SstV& v) {
+  double omega, F1, F2;
+  str_blend_fns(v, omega, F1, F2);
+  double alphas1 = (.025 + STR_RT / 6) / (1 + STR_RT / 6);
+  double alphas = sst_blend(alphas1, STR_ALPHAS2, F1);
+  double sk = sst_blend(STR_SK1, STR_SK2, F1), so = sst_blend(STR_SO1, STR_SO2, F1);
+  v.mi_t = fmin(alphas * v.r * v.k / v.om, STR_A1 * v.r * v.k / (omega * F2));
+  double aux = v.mi_L + sk * v.mi_t;
+  v.tkx = aux * v.dkx;
+  v.tky = aux * v.dky;
+  aux = v.mi_L + so * v.mi_t;
+  v.tox = aux * v.dox;
+  v.toy = aux * v.doy;
+}
+
+// sstTransSources (sstTrans.c:126-174).
+__device__ __forceinline__ void str_sources(SstV& v) {
+  double omega, F1, F2;
+  str_blend_fns(v, omega, F1, F2);
+  double alpha1 = (5. / 9.) * (0.1 + STR_RT / 2.7) / (1 + STR_RT / 2.7);
+  double alphas1 = (.025 + STR_RT / 6) / (1 + STR_RT / 6);
+  double a8 = STR_RT / 8;
+  a8 = a8 * a8;
+  a8 = a8 * a8;
+  double bs1 = 0.09 * (5. / 18. + a8) / (1 + a8);
+  double alpha = sst_blend(alpha1, STR_ALPHA2, F1), alphas = sst_blend(alphas1, STR_ALPHAS2, F1);
+  double b = sst_blend(STR_B1, STR_B2, F1), bs = sst_blend(bs1, STR_BS2, F1);
+  v.mi_t = fmin(alphas * v.r * v.k / v.om, STR_A1 * v.r * v.k / (omega * F2));
+  double n_t = v.mi_t / v.r;
+  double P = v.mi_t * omega * omega;
+  v.Qtk = P - bs * v.r * v.om * v.k;
+  v.Qto = alpha * P / n_t - b * v.r * v.om * v.om + 2 * (1 - F1) * v.r * STR_SO2 * (v.dkx * v.dox + v.dky * v.doy) / v.om;
+}
+
 // sstFlux (sst.c:749-786): eddy viscosity, k and omega diffusion fluxes.
-__device__ __forceinline__ void sst_flux(SstV& v) {
+__device__ __forceinline__ void sst_flux(const KP& kp, SstV& v) {
+  if (kp.sst_trans) {
+    str_flux(v);
+    return;
+  }
   double n_L = v.mi_L / v.r;
   double omega = fabs(v.duy - v.dvx);
   double sqrtk_term = sqrt(v.k) / (SST_BS * v.om * v.d);
@@ -322,6 +394,10 @@ __device__ __forceinline__ void sst_flux(SstV& v) {
 
 // sstSources (sst.c:698-747): Qtk, Qto.
 __device__ __forceinline__ void sst_sources(const KP& kp, SstV& v) {
+  if (kp.sst_trans) {
+    str_sources(v);
+    return;
+  }
   double n_L = v.mi_L / v.r;
   double omega = fabs(v.duy - v.dvx);
   double sqrtk_term = sqrt(v.k) / (SST_BS * v.om * v.d);
@@ -424,7 +500,7 @@ __device__ __forceinline__ void sst_face(const KP& kp, double dSx, double dSy, d
   v.om = (P1[6] + P0[6]) * 0.5;
   v.d = (d1 + d0) * 0.5;
   v.mi_L = sutherland(v.T);
-  sst_flux(v);
+  sst_flux(kp, v);
   const double mi = v.mi_L + v.mi_t;
   const double kk = gas_T2Cp<double>(kp.gas, v.T) * (v.mi_L / kp.pr + v.mi_t / kp.pr_t);
   mi_t = v.mi_t;
@@ -450,7 +526,7 @@ __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, 
     v.dox = gx[4] - (gx[4] * nx + gy[4] * ny) * nx; v.doy = gy[4] - (gx[4] * nx + gy[4] * ny) * ny;
     v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6];
     v.mi_L = sutherland(v.T);
-    sst_flux(v);
+    sst_flux(kp, v);
     Tau t = sst_tau(v.mi_L + v.mi_t, v);
     const double u = Pi[1], w = Pi[2];
     f[0] = (t.xx * nx + t.xy * ny) * dS2;
@@ -481,7 +557,7 @@ __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, 
     if (type != CSU_BC_WALL) edge_correct(gx[2], gy[2], (Tb - Pi[4]) / L, dx, dy, L, dTx, dTy);
     edge_correct(gx[3], gy[3], (kb - Pi[5]) / L, dx, dy, L, v.dkx, v.dky);
     edge_correct(gx[4], gy[4], (ob - Pi[6]) / L, dx, dy, L, v.dox, v.doy);
-    sst_flux(v);
+    sst_flux(kp, v);
     Tau t = sst_tau(v.mi_L + v.mi_t, v);
     f[0] = t.xx * dSx + t.xy * dSy;
     f[1] = t.xy * dSx + t.yy * dSy;
@@ -498,7 +574,7 @@ __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, 
   v.duy = gy[0]; v.dvy = gy[1]; v.dky = gy[3]; v.doy = gy[4];
   v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6];
   v.mi_L = sutherland(v.T);
-  sst_flux(v);
+  sst_flux(kp, v);
   const double kk = gas_T2Cp<double>(kp.gas, v.T) * (v.mi_L / kp.pr + v.mi_t / kp.pr_t);
   Tau t = sst_tau(v.mi_L + v.mi_t, v);
   const double u = Pi[1], w = Pi[2];

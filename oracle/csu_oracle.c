@@ -1117,9 +1117,94 @@ static double sst_F2(double n_L_term, double sqrtk_term)      /* sst.c:811-818 *
     return tanh(arg2 * arg2);
 }
 
-/* sstFlux (sst.c:749-786): eddy viscosity and the k, omega diffusion flux densities */
-static void sst_flux(sstv* v)
+/* ---- the transition variant, sstTrans.c: its own constants, Ry = Rt = 1e6
+   fixed (sstTrans.c:48,58,140,150), F1 without the omega = 0 guard ---- */
+#define STR_SK1 0.85
+#define STR_SO1 0.5
+#define STR_B1 0.075
+#define STR_SK2 1.0
+#define STR_SO2 0.856
+#define STR_B2 0.0828
+#define STR_BS2 0.09
+#define STR_ALPHAS2 1.0
+#define STR_ALPHA2 0.4403
+#define STR_A1 0.31
+#define STR_RY 1e6
+#define STR_RT 1e6
+
+/* sstTransF1 (sstTrans.c:96-115) */
+static double str_F1s(const sstv* v, double n_L_term, double sqrtk_term)
 {
+    double aux1 = 2 * v->r * STR_SO2 * (v->dkx * v->dox + v->dky * v->doy) / v->om;
+    double CD = fmax(aux1, 1.0e-20);
+    aux1 = fmax(sqrtk_term, n_L_term);
+    double aux2 = 4 * v->r * STR_SO2 * v->k / (CD * v->d * v->d);
+    double arg1 = fmin(aux1, aux2);
+    return tanh(arg1 * arg1 * arg1 * arg1);
+}
+
+/* common preamble of sstTransFlux / sstTransSources (sstTrans.c:36-59,128-151) */
+static void str_blend_fns(const sstv* v, double* omega, double* F1, double* F2)
+{
+    double n_L = v->mi_L / v->r;
+    *omega = fabs(v->duy - v->dvx);
+    double sqrtk = sqrt(v->k);
+    double sqrtk_term = sqrtk / (STR_BS2 * v->om * v->d);
+    double n_L_term = 500 * n_L / (v->d * v->d * v->om);
+    double Fs1 = str_F1s(v, n_L_term, sqrtk_term);
+    double aux = STR_RY / 120;
+    aux = aux * aux;
+    aux = aux * aux;
+    aux = aux * aux;
+    *F1 = fmax(Fs1, exp(-aux));
+    double arg2 = fmax(2 * sqrtk_term, n_L_term);   /* sstTransF2, sstTrans.c:117-124 */
+    *F2 = tanh(arg2 * arg2);
+}
+
+/* sstTransFlux (sstTrans.c:34-74) */
+static void str_flux(sstv* v)
+{
+    double omega, F1, F2;
+    str_blend_fns(v, &omega, &F1, &F2);
+    double alphas1 = (.025 + STR_RT / 6) / (1 + STR_RT / 6);          /* sstTransAlphas1 */
+    double alphas = sst_blend(alphas1, STR_ALPHAS2, F1);
+    double sk = sst_blend(STR_SK1, STR_SK2, F1), so = sst_blend(STR_SO1, STR_SO2, F1);
+    v->mi_t = fmin(alphas * v->r * v->k / v->om, STR_A1 * v->r * v->k / (omega * F2));
+    double aux = v->mi_L + sk * v->mi_t;
+    v->tkx = aux * v->dkx;
+    v->tky = aux * v->dky;
+    aux = v->mi_L + so * v->mi_t;
+    v->tox = aux * v->dox;
+    v->toy = aux * v->doy;
+}
+
+/* sstTransSources (sstTrans.c:126-174) */
+static void str_sources(sstv* v)
+{
+    double omega, F1, F2;
+    str_blend_fns(v, &omega, &F1, &F2);
+    double alpha1 = (5. / 9.) * (0.1 + STR_RT / 2.7) / (1 + STR_RT / 2.7);   /* sstTransAlpha1 */
+    double alphas1 = (.025 + STR_RT / 6) / (1 + STR_RT / 6);
+    double a8 = STR_RT / 8;                                                  /* sstTransBs1 */
+    a8 = a8 * a8;
+    a8 = a8 * a8;
+    double bs1 = 0.09 * (5. / 18. + a8) / (1 + a8);
+    double alpha = sst_blend(alpha1, STR_ALPHA2, F1), alphas = sst_blend(alphas1, STR_ALPHAS2, F1);
+    double b = sst_blend(STR_B1, STR_B2, F1), bs = sst_blend(bs1, STR_BS2, F1);
+    v->mi_t = fmin(alphas * v->r * v->k / v->om, STR_A1 * v->r * v->k / (omega * F2));
+    double n_t = v->mi_t / v->r;
+    double P = v->mi_t * omega * omega;
+    v->F1 = F1;
+    v->F2 = F2;
+    v->Qtk = P - bs * v->r * v->om * v->k;
+    v->Qto = alpha * P / n_t - b * v->r * v->om * v->om
+             + 2 * (1 - F1) * v->r * STR_SO2 * (v->dkx * v->dox + v->dky * v->doy) / v->om;
+}
+
+/* sstFlux (sst.c:749-786): eddy viscosity and the k, omega diffusion flux densities */
+static void sst_flux(const csu_oracle* o, sstv* v)
+{
+    if (o->prm.sst_trans) { str_flux(v); return; }
     double n_L = v->mi_L / v->r;
     double omega = fabs(v->duy - v->dvx);
     double sqrtk_term = sqrt(v->k) / (SST_BS * v->om * v->d);
@@ -1139,6 +1224,7 @@ static void sst_flux(sstv* v)
 /* sstSources (sst.c:698-747) */
 static void sst_sources(const csu_oracle* o, sstv* v)
 {
+    if (o->prm.sst_trans) { str_sources(v); return; }
     double n_L = v->mi_L / v->r;
     double omega = fabs(v->duy - v->dvx);
     double sqrtk_term = sqrt(v->k) / (SST_BS * v->om * v->d);
@@ -1207,7 +1293,7 @@ static void sst_inter(csu_oracle* o)
         v.om = (P1[6] + P0[6]) * 0.5;
         v.d = (o->wd[e1] + o->wd[e0]) * 0.5;
         v.mi_L = sutherland(v.T);
-        sst_flux(&v);
+        sst_flux(o, &v);
         double mi = v.mi_L + v.mi_t;
         double kk = gas_T2Cp(&o->gas, v.T) * (v.mi_L / o->prm.pr + v.mi_t / o->prm.pr_t);
         o->miT[f] = v.mi_t;
@@ -1295,7 +1381,7 @@ static void sst_bc_flux(const csu_oracle* o, int g, double* f)
         v.r = P[0]; v.T = P[4]; v.k = P[5]; v.om = P[6]; v.d = o->wd[e0];
         double u = P[1], w = P[2];
         v.mi_L = sutherland(v.T);
-        sst_flux(&v);
+        sst_flux(o, &v);
         sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
         f[1] = (txx * nx + txy * ny) * dS;
         f[2] = (txy * nx + tyy * ny) * dS;
@@ -1341,7 +1427,7 @@ static void sst_bc_flux(const csu_oracle* o, int g, double* f)
         v.dky = dkym + (dkl - (dkxm * dx + dkym * dy) / L) * dy / L;
         v.dox = doxm + (dol - (doxm * dx + doym * dy) / L) * dx / L;
         v.doy = doym + (dol - (doxm * dx + doym * dy) / L) * dy / L;
-        sst_flux(&v);
+        sst_flux(o, &v);
         sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
         f[1] = txx * dSx + txy * dSy;
         f[2] = txy * dSx + tyy * dSy;
@@ -1359,7 +1445,7 @@ static void sst_bc_flux(const csu_oracle* o, int g, double* f)
     v.r = P[0]; v.T = P[4]; v.k = P[5]; v.om = P[6]; v.d = o->wd[e0];
     double u = P[1], w = P[2];
     v.mi_L = sutherland(v.T);
-    sst_flux(&v);
+    sst_flux(o, &v);
     double kk = gas_T2Cp(&o->gas, v.T) * (v.mi_L / o->prm.pr + v.mi_t / o->prm.pr_t);
     sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
     f[1] = txx * dSx + txy * dSy;
@@ -1897,7 +1983,7 @@ int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
     /* SST: no transition model, no DPLUR (implicit.c:993-1000 has 4x4 and
        5x5 blocks only), no global-range limiter (limiter.c:37-38 sizes its
        range arrays for 5 variables) */
-    if (p->sst && (p->sst_trans || p->time_scheme == CSU_TIME_DPLUR || (p->order == 2 && p->limiter == 1)))
+    if (p->sst && (p->time_scheme == CSU_TIME_DPLUR || (p->order == 2 && p->limiter == 1)))
         return CSU_ERR_UNSUPPORTED;
     csu_oracle* o = calloc(1, sizeof(*o));
     o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;

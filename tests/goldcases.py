@@ -16,7 +16,7 @@ PLATE = ["plate_lam", "plate_lamT_ausmdv", "plate_lam_axi_roe", "plate_euler_axi
 SA = ["plate_sa_ausmdv", "plate_sa_blazek_wallT", "plate_sa_pup_lim1", "plate_sa_lusgs", "plate_sa_dplur"]
 NACA = ["naca_sa_lusgs", "naca_sst_lusgs"]
 SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_sst_pup2_t1", "plate_sst_lusgs",
-       "plate_sa_lusgs_colour"]
+       "plate_sa_lusgs_colour", "plate_sst_trans", "plate_sst_trans_lusgs_wallT"]
 PLATE = PLATE + SA + SST + NACA
 ALL = RAMP + PLATE
 # tube mode (time-accurate runs to tmax)

@@ -68,6 +68,10 @@ PLATE = [
     ("plate_sst_pup2_t1", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "flux": "AUSMpup2",
                                                   "turb1order": "1", "oWallFactor": "5"}, 20),
     ("plate_sst_lusgs", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
+    # SST with the transition variant's eddy viscosity and sources (sstTrans.c)
+    ("plate_sst_trans", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "sstTrans": "1"}, 20),
+    ("plate_sst_trans_lusgs_wallT", _PL, {"laminar": "0", "sst": "1", "sstTrans": "1", "timeScheme": "LUSGS",
+                                          "CFL": "10", "flux": "AUSMDV", "BC:plate": "wallT", "Twall": "320"}, 20),
     ("plate_sa_lusgs_colour", dict(_PL, jitter=0.1, renumber=True),
      {"laminar": "0", "sa": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
     # NACA 0012 hybrid O-mesh, Mach 0.15, Re 6e6, AoA 4 deg
