@@ -456,6 +456,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   kp.nc = nc; kp.nf = nf; kp.nb = nb; kp.nv = nv; kp.order = p->order; kp.flux = p->flux;
   kp.laminar = p->laminar; kp.sa = p->sa; kp.sst = p->sst; kp.limiter = p->limiter;
   kp.sst_trans = p->sst ? p->sst_trans : 0;
+  kp.sa_cc = p->sa ? p->sa_cc : 0;
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
   kp.lsq_ld = p->precision == CSU_PREC_FP64_LD; kp.store_grad = 0;
   kp.mixed = p->precision == CSU_PREC_MIXED;

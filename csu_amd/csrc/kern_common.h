@@ -15,6 +15,7 @@ struct KP {
   int mixed;      // 1: FP32 gradients, limiter, reconstruction and convective flux
   int store_grad; // write dPx/dPy/phi even when no viscous term reads them
   int sst_trans;  // SST: the transition variant's eddy viscosity and sources (sstTrans.c)
+  int sa_cc;      // SA: the saCC.c form, with density-gradient terms in the nu~ flux and source
   double rlim, pout, twall, efix, minf, limK, volmax, pr, pr_t, wimp;
   double sst_owall, sst_g1, sst_g2;  // SST: oWallFactor, blended gamma constants (sst.c:18-38)
   double pin[7];

@@ -141,9 +141,24 @@ __device__ __forceinline__ void sa_calc_face(double ni, double ni_L, double r, d
   ty = r * (ni_L + ni) * dniy / sig;
 }
 
-// saCalcSource (sa.c:292-349), Allmaras et al. 2012, without trip terms.
+// saCC_CalcFace (saCC.c:146-163): the nu~ diffusion flux density with the
+// density-gradient term 0.5 nu~^2 grad rho.
+__device__ __forceinline__ void sacc_calc_face(double ni, double ni_L, double r, double dnix, double dniy,
+                                               double drx, double dry, double& fv1, double& tx, double& ty) {
+  const double Cv1 = 7.1;
+  double X = ni / ni_L;
+  double aux = X * X * X;
+  fv1 = aux / (aux + Cv1 * Cv1 * Cv1);
+  const double sig = 2. / 3.;
+  tx = (r * (ni_L + ni) * dnix + 0.5 * ni * ni * drx) / sig;
+  ty = (r * (ni_L + ni) * dniy + 0.5 * ni * ni * dry) / sig;
+}
+
+// saCalcSource (sa.c:292-349), Allmaras et al. 2012, without trip terms;
+// cc: saCC_CalcSource (saCC.c:165-226), its guarded S~ branch and its
+// density-gradient terms.
 __device__ __forceinline__ double sa_source(double ni, double ni_L, double S, double d, double rho, double drx,
-                                            double dry, double dnix, double dniy) {
+                                            double dry, double dnix, double dniy, bool cc = false) {
   const double Cv1 = 7.1;
   double X = ni / ni_L;
   double aux = X * X * X;
@@ -153,8 +168,14 @@ __device__ __forceinline__ double sa_source(double ni, double ni_L, double S, do
   double Sbar = ni * fv2 / (k * k * d * d);
   double Stil;
   const double cv2 = 0.7, cv3 = 0.9;
-  if (Sbar >= -cv2 * S) Stil = S + Sbar;
-  else Stil = S + S * (cv2 * cv2 * S + cv3 * Sbar) / ((cv3 - 2 * cv2) * S - Sbar);
+  if (Sbar >= -cv2 * S) {
+    Stil = S + Sbar;
+  } else if (cc) {
+    const double aux2 = ((cv3 - 2 * cv2) * S - Sbar);
+    Stil = aux2 == 0 ? S : S + S * (cv2 * cv2 * S + cv3 * Sbar) / aux2;
+  } else {
+    Stil = S + S * (cv2 * cv2 * S + cv3 * Sbar) / ((cv3 - 2 * cv2) * S - Sbar);
+  }
   const double sig = 2. / 3., Cb1 = 0.1355, Cb2 = 0.622;
   const double Cw1 = Cb1 / (k * k) + (1. + Cb2) / sig;
   const double Cw2 = 0.3;
@@ -167,13 +188,19 @@ __device__ __forceinline__ double sa_source(double ni, double ni_L, double S, do
   aux = Cw3 * Cw3;
   aux = aux * aux * aux;
   double fw = g * pow((1. + aux) / (g * g * g * g * g * g + aux), 1. / 6.);
+  if (cc) {
+    double Qt = rho * (Cb1 * Stil * ni + Cb2 * (dnix * dnix + dniy * dniy) / sig - Cw1 * fw * (ni * ni / (d * d)));
+    Qt += (Cb2 / sig) * (ni * (dnix * drx + dniy * dry) + 0.25 * ni * ni * (drx * drx + dry * dry) / rho);
+    return Qt;
+  }
   return rho * (Cb1 * Stil * ni + Cb2 * (dnix * dnix + dniy * dniy) / sig - Cw1 * fw * (ni * ni / (d * d))) -
          (ni_L + ni) * (dnix * drx + dniy * dry) / sig;
 }
 
 // Interior SA viscous face flux F[0..3] (= reference faceFlux[1..4]) and
 // the face eddy viscosity, saInterFace (sa.c:138-221) or, with
-// viscBlazek, saInterFaceB (sa.c:26-134).  Gradients: u v T nu~.
+// viscBlazek, saInterFaceB (sa.c:26-134), or with saCC saCC_InterFace
+// (saCC.c:18-103, which ignores viscBlazek).  Gradients: u v T nu~ rho.
 __device__ __forceinline__ void sa_face(const KP& kp, double dSx, double dSy, double dx, double dy,
                                         const double* P0, const double* P1, const double* g0x, const double* g0y,
                                         const double* g1x, const double* g1y, double* F, double& mi_t) {
@@ -181,7 +208,7 @@ __device__ __forceinline__ void sa_face(const KP& kp, double dSx, double dSy, do
   double dTx = (g0x[2] + g1x[2]) * 0.5, dnx = (g0x[3] + g1x[3]) * 0.5;
   double duy = (g0y[0] + g1y[0]) * 0.5, dvy = (g0y[1] + g1y[1]) * 0.5;
   double dTy = (g0y[2] + g1y[2]) * 0.5, dny = (g0y[3] + g1y[3]) * 0.5;
-  if (kp.visc_blazek) {
+  if (kp.visc_blazek && !kp.sa_cc) {
     double L = sqrt(dx * dx + dy * dy);
     double dul = (P1[1] - P0[1]) / L, dvl = (P1[2] - P0[2]) / L, dTl = (P1[4] - P0[4]) / L, dnl = (P1[5] - P0[5]) / L;
     double a, b;
@@ -193,7 +220,10 @@ __device__ __forceinline__ void sa_face(const KP& kp, double dSx, double dSy, do
   double r = (P1[0] + P0[0]) * 0.5, u = (P1[1] + P0[1]) * 0.5, v = (P1[2] + P0[2]) * 0.5;
   double T = (P1[4] + P0[4]) * 0.5, n = (P1[5] + P0[5]) * 0.5;
   double mi_L = sutherland(T), n_L = mi_L / r, fv1, tx, ty;
-  sa_calc_face(n, n_L, r, dnx, dny, fv1, tx, ty);
+  if (kp.sa_cc)
+    sacc_calc_face(n, n_L, r, dnx, dny, (g0x[4] + g1x[4]) * 0.5, (g0y[4] + g1y[4]) * 0.5, fv1, tx, ty);
+  else
+    sa_calc_face(n, n_L, r, dnx, dny, fv1, tx, ty);
   mi_t = fv1 * r * n;
   double mi = mi_L + mi_t;
   double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
@@ -205,10 +235,12 @@ __device__ __forceinline__ void sa_face(const KP& kp, double dSx, double dSy, do
 }
 
 // SA boundary viscous flux f[0..3] (= reference f[1..4]),
-// saBoundaryFaceViscFlux (sa.c:370-692).  Gradients: u v T nu~.
+// saBoundaryFaceViscFlux (sa.c:370-692), or with saCC
+// saCC_BoundaryFaceViscFlux (saCC.c:247-589).  Gradients: u v T nu~; grx,
+// gry: the cell's density gradient.
 __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, double dSy, double nx, double ny,
                                            double dS2, double dx, double dy, const double* Pi, const double* gx,
-                                           const double* gy, double* f) {
+                                           const double* gy, double grx, double gry, double* f) {
   double fv1, tx, ty;
   if (type == CSU_BC_SYMMETRY) {
     double dux = gx[0] - (gx[0] * nx + gy[0] * ny) * nx, duy = gy[0] - (gx[0] * nx + gy[0] * ny) * ny;
@@ -229,7 +261,8 @@ __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, d
   if (type == CSU_BC_OUTLET) {
     double rho = Pi[0], u = Pi[1], v = Pi[2], T = Pi[4], n = Pi[5];
     double mi_L = sutherland(T), n_L = mi_L / rho;
-    sa_calc_face(n, n_L, rho, gx[3], gy[3], fv1, tx, ty);
+    if (kp.sa_cc) sacc_calc_face(n, n_L, rho, gx[3], gy[3], grx, gry, fv1, tx, ty);
+    else sa_calc_face(n, n_L, rho, gx[3], gy[3], fv1, tx, ty);
     double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
     double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
     Tau t = tau_of(mi, gx[0], gy[0], gx[1], gy[1]);
@@ -255,7 +288,17 @@ __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, d
   double T = inl ? kp.pin[4] : (type == CSU_BC_WALLT ? kp.twall : Pi[4]);
   double n = inl ? kp.pin[5] : 0.0;
   double mi_L = sutherland(T), n_L = mi_L / rho;
-  sa_calc_face(n, n_L, rho, dnx, dny, fv1, tx, ty);
+  if (kp.sa_cc) {
+    // density gradient per branch as saCC.c has it: one-sided to the
+    // inlet density (:334-353), the cell's gradient at a wallT, and at a
+    // wall the cell's d(rho)/dx in both components (:425-426)
+    double drx = grx, dry = gry;
+    if (inl) edge_correct(grx, gry, (kp.pin[0] - Pi[0]) / L, dx, dy, L, drx, dry);
+    else if (type == CSU_BC_WALL) dry = grx;
+    sacc_calc_face(n, n_L, rho, dnx, dny, drx, dry, fv1, tx, ty);
+  } else {
+    sa_calc_face(n, n_L, rho, dnx, dny, fv1, tx, ty);
+  }
   double mi_t = fv1 * rho * n;
   double mi = type == CSU_BC_WALL ? mi_L : mi_L + mi_t;
   Tau t = tau_of(mi, dux, duy, dvx, dvy);

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       // saInterSource (sa.c:224-260): uses grad rho, du/dy, dv/dx, grad nu~
       const double mi_L = sutherland(Pi[4]);
       const double S = fabs(guy - gvx);
-      const double Qt = sa_source(Pi[5], mi_L / Pi[0], S, g.wdist[i], Pi[0], grx, gry, gnx, gny);
+      const double Qt = sa_source(Pi[5], mi_L / Pi[0], S, g.wdist[i], Pi[0], grx, gry, gnx, gny, kp.sa_cc != 0);
       s.src[i] = Qt * g.omega[i];
     }
 #pragma unroll
@@ -238,7 +238,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
         s.bvflux[3 * nb + q] = f[3];
         s.bvflux[4 * nb + q] = f[4];
       } else if (NV == 5) {
-        sa_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, f);
+        sa_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, grx, gry,
+                   f);
         s.bvflux[3 * nb + q] = f[3];
       } else {
         laminar_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx[j], dy[j], Pi, gx3, gy3, f);
@@ -402,10 +403,11 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
     double P0[6], P1[6];
 #pragma unroll
     for (int m = 0; m < 6; m++) { P0[m] = s.P[m * nc + e0]; P1[m] = s.P[m * nc + e1]; }
-    const double g0x[4] = {s.dPx[nc + e0], s.dPx[2 * nc + e0], s.dTx[e0], s.dPx[4 * nc + e0]};
-    const double g0y[4] = {s.dPy[nc + e0], s.dPy[2 * nc + e0], s.dTy[e0], s.dPy[4 * nc + e0]};
-    const double g1x[4] = {s.dPx[nc + e1], s.dPx[2 * nc + e1], s.dTx[e1], s.dPx[4 * nc + e1]};
-    const double g1y[4] = {s.dPy[nc + e1], s.dPy[2 * nc + e1], s.dTy[e1], s.dPy[4 * nc + e1]};
+    const bool cc = kp.sa_cc != 0;
+    const double g0x[5] = {s.dPx[nc + e0], s.dPx[2 * nc + e0], s.dTx[e0], s.dPx[4 * nc + e0], cc ? s.dPx[e0] : 0.0};
+    const double g0y[5] = {s.dPy[nc + e0], s.dPy[2 * nc + e0], s.dTy[e0], s.dPy[4 * nc + e0], cc ? s.dPy[e0] : 0.0};
+    const double g1x[5] = {s.dPx[nc + e1], s.dPx[2 * nc + e1], s.dTx[e1], s.dPx[4 * nc + e1], cc ? s.dPx[e1] : 0.0};
+    const double g1y[5] = {s.dPy[nc + e1], s.dPy[2 * nc + e1], s.dTy[e1], s.dPy[4 * nc + e1], cc ? s.dPy[e1] : 0.0};
     double F[4], mi_t;
     sa_face(kp, dSx, dSy, g.xc[e1] - g.xc[e0], g.yc[e1] - g.yc[e0], P0, P1, g0x, g0y, g1x, g1y, F, mi_t);
 #pragma unroll

@@ -139,7 +139,7 @@ int csu_case_load(const char* wd, csu_case** out)
     if (p->sst == 1) p->nvar = 6;
     int axi = atoi(need(in, "axisymmetric"));
     if (g_missing[0]) { csu_case_free(c); return CSU_ERR_INPUT; }
-    if (sacc) { csu_case_free(c); return CSU_ERR_UNSUPPORTED; }
+    p->sa_cc = p->sa == 1 && sacc;   /* read only with SA (solver.c:719-730) */
 
     snprintf(path, sizeof path, "%smesh.su2", wd);
     rc = csu_mesh_read_su2(path, axi, c);

@@ -31,7 +31,7 @@ class MeshDesc(C.Structure):
 
 _PARAM_INTS = ["nvar", "order", "flux", "stages", "time_scheme", "dt_local", "dt_local_n",
                "laminar", "sa", "sst", "sst_trans", "tp", "limiter", "turb1order",
-               "visc_blazek", "nlinear", "precision", "reserved"]
+               "visc_blazek", "nlinear", "precision", "sa_cc"]
 _PARAM_DBLS = ["cfl", "rlim", "pout", "twall", "lim_k", "efix", "pr", "pr_t", "sref", "wimp",
                "turb_ratio", "gamma", "gas_r", "mach_inf"]
 

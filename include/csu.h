@@ -84,7 +84,7 @@ typedef struct csu_params {
     int32_t limiter;       /* 0 or 1 (limiter.c:45-94)                    */
     int32_t turb1order, visc_blazek, nlinear;
     int32_t precision;     /* CSU_PREC_*                                  */
-    int32_t reserved;
+    int32_t sa_cc;         /* SA: the saCC.c form (density-gradient terms) */
     double cfl, rlim, pout, twall, lim_k, efix, pr, pr_t, sref, wimp, turb_ratio;
     double gamma, gas_r;
     double mach_inf;       /* inlet Mach (AUSM+up reference Mach)         */

@@ -905,9 +905,21 @@ static void sa_calc_face(double ni, double ni_L, double r, double dnix, double d
     *ty = r * (ni_L + ni) * dniy / sig;
 }
 
-/* saCalcSource, sa.c:292-349 (Allmaras et al. 2012) */
+/* saCC_CalcFace, saCC.c:146-163: with the density-gradient term */
+static void sacc_calc_face(double ni, double ni_L, double r, double dnix, double dniy, double drx, double dry,
+                           double* fv1, double* tx, double* ty)
+{
+    double Cv1 = 7.1, X = ni / ni_L, aux = X * X * X;
+    *fv1 = aux / (aux + Cv1 * Cv1 * Cv1);
+    double sig = 2. / 3.;
+    *tx = (r * (ni_L + ni) * dnix + 0.5 * ni * ni * drx) / sig;
+    *ty = (r * (ni_L + ni) * dniy + 0.5 * ni * ni * dry) / sig;
+}
+
+/* saCalcSource, sa.c:292-349 (Allmaras et al. 2012); cc: saCC_CalcSource,
+   saCC.c:165-226 (guarded S~ branch, density-gradient terms) */
 static double sa_source(double ni, double ni_L, double S, double d, double rho, double drx, double dry,
-                        double dnix, double dniy)
+                        double dnix, double dniy, int cc)
 {
     double Cv1 = 7.1, X = ni / ni_L, aux = X * X * X;
     double fv1 = aux / (aux + Cv1 * Cv1 * Cv1);
@@ -916,7 +928,10 @@ static double sa_source(double ni, double ni_L, double S, double d, double rho, 
     double Sbar = ni * fv2 / (k * k * d * d), Stil;
     double cv2 = 0.7, cv3 = 0.9;
     if (Sbar >= -cv2 * S) Stil = S + Sbar;
-    else Stil = S + S * (cv2 * cv2 * S + cv3 * Sbar) / ((cv3 - 2 * cv2) * S - Sbar);
+    else if (cc) {
+        double aux2 = ((cv3 - 2 * cv2) * S - Sbar);
+        Stil = aux2 == 0 ? S : S + S * (cv2 * cv2 * S + cv3 * Sbar) / aux2;
+    } else Stil = S + S * (cv2 * cv2 * S + cv3 * Sbar) / ((cv3 - 2 * cv2) * S - Sbar);
     double sig = 2. / 3., Cb1 = 0.1355, Cb2 = 0.622;
     double Cw1 = Cb1 / (k * k) + (1. + Cb2) / sig;
     double Cw2 = 0.3;
@@ -929,12 +944,18 @@ static double sa_source(double ni, double ni_L, double S, double d, double rho, 
     aux = Cw3 * Cw3;
     aux = aux * aux * aux;
     double fw = g * pow((1. + aux) / (g * g * g * g * g * g + aux), 1. / 6.);
+    if (cc) {
+        double Qt = rho * (Cb1 * Stil * ni + Cb2 * (dnix * dnix + dniy * dniy) / sig - Cw1 * fw * (ni * ni / (d * d)));
+        Qt += (Cb2 / sig) * (ni * (dnix * drx + dniy * dry) + 0.25 * ni * ni * (drx * drx + dry * dry) / rho);
+        return Qt;
+    }
     return rho * (Cb1 * Stil * ni + Cb2 * (dnix * dnix + dniy * dniy) / sig - Cw1 * fw * (ni * ni / (d * d)))
            - (ni_L + ni) * (dnix * drx + dniy * dry) / sig;
 }
 
 /* saInterFace (sa.c:138-221) or saInterFaceB (sa.c:26-134), saInterSource
-   (sa.c:224-260) */
+   (sa.c:224-260); with saCC saCC_InterFace / saCC_InterSource
+   (saCC.c:18-137), which ignore viscBlazek */
 static void sa_inter(csu_oracle* o)
 {
     const int nc = o->nc, nf = o->nf, ns = o->nv + 1;
@@ -950,7 +971,7 @@ static void sa_inter(csu_oracle* o)
         double dTx = (gx[3 * nc + e0] + gx[3 * nc + e1]) * 0.5, dnx = (gx[4 * nc + e0] + gx[4 * nc + e1]) * 0.5;
         double duy = (gy[nc + e0] + gy[nc + e1]) * 0.5, dvy = (gy[2 * nc + e0] + gy[2 * nc + e1]) * 0.5;
         double dTy = (gy[3 * nc + e0] + gy[3 * nc + e1]) * 0.5, dny = (gy[4 * nc + e0] + gy[4 * nc + e1]) * 0.5;
-        if (o->prm.visc_blazek) {
+        if (o->prm.visc_blazek && !o->prm.sa_cc) {
             double x0, y0, x1, y1;
             cell_center(o, e0, &x0, &y0);
             cell_center(o, e1, &x1, &y1);
@@ -970,7 +991,10 @@ static void sa_inter(csu_oracle* o)
         double r = (P1[0] + P0[0]) * 0.5, u = (P1[1] + P0[1]) * 0.5, v = (P1[2] + P0[2]) * 0.5;
         double T = (P1[4] + P0[4]) * 0.5, n = (P1[5] + P0[5]) * 0.5;
         double mi_L = sutherland(T), n_L = mi_L / r, fv1, tx, ty;
-        sa_calc_face(n, n_L, r, dnx, dny, &fv1, &tx, &ty);
+        if (o->prm.sa_cc)
+            sacc_calc_face(n, n_L, r, dnx, dny, (gx[e0] + gx[e1]) * 0.5, (gy[e0] + gy[e1]) * 0.5, &fv1, &tx, &ty);
+        else
+            sa_calc_face(n, n_L, r, dnx, dny, &fv1, &tx, &ty);
         double mi_t = fv1 * r * n, mi = mi_L + mi_t;
         double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
         o->miT[f] = mi_t;
@@ -992,12 +1016,13 @@ static void sa_inter(csu_oracle* o)
         const double* P = o->P + (size_t)i * ns;
         double rho = P[0], T = P[4], n = P[5], mi_L = sutherland(T), n_L = mi_L / rho;
         double S = fabs(gy[nc + i] - gx[2 * nc + i]);
-        double Qt = sa_source(n, n_L, S, o->wd[i], rho, gx[i], gy[i], gx[4 * nc + i], gy[4 * nc + i]);
+        double Qt = sa_source(n, n_L, S, o->wd[i], rho, gx[i], gy[i], gx[4 * nc + i], gy[4 * nc + i], o->prm.sa_cc);
         o->R[(size_t)4 * nc + i] -= Qt * o->omega[i];
     }
 }
 
-/* saBoundaryFaceViscFlux, sa.c:370-692 */
+/* saBoundaryFaceViscFlux, sa.c:370-692; with saCC
+   saCC_BoundaryFaceViscFlux, saCC.c:247-589 */
 static void sa_bc_flux(const csu_oracle* o, int g, double* f)
 {
     const int nc = o->nc, ns = o->nv + 1, e0 = o->bface[3 * g], t = o->btype[g];
@@ -1031,7 +1056,8 @@ static void sa_bc_flux(const csu_oracle* o, int g, double* f)
         dux = gx[nc + e0]; dvx = gx[2 * nc + e0]; dTx = gx[3 * nc + e0]; dnx = gx[4 * nc + e0];
         duy = gy[nc + e0]; dvy = gy[2 * nc + e0]; dTy = gy[3 * nc + e0]; dny = gy[4 * nc + e0];
         double rho = P[0], u = P[1], v = P[2], T = P[4], n = P[5], mi_L = sutherland(T), n_L = mi_L / rho;
-        sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
+        if (o->prm.sa_cc) sacc_calc_face(n, n_L, rho, dnx, dny, gx[e0], gy[e0], &fv1, &tx, &ty);
+        else sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
         double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
         double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
         double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
@@ -1065,7 +1091,20 @@ static void sa_bc_flux(const csu_oracle* o, int g, double* f)
     double T = inl ? Pin[4] : (t == CSU_BC_WALLT ? o->prm.twall : P[4]);
     double n = inl ? Pin[5] : 0.0;
     double mi_L = sutherland(T), n_L = mi_L / rho;
-    sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
+    if (o->prm.sa_cc) {
+        /* saCC.c:334-353 (inlet), :425-426 (wall: d(rho)/dx in both), :494-495 (wallT) */
+        double drx = gx[e0], dry = gy[e0];
+        if (inl) {
+            double drl = (Pin[0] - P[0]) / L, drxm = gx[e0], drym = gy[e0];
+            drx = drxm + (drl - (drxm * dx + drym * dy) / L) * dx / L;
+            dry = drym + (drl - (drxm * dx + drym * dy) / L) * dy / L;
+        } else if (t == CSU_BC_WALL) {
+            dry = gx[e0];
+        }
+        sacc_calc_face(n, n_L, rho, dnx, dny, drx, dry, &fv1, &tx, &ty);
+    } else {
+        sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
+    }
     double mi_t = fv1 * rho * n, mi = t == CSU_BC_WALL ? mi_L : mi_L + mi_t;
     double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
     f[1] = txx * dSx + txy * dSy;

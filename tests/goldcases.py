@@ -13,7 +13,8 @@ RAMP = ["ramp_roe", "ramp_ausm_o1", "ramp_ausmdv_lim1", "ramp_ausmpup",
         "ramp_lusgs_ausmdv", "ramp_dplur_ausm", "ramp_lusgs_colour"]
 PLATE = ["plate_lam", "plate_lamT_ausmdv", "plate_lam_axi_roe", "plate_euler_axi",
          "plate_lam_pup_lim1", "plate_lusgs_lam", "plate_lusgs_lamT", "plate_dplur_lam"]
-SA = ["plate_sa_ausmdv", "plate_sa_blazek_wallT", "plate_sa_pup_lim1", "plate_sa_lusgs", "plate_sa_dplur"]
+SA = ["plate_sa_ausmdv", "plate_sa_blazek_wallT", "plate_sa_pup_lim1", "plate_sa_lusgs", "plate_sa_dplur",
+      "plate_sacc_ausmdv", "plate_sacc_lusgs_wallT"]
 NACA = ["naca_sa_lusgs", "naca_sst_lusgs"]
 SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_sst_pup2_t1", "plate_sst_lusgs",
        "plate_sa_lusgs_colour", "plate_sst_trans", "plate_sst_trans_lusgs_wallT"]

@@ -59,6 +59,11 @@ PLATE = [
     ("plate_sa_lusgs", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "timeScheme": "LUSGS", "CFL": "10"}, 20),
     ("plate_sa_dplur", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "timeScheme": "DPLUR", "CFL": "20",
                                               "Nlinear": "6"}, 20),
+    # SA in the saCC.c form (density-gradient terms); viscBlazek is ignored by it
+    ("plate_sacc_ausmdv", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "saCC": "1", "flux": "AUSMDV",
+                                                  "viscBlazek": "1"}, 20),
+    ("plate_sacc_lusgs_wallT", _PL, {"laminar": "0", "sa": "1", "saCC": "1", "timeScheme": "LUSGS", "CFL": "10",
+                                     "BC:plate": "wallT", "Twall": "320", "BC:top": "symmetry"}, 20),
     # Menter SST (nvar 6)
     ("plate_sst_ausm", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1"}, 20),
     ("plate_sst_ausmdv_wallT", _PL, {"laminar": "0", "sst": "1", "flux": "AUSMDV", "BC:plate": "wallT",
