@@ -299,10 +299,15 @@ def main():
         n = a.copy_calib * (1 << 20) // 8
         x = torch.ones(n, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         for _ in range(10):
             y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8}))
+        dt = (time.perf_counter() - t0) / 10
+        print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8,
+                          "copy_gb_per_s": 2 * n * 8 / dt / 1e9}))
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "

@@ -480,22 +480,18 @@ __device__ __forceinline__ T limiter_v2(T Ui, T Umin, T Umax, T d2, T ee) {
   return limiter_v2d<T>(Umax - Ui, Umin - Ui, d2, ee);
 }
 
-// limiterV2 from the differences to the neighbourhood max / min
+// limiterV2 from the differences to the neighbourhood max / min.  The
+// reference's two signed branches differ only in which difference they
+// use, so one straight-line evaluation with that difference selected gives
+// the same operations per lane without divergent branches (each holding
+// two divisions) in a wavefront.
 template <typename T>
 __device__ __forceinline__ T limiter_v2d(T d1max, T d1min, T d2, T ee) {
-  T ans;
-  if (d2 == (T)0) {
-    ans = (T)1;
-  } else if (d2 > (T)0) {
-    ans = (d1max * d1max + ee) * d2 + (T)2 * d2 * d2 * d1max;
-    ans /= d1max * d1max + (T)2 * d2 * d2 + d1max * d2 + ee;
-    ans /= d2;
-  } else {
-    ans = (d1min * d1min + ee) * d2 + (T)2 * d2 * d2 * d1min;
-    ans /= d1min * d1min + (T)2 * d2 * d2 + d1min * d2 + ee;
-    ans /= d2;
-  }
-  return ans;
+  const T d1 = d2 > (T)0 ? d1max : d1min;
+  T ans = (d1 * d1 + ee) * d2 + (T)2 * d2 * d2 * d1;
+  ans /= d1 * d1 + (T)2 * d2 * d2 + d1 * d2 + ee;
+  ans /= d2;
+  return d2 == (T)0 ? (T)1 : ans;
 }
 
 }  // namespace csu
