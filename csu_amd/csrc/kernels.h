@@ -97,10 +97,35 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       dy[j] = g.bym[q] - y0;
     }
   }
-  auto nval = [&](int j, int m) -> double {
-    const int n = nbr[j];
-    return n >= 0 ? s.P[m * nc + n] : s.Pb[m * nb + (-1 - n)];
-  };
+  // Every neighbour value the cell reads, loaded before any store: the
+  // compiler cannot tell the stores below from these loads, so loading
+  // them inside the variable loop would serialise one memory round trip
+  // per variable.  Slots past np read the cell itself and are not used.
+  double vnb[4][NV], vT[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const bool act = j < np;
+    const int n = act ? nbr[j] : i;
+    const double* base = n >= 0 ? s.P + n : s.Pb + (-1 - n);
+    const int st = n >= 0 ? nc : nb;
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) {
+      const int m = kk > 3 ? kk + 1 : kk;
+      vnb[j][kk] = base[m * st];
+    }
+    vT[j] = VISC ? base[4 * st] : 0.0;
+  }
+  TF ex[4], ey[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (MIX) {
+      ex[j] = g.edx32[j * nc + i];
+      ey[j] = g.edy32[j * nc + i];
+    } else {
+      ex[j] = g.edx[j * nc + i];
+      ey[j] = g.edy[j * nc + i];
+    }
+  }
   // Least squares (solver.c:1031-1074).  Default: gx = sum_j wx_j du_j with
   // the weights (c dx_j - b dy_j)/det, (a dy_j - b dx_j)/det precomputed per
   // cell in long double.  LD: the normal-equation sums of every call in
@@ -120,12 +145,15 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   } else {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
+      // all four slots exist in the [4][nc] arrays; past np the weight is 0
       if (MIX) {
-        wx[j] = j < np ? g.lwx32[j * nc + i] : 0.0f;
-        wy[j] = j < np ? g.lwy32[j * nc + i] : 0.0f;
+        const float ax = g.lwx32[j * nc + i], ay = g.lwy32[j * nc + i];
+        wx[j] = j < np ? ax : 0.0f;
+        wy[j] = j < np ? ay : 0.0f;
       } else {
-        wx[j] = j < np ? g.lwx[j * nc + i] : 0.0;
-        wy[j] = j < np ? g.lwy[j * nc + i] : 0.0;
+        const double ax = g.lwx[j * nc + i], ay = g.lwy[j * nc + i];
+        wx[j] = j < np ? ax : 0.0;
+        wy[j] = j < np ? ay : 0.0;
       }
     }
   }
@@ -158,7 +186,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     double vn[4], du[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      vn[j] = j < np ? nval(j, m) : u0;
+      vn[j] = j < np ? vnb[j][kk] : u0;
       du[j] = vn[j] - u0;
     }
     double gx, gy;
@@ -186,8 +214,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
-        if (MIX) d2[j] = tgx * g.edx32[j * nc + i] + tgy * g.edy32[j * nc + i];
-        else d2[j] = tgx * g.edx[j * nc + i] + tgy * g.edy[j * nc + i];
+        d2[j] = tgx * ex[j] + tgy * ey[j];
         TF p0 = limiter_v2d<TF>(d1max, d1min, d2[j], (TF)pref2);
         ph = (j == 0) ? p0 : fmin(ph, p0);
       }
@@ -205,7 +232,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   if (VISC) {
     double du[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) du[j] = j < np ? nval(j, 4) - Pi[4] : 0.0;
+    for (int j = 0; j < 4; j++) du[j] = j < np ? vT[j] - Pi[4] : 0.0;
     double gTx, gTy;
     grad(du, gTx, gTy);
     s.dTx[i] = gTx;
@@ -477,18 +504,24 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
   for (int kk = 0; kk < NV; kk++) R[kk] = 0.0;
   double dt_next = INFINITY;
   if (active) {
+    // all face indices, then all face fluxes, then the sums in slot order:
+    // two memory round trips instead of one pair per slot (slots without
+    // a face read face 0 and add nothing)
     int cf[4];
 #pragma unroll
+    for (int j = 0; j < 4; j++) cf[j] = g.cface[j * nc + i];
+    double fv[4][NV];
+#pragma unroll
     for (int j = 0; j < 4; j++) {
-      cf[j] = g.cface[j * nc + i];
+      const int f = cf[j], q = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) fv[j][kk] = MIX ? (double)s.fflux32[kk * nf + q] : s.fflux[kk * nf + q];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
       const int f = cf[j];
-      if (f > 0) {
 #pragma unroll
-        for (int kk = 0; kk < NV; kk++) R[kk] += MIX ? (double)s.fflux32[kk * nf + f - 1] : s.fflux[kk * nf + f - 1];
-      } else if (f < 0) {
-#pragma unroll
-        for (int kk = 0; kk < NV; kk++) R[kk] -= MIX ? (double)s.fflux32[kk * nf - f - 1] : s.fflux[kk * nf - f - 1];
-      }
+      for (int kk = 0; kk < NV; kk++) R[kk] = f > 0 ? R[kk] + fv[j][kk] : (f < 0 ? R[kk] - fv[j][kk] : R[kk]);
     }
     // boundary fluxes (k_bcflux), in the cell's slot order = marker order
     int gbq[4];
