@@ -62,8 +62,8 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int) -> float:
 # counters read for a plain 1 GiB copy (profiles/euler_ramp_4M_rk5_fp64.md).
 # A counter pass cannot run inside this timed process, so the figure is
 # reported for that workload only.
-MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 1182.2e6, ("k_flux", 2000, 1000): 775.5e6,
-                    ("k_update", 2000, 1000): 786.6e6}
+MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 1121.9e6, ("k_flux", 2000, 1000): 775.4e6,
+                    ("k_update", 2000, 1000): 787.2e6}
 
 
 _MESHES = {}

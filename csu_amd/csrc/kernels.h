@@ -344,74 +344,77 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, co
   }
 }
 
+// Convective flux of interior face f times dS, reference inter() face loop
+// (solver.c:550-644): the two edge states (order 2, reconstructed by k_grad
+// at edge s0 of e0 and edge s1 of e1) or the cell states (order 1),
+// rotated into the face frame, the Riemann flux, rotated back.  MIX: in
+// FP32 from FP32 edge states, as FP32 values.
+template <int NV, bool MIX>
+__device__ __forceinline__ void conv_flux(const KP& kp, const Geo& g, const St& s, int f, int e0, int e1, int s0,
+                                          int s1, typename std::conditional<MIX, float, double>::type* out) {
+  using TF = typename std::conditional<MIX, float, double>::type;
+  const int nc = kp.nc;
+  TF dSx, dSy, dS;
+  if (MIX) {
+    dSx = g.fdsx32[f]; dSy = g.fdsy32[f]; dS = g.fds32[f];
+  } else {
+    dSx = g.fdsx[f]; dSy = g.fdsy[f]; dS = g.fds[f];
+  }
+  TF PL[NV], PR[NV];
+  if (kp.order == 2) {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) {
+      if (MIX) {
+        PL[kk] = s.rec32[(s0 * NV + kk) * nc + e0];
+        PR[kk] = s.rec32[(s1 * NV + kk) * nc + e1];
+      } else {
+        PL[kk] = s.rec[(s0 * NV + kk) * nc + e0];
+        PR[kk] = s.rec[(s1 * NV + kk) * nc + e1];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < NV; kk++) {
+      const int m = kk > 3 ? kk + 1 : kk;
+      PL[kk] = (TF)s.P[m * nc + e0];
+      PR[kk] = (TF)s.P[m * nc + e1];
+    }
+  }
+  rotate<TF>(PL[1], PL[2], dSx, dSy, dS);
+  rotate<TF>(PR[1], PR[2], dSx, dSy, dS);
+  TF fl[NV];
+  constexpr int NSC = NV - 4;
+  flux_dispatch<TF, NSC>(kp.gas, kp.flux, (TF)kp.efix, (TF)kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4, PR[0],
+                         PR[1], PR[2], PR[3], PR + 4, NV == 6 ? PL[4] : (TF)0, NV == 6 ? PR[4] : (TF)0, fl);
+  rotate<TF>(fl[1], fl[2], dSx, -dSy, dS);
+#pragma unroll
+  for (int kk = 0; kk < NV; kk++) out[kk] = fl[kk] * dS;
+}
+
 // Interior face flux, reference inter() face loop (solver.c:550-644), and
 // for viscous runs the laminar face flux (laminar.c:17-75).
 // MIX: the convective flux in FP32 from FP32 edge states, stored in FP32.
 template <int NV, bool VISC, bool MIX = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
-  using TF = typename std::conditional<MIX, float, double>::type;
   const int f = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int nc = kp.nc, nf = kp.nf;
   if (f >= nf) return;
   const int e0 = g.fe0[f], e1 = g.fe1[f];
-  if (MIX) {
-    const TF dSx = g.fdsx32[f], dSy = g.fdsy32[f], dS = g.fds32[f];
-    TF PL[NV], PR[NV];
-    if (kp.order == 2) {
-      const int sl = g.fsl[f], s0 = sl & 3, s1 = sl >> 2;
+  {
+    const int sl = g.fsl[f];
+    if (MIX) {
+      float fl[NV];
+      conv_flux<NV, true>(kp, g, s, f, e0, e1, sl & 3, sl >> 2, fl);
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) {
-        PL[kk] = s.rec32[(s0 * NV + kk) * nc + e0];
-        PR[kk] = s.rec32[(s1 * NV + kk) * nc + e1];
-      }
+      for (int kk = 0; kk < NV; kk++) s.fflux32[kk * nf + f] = fl[kk];
     } else {
+      double fl[NV];
+      conv_flux<NV, false>(kp, g, s, f, e0, e1, sl & 3, sl >> 2, fl);
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) {
-        const int m = kk > 3 ? kk + 1 : kk;
-        PL[kk] = (TF)s.P[m * nc + e0];
-        PR[kk] = (TF)s.P[m * nc + e1];
-      }
+      for (int kk = 0; kk < NV; kk++) s.fflux[kk * nf + f] = fl[kk];
     }
-    rotate<TF>(PL[1], PL[2], dSx, dSy, dS);
-    rotate<TF>(PR[1], PR[2], dSx, dSy, dS);
-    TF fl[NV];
-    constexpr int NSC = NV - 4;
-    flux_dispatch<TF, NSC>(kp.gas, kp.flux, (TF)kp.efix, (TF)kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4, PR[0],
-                           PR[1], PR[2], PR[3], PR + 4, NV == 6 ? PL[4] : (TF)0, NV == 6 ? PR[4] : (TF)0, fl);
-    rotate<TF>(fl[1], fl[2], dSx, -dSy, dS);
-#pragma unroll
-    for (int kk = 0; kk < NV; kk++) s.fflux32[kk * nf + f] = fl[kk] * dS;
   }
-  const double dSx = g.fdsx[f], dSy = g.fdsy[f], dS = g.fds[f];
-  if (!MIX) {
-    double PL[NV], PR[NV];
-    if (kp.order == 2) {
-      // both sides' states were reconstructed at this edge by k_grad
-      const int sl = g.fsl[f], s0 = sl & 3, s1 = sl >> 2;
-#pragma unroll
-      for (int kk = 0; kk < NV; kk++) {
-        PL[kk] = s.rec[(s0 * NV + kk) * nc + e0];
-        PR[kk] = s.rec[(s1 * NV + kk) * nc + e1];
-      }
-    } else {
-#pragma unroll
-      for (int kk = 0; kk < NV; kk++) {
-        const int m = kk > 3 ? kk + 1 : kk;
-        PL[kk] = s.P[m * nc + e0];
-        PR[kk] = s.P[m * nc + e1];
-      }
-    }
-    rotate<double>(PL[1], PL[2], dSx, dSy, dS);
-    rotate<double>(PR[1], PR[2], dSx, dSy, dS);
-    double fl[NV];
-    constexpr int NSC = NV - 4;
-    flux_dispatch<double, NSC>(kp.gas, kp.flux, kp.efix, kp.minf, PL[0], PL[1], PL[2], PL[3], PL + 4,
-                               PR[0], PR[1], PR[2], PR[3], PR + 4, NV == 6 ? PL[4] : 0.0, NV == 6 ? PR[4] : 0.0,
-                               fl);
-    rotate<double>(fl[1], fl[2], dSx, -dSy, dS);
-#pragma unroll
-    for (int kk = 0; kk < NV; kk++) s.fflux[kk * nf + f] = fl[kk] * dS;
-  }
+  const double dSx = g.fdsx[f], dSy = g.fdsy[f];
   if (VISC && NV == 4) {
     double P0[5], P1[5];
 #pragma unroll
