@@ -13,45 +13,7 @@
 #include <math.h>
 #include "csu_host.h"
 
-/* ---- calorically / thermally perfect gas on the host (gasprop.c:14-171) ---- */
-typedef struct { double gamma, R, Cp, Cv, cc[6]; int tp; } hgas;
-
-static void hgas_init(hgas* g, double gamma, double R, int tp)
-{
-    g->gamma = gamma; g->R = R;
-    g->Cp = g->gamma * g->R / (g->gamma - 1);
-    g->Cv = g->Cp - g->R;
-    g->tp = tp;
-    g->cc[0] = 1.81716e3;
-    g->cc[1] = 9.890504e2 - 287.0530;
-    g->cc[2] = -9.595592e-3;
-    g->cc[3] = 1.041469e-4;
-    g->cc[4] = -4.433065e-8;
-    g->cc[5] = 5.879263e-12;
-}
-
-static double hgas_T2e(const hgas* g, double T)
-{
-    if (!g->tp) return g->Cv * T;
-    double P = 1, aux = g->cc[0];
-    for (int i = 1; i < 6; i++) { P *= T; aux += g->cc[i] * P; }
-    return aux;
-}
-
-static double hgas_T2Cv(const hgas* g, double T)
-{
-    if (!g->tp) return g->Cv;
-    double P = 1, aux = g->cc[1];
-    for (int i = 2; i < 6; i++) { P *= T; aux += i * g->cc[i] * P; }
-    return aux;
-}
-
-static double hgas_T2c(const hgas* g, double T)
-{
-    double Cv = hgas_T2Cv(g, T);
-    double gam = (Cv + g->R) / Cv;
-    return sqrt(gam * g->R * T);
-}
+#include "hgas.h"
 
 static double sutherland(double T) { return 1.458e-6 * T * sqrt(T) / (T + 110.4); }
 

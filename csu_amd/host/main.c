@@ -125,11 +125,17 @@ write:
     printf("main: saving the solution.\n");
     snprintf(path, sizeof path, "%ssolution.csv", wd);
     csu_write_solution(c, c->U, path);
-    /* solverWriteSolution2 clamps U in place (solverCalcPrimitive) before
-       the reference writes restart.csv */
-    rc = csu_dev_primitives(d, NULL);
+    /* solverWriteSolution2 clamps U in place (solverCalcPrimitive) and
+       writes the node averages of P; restart.csv then holds the clamped U */
+    double* Pc = malloc(sizeof(double) * (size_t)(c->prm.nvar + 1) * (size_t)c->mesh.ncell);
+    if (!Pc) { printf("Error: out of memory\n"); return 1; }
+    rc = csu_dev_primitives(d, Pc);
     if (!rc) rc = csu_dev_get_state(d, c->U);
     if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }
+    printf("main: saving the solution 2.\n");
+    snprintf(path, sizeof path, "%ssolution2.csv", wd);
+    if (!c->prm.sst) csu_write_solution2(c, Pc, path);
+    free(Pc);
     printf("main: saving the restart file.\n");
     snprintf(path, sizeof path, "%srestart.csv", wd);
     csu_write_restart(c, c->U, path);

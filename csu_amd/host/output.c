@@ -7,12 +7,16 @@
  *   solution.csv "0, Np, Nvar,\n" then per node the inverse-distance average
  *                of the surrounding cells' U (reference solver.c:222-325),
  *                velocity zeroed on wall nodes for viscous cases.
+ *   solution2.csv a header line, then per node the same average of the
+ *                primitive state and Mach, total enthalpy and entropy
+ *                (reference solver.c:2009-2220).
  */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
 #include "csu_host.h"
+#include "hgas.h"
 
 int csu_write_restart(const csu_case* c, const double* U, const char* path)
 {
@@ -109,5 +113,77 @@ int csu_write_solution(const csu_case* c, const double* U, const char* path)
     }
     fclose(f);
     free(Up); free(den);
+    return CSU_OK;
+}
+
+/* solverWriteSolution2 (solver.c:2009-2220) for Euler, laminar and SA runs:
+   P [ncell][nvar+1] = r u v p T [nu~] of the clamped state
+   (solverCalcPrimitive), averaged to the nodes with inverse-distance
+   weights; on wall (not wallT) nodes of viscous runs u = v = 0, and for SA
+   nu~ = 0 and T = Twall when Twall >= 0; then Mach, total enthalpy and the
+   entropy relative to the inlet's.  SST runs add eddy viscosity, F1, F2 and
+   two more per-cell fields there: not written (CSU_ERR_UNSUPPORTED). */
+int csu_write_solution2(const csu_case* c, const double* P, const char* path)
+{
+    const csu_mesh_desc* m = &c->mesh;
+    const csu_params* prm = &c->prm;
+    if (prm->sst) return CSU_ERR_UNSUPPORTED;
+    const int nv1 = prm->nvar + 1, np = m->nnode, nc = m->ncell;
+    double* Pn = calloc((size_t)nv1 * (size_t)np, sizeof(double));
+    double* den = calloc((size_t)np, sizeof(double));
+    if (!Pn || !den) { free(Pn); free(den); return CSU_ERR_NOMEM; }
+    for (int i = 0; i < nc; i++) {
+        const int32_t* v = m->cell_vert + 4 * i;
+        int nvert = v[3] >= 0 ? 4 : 3;
+        double xc = 0.0, yc = 0.0;
+        for (int k = 0; k < nvert; k++) { xc += m->node_xy[2 * v[k]]; yc += m->node_xy[2 * v[k] + 1]; }
+        xc /= nvert; yc /= nvert;
+        for (int j = 0; j < nvert; j++) {
+            int p = v[j];
+            double xp = m->node_xy[2 * p], yp = m->node_xy[2 * p + 1];
+            double L = sqrt((xp - xc) * (xp - xc) + (yp - yc) * (yp - yc));
+            for (int k = 0; k < nv1; k++) Pn[(size_t)k * np + p] += P[(size_t)i * nv1 + k] / L;
+            den[p] += 1 / L;
+        }
+    }
+    for (int p = 0; p < np; p++)
+        if (den[p] != 0)
+            for (int k = 0; k < nv1; k++) Pn[(size_t)k * np + p] /= den[p];
+    if (prm->laminar || prm->sa) {
+        for (int g = 0; g < m->nbface; g++) {
+            if (m->mark_type[m->bface_mark[g]] != CSU_BC_WALL) continue;
+            int q[2] = {m->bface[3 * g + 1], m->bface[3 * g + 2]};
+            for (int e = 0; e < 2; e++) {
+                Pn[(size_t)1 * np + q[e]] = 0.0;
+                Pn[(size_t)2 * np + q[e]] = 0.0;
+                if (prm->sa) {
+                    Pn[(size_t)5 * np + q[e]] = 0.0;
+                    if (prm->twall >= 0) Pn[(size_t)4 * np + q[e]] = prm->twall;
+                }
+            }
+        }
+    }
+    hgas gas;
+    hgas_init(&gas, prm->gamma, prm->gas_r, prm->tp);
+    /* the inlet condition's own p and T (conditionInit, solver.c:24-38) */
+    const char* ks_p = c->tube ? "pressure1" : "pressure";
+    const char* ks_T = c->tube ? "temperature1" : "temperature";
+    const char *sp = csu_input_get(c->input, ks_p), *sT = csu_input_get(c->input, ks_T);
+    double s0 = hgas_Tp2entropy(&gas, sT ? strtod(sT, NULL) : prm->pin[4], sp ? strtod(sp, NULL) : prm->pin[3]);
+    FILE* f = fopen(path, "w");
+    if (!f) { free(Pn); free(den); return CSU_ERR_IO; }
+    fprintf(f, prm->sa ? "r,u,v,p,T,n,mach,H,s,\n" : "r,u,v,p,T,mach,H,s,\n");
+    for (int p = 0; p < np; p++) {
+        const double* q = Pn + p;
+        double V2 = q[np] * q[np] + q[2 * (size_t)np] * q[2 * (size_t)np];
+        double T = q[4 * (size_t)np], pr = q[3 * (size_t)np];
+        double Q0 = sqrt(V2) / hgas_T2c(&gas, T);
+        double Q1 = hgas_T2e(&gas, T) + pr / q[0] + 0.5 * V2;
+        double Q2 = hgas_Tp2entropy(&gas, T, pr) - s0;
+        for (int k = 0; k < nv1; k++) fprintf(f, "%.10e, ", q[(size_t)k * np]);
+        fprintf(f, "%.10e, %.10e, %.10e, \n", Q0, Q1, Q2);
+    }
+    fclose(f);
+    free(Pn); free(den);
     return CSU_OK;
 }

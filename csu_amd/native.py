@@ -112,6 +112,7 @@ def lib():
     L.csu_write_restart.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_read_restart.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_write_solution.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
+    L.csu_write_solution2.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_wall_distance.argtypes = [C.POINTER(Case)]
     L.csu_wall_distance_dev.argtypes = [C.POINTER(MeshDesc), f64p]
     L.csu_dev_iterate_async.argtypes = [C.c_void_p, C.c_int32]
@@ -229,6 +230,11 @@ class HostCase:
     def write_solution(self, U: np.ndarray, path: str):
         U = np.ascontiguousarray(U, dtype=np.float64)
         check(lib().csu_write_solution(self._c, _p(U, C.c_double), path.encode()))
+
+    def write_solution2(self, P: np.ndarray, path: str):
+        """solution2.csv from the clamped primitive state P [ncell][nvar+1]."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        check(lib().csu_write_solution2(self._c, _p(P, C.c_double), path.encode()))
 
     def close(self):
         if self._c:

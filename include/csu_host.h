@@ -71,6 +71,10 @@ int  csu_wall_distance(csu_case* c);
 int  csu_write_restart(const csu_case* c, const double* U, const char* path);
 int  csu_read_restart(const csu_case* c, double* U, const char* path);
 int  csu_write_solution(const csu_case* c, const double* U, const char* path);
+/* solution2.csv (solverWriteSolution2, solver.c:2009-2220) from the clamped
+ * primitive state P [ncell][nvar+1] (csu_dev_primitives); Euler, laminar
+ * and SA runs (SST: CSU_ERR_UNSUPPORTED). */
+int  csu_write_solution2(const csu_case* c, const double* P, const char* path);
 
 /* Name of the input.dat key whose absence (or bad value) made the last
  * csu_case_load fail with CSU_ERR_INPUT. */

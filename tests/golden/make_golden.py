@@ -193,6 +193,38 @@ FRICTION = [
 ]
 
 
+# solution2.csv of the reference executable (solverWriteSolution2) after n
+# iterations: (name, kind, mesh kwargs, input overrides, n)
+SOLUTION2 = [
+    ("solution2_ramp", "ramp", dict(nx=12, ny=6), {}, 30),
+    ("solution2_plate_lam", "plate", dict(_PL, jitter=0.1), {}, 60),
+    ("solution2_plate_sa_tp", "plate", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "TP": "1"}, 60),
+    ("solution2_tube", "tube", dict(nx=100, ny=8), {}, 0),
+]
+
+
+def read_solution2(path):
+    lines = [ln for ln in open(path).read().splitlines() if ln.strip()]
+    return lines[0], np.array([[float(v) for v in ln.strip().rstrip(",").split(",")] for ln in lines[1:]])
+
+
+def make_solution2():
+    import subprocess
+    for name, kind, mk, inp, n in SOLUTION2:
+        d = case_dir(name)
+        if kind == "ramp":
+            cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=n, **inp))
+        elif kind == "plate":
+            cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=n, **inp))
+        else:
+            cases.write_case(d, cases.tube_mesh(**mk), cases.tube_input(threads=1, **inp))
+        subprocess.run([po.REF_EXE, d], capture_output=True, check=False, timeout=600)
+        head, S = read_solution2(d + "solution2.csv")
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kind=np.array(kind), mesh_kwargs=np.array(repr(mk)),
+                            inputs=np.array(repr(inp)), niter=n, header=np.array(head), solution2=S)
+        print("wrote", name, S.shape)
+
+
 def make_friction():
     import subprocess
     for name, mk, inp, nit in FRICTION:
@@ -214,3 +246,4 @@ if __name__ == "__main__":
     make_config0()
     make_friction()
     make_tube()
+    make_solution2()
