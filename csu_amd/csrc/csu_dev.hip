@@ -1081,6 +1081,20 @@ extern "C" int csu_dev_iterate(csu_dev* d, int32_t niter, double* res, double* c
   return prof_collect(d);
 }
 
+// New CFL number for the following iterations.  The next iteration takes its
+// time step from the current state with the new number, as a run restarted
+// from this state with the new CFL would (the chain of restart runs with
+// stepped CFL that ramps the reference's CFL, solver.c:1846-1862).
+extern "C" int csu_dev_set_cfl(csu_dev* d, double cfl) {
+  if (!d) return set_err(CSU_ERR_ARG, "null argument");
+  if (!(cfl > 0.0) || !std::isfinite(cfl)) return set_err(CSU_ERR_ARG, "CFL %g", cfl);
+  HIPCHK(hipStreamSynchronize(d->stream));
+  d->prm.cfl = cfl;
+  graphs_drop(d);  // the CFL is a kernel argument of the captured iterations
+  d->fresh = 1;
+  return CSU_OK;
+}
+
 // Time-accurate run to tmax, the tube branch of solverSolve
 // (solver.c:1912-1941).  The step control runs on the device
 // (k_time_clamp), so iterations are enqueued in chunks; the chunk length

@@ -6,6 +6,7 @@
 // and retires solves whose density residual has dropped by the requested
 // factor.  Contexts are independent, so the GPU overlaps their kernels and
 // each solve's numbers do not depend on what else is running.
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -26,6 +27,7 @@ struct Req {
   int max_iter;
   int keep;
   double t_submit;
+  double cfl_max = 0.0, cfl_growth = 1.0;  // CFL ramp: x growth per chunk up to cfl_max
 };
 
 struct Job {
@@ -34,6 +36,7 @@ struct Job {
   csu_job_result res{};
   int chunk_n = 0;  // iterations of the chunk in flight
   bool res0_set = false;
+  double cfl = 0.0;  // current CFL of a ramped job
 };
 
 }  // namespace
@@ -85,6 +88,17 @@ extern "C" int csu_server_submit(csu_server* s, const csu_mesh_desc* mesh, const
   return CSU_OK;
 }
 
+extern "C" int csu_server_set_ramp(csu_server* s, int32_t id, double cfl_max, double growth) {
+  if (!s || !(growth >= 1.0) || !(cfl_max > 0.0)) return CSU_ERR_ARG;
+  for (Req& r : s->queue)
+    if (r.id == id) {
+      r.cfl_max = cfl_max;
+      r.cfl_growth = growth;
+      return CSU_OK;
+    }
+  return CSU_ERR_ARG;  // unknown, or already admitted
+}
+
 extern "C" int csu_server_pending(const csu_server* s, int32_t* queued, int32_t* active) {
   if (!s) return CSU_ERR_ARG;
   if (queued) *queued = (int32_t)s->queue.size();
@@ -131,6 +145,7 @@ static void admit(csu_server* s) {
     j->res.t_submit = r.t_submit;
     j->res.t_admit = now(s);
     j->res.iter_conv = -1;
+    j->cfl = r.prm->cfl;
     s->active.push_back(j);
     int rc = csu_dev_create(r.mesh, r.prm, &j->dev);
     if (rc) { j->dev = nullptr; retire(s, s->active.size() - 1, rc); continue; }
@@ -161,6 +176,10 @@ static bool harvest(csu_server* s, size_t k) {
     }
   }
   if (j->res.converged || j->res.iters >= j->r.max_iter) { retire(s, k, CSU_OK); return true; }
+  if (j->r.cfl_growth > 1.0 && j->cfl < j->r.cfl_max) {
+    j->cfl = std::min(j->cfl * j->r.cfl_growth, j->r.cfl_max);
+    if ((rc = csu_dev_set_cfl(j->dev, j->cfl))) { retire(s, k, rc); return true; }
+  }
   if ((rc = enqueue_chunk(s, j))) { retire(s, k, rc); return true; }
   return false;
 }

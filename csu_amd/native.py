@@ -89,6 +89,7 @@ def lib():
     L.csu_dev_residual.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p]
     L.csu_dev_iterate.argtypes = [C.c_void_p, C.c_int32, f64p, f64p]
     L.csu_dev_primitives.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_set_cfl.argtypes = [C.c_void_p, C.c_double]
     L.csu_dev_advance_time.argtypes = [C.c_void_p, C.c_double, C.c_int32, f64p, i32p, i32p]
     L.csu_dev_iteration.argtypes = [C.c_void_p]
     L.csu_dev_sync.argtypes = [C.c_void_p]
@@ -129,6 +130,7 @@ def lib():
     L.csu_server_submit.argtypes = [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(Params), f64p, C.c_double,
                                     C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     L.csu_server_step.argtypes = [C.c_void_p, C.c_double]
+    L.csu_server_set_ramp.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
     L.csu_server_results.argtypes = [C.c_void_p, C.POINTER(JobResult), C.c_int32]
     L.csu_server_pending.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.csu_server_state.argtypes = [C.c_void_p, C.c_int32, f64p]
@@ -296,6 +298,10 @@ class Device:
                                     _p(coef, C.c_double) if want else None))
         return res, coef
 
+    def set_cfl(self, cfl: float):
+        """New CFL number from the next iteration on (CFL ramps)."""
+        check(lib().csu_dev_set_cfl(self._h, cfl))
+
     def primitives(self) -> np.ndarray:
         """Clamp the state in place (solverCalcPrimitive); P [ncell][nvar+1]."""
         P = np.empty((self.case.ncell, self.nvar + 1))
@@ -359,7 +365,8 @@ class Server:
         self._keep = {}
 
     def submit(self, case: "HostCase", U0=None, drop: float = 1e-8, max_iter: int = 5000,
-               keep_state: bool = False) -> int:
+               keep_state: bool = False, cfl_ramp=None) -> int:
+        """cfl_ramp: (cfl_max, growth per chunk) or None."""
         jid = C.c_int32()
         u = None
         if U0 is not None:
@@ -368,6 +375,8 @@ class Server:
                                       _p(u, C.c_double) if u is not None else None, drop, max_iter,
                                       1 if keep_state else 0, C.byref(jid)))
         self._keep[jid.value] = (case, u)
+        if cfl_ramp is not None:
+            check(lib().csu_server_set_ramp(self._h, jid.value, float(cfl_ramp[0]), float(cfl_ramp[1])))
         return jid.value
 
     def step(self, max_wait: float = 0.01) -> int:

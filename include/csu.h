@@ -125,6 +125,12 @@ int  csu_dev_residual(csu_dev* dev, double* R, double* dPx, double* dPy,
  * Iterations are counted across calls (the dt_local_n switch uses it). */
 int  csu_dev_iterate(csu_dev* dev, int32_t niter, double* res, double* coef);
 
+/* Change the CFL number between iterations (CFL ramps).  The next
+ * iteration computes its time step from the current state with the new
+ * number, as the reference does when a run is restarted from that state
+ * with a new CFL in input.dat (its way of ramping the CFL). */
+int  csu_dev_set_cfl(csu_dev* dev, double cfl);
+
 /* solverCalcPrimitive(solver, solver->U) (solver.c:1153-1213): the
  * positivity clamps applied to the device state in place, as the
  * reference's solverWriteSolution2 does before solverWriteReestart writes

@@ -52,6 +52,11 @@ int  csu_server_submit(csu_server* s, const csu_mesh_desc* mesh, const csu_param
  * solves.  Returns the number of solves retired by this call, or < 0. */
 int  csu_server_step(csu_server* s, double max_wait);
 
+/* CFL ramp for a job still in the queue: after each chunk its CFL is
+ * multiplied by growth (>= 1) up to cfl_max (csu_dev_set_cfl), the
+ * device-side form of the reference's chain of restarts at rising CFL. */
+int  csu_server_set_ramp(csu_server* s, int32_t id, double cfl_max, double growth);
+
 /* Move up to max finished results into out; returns the count. */
 int  csu_server_results(csu_server* s, csu_job_result* out, int32_t max);
 

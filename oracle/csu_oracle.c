@@ -2134,6 +2134,14 @@ int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
     return CSU_OK;
 }
 
+/* CFL change between iterations: the next solverCalcDt / implicitCalcD uses it */
+int oracle_set_cfl(csu_oracle* o, double cfl)
+{
+    if (!(cfl > 0.0)) return CSU_ERR_ARG;
+    o->prm.cfl = cfl;
+    return CSU_OK;
+}
+
 /* Time-accurate run, the tube branch of solverSolve (solver.c:1912-1941):
    global step from solverCalcDt, cut to tmax - t when t + dt passes tmax
    (that step is the last), solverStepRK, t += dt*stages/2.  The reference

@@ -23,6 +23,7 @@ int  oracle_residual(csu_oracle* o, double* R, double* dPx, double* dPy, double*
                      double* P, double* Pb);
 /* res [niter][nvar], coef [niter][4] (Cx_p Cx_v Cy_p Cy_v) */
 int  oracle_iterate(csu_oracle* o, int niter, double* res, double* coef);
+int  oracle_set_cfl(csu_oracle* o, double cfl);
 int  oracle_advance_time(csu_oracle* o, double tmax, int max_iter, double* t, int* niter, int* finished);
 int  oracle_wall_tensor(const csu_oracle* o, double* out);   /* [nbface][4] txx txy tyy yp */
 int  oracle_set_threads(int n);

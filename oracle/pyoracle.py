@@ -44,6 +44,7 @@ def lib():
         L.oracle_iterate.argtypes = [vp, C.c_int, f64, f64]
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_wall_tensor.argtypes = [vp, f64]
+        L.oracle_set_cfl.argtypes = [vp, C.c_double]
         L.oracle_advance_time.argtypes = [vp, C.c_double, C.c_int, f64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -96,6 +97,10 @@ class Oracle:
         coef = np.empty((n, 4))
         lib().oracle_iterate(self._h, n, _p(res), _p(coef))
         return res, coef
+
+    def set_cfl(self, cfl: float):
+        if lib().oracle_set_cfl(self._h, cfl) != 0:
+            raise RuntimeError("oracle_set_cfl failed")
 
     def advance_time(self, tmax: float, max_iter: int = 0, t0: float = 0.0):
         """Time-accurate run to tmax (tube mode).  Returns (t, steps, finished)."""

@@ -225,6 +225,45 @@ def make_solution2():
         print("wrote", name, S.shape)
 
 
+# CFL ramps as the reference does them: a run at one CFL, then a run
+# restarted from its final state with the next CFL.  The trace driver passes
+# the state in full precision, so the chain is exact.
+# (name, kind, mesh kwargs, input overrides, [(cfl, iterations), ...])
+CFL_CHAIN = [
+    ("cfl_ramp_rk", "ramp", dict(nx=12, ny=6), {}, [(0.5, 8), (0.9, 8), (1.4, 8)]),
+    ("cfl_ramp_lusgs", "ramp", dict(nx=16, ny=8, renumber=True), {"timeScheme": "LUSGS", "flux": "AUSMDV"},
+     [(5, 8), (10, 8), (20, 8)]),
+    ("cfl_plate_sa_lusgs", "plate", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "timeScheme": "LUSGS"},
+     [(10, 8), (30, 8)]),
+]
+
+
+def make_cfl_chain():
+    for name, kind, mk, inp, steps in CFL_CHAIN:
+        U, res, Us = None, [], []
+        mk_ = {k: v for k, v in mk.items() if k != "renumber"}
+        for cfl, n in steps:
+            d = case_dir(name)
+            over = dict(inp, CFL=cfl)
+            if kind == "ramp":
+                cases.write_case(d, cases.ramp_mesh(seed=0, **mk_), cases.ramp_input(threads=1, nmax=n, **over))
+            else:
+                cases.write_case(d, cases.plate_mesh(**mk_), cases.plate_input(threads=1, nmax=n, **over))
+            if mk.get("renumber"):
+                renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+            if U is None:
+                nv = 5 if inp.get("sa") == "1" else 4
+                nc = po.ref_trace(d, "mesh")["cellv"].size // 4
+            h = po.ref_trace(d, "hist", n, U=U) if U is not None else po.ref_trace(d, "hist", n)
+            res.append(h["res"].reshape(n, nv))
+            U = h["U"].reshape(nv, nc)
+            Us.append(U)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kind=np.array(kind), mesh_kwargs=np.array(repr(mk)),
+                            inputs=np.array(repr(inp)), steps=np.array(steps, dtype=np.float64),
+                            res=np.concatenate(res), U=U)
+        print("wrote", name, nc, "cells")
+
+
 def make_friction():
     import subprocess
     for name, mk, inp, nit in FRICTION:
@@ -247,3 +286,4 @@ if __name__ == "__main__":
     make_friction()
     make_tube()
     make_solution2()
+    make_cfl_chain()
