@@ -114,7 +114,12 @@ static int upload(csu_dev* d, const T** dst, const std::vector<T>& h) {
   T* p;
   int rc = dalloc(d, &p, h.size());
   if (rc) return rc;
-  if (!h.empty()) HIPCHK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  // on the context's own (non-blocking) stream: no legacy-stream work that
+  // would have to order itself after graph captures on other threads
+  if (!h.empty()) {
+    HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));  // h may be a temporary
+  }
   *dst = p;
   return CSU_OK;
 }
@@ -280,11 +285,17 @@ extern "C" int csu_wall_distance_dev(const csu_mesh_desc* m, double* wd) {
     return set_err(CSU_ERR_NOMEM, "hipMalloc failed");
   }
   int rc = CSU_OK;
-  if (hipMemcpy(dxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = set_err(CSU_ERR_HIP, "stream");
+  if (!rc && hipMemcpyAsync(dxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
     rc = set_err(CSU_ERR_HIP, "centre upload failed");
-  if (!rc) rc = wall_distance_launch(m, 0, dxy, dxy + nc, dd);
-  if (!rc && hipMemcpy(wd, dd, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+  if (!rc) rc = wall_distance_launch(m, st, dxy, dxy + nc, dd);
+  if (!rc && hipMemcpyAsync(wd, dd, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
     rc = set_err(CSU_ERR_HIP, "wall-distance download failed");
+  if (st) {
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = set_err(CSU_ERR_HIP, "wall distance");
+    (void)hipStreamDestroy(st);
+  }
   (void)hipFree(dxy);
   (void)hipFree(dd);
   return rc;
@@ -558,23 +569,23 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
       csu_dev_destroy(d);
       return rc;
     }
-    HIPCHK(hipMemset(s.wfric, 0, sizeof(double) * 2 * (size_t)(nb ? nb : 1)));
+    HIPCHK(hipMemsetAsync(s.wfric, 0, sizeof(double) * 2 * (size_t)(nb ? nb : 1), d->stream));
   }
   if (p->time_scheme != CSU_TIME_RK) {
     if ((rc = dalloc(d, &d->Dp, nc)) || (rc = dalloc(d, &d->dW0, nvc)) || (rc = dalloc(d, &d->dW1, nvc))) {
       csu_dev_destroy(d);
       return rc;
     }
-    HIPCHK(hipMemset(d->dW0, 0, nvc * sizeof(double)));
-    HIPCHK(hipMemset(d->dW1, 0, nvc * sizeof(double)));   // implicitInitDPLUR
+    HIPCHK(hipMemsetAsync(d->dW0, 0, nvc * sizeof(double), d->stream));
+    HIPCHK(hipMemsetAsync(d->dW1, 0, nvc * sizeof(double), d->stream));   // implicitInitDPLUR
   }
   if (p->time_scheme == CSU_TIME_DPLUR) {
     if ((rc = dalloc(d, &d->A, (size_t)4 * nv * nv * nc)) || (rc = dalloc(d, &d->jstate, 2))) {
       csu_dev_destroy(d);
       return rc;
     }
-    HIPCHK(hipMemset(d->A, 0, (size_t)4 * nv * nv * nc * sizeof(double)));
-    HIPCHK(hipMemset(d->jstate, 0, 2 * sizeof(int)));
+    HIPCHK(hipMemsetAsync(d->A, 0, (size_t)4 * nv * nv * nc * sizeof(double), d->stream));
+    HIPCHK(hipMemsetAsync(d->jstate, 0, 2 * sizeof(int), d->stream));
   }
   if (p->time_scheme == CSU_TIME_LUSGS) {
     // levels of the sweeps' dependency DAGs (lower / upper cell index)
@@ -611,15 +622,14 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
     csu_dev_destroy(d);
     return rc;
   }
-  HIPCHK(hipMemset(d->hctr, 0, sizeof(int)));
+  HIPCHK(hipMemsetAsync(d->hctr, 0, sizeof(int), d->stream));
   // free-stream initial state (solverInitU)
   std::vector<double> U0(nvc);
   for (int k = 0; k < nv; k++)
     for (int i = 0; i < nc; i++) U0[(size_t)k * nc + i] = p->uin[k];
-  HIPCHK(hipMemcpy(d->U, U0.data(), nvc * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(s.Pb, 0, sizeof(double) * (size_t)(nv + 1) * (nb ? nb : 1)));
+  HIPCHK(hipMemcpyAsync(d->U, U0.data(), nvc * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipMemsetAsync(s.Pb, 0, sizeof(double) * (size_t)(nv + 1) * (nb ? nb : 1), d->stream));
   // this context's work only: other contexts' streams keep running
-  HIPCHK(hipStreamSynchronize(nullptr));
   HIPCHK(hipStreamSynchronize(d->stream));
   *out = d;
   return CSU_OK;
