@@ -27,17 +27,41 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double w
   if (i < nc) {
     const double* P = s.P;
     double Lc = 0, Lv = 0;
+    // all slots' face data, then all states, then the sums in slot order
+    int cfj[4], fij[4], e0j[4], e1j[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int f = g.cface[j * nc + i];
-      if (f == 0) continue;
-      const int fi = f > 0 ? f - 1 : -f - 1;
-      const int e0 = g.fe0[fi], e1 = g.fe1[fi];
-      const double dSx = g.fdsx[fi], dSy = g.fdsy[fi], dS = g.fds[fi];
-      double r = (P[e0] + P[e1]) * 0.5;
-      double u = (P[nc + e0] + P[nc + e1]) * 0.5;
-      double v = (P[2 * nc + e0] + P[2 * nc + e1]) * 0.5;
-      double T = (P[4 * nc + e0] + P[4 * nc + e1]) * 0.5;
+      cfj[j] = g.cface[j * nc + i];
+      fij[j] = cfj[j] > 0 ? cfj[j] - 1 : (cfj[j] < 0 ? -cfj[j] - 1 : 0);
+    }
+    double sxj[4], syj[4], sj[4], Pa[4][4], Pb[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      e0j[j] = g.fe0[fij[j]];
+      e1j[j] = g.fe1[fij[j]];
+      sxj[j] = g.fdsx[fij[j]];
+      syj[j] = g.fdsy[fij[j]];
+      sj[j] = g.fds[fij[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int a = cfj[j] == 0 ? i : e0j[j], b = cfj[j] == 0 ? i : e1j[j];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int m = q == 3 ? 4 : q;  // r u v T
+        Pa[j][q] = P[m * nc + a];
+        Pb[j][q] = P[m * nc + b];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (cfj[j] == 0) continue;
+      const int fi = fij[j];
+      const double dSx = sxj[j], dSy = syj[j], dS = sj[j];
+      double r = (Pa[j][0] + Pb[j][0]) * 0.5;
+      double u = (Pa[j][1] + Pb[j][1]) * 0.5;
+      double v = (Pa[j][2] + Pb[j][2]) * 0.5;
+      double T = (Pa[j][3] + Pb[j][3]) * 0.5;
       double c = gas_T2c<double>(kp.gas, T);
       double nx = dSx / dS, ny = dSy / dS;
       Lc += (fabs(nx * u + ny * v) + c) * dS;
@@ -223,12 +247,32 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) acc[k] = DIR == 0 ? -R[k * nc + i] : dW[k * nc + i];
+  // Face data of all slots is loaded before any slot is used (one round
+  // trip per dependency level, not per slot); neighbour states are loaded
+  // per slot, only for the neighbours this sweep uses (loading them all up
+  // front needs ~200 VGPRs and halves the occupancy, measured slower).
+  int cfj[4], nbj[4], fij[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) cfj[j] = g.cface[j * nc + i];
+  double sxj[4], syj[4], sj[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    double dSx, dSy, dS;
-    int fi;
-    const int nb = face_from(g, nc, i, j, dSx, dSy, dS, fi);
-    if (nb < 0) continue;
+    const int f = cfj[j];
+    const int fi = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
+    fij[j] = fi;
+    sj[j] = g.fds[fi];
+    sxj[j] = g.fdsx[fi];
+    syj[j] = g.fdsy[fi];
+    const int e = (f > 0 ? g.fe1 : g.fe0)[fi];
+    nbj[j] = f == 0 ? i : e;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    // the slot's neighbour and outward face vector (face_from: the vector
+    // is negated when i is the face's e1, implicit.c:288-305)
+    if (cfj[j] == 0) continue;
+    const int nb = nbj[j], fi = fij[j];
+    const double dS = sj[j], dSx = cfj[j] > 0 ? sxj[j] : -sxj[j], dSy = cfj[j] > 0 ? syj[j] : -syj[j];
     if (DIR == 0 ? !(nb < i) : !(nb > i)) continue;
     const double nx = dSx / dS, ny = dSy / dS;
     double Pj[NV + 1];
