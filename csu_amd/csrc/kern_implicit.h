@@ -435,20 +435,30 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     double xi[NV], acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) { xi[k] = x[k * nc + i]; acc[k] = D * xi[k]; }
+    // neighbours of all slots first, then their x, then the products in
+    // slot order (slots without a face read the cell itself, skipped)
+    int cfj[4], nbj[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) cfj[j] = g.cface[j * nc + i];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      double dSx, dSy, dS;
-      int fi;
-      const int nb = face_from(g, nc, i, j, dSx, dSy, dS, fi);
-      if (nb < 0) continue;
-      const double* Aj = A + (size_t)j * NV * NV * nc;
-      double xj[NV];
+      const int f = cfj[j], fi = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
+      const int e = (f > 0 ? g.fe1 : g.fe0)[fi];
+      nbj[j] = f == 0 ? i : e;
+    }
+    double xj[4][NV];
 #pragma unroll
-      for (int n = 0; n < NV; n++) xj[n] = x[n * nc + nb];
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int n = 0; n < NV; n++) xj[j][n] = x[n * nc + nbj[j]];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (cfj[j] == 0) continue;
+      const double* Aj = A + (size_t)j * NV * NV * nc;
 #pragma unroll
       for (int k = 0; k < NV; k++)
 #pragma unroll
-        for (int n = 0; n < NV; n++) acc[k] += Aj[(size_t)(k * NV + n) * nc + i] * xj[n];
+        for (int n = 0; n < NV; n++) acc[k] += Aj[(size_t)(k * NV + n) * nc + i] * xj[j][n];
     }
     const double w = 2. / 3.;
 #pragma unroll
