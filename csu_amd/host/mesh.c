@@ -27,18 +27,58 @@
 #include <math.h>
 #include "csu_host.h"
 
-/* meshGetWord semantics: returns the delimiter, -1 at EOF */
-static int get_word(FILE* f, char* s, int cap)
+/* The whole file in memory, read token by token with meshGetWord's
+   semantics (mesh.c:10-42): a token ends at the first ' ', '\r' or '\n',
+   which is consumed, so consecutive delimiters give empty tokens; returns
+   the delimiter, -1 at the end of the file. */
+typedef struct { const char* p; const char* end; } rbuf;
+
+static int get_word(rbuf* f, char* s, int cap)
 {
-    int n = 0, c;
-    for (;;) {
-        c = fgetc(f);
-        if (c == EOF) break;
+    int n = 0, c = -1;
+    while (f->p < f->end) {
+        c = (unsigned char)*f->p++;
         if (c == ' ' || c == '\r' || c == '\n') break;
         if (n < cap - 1) s[n++] = (char)c;
+        c = -1;
     }
     s[n] = '\0';
     return c;
+}
+
+static char* slurp(const char* path, size_t* len)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) return NULL;
+    size_t cap = 1 << 20, n = 0;
+    char* b = malloc(cap);
+    for (;;) {
+        if (n == cap) { cap *= 2; b = realloc(b, cap); }
+        size_t r = fread(b + n, 1, cap - n, f);
+        n += r;
+        if (r == 0) break;
+    }
+    fclose(f);
+    *len = n;
+    return b;
+}
+
+/* Stable LSD radix sort of idx[0..n) by key[idx[.]] (8-bit digits, only as
+   many as the largest key needs): replaces the comparison sorts of the
+   face enumeration, whose keys make every order total. */
+static void radix_sort(const uint64_t* key, uint32_t* idx, size_t n)
+{
+    uint64_t mx = 0;
+    for (size_t i = 0; i < n; i++) if (key[i] > mx) mx = key[i];
+    uint32_t* tmp = malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (int shift = 0; shift < 64 && (mx >> shift) != 0; shift += 8) {
+        size_t cnt[257] = {0};
+        for (size_t i = 0; i < n; i++) cnt[((key[idx[i]] >> shift) & 255) + 1]++;
+        for (int b = 0; b < 256; b++) cnt[b + 1] += cnt[b];
+        for (size_t i = 0; i < n; i++) tmp[cnt[(key[idx[i]] >> shift) & 255]++] = idx[i];
+        memcpy(idx, tmp, sizeof(uint32_t) * n);
+    }
+    free(tmp);
 }
 
 static double tri_area(const double* xy, int p0, int p1, int p2)
@@ -64,22 +104,7 @@ static double cell_volume(const double* xy, const int32_t* v, int axi)
 }
 
 typedef struct { int64_t key; int64_t seq; int32_t cell, p0, p1; } dedge;
-
-static int cmp_dedge(const void* a, const void* b)
-{
-    const dedge* x = a; const dedge* y = b;
-    if (x->key != y->key) return x->key < y->key ? -1 : 1;
-    return x->seq < y->seq ? -1 : (x->seq > y->seq);
-}
-
 typedef struct { int64_t bucket; int64_t seq; int32_t e0, e1, p0, p1; } rface;
-
-static int cmp_rface(const void* a, const void* b)
-{
-    const rface* x = a; const rface* y = b;
-    if (x->bucket != y->bucket) return x->bucket < y->bucket ? -1 : 1;
-    return x->seq < y->seq ? -1 : (x->seq > y->seq);
-}
 
 static int64_t hash_bucket(int64_t size, int p0, int p1)
 {
@@ -98,8 +123,10 @@ static int err(const char* msg, const char* path)
 
 int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
 {
-    FILE* f = fopen(path, "r");
-    if (!f) return err("cannot open", path);
+    size_t flen = 0;
+    char* fbuf = slurp(path, &flen);
+    if (!fbuf) return err("cannot open", path);
+    rbuf rb = {fbuf, fbuf + flen}, *f = &rb;
     char s[128];
     int nelem = -1, npoin = -1, nmark = -1;
 
@@ -107,21 +134,21 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     if (strcmp(s, "NDIME=") == 0) get_word(f, s, sizeof s);
     get_word(f, s, sizeof s);
     if (strcmp(s, "NELEM=") == 0) { get_word(f, s, sizeof s); nelem = atoi(s); }
-    if (nelem <= 0) { fclose(f); return err("missing NELEM=", path); }
+    if (nelem <= 0) { free(fbuf); return err("missing NELEM=", path); }
 
     int32_t* cv = malloc(sizeof(int32_t) * 4 * (size_t)nelem);
     for (int i = 0; i < nelem; i++) {
         get_word(f, s, sizeof s);
         int type = atoi(s);
         int np = type == 5 ? 3 : type == 9 ? 4 : 0;
-        if (!np) { free(cv); fclose(f); return err("element type is not 5 or 9", path); }
+        if (!np) { free(cv); free(fbuf); return err("element type is not 5 or 9", path); }
         for (int k = 0; k < 4; k++) cv[4 * i + k] = -1;
         for (int k = 0; k < np; k++) { get_word(f, s, sizeof s); cv[4 * i + k] = atoi(s); }
         get_word(f, s, sizeof s);                       /* element index */
     }
     get_word(f, s, sizeof s);
     if (strcmp(s, "NPOIN=") == 0) { get_word(f, s, sizeof s); npoin = atoi(s); }
-    if (npoin <= 0) { free(cv); fclose(f); return err("missing NPOIN=", path); }
+    if (npoin <= 0) { free(cv); free(fbuf); return err("missing NPOIN=", path); }
     double* xy = malloc(sizeof(double) * 2 * (size_t)npoin);
     for (int i = 0; i < npoin; i++) {
         get_word(f, s, sizeof s); xy[2 * i] = strtod(s, NULL);
@@ -130,7 +157,7 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     }
     get_word(f, s, sizeof s);
     if (strcmp(s, "NMARK=") == 0) { get_word(f, s, sizeof s); nmark = atoi(s); }
-    if (nmark < 0) { free(cv); free(xy); fclose(f); return err("missing NMARK=", path); }
+    if (nmark < 0) { free(cv); free(xy); free(fbuf); return err("missing NMARK=", path); }
 
     char (*mname)[64] = calloc((size_t)(nmark > 0 ? nmark : 1), sizeof *mname);
     int* mcount = calloc((size_t)(nmark > 0 ? nmark : 1), sizeof(int));
@@ -158,7 +185,7 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
             nb++;
         }
     }
-    fclose(f);
+    free(fbuf);
     free(mcount);
 
     for (int i = 0; i < nelem; i++)
@@ -197,7 +224,17 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
             nde++;
         }
     }
-    qsort(de, nde, sizeof(dedge), cmp_dedge);
+    {
+        /* sorted on (key, seq): de is built in seq order, the sort is stable */
+        uint64_t* k = malloc(sizeof(uint64_t) * (nde ? nde : 1));
+        uint32_t* ix = malloc(sizeof(uint32_t) * (nde ? nde : 1));
+        for (size_t i = 0; i < nde; i++) { k[i] = (uint64_t)de[i].key; ix[i] = (uint32_t)i; }
+        radix_sort(k, ix, nde);
+        dedge* ds = malloc(sizeof(dedge) * (nde ? nde : 1));
+        for (size_t i = 0; i < nde; i++) ds[i] = de[ix[i]];
+        free(de); free(k); free(ix);
+        de = ds;
+    }
     rface* rf = malloc(sizeof(rface) * (nde / 2 + 1));
     size_t nf = 0;
     int64_t hsize = (int64_t)nelem + npoin;
@@ -221,7 +258,20 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
         a = b;
     }
     free(de);
-    qsort(rf, nf, sizeof(rface), cmp_rface);
+    {
+        /* sorted on (bucket, seq): both below 2^31, one 62-bit key */
+        uint64_t* k = malloc(sizeof(uint64_t) * (nf ? nf : 1));
+        uint32_t* ix = malloc(sizeof(uint32_t) * (nf ? nf : 1));
+        for (size_t i = 0; i < nf; i++) {
+            k[i] = ((uint64_t)rf[i].bucket << 31) | (uint64_t)rf[i].seq;
+            ix[i] = (uint32_t)i;
+        }
+        radix_sort(k, ix, nf);
+        rface* rs = malloc(sizeof(rface) * (nf + 1));
+        for (size_t i = 0; i < nf; i++) rs[i] = rf[ix[i]];
+        free(rf); free(k); free(ix);
+        rf = rs;
+    }
 
     int32_t* face = malloc(sizeof(int32_t) * 4 * (nf ? nf : 1));
     int32_t* nbr = malloc(sizeof(int32_t) * 4 * (size_t)nelem);
