@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
-HOST_SRC = ["input.c", "mesh.c", "case.c", "output.c"]
+HOST_SRC = ["input.c", "mesh.c", "case.c", "output.c", "order.c"]
 HIP_SRC = ["csu_dev.hip"]
 CXX_SRC = ["csu_serve.cpp"]   # host C++ over the C ABI (no device code)
 CXX = os.environ.get("CXX", "g++")

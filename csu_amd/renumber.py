@@ -29,28 +29,48 @@ from scipy.sparse.csgraph import reverse_cuthill_mckee
 
 
 def _read_elements(lines):
+    """Index of the NELEM= line, the element count and the vertex lists as
+    an [n, 4] array (-1 where a triangle has no fourth vertex)."""
     for k, ln in enumerate(lines):
         if ln.startswith("NELEM="):
             n = int(ln.split("=")[1])
-            cells = []
-            for ln2 in lines[k + 1:k + 1 + n]:
-                tok = ln2.split()
-                t = int(tok[0])
-                nv = 3 if t == 5 else 4
-                cells.append([int(x) for x in tok[1:1 + nv]])
+            block = "".join(lines[k + 1:k + 1 + n])
+            tok = np.array(block.split(), dtype=np.int64)
+            # each line: type (5 triangle, 9 quad), its vertices, an index;
+            # the type is the first character of the line
+            raw = np.frombuffer(block.encode(), dtype=np.uint8)
+            line_start = np.concatenate(([0], np.flatnonzero(raw == ord("\n"))[:-1] + 1))
+            types = np.where(raw[line_start] == ord("5"), 5, 9)
+            lens = np.where(types == 5, 5, 6)
+            cells = np.full((n, 4), -1, np.int64)
+            starts = np.concatenate(([0], np.cumsum(lens)[:-1]))
+            tri = types == 5
+            for j in range(3):
+                cells[:, j] = tok[starts + 1 + j]
+            cells[~tri, 3] = tok[starts[~tri] + 4]
             return k, n, cells
     raise ValueError("no NELEM= line")
 
 
 def cell_graph(cells) -> sp.csr_matrix:
     """Cells sharing an edge."""
-    a, b, c = [], [], []
-    for i, cv in enumerate(cells):
-        n = len(cv)
-        for j in range(n):
-            p, q = cv[j], cv[(j + 1) % n]
-            a.append(min(p, q)); b.append(max(p, q)); c.append(i)
-    a, b, c = np.array(a, np.int64), np.array(b, np.int64), np.array(c, np.int64)
+    cells = np.asarray(cells, dtype=np.int64) if not isinstance(cells, np.ndarray) else cells
+    if cells.ndim != 2:   # list of vertex lists of uneven length
+        cells = np.array([list(c) + [-1] * (4 - len(c)) for c in cells], np.int64)
+    nc = cells.shape[0]
+    quad = cells[:, 3] >= 0
+    ids = np.arange(nc, dtype=np.int64)
+    a_l, b_l, c_l = [], [], []
+    for j in range(4):
+        p = cells[:, j]
+        q = np.where(quad, cells[:, (j + 1) % 4], cells[:, (j + 1) % 3]) if j < 3 else cells[:, 0]
+        m = quad if j == 3 else np.ones(nc, bool)
+        p, q = p[m], q[m]
+        # edges in the order the cell lists them: j = 0, 1, 2 (, 3)
+        a_l.append(np.minimum(p, q)); b_l.append(np.maximum(p, q)); c_l.append(ids[m])
+    # the stable sort below pairs an edge's two cells in cell order
+    order = np.lexsort((np.concatenate([np.full(len(x), j) for j, x in enumerate(c_l)]), np.concatenate(c_l)))
+    a, b, c = np.concatenate(a_l)[order], np.concatenate(b_l)[order], np.concatenate(c_l)[order]
     key = a * (b.max() + 1) + b
     order = np.argsort(key, kind="stable")
     ks, cs = key[order], c[order]
@@ -67,17 +87,29 @@ def colour_order(cells) -> np.ndarray:
     g = cell_graph(cells)
     rcm = reverse_cuthill_mckee(g, symmetric_mode=True)
     nc = len(cells)
-    colour = np.full(nc, -1, np.int64)
-    indptr, indices = g.indptr, g.indices
-    for i in rcm:
-        used = {colour[j] for j in indices[indptr[i]:indptr[i + 1]] if colour[j] >= 0}
-        k = 0
-        while k in used:
-            k += 1
-        colour[i] = k
+    colour = greedy_colour(g, rcm)
     pos = np.empty(nc, np.int64)
     pos[rcm] = np.arange(nc)
     return np.lexsort((pos, colour))
+
+
+def greedy_colour(g: sp.csr_matrix, order: np.ndarray) -> np.ndarray:
+    """Visit cells in `order`; each takes the smallest colour none of its
+    already coloured neighbours has (csu_greedy_colour in the host library)."""
+    import ctypes as C
+    from csu_amd import native
+    n = g.shape[0]
+    indptr = np.ascontiguousarray(g.indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(g.indices, dtype=np.int32)
+    order = np.ascontiguousarray(order, dtype=np.int64)
+    colour = np.empty(n, np.int32)
+    L = native.lib()
+    rc = L.csu_greedy_colour(n, indptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                             indices.ctypes.data_as(C.POINTER(C.c_int32)),
+                             order.ctypes.data_as(C.POINTER(C.c_int64)), colour.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != 0:
+        raise RuntimeError(f"csu_greedy_colour failed ({rc})")
+    return colour.astype(np.int64)
 
 
 def lusgs_depth(cells, order=None) -> int:

@@ -76,6 +76,12 @@ int  csu_write_solution(const csu_case* c, const double* U, const char* path);
  * and SA runs (SST: CSU_ERR_UNSUPPORTED). */
 int  csu_write_solution2(const csu_case* c, const double* P, const char* path);
 
+/* Greedy colouring for the colour/RCM renumbering (csu_amd/renumber.py):
+ * cells visited in order[0..n) each take the smallest colour no coloured
+ * neighbour (CSR indptr/indices) has.  No counterpart in the reference. */
+int  csu_greedy_colour(int32_t n, const int64_t* indptr, const int32_t* indices, const int64_t* order,
+                       int32_t* colour);
+
 /* Name of the input.dat key whose absence (or bad value) made the last
  * csu_case_load fail with CSU_ERR_INPUT. */
 const char* csu_host_missing_key(void);
