@@ -248,6 +248,67 @@ def other_configs(log, n_plate: int = 500, n_naca: int = 500):
     return out
 
 
+def batch_mix(n: int, iters: int, slots: int, log, seed: int = 3):
+    """Many independent solves at once: n seeded meshes, cell counts
+    log-uniform in [10k, 400k], cycling Euler ramp / laminar plate / SA
+    NACA 0012, Mach U[0.2, 2.5], AoA U[-4, 10] deg (airfoils), Re log-U[1e5,
+    1e7] (viscous kinds, through the pressure), mixed precision (FP32 fluxes,
+    FP64 update), each run for `iters` iterations through the server with
+    `slots` solves on the GPU at a time.  Reports the aggregate
+    cell-iterations per second."""
+    from csu_amd import cases, native, renumber
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t0 = time.perf_counter()
+    for k in range(n):
+        kind = ("ramp", "plate", "naca")[k % 3]
+        ncell = float(np.exp(rng.uniform(np.log(1e4), np.log(4e5))))
+        mach = float(rng.uniform(0.2, 2.5))
+        re = float(np.exp(rng.uniform(np.log(1e5), np.log(1e7))))
+        aoa = float(rng.uniform(-4.0, 10.0))
+        d = tempfile.mkdtemp(prefix=f"bm{k}_", dir="/tmp") + "/"
+        if kind == "ramp":
+            ny = max(8, int(round(np.sqrt(ncell / 4))))
+            cases.write_case(d, cases.ramp_mesh(nx=2 * ny, ny=ny, seed=k),
+                             cases.ramp_input(threads=1, nmax=1, mach=f"{mach:.4f}", precision=1))
+        elif kind == "plate":
+            ny = max(8, int(round(np.sqrt(ncell / 5.6))))
+            p = 1526.1 * (re / 1e5) * (0.3 / mach)
+            cases.write_case(d, cases.plate_mesh(nx=5 * ny, ny=ny, seed=k),
+                             cases.plate_input(threads=1, nmax=1, mach=f"{mach:.4f}", pressure=f"{p:.6g}",
+                                               pout=f"{p:.6g}", precision=1))
+        else:
+            ni = max(64, int(round(ncell / 160)))
+            p = 183134.3 * (re / 6e6) * (0.15 / mach)
+            cases.write_case(d, cases.naca_mesh(ni=ni, nwall=40, nfar=60, seed=k),
+                             cases.naca_input(threads=1, nmax=1, aoa=aoa, mach=f"{mach:.4f}", pressure=f"{p:.6g}",
+                                              pout=f"{p:.6g}", precision=1))
+            renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+        reqs.append((kind, native.HostCase(d, host_wall_distance=False)))
+    log(f"batch mix: {n} cases, {sum(c.ncell for _, c in reqs)} cells prepared in {time.perf_counter() - t0:.1f} s")
+    srv = native.Server(max_slots=slots, hbm_budget=250e9, chunk=50)
+    t0 = time.perf_counter()
+    for _, c in reqs:
+        srv.submit(c, drop=1e-300, max_iter=iters)
+    res = []
+    while len(res) < n:
+        srv.step(0.002)
+        res += srv.results()
+    span = time.perf_counter() - t0
+    srv.close()
+    ci = sum(r["iters"] * r["ncell"] for r in res)
+    by = {}
+    for (kind, c), r in zip(reqs, sorted(res, key=lambda r: r["id"])):
+        by.setdefault(kind, [0, 0])
+        by[kind][0] += c.ncell
+        by[kind][1] += 1
+    return {"solves": n, "cells": int(sum(c.ncell for _, c in reqs)), "iterations_each": iters, "slots": slots,
+            "precision": "mixed (FP32 gradients/limiter/edge states/convective flux, FP64 update)",
+            "seconds": span, "cell_iter_per_s": ci / span, "kinds": {k: {"solves": v[1], "cells": v[0]}
+                                                                   for k, v in by.items()},
+            "errors": sum(r["status"] != 0 for r in res)}
+
+
 def cpu_baseline(nx: int, ny: int, iters: int):
     """Reference OpenMP build (oracle/_ref/csu_ref, compiled from the
     reference sources) on the host cores: Nc*N / (T(N) - T(0))."""
@@ -283,6 +344,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="also run the many-solve mix of N seeded cases (ramp / plate / SA airfoil, 10k-400k "
+                         "cells each) through the server and report its aggregate throughput")
+    ap.add_argument("--batch-iters", type=int, default=200)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the laminar-plate and NACA 0012 SA/SST measurements")
     ap.add_argument("--serve-n", type=int, default=24)
@@ -354,6 +419,7 @@ def main():
             "iteration_alg_bytes_per_cell": 2334.5,
             "iteration_frac": value * 2334.5 / (HBM_PEAK_GBS * 1e9)}
     others = None if a.no_configs else other_configs(log)
+    batch = batch_mix(a.batch, a.batch_iters, a.serve_slots, log) if a.batch else None
     serve = None if a.no_serve else serving(a.serve_n, a.serve_rate, a.serve_slots, log)
     cpu = None if a.no_cpu else cpu_baseline(200, 100, a.cpu_iters)
     line = {
@@ -368,6 +434,7 @@ def main():
         "roofline": roof,
         "mixed_precision": mixed,
         "other_configs": others,
+        "batch_mix": batch,
         "serving": serve,
         "cpu_baseline": cpu,
     }

@@ -133,13 +133,13 @@ static int enqueue_chunk(csu_server* s, Job* j) {
 
 static void admit(csu_server* s) {
   while (!s->queue.empty() && (int)s->active.size() < s->max_slots) {
-    const Req& r = s->queue.front();
-    const int64_t est = csu_dev_footprint(r.mesh, r.prm, s->chunk);
+    const int64_t est = csu_dev_footprint(s->queue.front().mesh, s->queue.front().prm, s->chunk);
     // a solve larger than the whole budget still runs, alone
     if (!s->active.empty() && s->used + est > s->budget) break;
     Job* j = new Job();
-    j->r = r;
+    j->r = s->queue.front();
     s->queue.pop_front();
+    const Req& r = j->r;  // the popped queue element no longer exists
     j->res.id = r.id;
     j->res.ncell = r.mesh->ncell;
     j->res.t_submit = r.t_submit;
