@@ -5,11 +5,15 @@
 // served), keeps a chunk of iterations enqueued on every active context,
 // and retires solves whose density residual has dropped by the requested
 // factor.  Contexts are independent, so the GPU overlaps their kernels and
-// each solve's numbers do not depend on what else is running.
+// each solve's numbers do not depend on what else is running.  Contexts are
+// created on a worker thread (host geometry, uploads, wall distance), so
+// admission overlaps with the iterations of the solves already running.
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 #include <string.h>
@@ -37,6 +41,8 @@ struct Job {
   int chunk_n = 0;  // iterations of the chunk in flight
   bool res0_set = false;
   double cfl = 0.0;  // current CFL of a ramped job
+  int64_t est = 0;   // footprint estimate charged while the context is created
+  int create_rc = 0;
 };
 
 }  // namespace
@@ -51,7 +57,33 @@ struct csu_server {
   std::deque<csu_job_result> done;
   std::map<int, std::vector<double>> states;
   std::vector<double> rbuf, cbuf;
+  // context creation on the worker thread: jobs handed over, jobs back
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job*> to_create, created;
+  int creating = 0;  // handed over and not yet taken back (main thread's count)
+  bool stop = false;
+  std::thread worker;
 };
+
+static void creator(csu_server* s) {
+  for (;;) {
+    Job* j;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [s] { return s->stop || !s->to_create.empty(); });
+      if (s->stop) return;
+      j = s->to_create.front();
+      s->to_create.pop_front();
+    }
+    int rc = csu_dev_create(j->r.mesh, j->r.prm, &j->dev);
+    if (rc) j->dev = nullptr;
+    if (!rc && j->r.U0) rc = csu_dev_set_state(j->dev, j->r.U0);
+    j->create_rc = rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->created.push_back(j);
+  }
+}
 
 static double now(const csu_server* s) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
@@ -66,12 +98,24 @@ extern "C" int csu_server_create(int32_t max_slots, int64_t hbm_budget, int32_t 
   s->budget = hbm_budget;
   s->chunk = chunk;
   s->t0 = std::chrono::steady_clock::now();
+  s->worker = std::thread(creator, s);
   *out = s;
   return CSU_OK;
 }
 
 extern "C" void csu_server_destroy(csu_server* s) {
   if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  s->worker.join();
+  for (Job* j : s->to_create) delete j;
+  for (Job* j : s->created) {
+    if (j->dev) csu_dev_destroy(j->dev);
+    delete j;
+  }
   for (Job* j : s->active) {
     csu_dev_destroy(j->dev);
     delete j;
@@ -102,7 +146,7 @@ extern "C" int csu_server_set_ramp(csu_server* s, int32_t id, double cfl_max, do
 extern "C" int csu_server_pending(const csu_server* s, int32_t* queued, int32_t* active) {
   if (!s) return CSU_ERR_ARG;
   if (queued) *queued = (int32_t)s->queue.size();
-  if (active) *active = (int32_t)s->active.size();
+  if (active) *active = (int32_t)s->active.size() + s->creating;
   return CSU_OK;
 }
 
@@ -131,28 +175,55 @@ static int enqueue_chunk(csu_server* s, Job* j) {
   return csu_dev_iterate_async(j->dev, n);
 }
 
+// Hand queued solves to the creator thread while slots and the budget allow
+// (contexts being created count against both, at their footprint estimate).
 static void admit(csu_server* s) {
-  while (!s->queue.empty() && (int)s->active.size() < s->max_slots) {
+  bool handed = false;
+  while (!s->queue.empty() && (int)s->active.size() + s->creating < s->max_slots) {
     const int64_t est = csu_dev_footprint(s->queue.front().mesh, s->queue.front().prm, s->chunk);
     // a solve larger than the whole budget still runs, alone
-    if (!s->active.empty() && s->used + est > s->budget) break;
+    if ((!s->active.empty() || s->creating) && s->used + est > s->budget) break;
     Job* j = new Job();
     j->r = s->queue.front();
     s->queue.pop_front();
-    const Req& r = j->r;  // the popped queue element no longer exists
-    j->res.id = r.id;
-    j->res.ncell = r.mesh->ncell;
-    j->res.t_submit = r.t_submit;
+    j->res.id = j->r.id;
+    j->res.ncell = j->r.mesh->ncell;
+    j->res.t_submit = j->r.t_submit;
     j->res.t_admit = now(s);
     j->res.iter_conv = -1;
-    j->cfl = r.prm->cfl;
-    s->active.push_back(j);
-    int rc = csu_dev_create(r.mesh, r.prm, &j->dev);
-    if (rc) { j->dev = nullptr; retire(s, s->active.size() - 1, rc); continue; }
-    s->used += csu_dev_bytes(j->dev);
-    if (r.U0 && (rc = csu_dev_set_state(j->dev, r.U0))) { retire(s, s->active.size() - 1, rc); continue; }
-    if ((rc = enqueue_chunk(s, j))) { retire(s, s->active.size() - 1, rc); continue; }
+    j->cfl = j->r.prm->cfl;
+    j->est = est;
+    s->used += est;
+    s->creating++;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->to_create.push_back(j);
+    }
+    handed = true;
   }
+  if (handed) s->cv.notify_one();
+}
+
+// Take back the contexts the creator has finished and start their first chunk.
+static int take_created(csu_server* s) {
+  std::deque<Job*> got;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    got.swap(s->created);
+  }
+  int retired = 0;
+  for (Job* j : got) {
+    s->creating--;
+    s->used -= j->est;
+    s->active.push_back(j);
+    int rc = j->create_rc;
+    if (!rc) {
+      s->used += csu_dev_bytes(j->dev);
+      rc = enqueue_chunk(s, j);
+    }
+    if (rc) { retire(s, s->active.size() - 1, rc); retired++; }
+  }
+  return retired;
 }
 
 // A finished chunk: read its history, decide, and feed the next chunk.
@@ -188,9 +259,12 @@ extern "C" int csu_server_step(csu_server* s, double max_wait) {
   if (!s) return -CSU_ERR_ARG;
   admit(s);
   const double t_end = now(s) + (max_wait > 0 ? max_wait : 0);
-  int retired = 0;
-  bool any = false;
-  while (!s->active.empty()) {
+  int retired = take_created(s);
+  bool any = retired > 0;
+  while (!s->active.empty() || s->creating) {
+    const int r0 = take_created(s);
+    retired += r0;
+    if (r0) any = true;
     for (size_t k = 0; k < s->active.size();) {
       int st = csu_dev_ready(s->active[k]->dev);
       if (st < 0) { retire(s, k, -st); retired++; any = true; continue; }
