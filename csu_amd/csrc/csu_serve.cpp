@@ -6,7 +6,7 @@
 // and retires solves whose density residual has dropped by the requested
 // factor.  Contexts are independent, so the GPU overlaps their kernels and
 // each solve's numbers do not depend on what else is running.  Contexts are
-// created on a worker thread (host geometry, uploads, wall distance), so
+// created on worker threads (host geometry, uploads, wall distance), so
 // admission overlaps with the iterations of the solves already running.
 #include <algorithm>
 #include <chrono>
@@ -63,7 +63,7 @@ struct csu_server {
   std::deque<Job*> to_create, created;
   int creating = 0;  // handed over and not yet taken back (main thread's count)
   bool stop = false;
-  std::thread worker;
+  std::vector<std::thread> workers;  // creators: up to 4, never more than the slots
 };
 
 static void creator(csu_server* s) {
@@ -98,7 +98,7 @@ extern "C" int csu_server_create(int32_t max_slots, int64_t hbm_budget, int32_t 
   s->budget = hbm_budget;
   s->chunk = chunk;
   s->t0 = std::chrono::steady_clock::now();
-  s->worker = std::thread(creator, s);
+  for (int k = 0; k < std::min(4, max_slots); k++) s->workers.emplace_back(creator, s);
   *out = s;
   return CSU_OK;
 }
@@ -110,7 +110,7 @@ extern "C" void csu_server_destroy(csu_server* s) {
     s->stop = true;
   }
   s->cv.notify_all();
-  s->worker.join();
+  for (std::thread& t : s->workers) t.join();
   for (Job* j : s->to_create) delete j;
   for (Job* j : s->created) {
     if (j->dev) csu_dev_destroy(j->dev);
@@ -201,7 +201,7 @@ static void admit(csu_server* s) {
     }
     handed = true;
   }
-  if (handed) s->cv.notify_one();
+  if (handed) s->cv.notify_all();
 }
 
 // Take back the contexts the creator has finished and start their first chunk.
