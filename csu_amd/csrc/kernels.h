@@ -115,9 +115,12 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
     }
     vT[j] = VISC ? base[4 * st] : 0.0;
   }
-  TF ex[4], ey[4];
+  // every cell has slots 0-2; slot 3 (quads only) is read only by quads,
+  // a branch that is uniform on triangle and quad meshes alike
+  TF ex[4] = {0, 0, 0, 0}, ey[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int j = 0; j < 4; j++) {
+    if (j == 3 && np < 4) break;
     if (MIX) {
       ex[j] = g.edx32[j * nc + i];
       ey[j] = g.edy32[j * nc + i];
@@ -145,15 +148,16 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   } else {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      // all four slots exist in the [4][nc] arrays; past np the weight is 0
+      // slot 3 only for quads (past np the weight is 0)
+      wx[j] = 0;
+      wy[j] = 0;
+      if (j == 3 && np < 4) break;
       if (MIX) {
-        const float ax = g.lwx32[j * nc + i], ay = g.lwy32[j * nc + i];
-        wx[j] = j < np ? ax : 0.0f;
-        wy[j] = j < np ? ay : 0.0f;
+        wx[j] = g.lwx32[j * nc + i];
+        wy[j] = g.lwy32[j * nc + i];
       } else {
-        const double ax = g.lwx[j * nc + i], ay = g.lwy[j * nc + i];
-        wx[j] = j < np ? ax : 0.0;
-        wy[j] = j < np ? ay : 0.0;
+        wx[j] = g.lwx[j * nc + i];
+        wy[j] = g.lwy[j * nc + i];
       }
     }
   }
