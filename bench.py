@@ -306,7 +306,8 @@ def batch_mix(n: int, iters: int, slots: int, log, seed: int = 3):
             "precision": "mixed (FP32 gradients/limiter/edge states/convective flux, FP64 update)",
             "seconds": span, "cell_iter_per_s": ci / span, "kinds": {k: {"solves": v[1], "cells": v[0]}
                                                                    for k, v in by.items()},
-            "errors": sum(r["status"] != 0 for r in res)}
+            "diverged": sum(r["status"] == native.CSU_ERR_DIVERGED for r in res),
+            "errors": sum(r["status"] not in (0, native.CSU_ERR_DIVERGED) for r in res)}
 
 
 def cpu_baseline(nx: int, ny: int, iters: int):

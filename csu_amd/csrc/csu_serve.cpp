@@ -10,6 +10,7 @@
 // admission overlaps with the iterations of the solves already running.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -156,7 +157,7 @@ static void retire(csu_server* s, size_t k, int status) {
   j->res.t_done = now(s);
   if (j->dev) {
     j->res.bytes = csu_dev_bytes(j->dev);
-    if (j->r.keep && status == CSU_OK) {
+    if (j->r.keep && (status == CSU_OK || status == CSU_ERR_DIVERGED)) {
       std::vector<double> U((size_t)j->r.prm->nvar * j->r.mesh->ncell);
       if (csu_dev_get_state(j->dev, U.data()) == CSU_OK) s->states[j->r.id] = std::move(U);
     }
@@ -234,9 +235,15 @@ static bool harvest(csu_server* s, size_t k) {
   s->cbuf.resize((size_t)j->chunk_n * 4);
   int rc = csu_dev_async_result(j->dev, s->rbuf.data(), s->cbuf.data());
   if (rc) { retire(s, k, rc); return true; }
+  bool diverged = false;
   for (int it = 0; it < j->chunk_n; it++) {
     const double r0 = s->rbuf[(size_t)it * nv];
     j->res.iters++;
+    if (!std::isfinite(r0)) {  // the state is gone: stop the solve
+      j->res.res_last = r0;
+      diverged = true;
+      break;
+    }
     if (!j->res0_set) { j->res.res0 = r0; j->res0_set = true; }
     j->res.res_last = r0;
     memcpy(j->res.coef_last, &s->cbuf[(size_t)it * 4], sizeof(double) * 4);
@@ -246,6 +253,7 @@ static bool harvest(csu_server* s, size_t k) {
       break;
     }
   }
+  if (diverged) { retire(s, k, CSU_ERR_DIVERGED); return true; }
   if (j->res.converged || j->res.iters >= j->r.max_iter) { retire(s, k, CSU_OK); return true; }
   if (j->r.cfl_growth > 1.0 && j->cfl < j->r.cfl_max) {
     j->cfl = std::min(j->cfl * j->r.cfl_growth, j->r.cfl_max);

@@ -15,7 +15,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "lib", "libcsu.so")
 EXE = os.path.join(PKG, "lib", "executable")
 
-CSU_OK, CSU_ERR_ARG, CSU_ERR_HIP, CSU_ERR_UNSUPPORTED, CSU_ERR_NOMEM, CSU_ERR_IO, CSU_ERR_INPUT = range(7)
+(CSU_OK, CSU_ERR_ARG, CSU_ERR_HIP, CSU_ERR_UNSUPPORTED, CSU_ERR_NOMEM, CSU_ERR_IO, CSU_ERR_INPUT,
+ CSU_ERR_DIVERGED) = range(8)
 
 i32p = C.POINTER(C.c_int32)
 f64p = C.POINTER(C.c_double)

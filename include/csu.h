@@ -34,6 +34,7 @@ extern "C" {
 #define CSU_ERR_NOMEM       4
 #define CSU_ERR_IO          5   /* host file input/output                 */
 #define CSU_ERR_INPUT       6   /* input.dat key missing or invalid       */
+#define CSU_ERR_DIVERGED    7   /* residual no longer finite (server)     */
 
 /* boundary types, reference boundary.c:479-511 */
 enum { CSU_BC_SYMMETRY = 0, CSU_BC_INLET = 1, CSU_BC_OUTLET = 2,

@@ -22,7 +22,9 @@ typedef struct csu_server csu_server;
 
 typedef struct {
     int32_t id;            /* submit order, from 0                                */
-    int32_t status;        /* CSU_OK, or the error code of its create / iterate   */
+    int32_t status;        /* CSU_OK, CSU_ERR_DIVERGED (non-finite residual: the  */
+                           /* solve is stopped; its state is kept like a finished */
+                           /* one's), or the error of create / iterate            */
     int32_t converged;     /* 1: drop reached; 0: iteration cap reached           */
     int32_t iters;         /* iterations run                                       */
     int32_t iter_conv;     /* first iteration (1-based) at the drop, -1 if none    */

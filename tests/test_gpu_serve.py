@@ -51,7 +51,8 @@ def test_mix_equals_alone(tmpcase):
     for i in mix:
         a, b = mix[i], alone[i]
         print(i, a["status"], a["converged"], a["iters"], a["iter_conv"], a["res0"], a["res_last"], flush=True)
-        assert a["status"] == 0 and b["status"] == 0
+        # a solve whose residual stops being finite is stopped, alone and in the mix alike
+        assert a["status"] == b["status"] and a["status"] in (0, native.CSU_ERR_DIVERGED)
         assert (a["iters"], a["iter_conv"], a["converged"]) == (b["iters"], b["iter_conv"], b["converged"])
         assert np.array_equal([a["res_last"]] + a["coef_last"], [b["res_last"]] + b["coef_last"], equal_nan=True)
         assert np.array_equal(smix[i], salone[i], equal_nan=True)
@@ -70,3 +71,18 @@ def test_served_equals_direct(tmpcase):
         assert res[i]["iters"] == 45 and res[i]["converged"] == 0
         assert res[i]["res_last"] == r[-1, 0] and res[i]["coef_last"] == list(f[-1])
         assert np.array_equal(states[i], dev.get_state())
+
+
+def test_diverged_solve_is_stopped(tmpcase):
+    """A solve whose residual stops being finite is retired with
+    CSU_ERR_DIVERGED at that iteration; the others run on unchanged."""
+    d = tmpcase("dv")
+    cases.ramp_case(d, nx=30, ny=15, threads=1, nmax=10, CFL=5, dtLocal=1)   # explicit RK far past stability
+    bad = native.HostCase(d)
+    good = _jobs(tmpcase)[1]
+    ref, _ = _serve([good], 1)
+    res, _ = _serve([bad, good], 2)
+    print(res[0], flush=True)
+    assert res[0]["status"] == native.CSU_ERR_DIVERGED and res[0]["iters"] < 300 and not res[0]["converged"]
+    assert not np.isfinite(res[0]["res_last"])
+    assert res[1]["status"] == 0 and res[1]["iters"] == ref[0]["iters"] and res[1]["res_last"] == ref[0]["res_last"]
