@@ -1206,6 +1206,26 @@ extern "C" int csu_dev_wall_tensor(csu_dev* d, double* out) {
   return CSU_OK;
 }
 
+extern "C" int csu_dev_sst_fields(csu_dev* d, double* out) {
+  if (!d || !out) return set_err(CSU_ERR_ARG, "null argument");
+  if (!d->kp.sst || d->nv != 6) return set_err(CSU_ERR_ARG, "not an SST run");
+  const size_t n = 5 * (size_t)d->nc;
+  double* buf = nullptr;
+  if (hipMalloc(&buf, n * sizeof(double)) != hipSuccess)
+    return set_err(CSU_ERR_NOMEM, "hipMalloc(%zu bytes) failed", n * sizeof(double));
+  hipLaunchKernelGGL(k_sst_fields, dim3(nblk(d->nc)), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->st.P, d->st.dPx,
+                     d->st.dPy, d->geo.wdist, buf);
+  std::vector<double> w(n);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(w.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost, d->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+  hipFree(buf);
+  if (e != hipSuccess) return set_err(CSU_ERR_HIP, "sst fields: %s", hipGetErrorString(e));
+  for (int i = 0; i < d->nc; i++)
+    for (int k = 0; k < 5; k++) out[5 * (size_t)i + k] = w[(size_t)k * d->nc + i];
+  return CSU_OK;
+}
+
 extern "C" int64_t csu_dev_bytes(const csu_dev* d) { return d ? (int64_t)d->bytes : -1; }
 
 // Device bytes csu_dev_create will allocate (plus a history ring of

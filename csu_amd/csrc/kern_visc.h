@@ -323,6 +323,7 @@ struct SstV {
   double dux, duy, dvx, dvy, dkx, dky, dox, doy;
   double r, T, k, om, d, mi_L, mi_t;
   double tkx, tky, tox, toy, Qtk, Qto;
+  double F1, F2;  // blending functions of the last sources evaluation
 };
 constexpr double SST_SK1 = 0.85, SST_SO1 = 0.5, SST_B1 = 0.075, SST_SK2 = 1.0, SST_SO2 = 0.856,
                  SST_B2 = 0.0828, SST_BS = 0.09, SST_A1 = 0.31;
@@ -409,6 +410,8 @@ __device__ __forceinline__ void str_sources(SstV& v) {
   v.mi_t = fmin(alphas * v.r * v.k / v.om, STR_A1 * v.r * v.k / (omega * F2));
   double n_t = v.mi_t / v.r;
   double P = v.mi_t * omega * omega;
+  v.F1 = F1;
+  v.F2 = F2;
   v.Qtk = P - bs * v.r * v.om * v.k;
   v.Qto = alpha * P / n_t - b * v.r * v.om * v.om + 2 * (1 - F1) * v.r * STR_SO2 * (v.dkx * v.dox + v.dky * v.doy) / v.om;
 }
@@ -455,6 +458,8 @@ __device__ __forceinline__ void sst_sources(const KP& kp, SstV& v) {
   double txy = v.mi_t * (v.duy + v.dvx);
   double P = v.dux * txx + v.dvy * tyy + (v.duy + v.dvx) * txy;
   P = fmin(P, 10 * SST_BS * v.r * v.om * v.k);
+  v.F1 = F1;
+  v.F2 = F2;
   double cd = 2 * (1 - F1) * v.r * SST_SO2 * (v.dkx * v.dox + v.dky * v.doy) / v.om;
   v.Qtk = P - SST_BS * v.r * v.om * v.k;
   v.Qto = gm * P / n_t - b * v.r * v.om * v.om + cd;

@@ -817,4 +817,28 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const in
   }
 }
 
+// Per-cell SST fields of the solution2.csv writer, reference sstInterMiT
+// (sst.c:964-1007): sstSources on the primitives and the last residual's
+// gradients of u, v, k, omega, with no omega = 0 guard.  out [5][nc] =
+// eddy viscosity, F1, F2, wall distance, omega.
+__global__ __launch_bounds__(CSU_BLOCK) void k_sst_fields(KP kp, const double* __restrict__ P,
+                                                          const double* __restrict__ gx,
+                                                          const double* __restrict__ gy,
+                                                          const double* __restrict__ wd, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc;
+  if (i >= nc) return;
+  SstV v;
+  v.dux = gx[nc + i]; v.dvx = gx[2 * nc + i]; v.dkx = gx[4 * nc + i]; v.dox = gx[5 * nc + i];
+  v.duy = gy[nc + i]; v.dvy = gy[2 * nc + i]; v.dky = gy[4 * nc + i]; v.doy = gy[5 * nc + i];
+  v.r = P[i]; v.T = P[4 * nc + i]; v.k = P[5 * nc + i]; v.om = P[6 * nc + i]; v.d = wd[i];
+  v.mi_L = sutherland(v.T);
+  sst_sources(kp, v);
+  out[i] = v.mi_t;
+  out[nc + i] = v.F1;
+  out[2 * nc + i] = v.F2;
+  out[3 * nc + i] = v.d;
+  out[4 * nc + i] = v.om;
+}
+
 }  // namespace csu

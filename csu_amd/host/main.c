@@ -134,7 +134,15 @@ write:
     if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }
     printf("main: saving the solution 2.\n");
     snprintf(path, sizeof path, "%ssolution2.csv", wd);
-    if (!c->prm.sst) csu_write_solution2(c, Pc, path);
+    double* X = NULL;
+    if (c->prm.sst) {
+        X = malloc(sizeof(double) * 5 * (size_t)c->mesh.ncell);
+        if (!X) { printf("Error: out of memory\n"); return 1; }
+        rc = csu_dev_sst_fields(d, X);
+        if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }
+    }
+    csu_write_solution2(c, Pc, X, path);
+    free(X);
     free(Pc);
     printf("main: saving the restart file.\n");
     snprintf(path, sizeof path, "%srestart.csv", wd);

@@ -116,22 +116,28 @@ int csu_write_solution(const csu_case* c, const double* U, const char* path)
     return CSU_OK;
 }
 
-/* solverWriteSolution2 (solver.c:2009-2220) for Euler, laminar and SA runs:
-   P [ncell][nvar+1] = r u v p T [nu~] of the clamped state
-   (solverCalcPrimitive), averaged to the nodes with inverse-distance
-   weights; on wall (not wallT) nodes of viscous runs u = v = 0, and for SA
-   nu~ = 0 and T = Twall when Twall >= 0; then Mach, total enthalpy and the
-   entropy relative to the inlet's.  SST runs add eddy viscosity, F1, F2 and
-   two more per-cell fields there: not written (CSU_ERR_UNSUPPORTED). */
-int csu_write_solution2(const csu_case* c, const double* P, const char* path)
+/* solverWriteSolution2 (solver.c:2009-2220): P [ncell][nvar+1] = r u v p T
+   [nu~ | k om] of the clamped state (solverCalcPrimitive), averaged to the
+   nodes with inverse-distance weights; on wall (not wallT) nodes of viscous
+   runs u = v = 0, and for SA nu~ = 0 (SST: k = 0) and T = Twall when
+   Twall >= 0; then Mach, total enthalpy and the entropy relative to the
+   inlet's.  SST runs add the node averages of the per-cell fields sst
+   [ncell][5] (sstInterMiT: eddy viscosity, F1, F2, wall distance, omega;
+   csu_dev_sst_fields); sst is ignored for the other models. */
+int csu_write_solution2(const csu_case* c, const double* P, const double* sst, const char* path)
 {
     const csu_mesh_desc* m = &c->mesh;
     const csu_params* prm = &c->prm;
-    if (prm->sst) return CSU_ERR_UNSUPPORTED;
+    if (prm->sst && !sst) return CSU_ERR_ARG;
     const int nv1 = prm->nvar + 1, np = m->nnode, nc = m->ncell;
-    double* Pn = calloc((size_t)nv1 * (size_t)np, sizeof(double));
+    const int nx = prm->sst ? 5 : 0;   /* SST: eddy viscosity, F1, F2, d, omega */
+    double* Pn = calloc((size_t)(nv1 + nx) * (size_t)np, sizeof(double));
     double* den = calloc((size_t)np, sizeof(double));
     if (!Pn || !den) { free(Pn); free(den); return CSU_ERR_NOMEM; }
+    double* Xn = Pn + (size_t)nv1 * np;
+    /* the SST fields take the same node weights in a pass of their own
+       (solver.c:2133-2172); summed in the same cell order, so one pass
+       gives the same sums */
     for (int i = 0; i < nc; i++) {
         const int32_t* v = m->cell_vert + 4 * i;
         int nvert = v[3] >= 0 ? 4 : 3;
@@ -143,20 +149,21 @@ int csu_write_solution2(const csu_case* c, const double* P, const char* path)
             double xp = m->node_xy[2 * p], yp = m->node_xy[2 * p + 1];
             double L = sqrt((xp - xc) * (xp - xc) + (yp - yc) * (yp - yc));
             for (int k = 0; k < nv1; k++) Pn[(size_t)k * np + p] += P[(size_t)i * nv1 + k] / L;
+            for (int k = 0; k < nx; k++) Xn[(size_t)k * np + p] += sst[(size_t)i * nx + k] / L;
             den[p] += 1 / L;
         }
     }
     for (int p = 0; p < np; p++)
         if (den[p] != 0)
-            for (int k = 0; k < nv1; k++) Pn[(size_t)k * np + p] /= den[p];
-    if (prm->laminar || prm->sa) {
+            for (int k = 0; k < nv1 + nx; k++) Pn[(size_t)k * np + p] /= den[p];
+    if (prm->laminar || prm->sa || prm->sst) {
         for (int g = 0; g < m->nbface; g++) {
             if (m->mark_type[m->bface_mark[g]] != CSU_BC_WALL) continue;
             int q[2] = {m->bface[3 * g + 1], m->bface[3 * g + 2]};
             for (int e = 0; e < 2; e++) {
                 Pn[(size_t)1 * np + q[e]] = 0.0;
                 Pn[(size_t)2 * np + q[e]] = 0.0;
-                if (prm->sa) {
+                if (prm->sa || prm->sst) {   /* nu~ or k; omega keeps its average */
                     Pn[(size_t)5 * np + q[e]] = 0.0;
                     if (prm->twall >= 0) Pn[(size_t)4 * np + q[e]] = prm->twall;
                 }
@@ -172,7 +179,8 @@ int csu_write_solution2(const csu_case* c, const double* P, const char* path)
     double s0 = hgas_Tp2entropy(&gas, sT ? strtod(sT, NULL) : prm->pin[4], sp ? strtod(sp, NULL) : prm->pin[3]);
     FILE* f = fopen(path, "w");
     if (!f) { free(Pn); free(den); return CSU_ERR_IO; }
-    fprintf(f, prm->sa ? "r,u,v,p,T,n,mach,H,s,\n" : "r,u,v,p,T,mach,H,s,\n");
+    fprintf(f, prm->sst ? "r,u,v,p,T,k,o,mach,H,s,miEddy,F1,F2,d,om2,\n"
+               : prm->sa ? "r,u,v,p,T,n,mach,H,s,\n" : "r,u,v,p,T,mach,H,s,\n");
     for (int p = 0; p < np; p++) {
         const double* q = Pn + p;
         double V2 = q[np] * q[np] + q[2 * (size_t)np] * q[2 * (size_t)np];
@@ -181,7 +189,9 @@ int csu_write_solution2(const csu_case* c, const double* P, const char* path)
         double Q1 = hgas_T2e(&gas, T) + pr / q[0] + 0.5 * V2;
         double Q2 = hgas_Tp2entropy(&gas, T, pr) - s0;
         for (int k = 0; k < nv1; k++) fprintf(f, "%.10e, ", q[(size_t)k * np]);
-        fprintf(f, "%.10e, %.10e, %.10e, \n", Q0, Q1, Q2);
+        fprintf(f, "%.10e, %.10e, %.10e, ", Q0, Q1, Q2);
+        for (int k = 0; k < nx; k++) fprintf(f, "%.10e, ", Xn[(size_t)k * np + p]);
+        fputc('\n', f);
     }
     fclose(f);
     free(Pn); free(den);

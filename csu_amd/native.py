@@ -114,11 +114,12 @@ def lib():
     L.csu_write_restart.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_read_restart.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_write_solution.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
-    L.csu_write_solution2.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
+    L.csu_write_solution2.argtypes = [C.POINTER(Case), f64p, f64p, C.c_char_p]
     L.csu_wall_distance.argtypes = [C.POINTER(Case)]
     L.csu_wall_distance_dev.argtypes = [C.POINTER(MeshDesc), f64p]
     L.csu_dev_iterate_async.argtypes = [C.c_void_p, C.c_int32]
     L.csu_dev_wall_tensor.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_sst_fields.argtypes = [C.c_void_p, f64p]
     L.csu_dev_ready.argtypes = [C.c_void_p]
     L.csu_dev_async_result.argtypes = [C.c_void_p, f64p, f64p]
     L.csu_dev_bytes.argtypes = [C.c_void_p]
@@ -234,10 +235,13 @@ class HostCase:
         U = np.ascontiguousarray(U, dtype=np.float64)
         check(lib().csu_write_solution(self._c, _p(U, C.c_double), path.encode()))
 
-    def write_solution2(self, P: np.ndarray, path: str):
-        """solution2.csv from the clamped primitive state P [ncell][nvar+1]."""
+    def write_solution2(self, P: np.ndarray, path: str, sst: np.ndarray | None = None):
+        """solution2.csv from the clamped primitive state P [ncell][nvar+1];
+        SST runs also take the per-cell fields sst [ncell][5]."""
         P = np.ascontiguousarray(P, dtype=np.float64)
-        check(lib().csu_write_solution2(self._c, _p(P, C.c_double), path.encode()))
+        X = None if sst is None else np.ascontiguousarray(sst, dtype=np.float64)
+        check(lib().csu_write_solution2(self._c, _p(P, C.c_double), None if X is None else _p(X, C.c_double),
+                                        path.encode()))
 
     def close(self):
         if self._c:
@@ -321,6 +325,13 @@ class Device:
 
     def set_profiling(self, on: bool):
         check(lib().csu_dev_set_profiling(self._h, 1 if on else 0))
+
+    def sst_fields(self) -> np.ndarray:
+        """SST: [ncell][5] eddy viscosity, F1, F2, wall distance, omega
+        (sstInterMiT) from the primitives and the last residual's gradients."""
+        out = np.empty((self.case.ncell, 5))
+        check(lib().csu_dev_sst_fields(self._h, _p(out, C.c_double)))
+        return out
 
     def wall_tensor(self) -> np.ndarray:
         """[nbface][4] txx txy tyy yp of the last residual (wall rows set)."""

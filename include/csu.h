@@ -172,6 +172,13 @@ int64_t csu_dev_footprint(const csu_mesh_desc* mesh, const csu_params* prm, int3
  * saCalcTensorWall (sa.c:795-852) for SA and SST.  Viscous runs only. */
 int  csu_dev_wall_tensor(csu_dev* dev, double* out);
 
+/* SST runs: the per-cell fields sstInterMiT (sst.c:964-1007) computes for
+ * solverWriteSolution2 (solver.c:2132), out [ncell][5] = eddy viscosity,
+ * F1, F2, wall distance, omega: sstSources on the current primitives (call
+ * csu_dev_primitives first, as the reference does) and the gradients of
+ * the last residual evaluation. */
+int  csu_dev_sst_fields(csu_dev* dev, double* out);
+
 /* Iterations done so far; local time step [ncell] of the last iteration. */
 int  csu_dev_iteration(const csu_dev* dev);
 int  csu_dev_get_dt(csu_dev* dev, double* dtl);

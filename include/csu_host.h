@@ -72,9 +72,9 @@ int  csu_write_restart(const csu_case* c, const double* U, const char* path);
 int  csu_read_restart(const csu_case* c, double* U, const char* path);
 int  csu_write_solution(const csu_case* c, const double* U, const char* path);
 /* solution2.csv (solverWriteSolution2, solver.c:2009-2220) from the clamped
- * primitive state P [ncell][nvar+1] (csu_dev_primitives); Euler, laminar
- * and SA runs (SST: CSU_ERR_UNSUPPORTED). */
-int  csu_write_solution2(const csu_case* c, const double* P, const char* path);
+ * primitive state P [ncell][nvar+1] (csu_dev_primitives); SST runs also
+ * take sst [ncell][5] (csu_dev_sst_fields), NULL for the other models. */
+int  csu_write_solution2(const csu_case* c, const double* P, const double* sst, const char* path);
 
 /* Greedy colouring for the colour/RCM renumbering (csu_amd/renumber.py):
  * cells visited in order[0..n) each take the smallest colour no coloured

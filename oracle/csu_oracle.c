@@ -2110,6 +2110,39 @@ int oracle_residual(csu_oracle* o, double* R, double* dPx, double* dPy, double* 
     return CSU_OK;
 }
 
+/* solverCalcPrimitive (solver.c:1153-1220) on its own, as solverWriteSolution2
+   calls it (solver.c:2035): the clamps write back into U, the gradients of
+   the last residual stay. */
+int oracle_primitives(csu_oracle* o, double* P)
+{
+    calc_primitive(o, o->U);
+    if (P) memcpy(P, o->P, sizeof(double) * (size_t)(o->nv + 1) * o->nc);
+    return CSU_OK;
+}
+
+/* sstInterMiT (sst.c:964-1007): sstSources per cell on the primitives and
+   the last residual's gradients of u, v, k, omega, without the omega = 0
+   guard of the residual's source term. */
+int oracle_sst_fields(const csu_oracle* o, double* out)
+{
+    if (!o->prm.sst) return CSU_ERR_UNSUPPORTED;
+    const size_t nc = o->nc;
+    const double *gx = o->dPx, *gy = o->dPy;
+    for (size_t i = 0; i < nc; i++) {
+        const double* P = o->P + i * (o->nv + 1);
+        sstv v;
+        memset(&v, 0, sizeof v);
+        v.dux = gx[nc + i]; v.dvx = gx[2 * nc + i]; v.dkx = gx[4 * nc + i]; v.dox = gx[5 * nc + i];
+        v.duy = gy[nc + i]; v.dvy = gy[2 * nc + i]; v.dky = gy[4 * nc + i]; v.doy = gy[5 * nc + i];
+        v.r = P[0]; v.T = P[4]; v.k = P[5]; v.om = P[6]; v.d = o->wd[i];
+        v.mi_L = sutherland(v.T);
+        sst_sources(o, &v);
+        double* q = out + 5 * i;
+        q[0] = v.mi_t; q[1] = v.F1; q[2] = v.F2; q[3] = v.d; q[4] = v.om;
+    }
+    return CSU_OK;
+}
+
 int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
 {
     for (int it = 0; it < niter; it++) {

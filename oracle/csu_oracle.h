@@ -25,6 +25,10 @@ int  oracle_residual(csu_oracle* o, double* R, double* dPx, double* dPy, double*
 int  oracle_iterate(csu_oracle* o, int niter, double* res, double* coef);
 int  oracle_set_cfl(csu_oracle* o, double cfl);
 int  oracle_advance_time(csu_oracle* o, double tmax, int max_iter, double* t, int* niter, int* finished);
+/* P [ncell][nvar+1]: the primitive pass alone (clamps U, keeps the gradients) */
+int  oracle_primitives(csu_oracle* o, double* P);
+/* SST: [ncell][5] eddy viscosity, F1, F2, wall distance, omega */
+int  oracle_sst_fields(const csu_oracle* o, double* out);
 int  oracle_wall_tensor(const csu_oracle* o, double* out);   /* [nbface][4] txx txy tyy yp */
 int  oracle_set_threads(int n);
 

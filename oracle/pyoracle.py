@@ -45,6 +45,8 @@ def lib():
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_wall_tensor.argtypes = [vp, f64]
         L.oracle_set_cfl.argtypes = [vp, C.c_double]
+        L.oracle_primitives.argtypes = [vp, f64]
+        L.oracle_sst_fields.argtypes = [vp, f64]
         L.oracle_advance_time.argtypes = [vp, C.c_double, C.c_int, f64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -118,6 +120,20 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle_wall_tensor failed ({rc})")
         return out[:self.nbface]
+
+    def primitives(self) -> np.ndarray:
+        """The primitive pass alone: clamps the state, keeps the gradients."""
+        P = np.empty((self.ncell, self.nvar + 1))
+        lib().oracle_primitives(self._h, _p(P))
+        return P
+
+    def sst_fields(self) -> np.ndarray:
+        """SST: [ncell][5] eddy viscosity, F1, F2, wall distance, omega."""
+        out = np.empty((self.ncell, 5))
+        rc = lib().oracle_sst_fields(self._h, _p(out))
+        if rc != 0:
+            raise RuntimeError(f"oracle_sst_fields failed ({rc})")
+        return out
 
     def close(self):
         if self._h:

@@ -200,6 +200,11 @@ SOLUTION2 = [
     ("solution2_plate_lam", "plate", dict(_PL, jitter=0.1), {}, 60),
     ("solution2_plate_sa_tp", "plate", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "TP": "1"}, 60),
     ("solution2_tube", "tube", dict(nx=100, ny=8), {}, 0),
+    # SST adds eddy viscosity, F1, F2, d, omega (sstInterMiT); Twall 320 on
+    # an adiabatic wall exercises the wall-node temperature override
+    ("solution2_plate_sst", "plate", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "Twall": "320"}, 40),
+    ("solution2_plate_sst_trans_lusgs", "plate", dict(_PL, jitter=0.1),
+     {"laminar": "0", "sst": "1", "sstTrans": "1", "timeScheme": "LUSGS", "CFL": "10"}, 40),
 ]
 
 
@@ -281,9 +286,7 @@ def make_friction():
 if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref is not built (needs the reference sources): make -C oracle ref")
-    make_small()
-    make_config0()
-    make_friction()
-    make_tube()
-    make_solution2()
-    make_cfl_chain()
+    steps = {"small": make_small, "config0": make_config0, "friction": make_friction, "tube": make_tube,
+             "solution2": make_solution2, "cfl_chain": make_cfl_chain}
+    for key in sys.argv[1:] or steps:   # default: every fixture set
+        steps[key]()

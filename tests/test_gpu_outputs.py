@@ -1,11 +1,13 @@
 """The drop-in executable's solution2.csv against the reference
-executable's on the same case (GPU; both print %.10e)."""
+executable's on the same case (GPU; both print %.10e), and the device's
+SST fields (sstInterMiT) against the oracle's."""
 import subprocess
 
 import numpy as np
 import pytest
 
 from csu_amd import native
+from oracle import pyoracle as po
 
 import goldcases
 from test_outputs import NAMES, read_solution2, write_case
@@ -27,3 +29,27 @@ def test_executable_solution2(tmpcase, name):
     e = float(np.max(np.abs(S - ref) / scale))
     print(name, "max rel", e, flush=True)
     assert e < 1e-9
+
+
+@pytest.mark.parametrize("name", ["solution2_plate_sst", "solution2_plate_sst_trans_lusgs"])
+def test_sst_fields_vs_oracle(tmpcase, name):
+    """Eddy viscosity, F1, F2, wall distance and omega per cell after the
+    fixture's iterations: 1e-8 relative per column (the converged-field bar)."""
+    g = goldcases.load(name)
+    d = write_case(g, tmpcase("sf"))
+    c = native.HostCase(d)
+    n = int(g["niter"])
+    o = po.Oracle(c)
+    o.set_state(c.initial_state())
+    o.iterate(n)
+    Po, Xo = o.primitives(), o.sst_fields()
+    dev = native.Device(c)
+    dev.set_state(c.initial_state())
+    dev.iterate(n, want=False)
+    Pd = dev.primitives()
+    Xd = dev.sst_fields()
+    for A, B, what in ((Pd, Po, "P"), (Xd, Xo, "fields")):
+        scale = np.maximum(np.abs(B).max(axis=0), 1e-300)
+        e = float(np.max(np.abs(A - B) / scale))
+        print(name, what, "max rel", e, flush=True)
+        assert e < 1e-8

@@ -4,7 +4,8 @@ Fixtures tests/golden/solution2_*.npz hold the reference executable's
 solution2.csv (solverWriteSolution2, solver.c:2009-2220) after n iterations
 (or, for the tube case, at the end of its time-accurate run).  The oracle
 reaches the same state bit for bit, so the host writer applied to the
-oracle's clamped primitive state must print the same numbers."""
+oracle's clamped primitive state (and, for SST, its sstInterMiT fields from
+the last residual's gradients) must print the same numbers."""
 import ast
 
 import numpy as np
@@ -15,7 +16,8 @@ from oracle import pyoracle as po
 
 import goldcases
 
-NAMES = ["solution2_ramp", "solution2_plate_lam", "solution2_plate_sa_tp", "solution2_tube"]
+NAMES = ["solution2_ramp", "solution2_plate_lam", "solution2_plate_sa_tp", "solution2_tube", "solution2_plate_sst",
+         "solution2_plate_sst_trans_lusgs"]
 
 
 def write_case(g, d):
@@ -48,8 +50,9 @@ def test_solution2_vs_reference(tmpcase, name):
         o.advance_time(c.tmax)
     else:
         o.iterate(int(g["niter"]))
-    P = o.residual()["P"]          # solverCalcPrimitive of the final state
-    c.write_solution2(P, d + "solution2.csv")
+    P = o.primitives()             # solverCalcPrimitive of the final state
+    X = o.sst_fields() if c.nvar == 6 else None
+    c.write_solution2(P, d + "solution2.csv", X)
     head, S = read_solution2(d + "solution2.csv")
     assert head == str(g["header"])
     ref = g["solution2"]
@@ -58,9 +61,43 @@ def test_solution2_vs_reference(tmpcase, name):
     assert np.max(np.abs(S - ref) / scale) < 1e-12
 
 
-def test_solution2_sst_not_written(tmpcase):
+def test_solution2_sst_needs_fields(tmpcase):
     d = tmpcase("s2s")
     cases.write_case(d, cases.plate_mesh(nx=8, ny=4, tri_from=2), cases.plate_input(threads=1, laminar="0", sst="1"))
     c = native.HostCase(d)
     with pytest.raises(native.CsuError):
         c.write_solution2(np.zeros((c.ncell, 7)), d + "solution2.csv")
+
+
+def test_sst_fields_pins(tmpcase):
+    """The sstInterMiT fields against what fixes them: the wall distance and
+    omega columns are the mesh's and the state's own; F1 and F2 lie in
+    [0, 1] and tend to 1 at the wall (tanh of a large argument); F2 and the
+    eddy viscosity equal Menter's (2003) closed forms
+    F2 = tanh(max(2 sqrt(k) / (0.09 om d), 500 nu / (d^2 om))^2) and
+    r a1 k / max(a1 om, |du/dy - dv/dx| F2) with the gradients of the last
+    residual."""
+    d = tmpcase("sf")
+    cases.write_case(d, cases.plate_mesh(nx=10, ny=6, tri_from=2, jitter=0.1),
+                     cases.plate_input(threads=1, nmax=20, laminar="0", sst="1"))
+    c = native.HostCase(d)
+    o = po.Oracle(c)
+    o.set_state(c.initial_state())
+    o.iterate(20)
+    r = o.residual()
+    P = o.primitives()
+    X = o.sst_fields()
+    assert np.array_equal(X[:, 3], c.arrays()["wall_dist"])
+    assert np.array_equal(X[:, 4], P[:, 6])
+    assert np.all((X[:, 1:3] >= 0) & (X[:, 1:3] <= 1))
+    near = X[:, 3] < np.sort(X[:, 3])[10]
+    assert np.median(X[near, 1]) > 0.99 and np.all(X[near, 2] > 0.99)
+    T = P[:, 4]
+    nu = 1.458e-6 * T ** 1.5 / (T + 110.4) / P[:, 0]
+    d, om, k = X[:, 3], P[:, 6], P[:, 5]
+    arg2 = np.maximum(2 * np.sqrt(k) / (0.09 * om * d), 500 * nu / (d * d * om))
+    assert np.allclose(X[:, 2], np.tanh(arg2 * arg2), rtol=1e-13, atol=0)
+    omega = np.abs(r["dPy"][1] - r["dPx"][2])
+    a1 = 0.31
+    mt = P[:, 0] * a1 * P[:, 5] / np.maximum(a1 * P[:, 6], omega * X[:, 2])
+    assert np.allclose(X[:, 0], mt, rtol=1e-14, atol=0)
