@@ -850,9 +850,22 @@ extern "C" int csu_dev_residual(csu_dev* d, double* R, double* dPx, double* dPy,
 
 // solverCalcPrimitive(solver, U) (solver.c:1153-1213) on the device state:
 // U clamped in place, P [nc][nv+1] to the host (NULL allowed).
+// solverCalcPrimitive: the cells, then the ghost states of the boundary
+// faces (its boundaryCalcPrimitive calls, solver.c:1215-1218); k_bcflux's
+// boundary fluxes are recomputed by the next stage before they are read.
+template <int NV>
+static int prim_ghost_pass(csu_dev* d) {
+  int rc = prim_pass<NV>(d);
+  if (rc || d->nb == 0) return rc;
+  Prof p(d, "k_bcflux");
+  hipLaunchKernelGGL(k_bcflux<NV>, dim3(nblk(d->nb)), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->geo, d->st);
+  HIPCHK(hipGetLastError());
+  return CSU_OK;
+}
+
 extern "C" int csu_dev_primitives(csu_dev* d, double* P) {
   if (!d) return set_err(CSU_ERR_ARG, "null argument");
-  int rc = d->nv == 6 ? prim_pass<6>(d) : d->nv == 5 ? prim_pass<5>(d) : prim_pass<4>(d);
+  int rc = d->nv == 6 ? prim_ghost_pass<6>(d) : d->nv == 5 ? prim_ghost_pass<5>(d) : prim_ghost_pass<4>(d);
   if (rc) return rc;
   d->fresh = 1;
   if (P) {
@@ -1223,6 +1236,32 @@ extern "C" int csu_dev_sst_fields(csu_dev* d, double* out) {
   if (e != hipSuccess) return set_err(CSU_ERR_HIP, "sst fields: %s", hipGetErrorString(e));
   for (int i = 0; i < d->nc; i++)
     for (int k = 0; k < 5; k++) out[5 * (size_t)i + k] = w[(size_t)k * d->nc + i];
+  return CSU_OK;
+}
+
+extern "C" int csu_dev_surf_faces(csu_dev* d, double* out) {
+  if (!d || !out) return set_err(CSU_ERR_ARG, "null argument");
+  if (d->nv != 5 && d->nv != 6) return set_err(CSU_ERR_ARG, "not an SA or SST run");
+  const int nb = d->nb;
+  if (nb == 0) return CSU_OK;
+  const size_t n = 11 * (size_t)nb;
+  double* buf = nullptr;
+  if (hipMalloc(&buf, n * sizeof(double)) != hipSuccess)
+    return set_err(CSU_ERR_NOMEM, "hipMalloc(%zu bytes) failed", n * sizeof(double));
+  if (d->nv == 6)
+    hipLaunchKernelGGL(k_surf<6>, dim3((nb + CSU_BLOCK - 1) / CSU_BLOCK), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->geo,
+                       d->st, buf);
+  else
+    hipLaunchKernelGGL(k_surf<5>, dim3((nb + CSU_BLOCK - 1) / CSU_BLOCK), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->geo,
+                       d->st, buf);
+  std::vector<double> w(n);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(w.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost, d->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+  hipFree(buf);
+  if (e != hipSuccess) return set_err(CSU_ERR_HIP, "surface faces: %s", hipGetErrorString(e));
+  for (int q = 0; q < nb; q++)
+    for (int k = 0; k < 11; k++) out[11 * (size_t)q + k] = w[(size_t)k * nb + q];
   return CSU_OK;
 }
 

@@ -240,7 +240,8 @@ __device__ __forceinline__ void sa_face(const KP& kp, double dSx, double dSy, do
 // gry: the cell's density gradient.
 __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, double dSy, double nx, double ny,
                                            double dS2, double dx, double dy, const double* Pi, const double* gx,
-                                           const double* gy, double grx, double gry, double* f) {
+                                           const double* gy, double grx, double gry, double* f,
+                                           double* mie = nullptr) {
   double fv1, tx, ty;
   if (type == CSU_BC_SYMMETRY) {
     double dux = gx[0] - (gx[0] * nx + gy[0] * ny) * nx, duy = gy[0] - (gx[0] * nx + gy[0] * ny) * ny;
@@ -250,6 +251,7 @@ __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, d
     double mi_L = sutherland(T), n_L = mi_L / rho;
     sa_calc_face(n, n_L, rho, 0.0, 0.0, fv1, tx, ty);  // tx, ty unused: f[3] = 0
     double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+    if (mie) *mie = mi_t;
     double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
     Tau t = tau_of(mi, dux, duy, dvx, dvy);
     f[0] = (t.xx * nx + t.xy * ny) * dS2;
@@ -264,6 +266,7 @@ __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, d
     if (kp.sa_cc) sacc_calc_face(n, n_L, rho, gx[3], gy[3], grx, gry, fv1, tx, ty);
     else sa_calc_face(n, n_L, rho, gx[3], gy[3], fv1, tx, ty);
     double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+    if (mie) *mie = mi_t;
     double k = gas_T2Cp<double>(kp.gas, T) * (mi_L / kp.pr + mi_t / kp.pr_t);
     Tau t = tau_of(mi, gx[0], gy[0], gx[1], gy[1]);
     f[0] = t.xx * dSx + t.xy * dSy;
@@ -301,6 +304,7 @@ __device__ __forceinline__ void sa_bc_face(const KP& kp, int type, double dSx, d
   }
   double mi_t = fv1 * rho * n;
   double mi = type == CSU_BC_WALL ? mi_L : mi_L + mi_t;
+  if (mie) *mie = type == CSU_BC_WALL ? 0.0 : mi_t;  // the wall reports 0 (sa.c:570)
   Tau t = tau_of(mi, dux, duy, dvx, dvy);
   f[0] = t.xx * dSx + t.xy * dSy;
   f[1] = t.xy * dSx + t.yy * dSy;
@@ -564,7 +568,8 @@ __device__ __forceinline__ void sst_face(const KP& kp, double dSx, double dSy, d
 // sstBoundaryFaceViscFlux (sst.c:202-547).  Gradients: u v T k om.
 __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, double dSy, double nx, double ny,
                                             double dS2, double dx, double dy, const double* Pi, double d,
-                                            const double* gx, const double* gy, double* f) {
+                                            const double* gx, const double* gy, double* f,
+                                            double* mie = nullptr) {
   SstV v;
   v.d = d;
   if (type == CSU_BC_SYMMETRY) {
@@ -575,6 +580,7 @@ __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, 
     v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6];
     v.mi_L = sutherland(v.T);
     sst_flux(kp, v);
+    if (mie) *mie = v.mi_t;
     Tau t = sst_tau(v.mi_L + v.mi_t, v);
     const double u = Pi[1], w = Pi[2];
     f[0] = (t.xx * nx + t.xy * ny) * dS2;
@@ -606,6 +612,7 @@ __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, 
     edge_correct(gx[3], gy[3], (kb - Pi[5]) / L, dx, dy, L, v.dkx, v.dky);
     edge_correct(gx[4], gy[4], (ob - Pi[6]) / L, dx, dy, L, v.dox, v.doy);
     sst_flux(kp, v);
+    if (mie) *mie = v.mi_t;
     Tau t = sst_tau(v.mi_L + v.mi_t, v);
     f[0] = t.xx * dSx + t.xy * dSy;
     f[1] = t.xy * dSx + t.yy * dSy;
@@ -623,6 +630,7 @@ __device__ __forceinline__ void sst_bc_face(const KP& kp, int type, double dSx, 
   v.r = Pi[0]; v.T = Pi[4]; v.k = Pi[5]; v.om = Pi[6];
   v.mi_L = sutherland(v.T);
   sst_flux(kp, v);
+  if (mie) *mie = v.mi_t;
   const double kk = gas_T2Cp<double>(kp.gas, v.T) * (v.mi_L / kp.pr + v.mi_t / kp.pr_t);
   Tau t = sst_tau(v.mi_L + v.mi_t, v);
   const double u = Pi[1], w = Pi[2];

@@ -841,4 +841,38 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_sst_fields(KP kp, const double* _
   out[4 * nc + i] = v.om;
 }
 
+// Per boundary face, what the surfData.csv writers take (saSolverWriteSurf
+// sa.c:854-960, saCC_SolverWriteSurf saCC.c:599-705, sstSolverWriteSurf
+// sst.c:820-962): the boundary viscous flux and eddy viscosity recomputed
+// from the current primitives and the last residual's gradients, the ghost
+// state of that residual and the wall distance of the face's cell.
+// out [11][nb] = f_x f_y f_e miEddy, ghost r u v p T (nu~ | k), d.
+template <int NV>
+__global__ __launch_bounds__(CSU_BLOCK) void k_surf(KP kp, Geo g, St s, double* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = kp.nc, nb = kp.nb;
+  if (q >= nb) return;
+  const int i = g.be0[q];
+  double Pi[NV + 1];
+#pragma unroll
+  for (int m = 0; m <= NV; m++) Pi[m] = s.P[m * nc + i];
+  const double gx[5] = {s.dPx[nc + i], s.dPx[2 * nc + i], s.dTx[i], s.dPx[4 * nc + i], NV == 6 ? s.dPx[5 * nc + i] : 0.0};
+  const double gy[5] = {s.dPy[nc + i], s.dPy[2 * nc + i], s.dTy[i], s.dPy[4 * nc + i], NV == 6 ? s.dPy[5 * nc + i] : 0.0};
+  const double dx = g.bxm[q] - g.xc[i], dy = g.bym[q] - g.yc[i];
+  const int t = g.btype[q];
+  double f[5], mie = 0.0;
+  if (NV == 6)
+    sst_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx, dy, Pi, g.wdist[i], gx, gy, f, &mie);
+  else
+    sa_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx, dy, Pi, gx, gy, s.dPx[i], s.dPy[i], f,
+               &mie);
+  out[q] = f[0];
+  out[nb + q] = f[1];
+  out[2 * nb + q] = f[2];
+  out[3 * nb + q] = mie;
+#pragma unroll
+  for (int m = 0; m < 6; m++) out[(4 + m) * nb + q] = s.Pb[m * nb + q];
+  out[10 * nb + q] = g.wdist[i];
+}
+
 }  // namespace csu

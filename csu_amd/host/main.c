@@ -4,9 +4,11 @@
  * Same process contract as the reference main.c:18-45 + solverSolve
  * (solver.c:1836-1911): reads <case>/input.dat, mesh.su2 (and restart.csv
  * when restart=1), runs Nmax iterations, prints one line per iteration and
- * appends it to convergence.csv, then writes solution.csv, restart.csv and
- * friction.csv.  With tube=1 it runs the time-accurate branch to tmax
- * instead (solver.c:1912-1941) and writes solution.csv and restart.csv.  The iterations run on the GPU through csu.h; the host only
+ * appends it to convergence.csv, then writes solution.csv, solution2.csv,
+ * restart.csv, friction.csv and, for SA and SST runs, surfData.csv.  With
+ * tube=1 it runs the time-accurate branch to tmax instead
+ * (solver.c:1912-1941) and writes solution.csv, solution2.csv and
+ * restart.csv.  The iterations run on the GPU through csu.h; the host only
  * formats the per-iteration records, which come back in blocks.
  *
  * As the reference, input errors print a message and exit with status 0
@@ -147,6 +149,15 @@ write:
     printf("main: saving the restart file.\n");
     snprintf(path, sizeof path, "%srestart.csv", wd);
     csu_write_restart(c, c->U, path);
+    if (c->prm.sa || c->prm.sst) {   /* solverWriteSurf (solver.c:1984-2007) */
+        double* F = malloc(sizeof(double) * 11 * (size_t)(c->mesh.nbface ? c->mesh.nbface : 1));
+        if (!F) { printf("Error: out of memory\n"); return 1; }
+        rc = csu_dev_surf_faces(d, F);
+        if (rc) { printf("Error: %s\n", csu_last_error()); return 1; }
+        snprintf(path, sizeof path, "%ssurfData.csv", wd);
+        csu_write_surf(c, F, path);
+        free(F);
+    }
 
     csu_dev_destroy(d);
     csu_case_free(c);

@@ -197,3 +197,104 @@ int csu_write_solution2(const csu_case* c, const double* P, const double* sst, c
     free(Pn); free(den);
     return CSU_OK;
 }
+
+/* surfData.csv for SA and SST runs: saSolverWriteSurf (sa.c:854-960),
+   saCC_SolverWriteSurf (saCC.c:599-705), sstSolverWriteSurf (sst.c:820-962).
+   For each marker named by the writeSurf key (default "wall"): per face,
+   from faces [nbface][11] (csu_dev_surf_faces): the ghost r u v p T and
+   nu~ (SST: k), Cp, Cfx, Cfy, the wall heat flux, y+, Mach and the eddy
+   viscosity; then averaged to the marker's nodes.  The nodes are numbered
+   as the reference numbers them, p0 of each face in file order and p1 of
+   the last face, so a closed marker's first node is left without faces
+   (0/0, as in the reference).  A face node outside that list is skipped:
+   the reference reads an unset index there. */
+int csu_write_surf(const csu_case* c, const double* faces, const char* path)
+{
+    const csu_mesh_desc* m = &c->mesh;
+    const csu_params* prm = &c->prm;
+    if (!(prm->sa || prm->sst)) return CSU_ERR_UNSUPPORTED;
+    if (!faces && m->nbface) return CSU_ERR_ARG;
+    const char* name = csu_input_get(c->input, "writeSurf");
+    if (!name) name = "wall";
+    hgas gas;
+    hgas_init(&gas, prm->gamma, prm->gas_r, prm->tp);
+    const double rin = prm->pin[0], uin = prm->pin[1], vin = prm->pin[2], pin = prm->pin[3];
+    const double qdin = 0.5 * rin * (uin * uin + vin * vin);
+    const int nb = m->nbface, np = m->nnode, NV = 13;
+    int* list = malloc(sizeof(int) * (size_t)(nb + 1));
+    int* pn = malloc(sizeof(int) * (size_t)(nb + 1));
+    int* cnt = malloc(sizeof(int) * (size_t)(nb + 1));
+    int* pn2 = malloc(sizeof(int) * (size_t)(np ? np : 1));
+    double* D = malloc(sizeof(double) * (size_t)NV * (nb ? nb : 1));
+    double* Dp = malloc(sizeof(double) * (size_t)NV * (nb + 1));
+    FILE* f = fopen(path, "w");
+    if (!list || !pn || !cnt || !pn2 || !D || !Dp || !f) {
+        free(list); free(pn); free(cnt); free(pn2); free(D); free(Dp);
+        if (f) fclose(f);
+        return f ? CSU_ERR_NOMEM : CSU_ERR_IO;
+    }
+    for (int mk = 0; mk < m->nmark; mk++) {
+        if (strcmp(c->mark_name[mk], name) != 0) continue;
+        fprintf(f, "x,y,rho,u,v,p,T,n,Cp,Cfx,Cfy,q,yplus,mach,miEddy,\n");
+        int ne = 0;
+        for (int g = 0; g < nb; g++)
+            if (m->bface_mark[g] == mk) list[ne++] = g;
+        if (ne == 0) continue;
+        for (int p = 0; p < np; p++) pn2[p] = -1;
+        for (int ii = 0; ii < ne; ii++) pn[ii] = m->bface[3 * list[ii] + 1];
+        pn[ne] = m->bface[3 * list[ne - 1] + 2];
+        for (int ii = 0; ii <= ne; ii++) pn2[pn[ii]] = ii;
+        for (int ii = 0; ii <= ne; ii++) {
+            cnt[ii] = 0;
+            for (int k = 0; k < NV; k++) Dp[(size_t)k * (ne + 1) + ii] = 0.0;
+        }
+        for (int ii = 0; ii < ne; ii++) {
+            const int g = list[ii];
+            const double* q = faces + 11 * (size_t)g;
+            const int p0 = m->bface[3 * g + 1], p1 = m->bface[3 * g + 2];
+            const double x0 = m->node_xy[2 * p0], y0 = m->node_xy[2 * p0 + 1];
+            const double x1 = m->node_xy[2 * p1], y1 = m->node_xy[2 * p1 + 1];
+            double dSx = y1 - y0, dSy = -(x1 - x0);
+            if (m->axi == 1) { double y = (y0 + y1) / 2; dSx = dSx * y; dSy = dSy * y; }   /* meshCalcDS */
+            const double dS = sqrt(dSx * dSx + dSy * dSy);
+            const double fx = q[0], fy = q[1], fe = q[2];
+            const double r = q[4], u = q[5], v = q[6], p = q[7], T = q[8];
+            const double tau = sqrt(fx * fx + fy * fy) / dS;
+            const double uplus = sqrt(tau / r);
+            const double mi_L = 1.458e-6 * T * sqrt(T) / (T + 110.4);   /* sutherland, solver.c:1222 */
+            double* Dc = D + ii;
+            Dc[0 * (size_t)ne] = r;
+            Dc[1 * (size_t)ne] = u;
+            Dc[2 * (size_t)ne] = v;
+            Dc[3 * (size_t)ne] = p;
+            Dc[4 * (size_t)ne] = T;
+            Dc[5 * (size_t)ne] = q[9];
+            Dc[6 * (size_t)ne] = (p - pin) / qdin;
+            Dc[7 * (size_t)ne] = fx / (dS * qdin);
+            Dc[8 * (size_t)ne] = fy / (dS * qdin);
+            Dc[9 * (size_t)ne] = (fe - u * fx - v * fy) / dS;
+            Dc[10 * (size_t)ne] = fabs(r * uplus * q[10]) / mi_L;
+            Dc[11 * (size_t)ne] = sqrt(u * u + v * v) / hgas_T2c(&gas, T);
+            Dc[12 * (size_t)ne] = q[3];
+        }
+        for (int ii = 0; ii < ne; ii++) {
+            const int g = list[ii];
+            const int a = pn2[m->bface[3 * g + 1]], b = pn2[m->bface[3 * g + 2]];
+            for (int k = 0; k < NV; k++) {
+                if (a >= 0) Dp[(size_t)k * (ne + 1) + a] += D[(size_t)k * ne + ii];
+                if (b >= 0) Dp[(size_t)k * (ne + 1) + b] += D[(size_t)k * ne + ii];
+            }
+            if (a >= 0) cnt[a] += 1;
+            if (b >= 0) cnt[b] += 1;
+        }
+        for (int ii = 0; ii <= ne; ii++) {
+            fprintf(f, "% .10e,", m->node_xy[2 * pn[ii]]);
+            fprintf(f, "% .10e,", m->node_xy[2 * pn[ii] + 1]);
+            for (int k = 0; k < NV; k++) fprintf(f, "% .10e,", Dp[(size_t)k * (ne + 1) + ii] / cnt[ii]);
+            fprintf(f, "\n");
+        }
+    }
+    fclose(f);
+    free(list); free(pn); free(cnt); free(pn2); free(D); free(Dp);
+    return CSU_OK;
+}

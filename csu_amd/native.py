@@ -120,6 +120,8 @@ def lib():
     L.csu_dev_iterate_async.argtypes = [C.c_void_p, C.c_int32]
     L.csu_dev_wall_tensor.argtypes = [C.c_void_p, f64p]
     L.csu_dev_sst_fields.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_surf_faces.argtypes = [C.c_void_p, f64p]
+    L.csu_write_surf.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_dev_ready.argtypes = [C.c_void_p]
     L.csu_dev_async_result.argtypes = [C.c_void_p, f64p, f64p]
     L.csu_dev_bytes.argtypes = [C.c_void_p]
@@ -243,6 +245,13 @@ class HostCase:
         check(lib().csu_write_solution2(self._c, _p(P, C.c_double), None if X is None else _p(X, C.c_double),
                                         path.encode()))
 
+    def write_surf(self, faces: np.ndarray, path: str):
+        """surfData.csv (SA/SST) from the per-face data [nbface][11]."""
+        F = np.ascontiguousarray(faces, dtype=np.float64).reshape(-1, 11)
+        if F.shape[0] == 0:
+            F = np.zeros((1, 11))
+        check(lib().csu_write_surf(self._c, _p(F, C.c_double), path.encode()))
+
     def close(self):
         if self._c:
             lib().csu_case_free(self._c)
@@ -332,6 +341,13 @@ class Device:
         out = np.empty((self.case.ncell, 5))
         check(lib().csu_dev_sst_fields(self._h, _p(out, C.c_double)))
         return out
+
+    def surf_faces(self) -> np.ndarray:
+        """SA/SST: [nbface][11] wall viscous flux x y e, eddy viscosity,
+        ghost r u v p T (nu~ | k), wall distance of the face's cell."""
+        out = np.empty((max(self.case.nbface, 1), 11))
+        check(lib().csu_dev_surf_faces(self._h, _p(out, C.c_double)))
+        return out[:self.case.nbface]
 
     def wall_tensor(self) -> np.ndarray:
         """[nbface][4] txx txy tyy yp of the last residual (wall rows set)."""

@@ -132,11 +132,11 @@ int  csu_dev_iterate(csu_dev* dev, int32_t niter, double* res, double* coef);
  * with a new CFL in input.dat (its way of ramping the CFL). */
 int  csu_dev_set_cfl(csu_dev* dev, double cfl);
 
-/* solverCalcPrimitive(solver, solver->U) (solver.c:1153-1213): the
- * positivity clamps applied to the device state in place, as the
- * reference's solverWriteSolution2 does before solverWriteReestart writes
- * restart.csv (main.c:29-31); P: [ncell][nvar+1] = r u v p T [nu~ | k om],
- * or NULL. */
+/* solverCalcPrimitive(solver, solver->U) (solver.c:1153-1218): the
+ * positivity clamps applied to the device state in place, and the boundary
+ * ghost states recomputed from it, as the reference's solverWriteSolution2
+ * does before solverWriteReestart writes restart.csv (main.c:29-31);
+ * P: [ncell][nvar+1] = r u v p T [nu~ | k om], or NULL. */
 int  csu_dev_primitives(csu_dev* dev, double* P);
 
 /* Time-accurate run: the tube branch of solverSolve (solver.c:1912-1941).
@@ -178,6 +178,15 @@ int  csu_dev_wall_tensor(csu_dev* dev, double* out);
  * csu_dev_primitives first, as the reference does) and the gradients of
  * the last residual evaluation. */
 int  csu_dev_sst_fields(csu_dev* dev, double* out);
+
+/* SA and SST runs: per boundary face, what the surfData.csv writers take
+ * (saSolverWriteSurf sa.c:854, saCC_SolverWriteSurf saCC.c:599,
+ * sstSolverWriteSurf sst.c:820), out [nbface][11] = the boundary viscous
+ * flux x, y, energy and the eddy viscosity of the boundary viscous-flux
+ * routine on the current primitives and the last residual's gradients,
+ * the ghost state of the last residual (r u v p T, nu~ or k) and the wall
+ * distance of the face's cell. */
+int  csu_dev_surf_faces(csu_dev* dev, double* out);
 
 /* Iterations done so far; local time step [ncell] of the last iteration. */
 int  csu_dev_iteration(const csu_dev* dev);

@@ -75,6 +75,11 @@ int  csu_write_solution(const csu_case* c, const double* U, const char* path);
  * primitive state P [ncell][nvar+1] (csu_dev_primitives); SST runs also
  * take sst [ncell][5] (csu_dev_sst_fields), NULL for the other models. */
 int  csu_write_solution2(const csu_case* c, const double* P, const double* sst, const char* path);
+/* surfData.csv (saSolverWriteSurf sa.c:854, saCC_SolverWriteSurf
+ * saCC.c:599, sstSolverWriteSurf sst.c:820) for SA and SST runs, from the
+ * per-face data of csu_dev_surf_faces ([nbface][11]); the marker is the
+ * writeSurf key's (default "wall").  Laminar runs: CSU_ERR_UNSUPPORTED. */
+int  csu_write_surf(const csu_case* c, const double* faces, const char* path);
 
 /* Greedy colouring for the colour/RCM renumbering (csu_amd/renumber.py):
  * cells visited in order[0..n) each take the smallest colour no coloured

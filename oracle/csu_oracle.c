@@ -426,7 +426,9 @@ static void bc_wall(const csu_oracle* o, const double* Pd, double* Pb, double nx
     Pb[3] = pb;
 }
 
-/* ---------------- primitive recovery: solver.c:1153-1220 ---------------- */
+/* ---------------- primitive recovery: solver.c:1153-1220 ----------------
+   the cells, then the boundary ghost states (boundaryCalcPrimitive, called
+   at the end of solverCalcPrimitive) */
 static void calc_primitive(csu_oracle* o, double* U)
 {
     const int nc = o->nc, nv = o->nv, ns = nv + 1;
@@ -1023,7 +1025,7 @@ static void sa_inter(csu_oracle* o)
 
 /* saBoundaryFaceViscFlux, sa.c:370-692; with saCC
    saCC_BoundaryFaceViscFlux, saCC.c:247-589 */
-static void sa_bc_flux(const csu_oracle* o, int g, double* f)
+static void sa_bc_flux(const csu_oracle* o, int g, double* f, double* mie)
 {
     const int nc = o->nc, ns = o->nv + 1, e0 = o->bface[3 * g], t = o->btype[g];
     const int p0 = o->bface[3 * g + 1], p1 = o->bface[3 * g + 2];
@@ -1043,6 +1045,7 @@ static void sa_bc_flux(const csu_oracle* o, int g, double* f)
         double rho = P[0], u = P[1], v = P[2], T = P[4], n = P[5], mi_L = sutherland(T), n_L = mi_L / rho;
         sa_calc_face(n, n_L, rho, 0.0, 0.0, &fv1, &tx, &ty);   /* tx, ty unused (f[4] = 0) */
         double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+        if (mie) *mie = mi_t;
         double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
         double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
         f[1] = (txx * nx + txy * ny) * dS;
@@ -1059,6 +1062,7 @@ static void sa_bc_flux(const csu_oracle* o, int g, double* f)
         if (o->prm.sa_cc) sacc_calc_face(n, n_L, rho, dnx, dny, gx[e0], gy[e0], &fv1, &tx, &ty);
         else sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
         double mi_t = fv1 * rho * n, mi = mi_L + mi_t;
+        if (mie) *mie = mi_t;
         double k = gas_T2Cp(&o->gas, T) * (mi_L / o->prm.pr + mi_t / o->prm.pr_t);
         double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
         f[1] = txx * dSx + txy * dSy;
@@ -1106,6 +1110,7 @@ static void sa_bc_flux(const csu_oracle* o, int g, double* f)
         sa_calc_face(n, n_L, rho, dnx, dny, &fv1, &tx, &ty);
     }
     double mi_t = fv1 * rho * n, mi = t == CSU_BC_WALL ? mi_L : mi_L + mi_t;
+    if (mie) *mie = t == CSU_BC_WALL ? 0.0 : mi_t;   /* sa.c:570 */
     double txx = 2 * mi * (dux - (dux + dvy) / 3), tyy = 2 * mi * (dvy - (dux + dvy) / 3), txy = mi * (duy + dvx);
     f[1] = txx * dSx + txy * dSy;
     f[2] = txy * dSx + tyy * dSy;
@@ -1122,7 +1127,7 @@ static void sa_boundary(csu_oracle* o)       /* saBoundary / saBoundaryFace, sa.
     const int nc = o->nc;
     for (int g = 0; g < o->nb; g++) {
         double f[5];
-        sa_bc_flux(o, g, f);
+        sa_bc_flux(o, g, f, NULL);
         int e0 = o->bface[3 * g];
         for (int k = 1; k < 5; k++) o->R[(size_t)k * nc + e0] -= f[k];
     }
@@ -1397,7 +1402,7 @@ static void sst_inter(csu_oracle* o)
 }
 
 /* sstBoundaryFaceViscFlux (sst.c:202-547): f[1..5] */
-static void sst_bc_flux(const csu_oracle* o, int g, double* f)
+static void sst_bc_flux(const csu_oracle* o, int g, double* f, double* mie)
 {
     const int nc = o->nc, ns = o->nv + 1, e0 = o->bface[3 * g], t = o->btype[g];
     const int p0 = o->bface[3 * g + 1], p1 = o->bface[3 * g + 2];
@@ -1421,6 +1426,7 @@ static void sst_bc_flux(const csu_oracle* o, int g, double* f)
         double u = P[1], w = P[2];
         v.mi_L = sutherland(v.T);
         sst_flux(o, &v);
+        if (mie) *mie = v.mi_t;
         sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
         f[1] = (txx * nx + txy * ny) * dS;
         f[2] = (txy * nx + tyy * ny) * dS;
@@ -1467,6 +1473,7 @@ static void sst_bc_flux(const csu_oracle* o, int g, double* f)
         v.dox = doxm + (dol - (doxm * dx + doym * dy) / L) * dx / L;
         v.doy = doym + (dol - (doxm * dx + doym * dy) / L) * dy / L;
         sst_flux(o, &v);
+        if (mie) *mie = v.mi_t;
         sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
         f[1] = txx * dSx + txy * dSy;
         f[2] = txy * dSx + tyy * dSy;
@@ -1485,6 +1492,7 @@ static void sst_bc_flux(const csu_oracle* o, int g, double* f)
     double u = P[1], w = P[2];
     v.mi_L = sutherland(v.T);
     sst_flux(o, &v);
+    if (mie) *mie = v.mi_t;
     double kk = gas_T2Cp(&o->gas, v.T) * (v.mi_L / o->prm.pr + v.mi_t / o->prm.pr_t);
     sst_tau(v.mi_L + v.mi_t, &v, &txx, &tyy, &txy);
     f[1] = txx * dSx + txy * dSy;
@@ -1499,7 +1507,7 @@ static void sst_boundary(csu_oracle* o)      /* sstBoundary / sstBoundaryFace, s
     const int nc = o->nc;
     for (int g = 0; g < o->nb; g++) {
         double f[6];
-        sst_bc_flux(o, g, f);
+        sst_bc_flux(o, g, f, NULL);
         int e0 = o->bface[3 * g];
         for (int k = 1; k < 6; k++) o->R[(size_t)k * nc + e0] -= f[k];
     }
@@ -2111,8 +2119,8 @@ int oracle_residual(csu_oracle* o, double* R, double* dPx, double* dPy, double* 
 }
 
 /* solverCalcPrimitive (solver.c:1153-1220) on its own, as solverWriteSolution2
-   calls it (solver.c:2035): the clamps write back into U, the gradients of
-   the last residual stay. */
+   calls it (solver.c:2035): the clamps write back into U and the ghost
+   states follow the clamped cells; the gradients of the last residual stay. */
 int oracle_primitives(csu_oracle* o, double* P)
 {
     calc_primitive(o, o->U);
@@ -2139,6 +2147,29 @@ int oracle_sst_fields(const csu_oracle* o, double* out)
         sst_sources(o, &v);
         double* q = out + 5 * i;
         q[0] = v.mi_t; q[1] = v.F1; q[2] = v.F2; q[3] = v.d; q[4] = v.om;
+    }
+    return CSU_OK;
+}
+
+/* Per boundary face, what the surfData.csv writers take (saSolverWriteSurf
+   sa.c:854-960, saCC_SolverWriteSurf saCC.c:599-705, sstSolverWriteSurf
+   sst.c:820-962): the boundary viscous flux (reference f[1..3]) and eddy
+   viscosity of saBoundaryFaceViscFlux / saCC_BoundaryFaceViscFlux /
+   sstBoundaryFaceViscFlux on the current primitives and the last residual's
+   gradients, the ghost state of the last residual (r u v p T and nu~ or
+   k) and the wall distance of the face's cell.  out [nbface][11]. */
+int oracle_surf_faces(const csu_oracle* o, double* out)
+{
+    if (!(o->prm.sa || o->prm.sst)) return CSU_ERR_UNSUPPORTED;
+    const int ns = o->nv + 1;
+    for (int g = 0; g < o->nb; g++) {
+        double f[6] = {0, 0, 0, 0, 0, 0}, mie = 0.0;
+        if (o->prm.sst) sst_bc_flux(o, g, f, &mie);
+        else sa_bc_flux(o, g, f, &mie);
+        double* q = out + 11 * (size_t)g;
+        q[0] = f[1]; q[1] = f[2]; q[2] = f[3]; q[3] = mie;
+        for (int m = 0; m < 6; m++) q[4 + m] = o->Pb[(size_t)g * ns + m];
+        q[10] = o->wd[o->bface[3 * g]];
     }
     return CSU_OK;
 }

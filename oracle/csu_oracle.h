@@ -29,6 +29,9 @@ int  oracle_advance_time(csu_oracle* o, double tmax, int max_iter, double* t, in
 int  oracle_primitives(csu_oracle* o, double* P);
 /* SST: [ncell][5] eddy viscosity, F1, F2, wall distance, omega */
 int  oracle_sst_fields(const csu_oracle* o, double* out);
+/* SA/SST: [nbface][11] wall viscous flux x y e, eddy viscosity, ghost
+   r u v p T (nu~ | k), wall distance of the face's cell */
+int  oracle_surf_faces(const csu_oracle* o, double* out);
 int  oracle_wall_tensor(const csu_oracle* o, double* out);   /* [nbface][4] txx txy tyy yp */
 int  oracle_set_threads(int n);
 

@@ -47,6 +47,7 @@ def lib():
         L.oracle_set_cfl.argtypes = [vp, C.c_double]
         L.oracle_primitives.argtypes = [vp, f64]
         L.oracle_sst_fields.argtypes = [vp, f64]
+        L.oracle_surf_faces.argtypes = [vp, f64]
         L.oracle_advance_time.argtypes = [vp, C.c_double, C.c_int, f64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -134,6 +135,15 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle_sst_fields failed ({rc})")
         return out
+
+    def surf_faces(self) -> np.ndarray:
+        """SA/SST: [nbface][11] wall viscous flux x y e, eddy viscosity,
+        ghost r u v p T (nu~ | k), wall distance of the face's cell."""
+        out = np.empty((max(self.nbface, 1), 11))
+        rc = lib().oracle_surf_faces(self._h, _p(out))
+        if rc != 0:
+            raise RuntimeError(f"oracle_surf_faces failed ({rc})")
+        return out[:self.nbface]
 
     def close(self):
         if self._h:

@@ -230,6 +230,46 @@ def make_solution2():
         print("wrote", name, S.shape)
 
 
+# surfData.csv of SA and SST runs (the reference executable's; writeSurf
+# names the marker).  The airfoil marker is closed, so its first node has
+# no faces in the reference's node numbering (its row is NaN).  A marker
+# whose faces do not run head to tail in file order (the airfoil case's
+# far-field outlet) makes the reference read unset node indices: it ends
+# with a segmentation fault, so no fixture of that kind exists.
+# (name, kind, mesh kwargs, input overrides, iterations)
+SURF = [
+    ("surf_plate_sa", "plate", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "writeSurf": "plate"}, 40),
+    ("surf_plate_sacc_wallT", "plate", _PL, {"laminar": "0", "sa": "1", "saCC": "1", "flux": "AUSMDV",
+                                             "BC:plate": "wallT", "Twall": "320", "writeSurf": "plate"}, 40),
+    ("surf_plate_sst", "plate", dict(_PL, jitter=0.1), {"laminar": "0", "sst": "1", "writeSurf": "plate"}, 40),
+    ("surf_plate_sst_trans_lusgs", "plate", dict(_PL, jitter=0.1),
+     {"laminar": "0", "sst": "1", "sstTrans": "1", "timeScheme": "LUSGS", "CFL": "10", "writeSurf": "plate"}, 40),
+    ("surf_naca_sa", "naca", dict(ni=48, nwall=8, nfar=8), {"aoa": 4.0, "CFL": "5", "writeSurf": "airfoil"}, 30),
+    ("surf_naca_sst", "naca", dict(ni=48, nwall=8, nfar=8),
+     {"aoa": 4.0, "sa": "0", "sst": "1", "CFL": "5", "writeSurf": "airfoil"}, 30),
+]
+
+
+def read_surf(path):
+    lines = [ln for ln in open(path).read().splitlines() if ln.strip()]
+    return lines[0], np.array([[float(v) for v in ln.strip().rstrip(",").split(",")] for ln in lines[1:]])
+
+
+def make_surf():
+    import subprocess
+    for name, kind, mk, inp, n in SURF:
+        d = case_dir(name)
+        if kind == "plate":
+            cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=n, **inp))
+        else:
+            cases.write_case(d, cases.naca_mesh(**mk), cases.naca_input(threads=1, nmax=n, **inp))
+        subprocess.run([po.REF_EXE, d], capture_output=True, check=False, timeout=600)
+        head, S = read_surf(d + "surfData.csv")
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kind=np.array(kind), mesh_kwargs=np.array(repr(mk)),
+                            inputs=np.array(repr(inp)), niter=n, header=np.array(head), surf=S)
+        print("wrote", name, S.shape, "NaN rows", int(np.isnan(S).any(axis=1).sum()))
+
+
 # CFL ramps as the reference does them: a run at one CFL, then a run
 # restarted from its final state with the next CFL.  The trace driver passes
 # the state in full precision, so the chain is exact.
@@ -287,6 +327,6 @@ if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref is not built (needs the reference sources): make -C oracle ref")
     steps = {"small": make_small, "config0": make_config0, "friction": make_friction, "tube": make_tube,
-             "solution2": make_solution2, "cfl_chain": make_cfl_chain}
+             "solution2": make_solution2, "cfl_chain": make_cfl_chain, "surf": make_surf}
     for key in sys.argv[1:] or steps:   # default: every fixture set
         steps[key]()

@@ -10,7 +10,7 @@ from csu_amd import native
 from oracle import pyoracle as po
 
 import goldcases
-from test_outputs import NAMES, read_solution2, write_case
+from test_outputs import NAMES, SURF, read_solution2, surf_close, write_case
 
 pytestmark = pytest.mark.gpu
 
@@ -53,3 +53,42 @@ def test_sst_fields_vs_oracle(tmpcase, name):
         e = float(np.max(np.abs(A - B) / scale))
         print(name, what, "max rel", e, flush=True)
         assert e < 1e-8
+
+
+@pytest.mark.parametrize("name", SURF)
+def test_executable_surf(tmpcase, name):
+    """surfData.csv of the drop-in executable against the reference
+    executable's (both %.10e; NaN where the reference's node numbering
+    leaves a node without faces)."""
+    g = goldcases.load(name)
+    d = write_case(g, tmpcase("xs"))
+    r = subprocess.run([native.EXE, d], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], flush=True)
+    assert r.returncode == 0
+    head, S = read_solution2(d + "surfData.csv")
+    assert head == str(g["header"])
+    e = surf_close(S, g["surf"], 1)
+    print(name, "max rel", e, flush=True)
+    assert e < 1e-9
+
+
+@pytest.mark.parametrize("name", ["surf_plate_sacc_wallT", "surf_naca_sst"])
+def test_surf_faces_vs_oracle(tmpcase, name):
+    g = goldcases.load(name)
+    d = write_case(g, tmpcase("sf"))
+    c = native.HostCase(d)
+    n = int(g["niter"])
+    o = po.Oracle(c)
+    o.set_state(c.initial_state())
+    o.iterate(n)
+    o.primitives()
+    Fo = o.surf_faces()
+    dev = native.Device(c)
+    dev.set_state(c.initial_state())
+    dev.iterate(n, want=False)
+    dev.primitives()
+    Fd = dev.surf_faces()
+    scale = np.maximum(np.abs(Fo).max(axis=0), 1e-300)
+    e = float(np.max(np.abs(Fd - Fo) / scale))
+    print(name, "max rel", e, flush=True)
+    assert e < 1e-8

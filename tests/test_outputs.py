@@ -1,11 +1,15 @@
-"""Output files in the reference's formats (CPU): solution2.csv.
+"""Output files in the reference's formats (CPU): solution2.csv and
+surfData.csv.
 
 Fixtures tests/golden/solution2_*.npz hold the reference executable's
 solution2.csv (solverWriteSolution2, solver.c:2009-2220) after n iterations
 (or, for the tube case, at the end of its time-accurate run).  The oracle
 reaches the same state bit for bit, so the host writer applied to the
 oracle's clamped primitive state (and, for SST, its sstInterMiT fields from
-the last residual's gradients) must print the same numbers."""
+the last residual's gradients) must print the same numbers.  Fixtures tests/golden/surf_*.npz hold the
+reference executable's surfData.csv of SA and SST runs (saSolverWriteSurf
+sa.c:854, saCC_SolverWriteSurf saCC.c:599, sstSolverWriteSurf sst.c:820);
+the host writer applied to the oracle's boundary-face data prints them."""
 import ast
 
 import numpy as np
@@ -18,6 +22,8 @@ import goldcases
 
 NAMES = ["solution2_ramp", "solution2_plate_lam", "solution2_plate_sa_tp", "solution2_tube", "solution2_plate_sst",
          "solution2_plate_sst_trans_lusgs"]
+SURF = ["surf_plate_sa", "surf_plate_sacc_wallT", "surf_plate_sst", "surf_plate_sst_trans_lusgs", "surf_naca_sa",
+        "surf_naca_sst"]
 
 
 def write_case(g, d):
@@ -29,6 +35,8 @@ def write_case(g, d):
         cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=max(n, 1), **inp))
     elif kind == "plate":
         cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=max(n, 1), **inp))
+    elif kind == "naca":
+        cases.write_case(d, cases.naca_mesh(**mk), cases.naca_input(threads=1, nmax=max(n, 1), **inp))
     else:
         cases.write_case(d, cases.tube_mesh(**mk), cases.tube_input(threads=1, **inp))
     return d
@@ -101,3 +109,55 @@ def test_sst_fields_pins(tmpcase):
     a1 = 0.31
     mt = P[:, 0] * a1 * P[:, 5] / np.maximum(a1 * P[:, 6], omega * X[:, 2])
     assert np.allclose(X[:, 0], mt, rtol=1e-14, atol=0)
+
+
+def surf_close(S, ref, tol):
+    """Same NaN cells, and the rest within tol of each column's scale."""
+    assert S.shape == ref.shape
+    assert np.array_equal(np.isnan(S), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    scale = np.maximum(np.nanmax(np.abs(ref), axis=0), 1e-300)
+    e = np.where(ok, np.abs(np.where(ok, S, 0) - np.where(ok, ref, 0)) / scale, 0)
+    return float(e.max())
+
+
+@pytest.mark.parametrize("name", SURF)
+def test_surf_vs_reference(tmpcase, name):
+    g = goldcases.load(name)
+    d = write_case(g, tmpcase("sd"))
+    c = native.HostCase(d)
+    o = po.Oracle(c)
+    o.set_state(c.initial_state())
+    o.iterate(int(g["niter"]))
+    o.primitives()                 # solverWriteSolution2 runs first
+    c.write_surf(o.surf_faces(), d + "surfData.csv")
+    head, S = read_solution2(d + "surfData.csv")
+    assert head == str(g["header"])
+    assert surf_close(S, g["surf"], 1) < 1e-12
+
+
+def test_surf_wall_rows(tmpcase):
+    """On a no-slip wall the ghost velocity is zero, so are u, v and Mach
+    at every node; y+ = rho u_tau d / mu with u_tau from the wall shear."""
+    g = goldcases.load("surf_plate_sa")
+    d = write_case(g, tmpcase("sw"))
+    c = native.HostCase(d)
+    o = po.Oracle(c)
+    o.set_state(c.initial_state())
+    o.iterate(5)
+    o.primitives()
+    F = o.surf_faces()
+    a = c.arrays()
+    wall = a["mark_type"][a["bface_mark"]] == 3
+    assert wall.any() and np.all(F[wall, 5:7] == 0)
+    c.write_surf(F, d + "surfData.csv")
+    _, S = read_solution2(d + "surfData.csv")
+    assert np.all(S[:, [3, 4, 13]] == 0) and np.all(S[:, 14] == 0)   # u v mach, SA wall miEddy
+
+
+def test_surf_unsupported(tmpcase):
+    d = tmpcase("su")
+    cases.write_case(d, cases.plate_mesh(nx=8, ny=4, tri_from=2), cases.plate_input(threads=1))
+    c = native.HostCase(d)
+    with pytest.raises(native.CsuError):
+        c.write_surf(np.zeros((c.nbface, 11)), d + "surfData.csv")
