@@ -173,8 +173,6 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
   if (p->nvar != (p->sa ? 5 : p->sst ? 6 : 4)) return set_err(CSU_ERR_ARG, "nvar %d does not match the model", p->nvar);
   if ((p->sa || p->sst) && p->flux == CSU_FLUX_ROE)
     return set_err(CSU_ERR_UNSUPPORTED, "Roe has no turbulence-scalar form in the reference (flux.c:1325-1366)");
-  if (p->sst && p->time_scheme == CSU_TIME_DPLUR)
-    return set_err(CSU_ERR_UNSUPPORTED, "DPLUR with SST: the reference has 4x4 and 5x5 blocks only (implicit.c:993-1000)");
   if (p->sst && p->order == 2 && p->limiter == 1)
     return set_err(CSU_ERR_UNSUPPORTED, "limiter 1 with SST: the reference's range arrays hold 5 variables (limiter.c:37-38)");
   if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);
@@ -1035,7 +1033,7 @@ static int implicit_iteration(csu_dev* d) {
     hipLaunchKernelGGL(k_jac_reset, dim3(1), dim3(1), 0, d->stream, d->jstate);
     for (int mm = 0; mm < d->prm.nlinear; mm++) {
       hipLaunchKernelGGL(k_jacobi<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->A, d->Dp, d->dtg,
-                         d->R, d->dW0, d->dW1, d->jstate, red_min);
+                         d->R, d->dW0, d->dW1, d->jstate, red_min, d->st.dq);
       hipLaunchKernelGGL(k_jac_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, red_min, d->jstate);
     }
     hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW0, d->dW1,

@@ -348,7 +348,12 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs_uinit(KP kp, Geo g, const d
 }
 
 // DPLUR finite-difference blocks of every interior neighbour
-// (implicitUpdateA, implicit.c:689-813), h = 1e-6.  Layout [4 slots][NV*NV][nc].
+// (implicitUpdateA, implicit.c:689-813; implicitUpdateA_sa :815-936),
+// h = 1e-6.  Layout [4 slots][NV*NV][nc].  SST (NV 6) has no DPLUR in the
+// reference (implicitCalcDPLUR builds 4x4 blocks for it); its 6x6 blocks
+// here take implicitUpdateA_sa's form with two convected scalars (the
+// fluxes of implicitAuxCalcFlux_sst2, implicit.c:1036-1046) and k taken out
+// of the internal energy, as implicitCalcDeltaFlux_sst does (:1048-1117).
 template <int NV, bool VISC>
 __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const double* __restrict__ U,
                                                       double* __restrict__ A) {
@@ -371,8 +376,9 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
     double V = (nx * Uj[1] + ny * Uj[2]) / Uj[0];
     double F0[NV];
     F0[0] = Uj[0] * V; F0[1] = Uj[1] * V + nx * P3; F0[2] = Uj[2] * V + ny * P3; F0[3] = (Uj[3] + P3) * V;
-    if (NV == 5) F0[4] = Uj[4] * V;
-    const double ee0 = E - 0.5 * (u * u + v * v);
+    if (NV >= 5) F0[4] = Uj[4] * V;
+    if (NV == 6) F0[5] = Uj[5] * V;
+    const double ee0 = NV == 6 ? E - 0.5 * (u * u + v * v) - Uj[4] / rho : E - 0.5 * (u * u + v * v);
     const double cv = gas_T2Cv<double>(kp.gas, P4);
     double* Aj = A + (size_t)j * NV * NV * nc;
 #pragma unroll
@@ -381,20 +387,22 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
 #pragma unroll
       for (int n = 0; n < NV; n++) Up[n] = mm == n ? Uj[n] + h : Uj[n];
       rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
-      double ee1 = E - 0.5 * (u * u + v * v);
+      double ee1 = NV == 6 ? E - 0.5 * (u * u + v * v) - Up[4] / rho : E - 0.5 * (u * u + v * v);
       double T = P4 + (ee1 - ee0) / cv;
       double p = kp.gas.R * rho * T;
       double Vp = (nx * Up[1] + ny * Up[2]) / Up[0];
       double F[NV];
       F[0] = Up[0] * Vp; F[1] = Up[1] * Vp + nx * p; F[2] = Up[2] * Vp + ny * p; F[3] = (Up[3] + p) * Vp;
-      if (NV == 5) F[4] = Up[4] * Vp;
+      if (NV >= 5) F[4] = Up[4] * Vp;
+      if (NV == 6) F[5] = Up[5] * Vp;
 #pragma unroll
       for (int n = 0; n < NV; n++) Aj[(size_t)(n * NV + mm) * nc + i] = 0.5 * (F[n] - F0[n]) * dS / h;
     }
     double c = gas_T2c<double>(kp.gas, P4);
     double ra = kp.wimp * (fabs(nx * s.P[nc + nb] + ny * s.P[2 * nc + nb]) + c) * dS;
-    if (NV == 5) {
-      // implicitUpdateA_sa (implicit.c:815-936)
+    if (NV >= 5) {
+      // implicitUpdateA_sa (implicit.c:815-936); SST the same with its
+      // face eddy viscosity
       double r = (s.P[i] + s.P[nb]) * 0.5;
       double T = (s.P[4 * nc + i] + P4) * 0.5;
       double mi = sutherland(T);
@@ -418,11 +426,15 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
 // y = D x + sum_j A_j x_j;  x_new = w (-R - y)/D + x.  Buffers buf[cur] ->
 // buf[1-cur]; state[0] = converged flag, state[1] = cur.  A sweep after
 // convergence in the same implicit step does nothing (the reference breaks).
+// SST: the diagonal is LU-SGS's block (implicit.c:326-351), D with the
+// k-omega source Jacobians dq; y takes its product and x_new solves with it
+// by Cramer's rule (D alone where its determinant is below 1e-14).
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double* __restrict__ A,
                                                       const double* __restrict__ Dp, const double* __restrict__ dtg,
                                                       const double* __restrict__ R, double* buf0, double* buf1,
-                                                      const int* __restrict__ state, double* __restrict__ red_max) {
+                                                      const int* __restrict__ state, double* __restrict__ red_max,
+                                                      const double* __restrict__ dq) {
   __shared__ double smax[CSU_BLOCK];
   if (state[0]) return;
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
@@ -435,6 +447,18 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     double xi[NV], acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) { xi[k] = x[k * nc + i]; acc[k] = D * xi[k]; }
+    double qa = D, qb = 0.0, qc = 0.0, qd = D, q0 = 0.0, q3 = 0.0, det = 0.0;
+    bool blk = false;
+    if (NV == 6) {
+      q0 = dq[i]; q3 = dq[3 * nc + i];
+      qa = D - dq[nc + i]; qb = -dq[2 * nc + i]; qc = -dq[4 * nc + i]; qd = D - dq[5 * nc + i];
+      det = qa * qd - qb * qc;
+      blk = !(fabs(det) < 1e-14);
+      if (blk) {
+        acc[4] = qa * xi[4] + qb * xi[5] - q0 * xi[0];
+        acc[5] = qc * xi[4] + qd * xi[5] - q3 * xi[0];
+      }
+    }
     // neighbours of all slots first, then their x, then the products in
     // slot order (slots without a face read the cell itself, skipped)
     int cfj[4], nbj[4];
@@ -461,11 +485,27 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
         for (int n = 0; n < NV; n++) acc[k] += Aj[(size_t)(k * NV + n) * nc + i] * xj[j][n];
     }
     const double w = 2. / 3.;
+    if (NV == 6 && blk) {
+      double z[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double yk = w * (-R[k * nc + i] - acc[k]) / D + xi[k];
-      y[k * nc + i] = yk;
-      if (k == 0) diff = fabs(yk - xi[0]);
+      for (int k = 0; k < 4; k++) z[k] = (-R[k * nc + i] - acc[k]) / D;
+      const double Ak = -R[4 * nc + i] - acc[4] + q0 * z[0];
+      const double Bo = -R[5 * nc + i] - acc[5] + q3 * z[0];
+      z[4] = (Ak * qd - Bo * qb) / det;
+      z[5] = (-Ak * qc + Bo * qa) / det;
+#pragma unroll
+      for (int k = 0; k < NV; k++) {
+        const double yk = w * z[k] + xi[k];
+        y[k * nc + i] = yk;
+        if (k == 0) diff = fabs(yk - xi[0]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NV; k++) {
+        double yk = w * (-R[k * nc + i] - acc[k]) / D + xi[k];
+        y[k * nc + i] = yk;
+        if (k == 0) diff = fabs(yk - xi[0]);
+      }
     }
   }
   smax[t] = diff;

@@ -1884,6 +1884,13 @@ static void lusgs(csu_oracle* o)
 
 /* implicitUpdateA, implicit.c:689-813: finite-difference flux Jacobian
    blocks of every interior neighbour, h = 1e-6 */
+/* SST (nv 6) has no DPLUR in the reference: implicitCalcDPLUR builds 4x4
+   blocks for it (implicit.c:993-1000) and implicitMultA then reads past
+   them.  This build's SST form, restated here: the blocks of
+   implicitUpdateA_sa with two convected scalars (implicitAuxCalcFlux_sst2,
+   implicit.c:1036-1046) and k taken out of the internal energy as
+   implicitCalcDeltaFlux_sst does (:1048-1117); the diagonal is LU-SGS's
+   k-omega block with the source Jacobians (implicit.c:326-351). */
 static void dplur_blocks(csu_oracle* o)
 {
     const int nc = o->nc, nv = o->nv, ns = nv + 1;
@@ -1897,23 +1904,29 @@ static void dplur_blocks(csu_oracle* o)
             double* A = o->A + ((size_t)i * 4 + s) * nv * nv;
             double dS = sqrt(dSx * dSx + dSy * dSy), nx = dSx / dS, ny = dSy / dS;
             const double* Pj = o->P + (size_t)j * ns;
-            double Uj[5], F0[5], F[5], Up[5];
+            double Uj[6] = {0}, F0[6], F[6], Up[6];
             for (int k = 0; k < nv; k++) Uj[k] = o->U[(size_t)k * nc + j];
             double rho = Uj[0], u = Uj[1] / rho, v = Uj[2] / rho, E = Uj[3] / rho;
-            if (nv == 5) aux_flux_sa(Uj, Pj[3], nx, ny, F0); else aux_flux(Uj, Pj[3], nx, ny, F0);
+            if (nv == 6) aux_flux_sst(Uj, Pj[3], nx, ny, F0);
+            else if (nv == 5) aux_flux_sa(Uj, Pj[3], nx, ny, F0);
+            else aux_flux(Uj, Pj[3], nx, ny, F0);
             double ee0 = E - 0.5 * (u * u + v * v);
+            if (nv == 6) ee0 = ee0 - Uj[4] / rho;
             for (int m = 0; m < nv; m++) {
                 for (int n = 0; n < nv; n++) Up[n] = m == n ? Uj[n] + h : Uj[n];
                 rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
                 double ee1 = E - 0.5 * (u * u + v * v);
+                if (nv == 6) ee1 = ee1 - Up[4] / rho;
                 double T = Pj[4] + (ee1 - ee0) / gas_T2Cv(&o->gas, Pj[4]);
                 double p = o->gas.R * rho * T;
-                if (nv == 5) aux_flux_sa(Up, p, nx, ny, F); else aux_flux(Up, p, nx, ny, F);
+                if (nv == 6) aux_flux_sst(Up, p, nx, ny, F);
+                else if (nv == 5) aux_flux_sa(Up, p, nx, ny, F);
+                else aux_flux(Up, p, nx, ny, F);
                 for (int n = 0; n < nv; n++) A[n * nv + m] = 0.5 * (F[n] - F0[n]) * dS / h;
             }
             double c = gas_T2c(&o->gas, Pj[4]);
             double ra = o->prm.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
-            if (o->prm.sa) {     /* implicitUpdateA_sa, implicit.c:815-936 */
+            if (o->prm.sa || o->prm.sst) {     /* implicitUpdateA_sa, implicit.c:815-936 */
                 const double* Pi = o->P + (size_t)i * ns;
                 double r = (Pi[0] + Pj[0]) * 0.5, T = (Pi[4] + Pj[4]) * 0.5, mi = sutherland(T), x0, y0, x1, y1;
                 cell_center(o, i, &x0, &y0);
@@ -1933,6 +1946,18 @@ static void dplur_blocks(csu_oracle* o)
     }
 }
 
+/* SST: the k-omega 2x2 of the DPLUR block diagonal; 0 where its
+   determinant is below 1e-14 (D alone there, as lusgs_diag) */
+static int sst_kw_block(const csu_oracle* o, int i, double* a, double* b, double* c, double* d, double* det)
+{
+    *a = o->D[i] - o->dQkdrk[i];
+    *b = -o->dQkdro[i];
+    *c = -o->dQodrk[i];
+    *d = o->D[i] - o->dQodro[i];
+    *det = *a * *d - *b * *c;
+    return !(fabs(*det) < 1e-14);
+}
+
 /* implicitCalcDPLUR, implicit.c:990-1034: damped Jacobi, w = 2/3 */
 static void dplur(csu_oracle* o)
 {
@@ -1947,6 +1972,11 @@ static void dplur(csu_oracle* o)
 #pragma omp parallel for
         for (int i = 0; i < nc; i++) {
             for (int k = 0; k < nv; k++) y[(size_t)k * nc + i] = o->D[i] * x[(size_t)k * nc + i];
+            double a, b, c, d, det;
+            if (nv == 6 && sst_kw_block(o, i, &a, &b, &c, &d, &det)) {   /* k-omega rows of the block diagonal */
+                y[4 * (size_t)nc + i] = a * x[4 * (size_t)nc + i] + b * x[5 * (size_t)nc + i] - o->dQkdr[i] * x[i];
+                y[5 * (size_t)nc + i] = c * x[4 * (size_t)nc + i] + d * x[5 * (size_t)nc + i] - o->dQodr[i] * x[i];
+            }
             for (int s = 0; s < 4; s++) {
                 double dSx, dSy;
                 int j = face_from(o, i, s, &dSx, &dSy);
@@ -1957,9 +1987,21 @@ static void dplur(csu_oracle* o)
             }
         }
 #pragma omp parallel for
-        for (int i = 0; i < nc; i++)
+        for (int i = 0; i < nc; i++) {
+            double a, b, c, d, det;
+            if (nv == 6 && sst_kw_block(o, i, &a, &b, &c, &d, &det)) {
+                double z[6];
+                for (int k = 0; k < 4; k++) z[k] = (-o->R[(size_t)k * nc + i] - y[(size_t)k * nc + i]) / o->D[i];
+                double A = -o->R[4 * (size_t)nc + i] - y[4 * (size_t)nc + i] + o->dQkdr[i] * z[0];
+                double B = -o->R[5 * (size_t)nc + i] - y[5 * (size_t)nc + i] + o->dQodr[i] * z[0];
+                z[4] = (A * d - B * b) / det;
+                z[5] = (-A * c + B * a) / det;
+                for (int k = 0; k < nv; k++) y[(size_t)k * nc + i] = w * z[k] + x[(size_t)k * nc + i];
+                continue;
+            }
             for (int k = 0; k < nv; k++)
                 y[(size_t)k * nc + i] = w * (-o->R[(size_t)k * nc + i] - y[(size_t)k * nc + i]) / o->D[i] + x[(size_t)k * nc + i];
+        }
         double ans = fabs(y[0] - x[0]);
         for (int i = 1; i < nc; i++) ans = fmax(ans, fabs(y[i] - x[i]));
         if (ans < 1e-14) break;
@@ -2027,10 +2069,9 @@ static void* dup(const void* p, size_t n) { void* q = malloc(n ? n : 1); if (n) 
 int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
 {
     if ((p->sa || p->sst) && (!m->wall_dist || p->flux == CSU_FLUX_ROE)) return CSU_ERR_UNSUPPORTED;
-    /* SST: no transition model, no DPLUR (implicit.c:993-1000 has 4x4 and
-       5x5 blocks only), no global-range limiter (limiter.c:37-38 sizes its
-       range arrays for 5 variables) */
-    if (p->sst && (p->time_scheme == CSU_TIME_DPLUR || (p->order == 2 && p->limiter == 1)))
+    /* SST: no global-range limiter (limiter.c:37-38 sizes its range arrays
+       for 5 variables) */
+    if (p->sst && p->order == 2 && p->limiter == 1)
         return CSU_ERR_UNSUPPORTED;
     csu_oracle* o = calloc(1, sizeof(*o));
     o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;
