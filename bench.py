@@ -36,34 +36,40 @@ METRIC = "cell-iters/s + solves/hr, p50/p99 time-to-converge vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int) -> float:
+def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int, pu: bool = False) -> float:
     """Compulsory HBM bytes of one launch: every array the kernel must touch,
-    each element counted once (neighbour re-reads are cache traffic)."""
+    each element counted once (neighbour re-reads are cache traffic).
+    pu: the stages recover the primitives from the conservative state (the
+    inviscid RK form), so k_grad reads U instead of P and k_update writes no
+    P."""
     ns = nv + 1
     if kernel == "k_prim":
         return nc * (8 * nv + 8 * ns)
     edges = 2 * nf + nb                 # cell-edge slots (interior faces count twice)
     if kernel == "k_grad":
-        # own P, cnp, cnbr, omega; per slot LSQ weights and edge offsets; the
-        # edge states written; ghost states read
-        return nc * (8 * ns + 4 + 16 + 8) + edges * (16 + 16 + 8 * nv) + nb * 8 * ns
+        # own P (or U), cnp, cnbr, omega; per slot LSQ weights and edge
+        # offsets; the edge states written; ghost states read
+        return nc * (8 * (nv if pu else ns) + 4 + 16 + 8) + edges * (16 + 16 + 8 * nv) + nb * 8 * ns
     if kernel == "k_flux":
         # e0, e1, edge slots, face vector; both edge states; the flux
         return nf * (8 + 1 + 24 + 2 * 8 * nv + 8 * nv)
     if kernel == "k_update":
-        # face list, boundary mask, U^n, U out, P out, omega, dt; each face
-        # flux once
-        return nc * (16 + 1 + 8 * nv * 2 + 8 * ns + 8 + 8) + nf * 8 * nv + nb * 8 * nv
+        # face list, boundary mask, U^n, U out, P out (not with pu), omega,
+        # dt; each face flux once
+        return nc * (16 + 1 + 8 * nv * 2 + (0 if pu else 8 * ns) + 8 + 8) + nf * 8 * nv + nb * 8 * nv
     return 0.0
 
 
 # HBM bytes per launch on the default 2000x1000x2 workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) passes, both scaled by what the
-# counters read for a plain 1 GiB copy (profiles/euler_ramp_4M_rk5_fp64.md).
+# counters read for a plain 1 GiB copy (profiles/euler_ramp_4M_rk5_fp64.md),
+# for the primitives-from-state stages (True) and the stored form (False).
 # A counter pass cannot run inside this timed process, so the figure is
 # reported for that workload only.
-MEASURED_TRAFFIC = {("k_grad", 2000, 1000): 1121.9e6, ("k_flux", 2000, 1000): 775.4e6,
-                    ("k_update", 2000, 1000): 787.2e6}
+MEASURED_TRAFFIC = {("k_grad", 2000, 1000, True): 993.8e6, ("k_flux", 2000, 1000, True): 775.5e6,
+                    ("k_update", 2000, 1000, True): 626.6e6,
+                    ("k_grad", 2000, 1000, False): 1121.9e6, ("k_flux", 2000, 1000, False): 775.4e6,
+                    ("k_update", 2000, 1000, False): 787.2e6}
 
 
 _MESHES = {}
@@ -410,10 +416,13 @@ def main():
     kt = big["kernels"]
     dom = max((k for k in kt if alg_bytes(k, 1, 1, 1, 4) > 0), key=lambda k: kt[k][0] * kt[k][1])
     avg_ms = kt[dom][0]
-    traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"])
+    # the default workload runs the primitives-from-state stages unless the
+    # stored form is selected through the library's environment switch
+    pu = os.environ.get("CSU_STORED_PRIMITIVES", "0") != "1"
+    traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"], pu)
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((dom, a.nx, a.ny)),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((dom, a.nx, a.ny, pu)),
             "kernel": dom,
             "kernel_avg_ms": avg_ms,
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},

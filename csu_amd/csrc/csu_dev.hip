@@ -68,6 +68,7 @@ struct csu_dev {
   double *res_hist, *coef_hist;
   int hist_cap;
   int fresh;                   // state set from the host: P and dt not yet derived
+  int pu;                      // RK stages recover the primitives from the state (k_grad's PU form)
   // implicit schemes
   double *Dp, *dW0, *dW1, *A;  // diagonal (without the dt term), increments, DPLUR blocks
   int* jstate;                 // DPLUR: [converged, current buffer]
@@ -469,6 +470,15 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
   kp.lsq_ld = p->precision == CSU_PREC_FP64_LD; kp.store_grad = 0;
   kp.mixed = p->precision == CSU_PREC_MIXED;
+  // inviscid ideal-gas second-order RK with the default limiter, plain FP64:
+  // the stage kernels recover the primitives from the state (k_grad PU);
+  // first order, the global-range limiter and the axisymmetric source read
+  // stored primitives, so those keep them.  CSU_STORED_PRIMITIVES=1 in the
+  // environment selects the stored form everywhere (same results; for
+  // comparisons of the two forms).
+  const char* sp = getenv("CSU_STORED_PRIMITIVES");
+  d->pu = nv == 4 && !p->laminar && p->time_scheme == CSU_TIME_RK && p->order == 2 && p->limiter == 0 &&
+          m->axi == 0 && p->tp == 0 && p->precision == CSU_PREC_FP64 && !(sp && sp[0] == '1');
   kp.rlim = p->rlim; kp.pout = p->pout; kp.twall = p->twall; kp.efix = p->efix; kp.minf = p->mach_inf;
   kp.sst_owall = p->sst_owallfactor;
   {
@@ -742,10 +752,40 @@ extern "C" int csu_dev_kernel_times(csu_dev* d, const char** names, double* avg_
 // orchestration
 // ---------------------------------------------------------------------------
 // One residual evaluation (solverCalcR) with P valid, then the fused update.
+// Us non-null: the primitives-from-state form (k_grad's PU), where Us is the
+// stage's conservative state and P is neither read nor written.
 template <int NV, bool VISC>
-static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, double* Uout) {
+static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, double* Uout,
+                 const double* Us = nullptr) {
   const KP& kp = d->kp;
   const int nc = d->nc;
+  if constexpr (NV == 4 && !VISC) {
+    if (Us) {
+      St st = d->st;
+      st.Us = Us;
+      if (d->nb > 0) {
+        Prof p(d, "k_bcflux");
+        hipLaunchKernelGGL((k_bcflux<4, true>), dim3(nblk(d->nb)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, st);
+      }
+      {
+        Prof p(d, "k_grad");
+        hipLaunchKernelGGL((k_grad<4, false, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+                           d->geo, st);
+      }
+      if (d->nf > 0) {
+        Prof p(d, "k_flux");
+        hipLaunchKernelGGL((k_flux<4, false>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, st);
+      }
+      {
+        Prof p(d, "k_update");
+        hipLaunchKernelGGL((k_update<4, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+                           d->geo, st, flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
+                           d->red + (size_t)NV * d->nred);
+      }
+      HIPCHK(hipGetLastError());
+      return CSU_OK;
+    }
+  }
   if (kp.order == 2 && kp.limiter == 1) {
     Prof p(d, "k_lim1");
     hipLaunchKernelGGL(k_lim1_partial<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->st.P, d->red);
@@ -941,14 +981,18 @@ static int rk_iteration(csu_dev* d, int it_index) {
   int first_flags = F_CLAMP_UN;
   if (d->fresh) {
     // time step from the unclamped state (solverCalcDt), then primitives
+    // (the primitives-from-state form recovers them in the stage kernels
+    // and clamps U^n in the first update, as after any other iteration)
     Prof p(d, "k_dt");
     hipLaunchKernelGGL(k_dt<NV>, dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, d->kp, d->geo, cur_local, d->prm.cfl,
                        d->U, d->dtl, red_dt);
     if (!cur_local)
       hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nblk(nc), d->prm.cfl, red_dt, d->dtg);
-    int rc = prim_pass<NV>(d);
-    if (rc) return rc;
-    first_flags = 0;
+    if (!d->pu) {
+      int rc = prim_pass<NV>(d);
+      if (rc) return rc;
+      first_flags = 0;
+    }
     d->fresh = 0;
   }
   if (d->st.tube)
@@ -957,7 +1001,7 @@ static int rk_iteration(csu_dev* d, int it_index) {
     const bool last = s == d->prm.stages - 1;
     int flags = F_RK | (cur_local ? F_CUR_LOCAL : 0) | (s == 0 ? first_flags : 0);
     if (last) flags |= F_LAST | F_STORE_R | (next_local ? F_NEXT_LOCAL : 0);
-    int rc = stage<NV, VISC>(d, flags, sigma, a[s], d->U, d->Uaux);
+    int rc = stage<NV, VISC>(d, flags, sigma, a[s], d->U, d->Uaux, d->pu ? (s == 0 ? d->U : d->Uaux) : nullptr);
     if (rc) return rc;
   }
   std::swap(d->U, d->Uaux);

@@ -62,6 +62,7 @@ struct St {
   double *wten;         // viscous: wall shear tensor txx txy tyy yp per boundary face [4][nb]
   double *bflux;        // convective boundary-face flux times dS [NV][nb]
   const double *tube;   // time-accurate run: t, tmax, state (0 running, 1 last step, 2 past it), steps
+  const double *Us;     // primitives-from-state stages: the stage's conservative state [NV][nc]
 };
 
 // Primitive recovery of one cell with the positivity clamps, reference
