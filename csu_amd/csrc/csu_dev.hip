@@ -470,15 +470,16 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   kp.turb1order = p->turb1order; kp.axi = m->axi; kp.visc_blazek = p->visc_blazek;
   kp.lsq_ld = p->precision == CSU_PREC_FP64_LD; kp.store_grad = 0;
   kp.mixed = p->precision == CSU_PREC_MIXED;
-  // inviscid ideal-gas second-order RK with the default limiter, plain FP64:
-  // the stage kernels recover the primitives from the state (k_grad PU);
+  // inviscid ideal-gas second-order RK with the default limiter (any
+  // precision): the stage kernels recover the primitives from the state
+  // (k_grad PU);
   // first order, the global-range limiter and the axisymmetric source read
   // stored primitives, so those keep them.  CSU_STORED_PRIMITIVES=1 in the
   // environment selects the stored form everywhere (same results; for
   // comparisons of the two forms).
   const char* sp = getenv("CSU_STORED_PRIMITIVES");
   d->pu = nv == 4 && !p->laminar && p->time_scheme == CSU_TIME_RK && p->order == 2 && p->limiter == 0 &&
-          m->axi == 0 && p->tp == 0 && p->precision == CSU_PREC_FP64 && !(sp && sp[0] == '1');
+          m->axi == 0 && p->tp == 0 && !(sp && sp[0] == '1');
   kp.rlim = p->rlim; kp.pout = p->pout; kp.twall = p->twall; kp.efix = p->efix; kp.minf = p->mach_inf;
   kp.sst_owall = p->sst_owallfactor;
   {
@@ -769,18 +770,33 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
       }
       {
         Prof p(d, "k_grad");
-        hipLaunchKernelGGL((k_grad<4, false, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
-                           d->geo, st);
+        if (kp.mixed)
+          hipLaunchKernelGGL((k_grad<4, false, false, true, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+                             d->geo, st);
+        else if (kp.lsq_ld)
+          hipLaunchKernelGGL((k_grad<4, false, true, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+                             d->geo, st);
+        else
+          hipLaunchKernelGGL((k_grad<4, false, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+                             d->geo, st);
       }
       if (d->nf > 0) {
         Prof p(d, "k_flux");
-        hipLaunchKernelGGL((k_flux<4, false>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, st);
+        if (kp.mixed)
+          hipLaunchKernelGGL((k_flux<4, false, true>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, st);
+        else
+          hipLaunchKernelGGL((k_flux<4, false>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, st);
       }
       {
         Prof p(d, "k_update");
-        hipLaunchKernelGGL((k_update<4, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
-                           d->geo, st, flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
-                           d->red + (size_t)NV * d->nred);
+        if (kp.mixed)
+          hipLaunchKernelGGL((k_update<4, false, true, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+                             d->geo, st, flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
+                             d->red + (size_t)NV * d->nred);
+        else
+          hipLaunchKernelGGL((k_update<4, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+                             d->geo, st, flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
+                             d->red + (size_t)NV * d->nred);
       }
       HIPCHK(hipGetLastError());
       return CSU_OK;

@@ -449,12 +449,14 @@ def test_long_run_fields_and_cf_1e8(tmpcase, kind):
     assert cf.max() < 1e-8
 
 
-@pytest.mark.parametrize("kind,inp", [
-    ("ramp", {}),
-    ("ramp", {"flux": "AUSMDV", "dtLocal": 0, "stages": 3}),
-    ("ramp", {"mach": 0.6, "BC:outlet": "outlet", "flux": "AUSMpup2"}),
+@pytest.mark.parametrize("kind,inp,prec", [
+    ("ramp", {}, 0),
+    ("ramp", {"flux": "AUSMDV", "dtLocal": 0, "stages": 3}, 0),
+    ("ramp", {"mach": 0.6, "BC:outlet": "outlet", "flux": "AUSMpup2"}, 0),
+    ("ramp", {}, 1),
+    ("ramp", {"flux": "AUSM"}, 2),
 ])
-def test_primitives_from_state_is_bitwise(tmpcase, kind, inp, monkeypatch):
+def test_primitives_from_state_is_bitwise(tmpcase, kind, inp, prec, monkeypatch):
     """Inviscid second-order RK stages recover the primitives from the
     state instead of storing them: bit for bit the stored form's histories
     and state (the same function of the same unclamped stage values),
@@ -464,9 +466,11 @@ def test_primitives_from_state_is_bitwise(tmpcase, kind, inp, monkeypatch):
     U0 = c.initial_state() * (1 + 0.2 * rng.standard_normal((c.nvar, c.ncell)))
     U0[0, :50] = -1.0   # density clamps on the first residual
     out = []
+    prm = native.Params.from_buffer_copy(c.params)
+    prm.precision = prec
     for stored in ("0", "1"):
         monkeypatch.setenv("CSU_STORED_PRIMITIVES", stored)
-        dev = native.Device(c)
+        dev = native.Device(c, params=prm)
         dev.set_state(U0)
         r1, c1 = dev.iterate(7)
         dev.set_state(dev.get_state())   # fresh again mid-run
