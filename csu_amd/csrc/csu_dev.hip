@@ -1073,16 +1073,19 @@ static int implicit_iteration(csu_dev* d) {
   }
   if (d->prm.time_scheme == CSU_TIME_LUSGS) {
     Prof p(d, "k_lusgs");
+    // a level holds a fraction of the cells (a colour of the renumbered
+    // mesh): 64-thread workgroups spread it over every CU
+    constexpr int LB = 64;
     for (size_t l = 0; l + 1 < d->lvlL.size(); l++) {
       int off = d->lvlL[l], cnt = d->lvlL[l + 1] - off;
-      hipLaunchKernelGGL((k_lusgs<NV, VISC, 0>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+      hipLaunchKernelGGL((k_lusgs<NV, VISC, 0>), dim3((cnt + LB - 1) / LB), dim3(LB), 0, d->stream, kp, d->geo, d->st,
                          d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0);
     }
     hipLaunchKernelGGL(k_lusgs_uinit<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->Dp, d->dtg, d->dW0,
                        d->dW1, d->st.dq);
     for (size_t l = 0; l + 1 < d->lvlU.size(); l++) {
       int off = d->lvlU[l], cnt = d->lvlU[l + 1] - off;
-      hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3(nblk(cnt)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st,
+      hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3((cnt + LB - 1) / LB), dim3(LB), 0, d->stream, kp, d->geo, d->st,
                          d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1);
     }
     hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW1, d->dW1,
