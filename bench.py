@@ -188,6 +188,14 @@ def _config_solve(d: str, n: int, precision: int):
     t1 = time.perf_counter()
     out = dict(nc=case.ncell, seconds=t1 - t0, res=res, coef=coef, U=dev.get_state(),
                wt=dev.wall_tensor() if (case.params.laminar or case.params.sa or case.params.sst) else None)
+    # per-kernel time per iteration from the library's own HIP events over
+    # 20 more iterations (outside the timed region; eager launches, so the
+    # sum exceeds the graph-replayed iteration by the launch gaps)
+    dev.set_profiling(True)
+    dev.iterate(20, want=False)
+    dev.sync()
+    out["kernels_ms_per_iter"] = {k: round(ms * cnt / 20, 5) for k, (ms, cnt) in dev.kernel_times().items()}
+    dev.set_profiling(False)
     dev.close()
     case.close()
     return out
@@ -243,7 +251,8 @@ def other_configs(log, n_plate: int = 500, n_naca: int = 500):
         entry = {"workload": label, "cells": r64["nc"], "iterations": n}
         for tag, r in (("fp64", r64), ("mixed", r32)):
             entry[tag] = {"cell_iter_per_s": r["nc"] * n / r["seconds"], "ms_per_iter": 1e3 * r["seconds"] / n,
-                          "density_residual_drop": float(r["res"][-1, 0] / np.max(r["res"][:, 0]))}
+                          "density_residual_drop": float(r["res"][-1, 0] / np.max(r["res"][:, 0])),
+                          "kernels_ms_per_iter": r["kernels_ms_per_iter"]}
         entry["mixed_vs_fp64"] = {
             "CL_rel": abs(L32 / L64 - 1) if L64 else None, "CD_rel": abs(D32 / D64 - 1) if D64 else None,
             "Cf_linf_rel": float(np.abs(wt32[m] - wt64[m]).max() / np.abs(wt64[m]).max()) if m.any() else None,
