@@ -71,7 +71,8 @@ struct csu_dev {
   int pu;                      // RK stages recover the primitives from the state (k_grad's PU form)
   // implicit schemes
   double *Dp, *dW0, *dW1, *A;  // diagonal (without the dt term), increments, DPLUR blocks
-  int* jstate;                 // DPLUR: [converged, current buffer]
+  int* jstate;                 // DPLUR: [unused, buffer holding the last result]
+  double* jred;                // DPLUR: per-sweep block maxima of |dx_rho| [nlinear][cell blocks]
   std::vector<int> lvlL, lvlU; // LU-SGS level offsets (forward / backward)
   int *cellsL, *cellsU;        // cells sorted by level
   // iteration graphs (captured once, replayed): RK keyed by local time
@@ -310,7 +311,7 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   d->nmark = m->nmark; d->iter = 0; d->prm = *p; d->stream = nullptr; d->prof = 0;
   d->visc = (p->laminar || p->sa || p->sst) ? 1 : 0;
   d->hist_cap = 0; d->res_hist = d->coef_hist = nullptr; d->fresh = 1;
-  d->Dp = d->dW0 = d->dW1 = d->A = nullptr; d->jstate = nullptr; d->cellsL = d->cellsU = nullptr;
+  d->Dp = d->dW0 = d->dW1 = d->A = nullptr; d->jstate = nullptr; d->jred = nullptr; d->cellsL = d->cellsU = nullptr;
   d->hctr = nullptr; d->use_graphs = 1; d->bytes = 0; d->tstate = nullptr;
   d->hres = d->hcoef = nullptr; d->hcap = 0; d->async_start = 0; d->async_n = 0;
   for (int k = 0; k < 4; k++) { d->gexec[k] = nullptr; d->gU[k] = nullptr; }
@@ -589,7 +590,8 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
     HIPCHK(hipMemsetAsync(d->dW1, 0, nvc * sizeof(double), d->stream));   // implicitInitDPLUR
   }
   if (p->time_scheme == CSU_TIME_DPLUR) {
-    if ((rc = dalloc(d, &d->A, (size_t)4 * nv * nv * nc)) || (rc = dalloc(d, &d->jstate, 2))) {
+    if ((rc = dalloc(d, &d->A, (size_t)4 * nv * nv * nc)) || (rc = dalloc(d, &d->jstate, 2)) ||
+        (rc = dalloc(d, &d->jred, (size_t)(p->nlinear > 0 ? p->nlinear : 1) * nblk(nc)))) {
       csu_dev_destroy(d);
       return rc;
     }
@@ -1089,18 +1091,18 @@ static int implicit_iteration(csu_dev* d) {
                          d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1);
     }
     hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW1, d->dW1,
-                       (const int*)nullptr);
+                       (const int*)nullptr, (const double*)nullptr, 0, 0);
   } else {
     Prof p(d, "k_dplur");
     hipLaunchKernelGGL((k_blocks<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->U, d->A);
-    hipLaunchKernelGGL(k_jac_reset, dim3(1), dim3(1), 0, d->stream, d->jstate);
-    for (int mm = 0; mm < d->prm.nlinear; mm++) {
+    // one launch per sweep: each tests the previous sweep's maximum itself
+    for (int mm = 0; mm < d->prm.nlinear; mm++)
       hipLaunchKernelGGL(k_jacobi<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->A, d->Dp, d->dtg,
-                         d->R, d->dW0, d->dW1, d->jstate, red_min, d->st.dq);
-      hipLaunchKernelGGL(k_jac_final, dim3(1), dim3(CSU_BLOCK), 0, d->stream, nb, red_min, d->jstate);
-    }
+                         d->R, d->dW0, d->dW1, (const int*)d->jstate, d->jred, mm, d->st.dq);
     hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW0, d->dW1,
-                       (const int*)d->jstate);
+                       (const int*)d->jstate, (const double*)d->jred, d->prm.nlinear, nb);
+    hipLaunchKernelGGL(k_jac_base, dim3(1), dim3(64), 0, d->stream, d->jstate, (const double*)d->jred,
+                       d->prm.nlinear, nb);
   }
   iteration_tail<NV>(d, nb);
   HIPCHK(hipGetLastError());
@@ -1341,7 +1343,7 @@ extern "C" int64_t csu_dev_footprint(const csu_mesh_desc* m, const csu_params* p
   if (p->order == 2) b += 32 * nv * nc;                                                   // edge states
   if (visc) b += nc * (16 + 16 + 48) + nf * (8 * (nv - 1) + 8) + nb * (8 * (nv - 1) + 16);
   if (p->time_scheme != CSU_TIME_RK) b += nc * (8 + 16 * nv + 8);
-  if (p->time_scheme == CSU_TIME_DPLUR) b += 32 * nv * nv * nc;
+  if (p->time_scheme == CSU_TIME_DPLUR) b += 32 * nv * nv * nc + 8.0 * (p->nlinear > 0 ? p->nlinear : 1) * (nc / 256 + 1);
   b += (nv + 4) * 8 * (double)(hist > 0 ? hist : 1);
   // allocation granularity: every array rounds up (at most 2 MiB slack per large array)
   return (int64_t)(b * 1.02) + (int64_t)64 * 4096;

@@ -422,10 +422,14 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
   }
 }
 
-// One damped Jacobi sweep (implicitMultA + update, implicit.c:938-1022):
-// y = D x + sum_j A_j x_j;  x_new = w (-R - y)/D + x.  Buffers buf[cur] ->
-// buf[1-cur]; state[0] = converged flag, state[1] = cur.  A sweep after
-// convergence in the same implicit step does nothing (the reference breaks).
+// One damped Jacobi sweep m (implicitMultA + update, implicit.c:938-1022):
+// y = D x + sum_j A_j x_j;  x_new = w (-R - y)/D + x, from buf[(state[1] +
+// m) & 1] into the other buffer; state[1] is the buffer holding the last
+// implicit step's result.  The block maxima of |dx_rho| go to red_sw[m][.];
+// each sweep first reduces the previous sweep's (the reference's test
+// after every sweep, implicit.c:1026-1031), and once that is below 1e-14
+// (or marked -1: the sweeps had already stopped) it writes -1 and does
+// nothing: the reference breaks out of its loop.
 // SST: the diagonal is LU-SGS's block (implicit.c:326-351), D with the
 // k-omega source Jacobians dq; y takes its product and x_new solves with it
 // by Cramer's rule (D alone where its determinant is below 1e-14).
@@ -433,14 +437,31 @@ template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double* __restrict__ A,
                                                       const double* __restrict__ Dp, const double* __restrict__ dtg,
                                                       const double* __restrict__ R, double* buf0, double* buf1,
-                                                      const int* __restrict__ state, double* __restrict__ red_max,
-                                                      const double* __restrict__ dq) {
+                                                      const int* __restrict__ state, double* __restrict__ red_sw,
+                                                      int m, const double* __restrict__ dq) {
   __shared__ double smax[CSU_BLOCK];
-  if (state[0]) return;
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
-  const int nc = kp.nc;
-  const double* x = state[1] ? buf1 : buf0;
-  double* y = state[1] ? buf0 : buf1;
+  const int nc = kp.nc, nblocks = gridDim.x;
+  if (m > 0) {
+    const double* prev = red_sw + (size_t)(m - 1) * nblocks;
+    double pm = 0.0;
+    for (int b = t; b < nblocks; b += CSU_BLOCK) pm = fmax(pm, prev[b]);
+    smax[t] = pm;
+    __syncthreads();
+    for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
+      if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
+      __syncthreads();
+    }
+    const bool stopped = smax[0] < 1e-14;
+    __syncthreads();  // smax is reused below
+    if (stopped) {
+      if (t == 0) red_sw[(size_t)m * nblocks + blockIdx.x] = -1.0;
+      return;
+    }
+  }
+  const int par = (state[1] + m) & 1;
+  const double* x = par ? buf1 : buf0;
+  double* y = par ? buf0 : buf1;
   double diff = 0.0;
   if (i < nc) {
     const double D = diag(Dp, g.omega, dtg, i);
@@ -514,40 +535,31 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
     __syncthreads();
   }
-  if (t == 0) red_max[blockIdx.x] = smax[0];
+  if (t == 0) red_sw[(size_t)m * nblocks + blockIdx.x] = smax[0];
 }
 
-// After a Jacobi sweep: flip the current buffer and test max|dx_rho| < 1e-14.
-__global__ __launch_bounds__(CSU_BLOCK) void k_jac_final(int nblocks, const double* __restrict__ red_max, int* state) {
-  __shared__ double smax[CSU_BLOCK];
-  if (state[0]) return;
-  const int t = threadIdx.x;
-  double m = 0.0;
-  for (int b = t; b < nblocks; b += CSU_BLOCK) m = fmax(m, red_max[b]);
-  smax[t] = m;
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
-    __syncthreads();
-  }
-  if (t == 0) {
-    state[1] = 1 - state[1];
-    if (smax[0] < 1e-14) state[0] = 1;
-  }
+// Buffer of the DPLUR result: sweeps 0..k-1 ran (k = the first sweep marked
+// -1, or nlin), so the last one wrote buf[(state[1] + k) & 1].
+__device__ __forceinline__ int jac_result(const int* state, const double* red_sw, int nlin, int nblocks) {
+  int k = 0;
+  while (k < nlin && !(red_sw[(size_t)k * nblocks] < 0.0)) k++;
+  return (state[1] + k) & 1;
 }
 
-__global__ void k_jac_reset(int* state) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) state[0] = 0;
+// After the update: the result buffer becomes the next step's start.
+__global__ void k_jac_base(int* state, const double* __restrict__ red_sw, int nlin, int nblocks) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) state[1] = jac_result(state, red_sw, nlin, nblocks);
 }
 
-// solverUpdateUImplicit (solver.c:1945-1955): U += dW; for DPLUR dW is
-// buf[state[1]].
+// solverUpdateUImplicit (solver.c:1945-1955): U += dW; for DPLUR (red_sw
+// non-null) dW is the sweeps' result buffer.
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_upd_imp(int nc, double* __restrict__ U, const double* buf0,
-                                                       const double* buf1, const int* __restrict__ state) {
+                                                       const double* buf1, const int* __restrict__ state,
+                                                       const double* __restrict__ red_sw, int nlin, int nblocks) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nc) return;
-  const double* dW = (state && state[1]) ? buf1 : buf0;
+  const double* dW = (red_sw && jac_result(state, red_sw, nlin, nblocks)) ? buf1 : buf0;
 #pragma unroll
   for (int k = 0; k < NV; k++) U[k * nc + i] += dW[k * nc + i];
 }
