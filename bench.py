@@ -391,7 +391,7 @@ def main():
                           "copy_gb_per_s": 2 * n * 8 / dt / 1e9}))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
+        print(f"bench.py: the benchmark runs on one GPU only (asked for {a.gpus}); "
               "multi-GPU sharding is a design note, not a benchmark", file=sys.stderr)
         sys.exit(2)
 

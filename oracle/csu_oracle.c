@@ -1886,7 +1886,7 @@ static void lusgs(csu_oracle* o)
    blocks of every interior neighbour, h = 1e-6 */
 /* SST (nv 6) has no DPLUR in the reference: implicitCalcDPLUR builds 4x4
    blocks for it (implicit.c:993-1000) and implicitMultA then reads past
-   them.  This build's SST form, restated here: the blocks of
+   them.  The device library's SST form, restated here: the blocks of
    implicitUpdateA_sa with two convected scalars (implicitAuxCalcFlux_sst2,
    implicit.c:1036-1046) and k taken out of the internal energy as
    implicitCalcDeltaFlux_sst does (:1048-1117); the diagonal is LU-SGS's

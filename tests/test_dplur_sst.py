@@ -1,6 +1,7 @@
 """DPLUR for SST (CPU).  The reference has no SST form of DPLUR
-(implicitCalcDPLUR, implicit.c:990-1000, builds 4x4 blocks for it); this
-build's form is pinned by what fixes any consistent implicit scheme: its
+(implicitCalcDPLUR, implicit.c:990-1000, builds 4x4 blocks for it); the
+SST form of the device library is pinned by what fixes any consistent
+implicit scheme: its
 fixed point is the steady state R(U) = 0, so run to convergence it reaches
 the state the reference's own LU-SGS reaches on the same case.  The device
 is held to the oracle's restatement of the form (tests/test_gpu_euler.py)."""

@@ -297,8 +297,8 @@ def test_sst_history_vs_oracle(tmpcase, inp):
 @pytest.mark.parametrize("inp", [{}, {"sstTrans": "1", "flux": "AUSMDV"}])
 def test_sst_dplur_history_vs_oracle(tmpcase, inp):
     """DPLUR for SST (6x6 finite-difference blocks, k-omega source block on
-    the diagonal; no reference counterpart, the oracle restates this
-    build's form): GPU against the oracle over 40 iterations."""
+    the diagonal; no reference counterpart, the oracle restates the device
+    library's form): GPU against the oracle over 40 iterations."""
     d = tmpcase("kd")
     print("sst dplur plate", inp, flush=True)
     cases.write_case(d, cases.plate_mesh(nx=40, ny=16, y1=2e-3, growth=1.15, jitter=0.1),
