@@ -161,3 +161,25 @@ def test_surf_unsupported(tmpcase):
     c = native.HostCase(d)
     with pytest.raises(native.CsuError):
         c.write_surf(np.zeros((c.nbface, 11)), d + "surfData.csv")
+
+
+def test_surf_marker_not_a_chain(tmpcase):
+    """A marker whose faces do not run head to tail in file order (the
+    airfoil case's far-field outlet): the reference reads unset node
+    indices there and ends with a segmentation fault; this writer numbers
+    the nodes as the reference does, skips the faces' unnumbered nodes and
+    writes one row per numbered node."""
+    d = tmpcase("sn")
+    cases.write_case(d, cases.naca_mesh(ni=48, nwall=8, nfar=8),
+                     cases.naca_input(threads=1, nmax=2, aoa=4.0, writeSurf="outlet"))
+    c = native.HostCase(d)
+    o = po.Oracle(c)
+    o.set_state(c.initial_state())
+    o.iterate(2)
+    o.primitives()
+    c.write_surf(o.surf_faces(), d + "surfData.csv")
+    head, S = read_solution2(d + "surfData.csv")
+    a = c.arrays()
+    mk = [k for k in range(c.nmark) if a["mark_type"][k] == 2]
+    ne = int(np.sum(a["bface_mark"] == mk[0])) if mk else 0
+    assert head.startswith("x,y,rho,u,v,p,T,n,") and S.shape == (ne + 1, 15)
