@@ -69,6 +69,7 @@ struct csu_dev {
   int hist_cap;
   int fresh;                   // state set from the host: P and dt not yet derived
   int pu;                      // RK stages recover the primitives from the state (k_grad's PU form)
+  int sblk;                    // threads per workgroup of k_grad and k_flux
   // implicit schemes
   double *Dp, *dW0, *dW1, *A;  // diagonal (without the dt term), increments, DPLUR blocks
   int* jstate;                 // DPLUR: [unused, buffer holding the last result]
@@ -478,6 +479,9 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   // stored primitives, so those keep them.  CSU_STORED_PRIMITIVES=1 in the
   // environment selects the stored form everywhere (same results; for
   // comparisons of the two forms).
+  // small meshes: 64-thread workgroups for the gradient and face kernels
+  // (no shared memory), so that a few hundred cell tiles reach every CU
+  d->sblk = nc <= 262144 ? 64 : CSU_BLOCK;
   const char* sp = getenv("CSU_STORED_PRIMITIVES");
   d->pu = nv == 4 && !p->laminar && p->time_scheme == CSU_TIME_RK && p->order == 2 && p->limiter == 0 &&
           m->axi == 0 && p->tp == 0 && !(sp && sp[0] == '1');
@@ -773,21 +777,21 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
       {
         Prof p(d, "k_grad");
         if (kp.mixed)
-          hipLaunchKernelGGL((k_grad<4, false, false, true, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+          hipLaunchKernelGGL((k_grad<4, false, false, true, true>), dim3((nc + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp,
                              d->geo, st);
         else if (kp.lsq_ld)
-          hipLaunchKernelGGL((k_grad<4, false, true, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+          hipLaunchKernelGGL((k_grad<4, false, true, false, true>), dim3((nc + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp,
                              d->geo, st);
         else
-          hipLaunchKernelGGL((k_grad<4, false, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
+          hipLaunchKernelGGL((k_grad<4, false, false, false, true>), dim3((nc + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp,
                              d->geo, st);
       }
       if (d->nf > 0) {
         Prof p(d, "k_flux");
         if (kp.mixed)
-          hipLaunchKernelGGL((k_flux<4, false, true>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, st);
+          hipLaunchKernelGGL((k_flux<4, false, true>), dim3((d->nf + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo, st);
         else
-          hipLaunchKernelGGL((k_flux<4, false>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, st);
+          hipLaunchKernelGGL((k_flux<4, false>), dim3((d->nf + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo, st);
       }
       {
         Prof p(d, "k_update");
@@ -817,20 +821,20 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
   {
     Prof p(d, "k_grad");
     if (kp.mixed)
-      hipLaunchKernelGGL((k_grad<NV, VISC, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo,
+      hipLaunchKernelGGL((k_grad<NV, VISC, false, true>), dim3((nc + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo,
                          d->st);
     else if (kp.lsq_ld)
-      hipLaunchKernelGGL((k_grad<NV, VISC, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
+      hipLaunchKernelGGL((k_grad<NV, VISC, true>), dim3((nc + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo, d->st);
     else
-      hipLaunchKernelGGL((k_grad<NV, VISC, false>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
+      hipLaunchKernelGGL((k_grad<NV, VISC, false>), dim3((nc + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo, d->st);
   }
   if (d->nf > 0) {
     Prof p(d, "k_flux");
     if (kp.mixed)
-      hipLaunchKernelGGL((k_flux<NV, VISC, true>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo,
+      hipLaunchKernelGGL((k_flux<NV, VISC, true>), dim3((d->nf + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo,
                          d->st);
     else
-      hipLaunchKernelGGL((k_flux<NV, VISC>), dim3(nblk(d->nf)), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st);
+      hipLaunchKernelGGL((k_flux<NV, VISC>), dim3((d->nf + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo, d->st);
   }
   {
     Prof p(d, "k_update");
