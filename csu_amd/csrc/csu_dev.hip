@@ -8,6 +8,7 @@
 #include <string.h>
 #include <math.h>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "csu.h"
@@ -16,6 +17,8 @@
 #include "kern_setup.h"
 
 using namespace csu;
+
+#define CSU_MAX_DEVICES 64
 
 // ---------------------------------------------------------------------------
 // error reporting
@@ -54,6 +57,7 @@ struct DevBuf {
 
 struct csu_dev {
   int nc, nf, nb, nn, nv, nmark, iter, visc;
+  int device;                  // HIP device the context was created on
   csu_params prm;
   KP kp;
   Geo geo;
@@ -310,6 +314,11 @@ extern "C" int csu_dev_create(const csu_mesh_desc* m, const csu_params* p, csu_d
   csu_dev* d = new csu_dev();
   d->nc = m->ncell; d->nf = m->nface; d->nb = m->nbface; d->nn = m->nnode; d->nv = p->nvar;
   d->nmark = m->nmark; d->iter = 0; d->prm = *p; d->stream = nullptr; d->prof = 0;
+  d->device = 0;
+  if (hipGetDevice(&d->device) != hipSuccess) {
+    delete d;
+    return set_err(CSU_ERR_HIP, "hipGetDevice failed");
+  }
   d->visc = (p->laminar || p->sa || p->sst) ? 1 : 0;
   d->hist_cap = 0; d->res_hist = d->coef_hist = nullptr; d->fresh = 1;
   d->Dp = d->dW0 = d->dW1 = d->A = nullptr; d->jstate = nullptr; d->jred = nullptr; d->cellsL = d->cellsU = nullptr;
@@ -1329,6 +1338,61 @@ extern "C" int csu_dev_surf_faces(csu_dev* d, double* out) {
   if (e != hipSuccess) return set_err(CSU_ERR_HIP, "surface faces: %s", hipGetErrorString(e));
   for (int q = 0; q < nb; q++)
     for (int k = 0; k < 11; k++) out[11 * (size_t)q + k] = w[(size_t)k * nb + q];
+  return CSU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free surface data
+// ---------------------------------------------------------------------------
+// A fixed staging array in device memory (one copy per GPU, held until the
+// process ends) replaces the per-call buffer of csu_dev_surf_faces.  The
+// array's name is only a host-side stand-in: host code uses the address
+// hipGetSymbolAddress returns on each device.  k_surf writes all 11 entries
+// of every face it covers, so nothing in it needs clearing between calls.
+static __device__ __align__(256) double scratch[11 * CSU_SURF_NOALLOC_MAX_NB];
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[CSU_MAX_DEVICES];
+
+// The context's device's copy of `scratch` (looked up once per device, with
+// that device current).
+static int scratch_on(const csu_dev* d, double** out) {
+  if (d->device < 0 || d->device >= CSU_MAX_DEVICES) return set_err(CSU_ERR_ARG, "device %d", d->device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[d->device]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != d->device) HIPCHK(hipSetDevice(d->device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != d->device) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !ptr)
+      return set_err(CSU_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", d->device, hipGetErrorString(e));
+    g_scratch_dev[d->device] = (double*)ptr;
+  }
+  *out = g_scratch_dev[d->device];
+  return CSU_OK;
+}
+
+extern "C" int csu_dev_surf_faces_noalloc(csu_dev* d, double* out) {
+  if (!d || !out) return set_err(CSU_ERR_ARG, "null argument");
+  if (d->nv != 5 && d->nv != 6) return set_err(CSU_ERR_ARG, "not an SA or SST run");
+  const int nb = d->nb;
+  if (nb > CSU_SURF_NOALLOC_MAX_NB)
+    return set_err(CSU_ERR_ARG, "%d boundary faces: the allocation-free form stages at most %d", nb,
+                   CSU_SURF_NOALLOC_MAX_NB);
+  if (nb == 0) return CSU_OK;
+  double* buf = nullptr;
+  int rc = scratch_on(d, &buf);
+  if (rc) return rc;
+  if (d->nv == 6)
+    hipLaunchKernelGGL((k_surf<6, true>), dim3((nb + CSU_BLOCK - 1) / CSU_BLOCK), dim3(CSU_BLOCK), 0, d->stream,
+                       d->kp, d->geo, d->st, buf);
+  else
+    hipLaunchKernelGGL((k_surf<5, true>), dim3((nb + CSU_BLOCK - 1) / CSU_BLOCK), dim3(CSU_BLOCK), 0, d->stream,
+                       d->kp, d->geo, d->st, buf);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, buf, 11 * sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
   return CSU_OK;
 }
 

@@ -900,8 +900,9 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_sst_fields(KP kp, const double* _
 // sst.c:820-962): the boundary viscous flux and eddy viscosity recomputed
 // from the current primitives and the last residual's gradients, the ghost
 // state of that residual and the wall distance of the face's cell.
-// out [11][nb] = f_x f_y f_e miEddy, ghost r u v p T (nu~ | k), d.
-template <int NV>
+// out [11][nb] = f_x f_y f_e miEddy, ghost r u v p T (nu~ | k), d; with
+// AOS, [nb][11] (the host layout, copied out without a transpose).
+template <int NV, bool AOS = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_surf(KP kp, Geo g, St s, double* __restrict__ out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = kp.nc, nb = kp.nb;
@@ -920,13 +921,16 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_surf(KP kp, Geo g, St s, double* 
   else
     sa_bc_face(kp, t, g.bdsx[q], g.bdsy[q], g.bnx[q], g.bny[q], g.bds2[q], dx, dy, Pi, gx, gy, s.dPx[i], s.dPy[i], f,
                &mie);
-  out[q] = f[0];
-  out[nb + q] = f[1];
-  out[2 * nb + q] = f[2];
-  out[3 * nb + q] = mie;
+  // every one of the 11 entries of face q is written
+  const size_t cs = AOS ? 1 : (size_t)nb, rs = AOS ? 11 : 1;
+  double* o = out + (size_t)q * rs;
+  o[0] = f[0];
+  o[cs] = f[1];
+  o[2 * cs] = f[2];
+  o[3 * cs] = mie;
 #pragma unroll
-  for (int m = 0; m < 6; m++) out[(4 + m) * nb + q] = s.Pb[m * nb + q];
-  out[10 * nb + q] = g.wdist[i];
+  for (int m = 0; m < 6; m++) o[(4 + m) * cs] = s.Pb[m * nb + q];
+  o[10 * cs] = g.wdist[i];
 }
 
 }  // namespace csu

@@ -121,6 +121,7 @@ def lib():
     L.csu_dev_wall_tensor.argtypes = [C.c_void_p, f64p]
     L.csu_dev_sst_fields.argtypes = [C.c_void_p, f64p]
     L.csu_dev_surf_faces.argtypes = [C.c_void_p, f64p]
+    L.csu_dev_surf_faces_noalloc.argtypes = [C.c_void_p, f64p]
     L.csu_write_surf.argtypes = [C.POINTER(Case), f64p, C.c_char_p]
     L.csu_dev_ready.argtypes = [C.c_void_p]
     L.csu_dev_async_result.argtypes = [C.c_void_p, f64p, f64p]
@@ -347,6 +348,12 @@ class Device:
         ghost r u v p T (nu~ | k), wall distance of the face's cell."""
         out = np.empty((max(self.case.nbface, 1), 11))
         check(lib().csu_dev_surf_faces(self._h, _p(out, C.c_double)))
+        return out[:self.case.nbface]
+
+    def surf_faces_noalloc(self) -> np.ndarray:
+        """surf_faces through the allocation-free entry point."""
+        out = np.empty((max(self.case.nbface, 1), 11))
+        check(lib().csu_dev_surf_faces_noalloc(self._h, _p(out, C.c_double)))
         return out[:self.case.nbface]
 
     def wall_tensor(self) -> np.ndarray:

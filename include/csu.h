@@ -188,6 +188,19 @@ int  csu_dev_sst_fields(csu_dev* dev, double* out);
  * distance of the face's cell. */
 int  csu_dev_surf_faces(csu_dev* dev, double* out);
 
+/* Allocation-free form of csu_dev_surf_faces: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): the
+ * per-face data is staged in a fixed device array of the library
+ * (11 x CSU_SURF_NOALLOC_MAX_NB doubles = 5.8 MB, one per GPU, held until
+ * the process ends) instead of a buffer allocated per call; nothing is
+ * allocated.  Same output as csu_dev_surf_faces.  Contexts with more than
+ * CSU_SURF_NOALLOC_MAX_NB boundary faces are refused (CSU_ERR_ARG).  All
+ * calls on one GPU share that array: calls that overlap on one GPU (from
+ * several host threads) race. */
+#define CSU_SURF_NOALLOC_MAX_NB 65536
+int  csu_dev_surf_faces_noalloc(csu_dev* dev, double* out);
+
 /* Iterations done so far; local time step [ncell] of the last iteration. */
 int  csu_dev_iteration(const csu_dev* dev);
 int  csu_dev_get_dt(csu_dev* dev, double* dtl);
