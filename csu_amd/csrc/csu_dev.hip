@@ -116,6 +116,16 @@ static int dalloc(csu_dev* d, T** p, size_t n) {
   return CSU_OK;
 }
 
+// Free one of the context's own allocations before the context ends.
+static void dfree(csu_dev* d, void* p, size_t bytes) {
+  if (!p) return;
+  auto it = std::find(d->buf.owned.begin(), d->buf.owned.end(), p);
+  if (it == d->buf.owned.end()) return;
+  d->buf.owned.erase(it);
+  (void)hipFree(p);
+  d->bytes -= bytes;
+}
+
 template <typename T>
 static int upload(csu_dev* d, const T** dst, const std::vector<T>& h) {
   T* p;
@@ -1137,9 +1147,14 @@ static int iterate_implicit(csu_dev* d, int niter) {
 // Enqueue niter iterations on the context's stream (no host wait).
 static int enqueue_iterations(csu_dev* d, int niter) {
   if (d->hist_cap < niter) {
-    // a new history ring: device slots continue from the iteration count
+    // a new history ring (the old one freed): device slots continue from
+    // the iteration count
     HIPCHK(hipStreamSynchronize(d->stream));
     graphs_drop(d);
+    dfree(d, d->res_hist, sizeof(double) * (size_t)d->hist_cap * d->nv);
+    dfree(d, d->coef_hist, sizeof(double) * (size_t)d->hist_cap * 4);
+    d->res_hist = d->coef_hist = nullptr;
+    d->hist_cap = 0;
     int rc;
     if ((rc = dalloc(d, &d->res_hist, (size_t)niter * d->nv)) || (rc = dalloc(d, &d->coef_hist, (size_t)niter * 4)))
       return rc;

@@ -43,6 +43,7 @@ struct Job {
   bool res0_set = false;
   double cfl = 0.0;  // current CFL of a ramped job
   int64_t est = 0;   // footprint estimate charged while the context is created
+  int64_t charged = 0;  // device bytes of the context charged against the budget
   int create_rc = 0;
 };
 
@@ -80,6 +81,10 @@ static void creator(csu_server* s) {
     int rc = csu_dev_create(j->r.mesh, j->r.prm, &j->dev);
     if (rc) j->dev = nullptr;
     if (!rc && j->r.U0) rc = csu_dev_set_state(j->dev, j->r.U0);
+    if (rc && j->dev) {  // a context that cannot run holds no memory
+      csu_dev_destroy(j->dev);
+      j->dev = nullptr;
+    }
     j->create_rc = rc;
     std::lock_guard<std::mutex> lk(s->mu);
     s->created.push_back(j);
@@ -161,9 +166,9 @@ static void retire(csu_server* s, size_t k, int status) {
       std::vector<double> U((size_t)j->r.prm->nvar * j->r.mesh->ncell);
       if (csu_dev_get_state(j->dev, U.data()) == CSU_OK) s->states[j->r.id] = std::move(U);
     }
-    s->used -= csu_dev_bytes(j->dev);
     csu_dev_destroy(j->dev);
   }
+  s->used -= j->charged;  // exactly what take_created added
   s->done.push_back(j->res);
   delete j;
   s->active.erase(s->active.begin() + (long)k);
@@ -219,8 +224,10 @@ static int take_created(csu_server* s) {
     s->active.push_back(j);
     int rc = j->create_rc;
     if (!rc) {
-      s->used += csu_dev_bytes(j->dev);
+      // charged after the first chunk, which allocates the history ring
       rc = enqueue_chunk(s, j);
+      j->charged = csu_dev_bytes(j->dev);
+      s->used += j->charged;
     }
     if (rc) { retire(s, s->active.size() - 1, rc); retired++; }
   }

@@ -444,7 +444,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
   const int nc = kp.nc, nblocks = gridDim.x;
   if (m > 0) {
     const double* prev = red_sw + (size_t)(m - 1) * nblocks;
-    double pm = 0.0;
+    // NaN is fmax's identity: the maximum is NaN only when every cell's
+    // change is, which the reference's loop (implicit.c:1023-1031) does not
+    // take as converged
+    double pm = __builtin_nan("");
     for (int b = t; b < nblocks; b += CSU_BLOCK) pm = fmax(pm, prev[b]);
     smax[t] = pm;
     __syncthreads();
@@ -462,7 +465,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
   const int par = (state[1] + m) & 1;
   const double* x = par ? buf1 : buf0;
   double* y = par ? buf0 : buf1;
-  double diff = 0.0;
+  double diff = __builtin_nan("");  // lanes past the last cell: fmax's identity
   if (i < nc) {
     const double D = diag(Dp, g.omega, dtg, i);
     double xi[NV], acc[NV];

@@ -52,8 +52,14 @@ static char* slurp(const char* path, size_t* len)
     if (!f) return NULL;
     size_t cap = 1 << 20, n = 0;
     char* b = malloc(cap);
+    if (!b) { fclose(f); return NULL; }
     for (;;) {
-        if (n == cap) { cap *= 2; b = realloc(b, cap); }
+        if (n == cap) {
+            char* nb = realloc(b, cap * 2);
+            if (!nb) { free(b); fclose(f); return NULL; }
+            b = nb;
+            cap *= 2;
+        }
         size_t r = fread(b + n, 1, cap - n, f);
         n += r;
         if (r == 0) break;
@@ -125,7 +131,7 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
 {
     size_t flen = 0;
     char* fbuf = slurp(path, &flen);
-    if (!fbuf) return err("cannot open", path);
+    if (!fbuf) return err("cannot open or read into memory", path);
     rbuf rb = {fbuf, fbuf + flen}, *f = &rb;
     char s[128];
     int nelem = -1, npoin = -1, nmark = -1;
@@ -143,7 +149,13 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
         int np = type == 5 ? 3 : type == 9 ? 4 : 0;
         if (!np) { free(cv); free(fbuf); return err("element type is not 5 or 9", path); }
         for (int k = 0; k < 4; k++) cv[4 * i + k] = -1;
-        for (int k = 0; k < np; k++) { get_word(f, s, sizeof s); cv[4 * i + k] = atoi(s); }
+        for (int k = 0; k < np; k++) {
+            get_word(f, s, sizeof s);
+            cv[4 * i + k] = atoi(s);
+            /* -1 marks the missing 4th vertex of a triangle: no element may
+               name a negative vertex (a quad would turn into a triangle) */
+            if (cv[4 * i + k] < 0) { free(cv); free(fbuf); return err("negative vertex index", path); }
+        }
         get_word(f, s, sizeof s);                       /* element index */
     }
     get_word(f, s, sizeof s);

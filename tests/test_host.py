@@ -108,3 +108,19 @@ def test_solution_csv_inverse_distance(tmp_path):
     assert rows[0] == f"0, {c.nnode}, 4,"
     vals = np.array([[float(x) for x in r.split(",")[:-1]] for r in rows[1:]])
     assert np.allclose(vals, U[:, 0][None, :], rtol=1e-10)
+
+
+@pytest.mark.parametrize("bad", ["9 0 1 4 -1 0", "5 0 -2 4 0", "9 0 1 4 3 0\n9 0 1 4 7 1"])
+def test_su2_vertex_index_out_of_range(tmp_path, bad):
+    """A negative vertex index (a quad whose 4th index is -1 included) or one
+    past the node count is refused, not read as a triangle or out of the
+    coordinate array."""
+    d = str(tmp_path) + "/"
+    nel = bad.count("\n") + 1
+    text = (f"NDIME= 2\nNELEM= {nel}\n{bad}\nNPOIN= 5\n0 0 0\n1 0 1\n2 0 2\n1 1 3\n0 1 4\n"
+            "NMARK= 1\nMARKER_TAG= wall\nMARKER_ELEMS= 1\n3 0 1 \n")
+    _write(d + "mesh.su2", text)
+    cases.write_input(d + "input.dat", cases.ramp_input(nmax=1))
+    with pytest.raises(native.CsuError) as e:
+        native.HostCase(d)
+    assert e.value.code == native.CSU_ERR_IO
