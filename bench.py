@@ -263,18 +263,48 @@ def other_configs(log, n_plate: int = 500, n_naca: int = 500):
     return out
 
 
-def batch_mix(n: int, iters: int, slots: int, log, seed: int = 3):
-    """Many independent solves at once: n seeded meshes, cell counts
-    log-uniform in [10k, 400k], cycling Euler ramp / laminar plate / SA
-    NACA 0012, Mach U[0.2, 2.5], AoA U[-4, 10] deg (airfoils), Re log-U[1e5,
-    1e7] (viscous kinds, through the pressure), mixed precision (FP32 fluxes,
-    FP64 update), each run for `iters` iterations through the server with
-    `slots` solves on the GPU at a time.  Reports the aggregate
-    cell-iterations per second."""
+def batch_mix(reqs, iters: int, slots: int, log):
+    """The many-solve mix (batch_cases) through the server instead: every
+    solve its own context, `slots` of them on the GPU at a time, their
+    launches overlapping.  Reports the aggregate cell-iterations per
+    second."""
+    from csu_amd import native
+    n = len(reqs)
+    srv = native.Server(max_slots=slots, hbm_budget=250e9, chunk=50)
+    t0 = time.perf_counter()
+    for _, c in reqs:
+        srv.submit(c, drop=1e-300, max_iter=iters)
+    res = []
+    while len(res) < n:
+        srv.step(0.002)
+        res += srv.results()
+    span = time.perf_counter() - t0
+    srv.close()
+    ci = sum(r["iters"] * r["ncell"] for r in res)
+    by = {}
+    for (kind, c), r in zip(reqs, sorted(res, key=lambda r: r["id"])):
+        by.setdefault(kind, [0, 0])
+        by[kind][0] += c.ncell
+        by[kind][1] += 1
+    return {"solves": n, "cells": int(sum(c.ncell for _, c in reqs)), "iterations_each": iters, "slots": slots,
+            "precision": "mixed (FP32 gradients/limiter/edge states/convective flux, FP64 update)",
+            "seconds": span, "cell_iter_per_s": ci / span, "kinds": {k: {"solves": v[1], "cells": v[0]}
+                                                                   for k, v in by.items()},
+            "diverged": sum(r["status"] == native.CSU_ERR_DIVERGED for r in res),
+            "errors": sum(r["status"] not in (0, native.CSU_ERR_DIVERGED) for r in res)}
+
+
+def batch_cases(n: int, seed: int = 3):
+    """The many-solve mix: n seeded meshes, cell counts log-uniform in
+    [10k, 400k], cycling Euler ramp / laminar plate / SA NACA 0012, Mach
+    U[0.2, 2.5], AoA U[-4, 10] deg (airfoils), Re log-U[1e5, 1e7] (viscous
+    kinds, through the pressure), mixed precision (FP32 fluxes, FP64
+    update).  Ramps: Roe, RK5 at CFL 0.5; plates: AUSM, laminar, RK5 at CFL
+    0.5; airfoils: SA, AUSMDV, LU-SGS at CFL 20 on colour/RCM-renumbered
+    meshes."""
     from csu_amd import cases, native, renumber
     rng = np.random.default_rng(seed)
     reqs = []
-    t0 = time.perf_counter()
     for k in range(n):
         kind = ("ramp", "plate", "naca")[k % 3]
         ncell = float(np.exp(rng.uniform(np.log(1e4), np.log(4e5))))
@@ -300,29 +330,65 @@ def batch_mix(n: int, iters: int, slots: int, log, seed: int = 3):
                                               pout=f"{p:.6g}", precision=1))
             renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
         reqs.append((kind, native.HostCase(d, host_wall_distance=False)))
-    log(f"batch mix: {n} cases, {sum(c.ncell for _, c in reqs)} cells prepared in {time.perf_counter() - t0:.1f} s")
-    srv = native.Server(max_slots=slots, hbm_budget=250e9, chunk=50)
+    return reqs
+
+
+# Algorithmic bytes per cell-iteration of the many-solve mix's kinds in
+# mixed precision (FP32 gradients, limiter, edge states): the 5-stage
+# inviscid stage figure for ramps (1,854.5 B), the same for the plates (a
+# lower bound: the viscous face and gradient terms add to it), LU-SGS with
+# SA for the airfoils (~930 B).
+MIX_ALG_BYTES = {"ramp": 1854.5, "plate": 1854.5, "naca": 930.0}
+
+
+def batch_mix_batched(reqs, iters: int, log, tune: bool = True):
+    """The many-solve mix as batches: one concatenated context per kind of
+    solve (the kinds need different kernels), each one set of launches per
+    iteration over all of its solves, captured in graphs; the three run
+    concurrently on their own streams.  Reports the aggregate
+    cell-iterations per second, each batch's rate, the autotuner's choices
+    and the fraction of the HBM roofline on the kinds' algorithmic bytes."""
+    from csu_amd import native
+    groups = {}
+    for kind, c in reqs:
+        groups.setdefault(kind, []).append(c)
     t0 = time.perf_counter()
-    for _, c in reqs:
-        srv.submit(c, drop=1e-300, max_iter=iters)
-    res = []
-    while len(res) < n:
-        srv.step(0.002)
-        res += srv.results()
+    batches = {kind: native.Batch(cs) for kind, cs in groups.items()}
+    log(f"batched mix: {len(batches)} batches created in {time.perf_counter() - t0:.1f} s")
+    shapes = {kind: (b.autotune(3) if tune else b.launch_shape()) for kind, b in batches.items()}
+    for b in batches.values():      # first iteration and graph capture, outside the timed region
+        b.iterate(2, want=False)
+        b.sync()
+    per = {}
+    t0 = time.perf_counter()
+    for b in batches.values():
+        b.iterate_async(iters)
+    for kind, b in batches.items():
+        b.sync()
+        per[kind] = time.perf_counter() - t0
     span = time.perf_counter() - t0
-    srv.close()
-    ci = sum(r["iters"] * r["ncell"] for r in res)
-    by = {}
-    for (kind, c), r in zip(reqs, sorted(res, key=lambda r: r["id"])):
-        by.setdefault(kind, [0, 0])
-        by[kind][0] += c.ncell
-        by[kind][1] += 1
-    return {"solves": n, "cells": int(sum(c.ncell for _, c in reqs)), "iterations_each": iters, "slots": slots,
-            "precision": "mixed (FP32 gradients/limiter/edge states/convective flux, FP64 update)",
-            "seconds": span, "cell_iter_per_s": ci / span, "kinds": {k: {"solves": v[1], "cells": v[0]}
-                                                                   for k, v in by.items()},
-            "diverged": sum(r["status"] == native.CSU_ERR_DIVERGED for r in res),
-            "errors": sum(r["status"] not in (0, native.CSU_ERR_DIVERGED) for r in res)}
+    cells = {kind: sum(c.ncell for c in cs) for kind, cs in groups.items()}
+    # each batch alone, for the per-batch rates
+    alone = {}
+    for kind, b in batches.items():
+        ta = time.perf_counter()
+        b.iterate(iters, want=False)
+        b.sync()
+        alone[kind] = time.perf_counter() - ta
+    bytes_total = sum(cells[k] * iters * MIX_ALG_BYTES[k] for k in cells)
+    out = {"solves": len(reqs), "cells": int(sum(cells.values())), "iterations_each": iters,
+           "form": "one concatenated batch context per kind (csu_batch_create), graph-replayed iterations, "
+                   "the kinds' batches concurrent on their own streams",
+           "precision": "mixed (FP32 gradients/limiter/edge states/convective flux, FP64 update)",
+           "seconds": span, "cell_iter_per_s": sum(cells.values()) * iters / span,
+           "hbm_frac_alg": bytes_total / span / (HBM_PEAK_GBS * 1e9),
+           "batches": {k: {"solves": len(groups[k]), "cells": cells[k],
+                           "cell_iter_per_s_alone": cells[k] * iters / alone[k],
+                           "launch_shape": shapes[k]} for k in groups}}
+    for b in batches.values():
+        b.close()
+    log(f"batched mix: {out['cell_iter_per_s']:.3e} cell-iter/s over {out['cells']} cells")
+    return out
 
 
 def cpu_baseline(nx: int, ny: int, iters: int):
@@ -360,10 +426,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
-    ap.add_argument("--batch", type=int, default=0, metavar="N",
-                    help="also run the many-solve mix of N seeded cases (ramp / plate / SA airfoil, 10k-400k "
-                         "cells each) through the server and report its aggregate throughput")
+    ap.add_argument("--batch", type=int, default=96, metavar="N",
+                    help="the many-solve mix of N seeded cases (ramp / plate / SA airfoil, 10k-400k cells each): "
+                         "batched contexts, and through the server (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=200)
+    ap.add_argument("--no-batch-server", action="store_true",
+                    help="measure the many-solve mix in batched contexts only, not through the server")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the laminar-plate and NACA 0012 SA/SST measurements")
     ap.add_argument("--serve-n", type=int, default=24)
@@ -438,7 +506,18 @@ def main():
             "iteration_alg_bytes_per_cell": 2334.5,
             "iteration_frac": value * 2334.5 / (HBM_PEAK_GBS * 1e9)}
     others = None if a.no_configs else other_configs(log)
-    batch = batch_mix(a.batch, a.batch_iters, a.serve_slots, log) if a.batch else None
+    batch = None
+    if a.batch:
+        t0 = time.perf_counter()
+        reqs = batch_cases(a.batch)
+        log(f"batch mix: {a.batch} cases, {sum(c.ncell for _, c in reqs)} cells prepared in "
+            f"{time.perf_counter() - t0:.1f} s")
+        batch = batch_mix_batched(reqs, a.batch_iters, log)
+        if not a.no_batch_server:
+            batch["server_form"] = batch_mix(reqs, a.batch_iters, a.serve_slots, log)
+            log(f"batch mix through the server: {batch['server_form']['cell_iter_per_s']:.3e} cell-iter/s")
+        for _, c in reqs:
+            c.close()
     serve = None if a.no_serve else serving(a.serve_n, a.serve_rate, a.serve_slots, log)
     cpu = None if a.no_cpu else cpu_baseline(200, 100, a.cpu_iters)
     line = {

@@ -33,6 +33,8 @@ struct Req {
   int keep;
   double t_submit;
   double cfl_max = 0.0, cfl_growth = 1.0;  // CFL ramp: x growth per chunk up to cfl_max
+  int peak_ref = 0;          // drops measured from the largest residual so far (1) or the first (0)
+  double switch_drop = 0.0;  // mixed precision: FP64 from this drop on (0: never)
 };
 
 struct Job {
@@ -41,6 +43,7 @@ struct Job {
   csu_job_result res{};
   int chunk_n = 0;  // iterations of the chunk in flight
   bool res0_set = false;
+  double peak = 0.0;  // largest density residual so far
   double cfl = 0.0;  // current CFL of a ramped job
   int64_t est = 0;   // footprint estimate charged while the context is created
   int64_t charged = 0;  // device bytes of the context charged against the budget
@@ -149,6 +152,17 @@ extern "C" int csu_server_set_ramp(csu_server* s, int32_t id, double cfl_max, do
   return CSU_ERR_ARG;  // unknown, or already admitted
 }
 
+extern "C" int csu_server_set_options(csu_server* s, int32_t id, int32_t peak_ref, double switch_drop) {
+  if (!s || !(switch_drop >= 0.0)) return CSU_ERR_ARG;
+  for (Req& r : s->queue)
+    if (r.id == id) {
+      r.peak_ref = peak_ref ? 1 : 0;
+      r.switch_drop = switch_drop;
+      return CSU_OK;
+    }
+  return CSU_ERR_ARG;  // unknown, or already admitted
+}
+
 extern "C" int csu_server_pending(const csu_server* s, int32_t* queued, int32_t* active) {
   if (!s) return CSU_ERR_ARG;
   if (queued) *queued = (int32_t)s->queue.size();
@@ -197,6 +211,7 @@ static void admit(csu_server* s) {
     j->res.t_submit = j->r.t_submit;
     j->res.t_admit = now(s);
     j->res.iter_conv = -1;
+    j->res.iter_switch = -1;
     j->cfl = j->r.prm->cfl;
     j->est = est;
     s->used += est;
@@ -252,9 +267,12 @@ static bool harvest(csu_server* s, size_t k) {
       break;
     }
     if (!j->res0_set) { j->res.res0 = r0; j->res0_set = true; }
+    j->peak = std::max(j->peak, r0);
+    j->res.res_peak = j->peak;
     j->res.res_last = r0;
     memcpy(j->res.coef_last, &s->cbuf[(size_t)it * 4], sizeof(double) * 4);
-    if (j->res.iter_conv < 0 && r0 <= j->r.drop * j->res.res0) {
+    const double ref = j->r.peak_ref ? j->peak : j->res.res0;
+    if (j->res.iter_conv < 0 && r0 <= j->r.drop * ref) {
       j->res.iter_conv = j->res.iters;
       j->res.converged = 1;
       break;
@@ -262,6 +280,12 @@ static bool harvest(csu_server* s, size_t k) {
   }
   if (diverged) { retire(s, k, CSU_ERR_DIVERGED); return true; }
   if (j->res.converged || j->res.iters >= j->r.max_iter) { retire(s, k, CSU_OK); return true; }
+  if (j->r.switch_drop > 0.0 && j->res.iter_switch < 0 && j->r.prm->precision == CSU_PREC_MIXED &&
+      j->res.res_last <= j->r.switch_drop * (j->r.peak_ref ? j->peak : j->res.res0)) {
+    // mixed precision has done its part: FP64 to the converged state
+    if ((rc = csu_dev_set_precision(j->dev, CSU_PREC_FP64))) { retire(s, k, rc); return true; }
+    j->res.iter_switch = j->res.iters;
+  }
   if (j->r.cfl_growth > 1.0 && j->cfl < j->r.cfl_max) {
     j->cfl = std::min(j->cfl * j->r.cfl_growth, j->r.cfl_max);
     if ((rc = csu_dev_set_cfl(j->dev, j->cfl))) { retire(s, k, rc); return true; }

@@ -494,4 +494,15 @@ __device__ __forceinline__ T limiter_v2d(T d1max, T d1min, T d2, T ee) {
   return d2 == (T)0 ? (T)1 : ans;
 }
 
+// Barth-Jespersen limiter of one edge, the reference's limiterBJ
+// (solver.c:1132-1151, which its limiter loop never calls): the largest
+// phi <= 1 keeping the edge value within the neighbourhood's range.  d1max
+// = Umax - Ui, d1min = Umin - Ui; straight-line like limiter_v2d.
+template <typename T>
+__device__ __forceinline__ T limiter_bj(T d1max, T d1min, T d2) {
+  const T d1 = d2 > (T)0 ? d1max : d1min;
+  const T ans = fmin((T)1, d1 / d2);
+  return d2 == (T)0 ? (T)1 : ans;
+}
+
 }  // namespace csu

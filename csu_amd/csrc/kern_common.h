@@ -7,10 +7,20 @@
 namespace csu {
 
 #define CSU_BLOCK 256
+// A batch concatenates several meshes; every solve's cells, interior faces
+// and boundary faces start on a multiple of CSU_CHUNK elements, so every
+// chunk -- and every workgroup of up to CSU_CHUNK threads -- belongs to one
+// solve, and a solve's per-block reduction partials are those of its own
+// run alone.
+#define CSU_CHUNK 256
 
-// Scalar options and free-stream data, passed by value.
+// Scalar options and free-stream data, passed by value.  nc, nf, nb are the
+// array strides (the whole batch); cend, fend, bend the end of the
+// workgroup's solve (nc, nf, nb for a single solve).
 struct KP {
   int nc, nf, nb, nv, order, flux, laminar, sa, sst, limiter, turb1order, axi, visc_blazek;
+  int cend, fend, bend;
+  int xcd;        // 1: cell and face tiles mapped XCD-contiguously (xcd_tile); 0: in workgroup order
   int lsq_ld;     // 1: long double LSQ sums emulated per call; 0: precomputed weights
   int mixed;      // 1: FP32 gradients, limiter, reconstruction and convective flux
   int store_grad; // write dPx/dPy/phi even when no viscous term reads them
@@ -33,8 +43,36 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
+// Batch: solve of the workgroup whose first element is e0 (uniform across
+// the workgroup), from a per-chunk table.
+__device__ __forceinline__ int solve_of(const int* solk, int e0) { return solk[e0 / CSU_CHUNK]; }
+
+// One solve of a batch: its element ranges in the concatenated arrays and
+// what differs between solves (free stream, outlet pressure, wall
+// temperature, AUSM+up reference Mach, limiter volume scale, CFL, reference
+// length of the force coefficients).
+struct SolveP {
+  int c0, c1, f0, f1, b0, b1, w0, w1;  // [begin, end): cells, faces, boundary faces, wall-list entries
+  int active;                          // 0: the state is held (a converged or finished solve)
+  int pad_;
+  double cfl, sref, pout, twall, minf, volmax;
+  double pin[7];
+};
+
+// Batch: the parameters of solve q in the workgroup's copy of kp.
+__device__ __forceinline__ void solve_kp(KP& kp, const SolveP& q) {
+  kp.cend = q.c1; kp.fend = q.f1; kp.bend = q.b1;
+  kp.pout = q.pout; kp.twall = q.twall; kp.minf = q.minf; kp.volmax = q.volmax;
+#pragma unroll
+  for (int k = 0; k < 7; k++) kp.pin[k] = q.pin[k];
+}
+
 // Geometry (device pointers, read-only), passed by value.
 struct Geo {
+  // batch (sol non-null): per-solve table, the solve of every CSU_CHUNK
+  // cells / faces / boundary faces, and of every cell
+  const SolveP* sol;
+  const int *csolk, *fsolk, *bsolk, *csol;
   const int *cnp, *cnbr, *cface;                        // [nc], [4][nc], [4][nc]
   const double *xc, *yc, *omega, *dsxm, *dsym, *dslat;  // [nc]
   const double *edx, *edy;                              // [4][nc] own-edge midpoint offsets
@@ -45,7 +83,7 @@ struct Geo {
   const float *fdsx32, *fdsy32, *fds32;                 // mixed precision face vectors
   const double *wdist;                                  // [nc] wall distance (SA/SST)
   const int *fe0, *fe1;                                 // [nf]
-  const double *fdsx, *fdsy, *fds, *fdx0, *fdy0, *fdx1, *fdy1;
+  const double *fdsx, *fdsy, *fds;
   const int *be0, *btype, *bdonor;                      // [nb]
   const double *bdsx, *bdsy, *bds, *bxm, *bym, *bnx, *bny, *bds2;
 };
@@ -53,7 +91,8 @@ struct Geo {
 // Mutable per-stage arrays, passed by value.
 struct St {
   double *P, *Pb, *dPx, *dPy, *phi, *dTx, *dTy;
-  double *fflux, *vflux, *bvflux, *wfric, *pref2, *miT;
+  double *fflux, *vflux, *bvflux, *wfric, *miT;
+  double *pref2;        // limiter reference values [solves][6]
   double *src;          // SA: Qt*Omega [nc]; SST: Qtk*Omega, Qto*Omega [2][nc]
   double *dq;           // SST source Jacobians dQk/d(r, rk, ro), dQo/d(r, rk, ro) [6][nc]
   double *rec;          // second order: reconstructed state at each cell edge [4][NV][nc]

@@ -23,8 +23,9 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double w
   __shared__ double smin[CSU_BLOCK];
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc;
+  if (g.sol) kp.cend = g.sol[solve_of(g.csolk, blockIdx.x * blockDim.x)].c1;
   double val = INFINITY;
-  if (i < nc) {
+  if (i < kp.cend) {
     const double* P = s.P;
     double Lc = 0, Lv = 0;
     // all slots' face data, then all states, then the sums in slot order
@@ -106,9 +107,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double w
   if (t == 0) red_min[blockIdx.x] = smin[0];
 }
 
-// diagonal with the global-time-step term (implicit.c:176-180)
-__device__ __forceinline__ double diag(const double* Dp, const double* omega, const double* dtg, int i) {
-  return Dp[i] + omega[i] / dtg[0];
+// diagonal with the global-time-step term (implicit.c:176-180); dtg holds
+// one step per solve
+__device__ __forceinline__ double diag(const double* Dp, const double* omega, const double* dtg, int i, int sid) {
+  return Dp[i] + omega[i] / dtg[sid];
 }
 
 // implicitCalcDeltaFlux_sa (implicit.c:495-553): with the positivity clamps
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
 #pragma unroll
     for (int k = 0; k < NV; k++) acc[k] -= 0.5 * (dF[k] * dS - ra * d[k]);
   }
-  const double D = diag(Dp, g.omega, dtg, i);
+  const double D = diag(Dp, g.omega, dtg, i, g.sol ? g.csol[i] : 0);
   if (NV == 6) {
     // SST: k-omega block with the source Jacobians, Cramer's rule
     // (implicit.c:326-351, 447-472).  The reference divides by a long double
@@ -333,8 +335,13 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs_uinit(KP kp, Geo g, const d
                                                            double* __restrict__ dW1, const double* __restrict__ dq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = kp.nc;
-  if (i >= nc) return;
-  const double D = diag(Dp, g.omega, dtg, i);
+  int sid = 0;
+  if (g.sol) {
+    sid = solve_of(g.csolk, blockIdx.x * blockDim.x);
+    kp.cend = g.sol[sid].c1;
+  }
+  if (i >= kp.cend) return;
+  const double D = diag(Dp, g.omega, dtg, i, sid);
 #pragma unroll
   for (int k = 0; k < NV; k++) dW1[k * nc + i] = D * dW0[k * nc + i];
   if (NV == 6) {  // implicit.c:372-392: the k-omega block times dW0
@@ -359,7 +366,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
                                                       double* __restrict__ A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = kp.nc;
-  if (i >= nc) return;
+  if (g.sol) kp.cend = g.sol[solve_of(g.csolk, blockIdx.x * blockDim.x)].c1;
+  if (i >= kp.cend) return;
   const double h = 1.0e-6;
 #pragma unroll 1
   for (int j = 0; j < 4; j++) {
@@ -441,9 +449,18 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
                                                       int m, const double* __restrict__ dq) {
   __shared__ double smax[CSU_BLOCK];
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
-  const int nc = kp.nc, nblocks = gridDim.x;
+  const int nc = kp.nc, stride = gridDim.x;
+  // batch: the solve's own blocks and buffer state
+  int sid = 0, b0 = 0, nblocks = gridDim.x;
+  if (g.sol) {
+    sid = solve_of(g.csolk, blockIdx.x * blockDim.x);
+    const SolveP& q = g.sol[sid];
+    kp.cend = q.c1;
+    b0 = q.c0 / CSU_BLOCK;
+    nblocks = (q.c1 - q.c0 + CSU_BLOCK - 1) / CSU_BLOCK;
+  }
   if (m > 0) {
-    const double* prev = red_sw + (size_t)(m - 1) * nblocks;
+    const double* prev = red_sw + (size_t)(m - 1) * stride + b0;
     // NaN is fmax's identity: the maximum is NaN only when every cell's
     // change is, which the reference's loop (implicit.c:1023-1031) does not
     // take as converged
@@ -458,16 +475,16 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     const bool stopped = smax[0] < 1e-14;
     __syncthreads();  // smax is reused below
     if (stopped) {
-      if (t == 0) red_sw[(size_t)m * nblocks + blockIdx.x] = -1.0;
+      if (t == 0) red_sw[(size_t)m * stride + blockIdx.x] = -1.0;
       return;
     }
   }
-  const int par = (state[1] + m) & 1;
+  const int par = (state[2 * sid + 1] + m) & 1;
   const double* x = par ? buf1 : buf0;
   double* y = par ? buf0 : buf1;
   double diff = __builtin_nan("");  // lanes past the last cell: fmax's identity
-  if (i < nc) {
-    const double D = diag(Dp, g.omega, dtg, i);
+  if (i < kp.cend) {
+    const double D = diag(Dp, g.omega, dtg, i, sid);
     double xi[NV], acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) { xi[k] = x[k * nc + i]; acc[k] = D * xi[k]; }
@@ -538,31 +555,48 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
     __syncthreads();
   }
-  if (t == 0) red_sw[(size_t)m * nblocks + blockIdx.x] = smax[0];
+  if (t == 0) red_sw[(size_t)m * stride + blockIdx.x] = smax[0];
 }
 
-// Buffer of the DPLUR result: sweeps 0..k-1 ran (k = the first sweep marked
-// -1, or nlin), so the last one wrote buf[(state[1] + k) & 1].
-__device__ __forceinline__ int jac_result(const int* state, const double* red_sw, int nlin, int nblocks) {
+// Buffer of the DPLUR result of solve sid, whose first block is b0 (rows of
+// red_sw have length stride): sweeps 0..k-1 ran (k = the first sweep marked
+// -1, or nlin), so the last one wrote buf[(state[2 sid + 1] + k) & 1].
+__device__ __forceinline__ int jac_result(const int* state, const double* red_sw, int nlin, int stride, int sid,
+                                          int b0) {
   int k = 0;
-  while (k < nlin && !(red_sw[(size_t)k * nblocks] < 0.0)) k++;
-  return (state[1] + k) & 1;
+  while (k < nlin && !(red_sw[(size_t)k * stride + b0] < 0.0)) k++;
+  return (state[2 * sid + 1] + k) & 1;
 }
 
-// After the update: the result buffer becomes the next step's start.
-__global__ void k_jac_base(int* state, const double* __restrict__ red_sw, int nlin, int nblocks) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) state[1] = jac_result(state, red_sw, nlin, nblocks);
+// After the update: the result buffer becomes the next step's start (one
+// thread per solve).
+__global__ void k_jac_base(int* state, const double* __restrict__ red_sw, int nlin, int stride,
+                           const SolveP* __restrict__ sol, int nsolve) {
+  const int sid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sid >= nsolve) return;
+  const int b0 = sol ? sol[sid].c0 / CSU_BLOCK : 0;
+  state[2 * sid + 1] = jac_result(state, red_sw, nlin, stride, sid, b0);
 }
 
 // solverUpdateUImplicit (solver.c:1945-1955): U += dW; for DPLUR (red_sw
-// non-null) dW is the sweeps' result buffer.
+// non-null) dW is the sweeps' result buffer.  Batch: a solve that is not
+// active keeps its state.
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_upd_imp(int nc, double* __restrict__ U, const double* buf0,
                                                        const double* buf1, const int* __restrict__ state,
-                                                       const double* __restrict__ red_sw, int nlin, int nblocks) {
+                                                       const double* __restrict__ red_sw, int nlin, int stride,
+                                                       const SolveP* __restrict__ sol, const int* __restrict__ csolk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
-  const double* dW = (red_sw && jac_result(state, red_sw, nlin, nblocks)) ? buf1 : buf0;
+  int sid = 0, b0 = 0, cend = nc;
+  if (sol) {
+    sid = solve_of(csolk, blockIdx.x * blockDim.x);
+    const SolveP& q = sol[sid];
+    if (!q.active) return;
+    b0 = q.c0 / CSU_BLOCK;
+    cend = q.c1;
+  }
+  if (i >= cend) return;
+  const double* dW = (red_sw && jac_result(state, red_sw, nlin, stride, sid, b0)) ? buf1 : buf0;
 #pragma unroll
   for (int k = 0; k < NV; k++) U[k * nc + i] += dW[k * nc + i];
 }

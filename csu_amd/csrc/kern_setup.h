@@ -4,6 +4,13 @@
 
 namespace csu {
 
+// FP32 copy of an FP64 array (round to nearest, as a host conversion does).
+__global__ __launch_bounds__(CSU_BLOCK) void k_to_f32(const double* __restrict__ src, float* __restrict__ dst,
+                                                      size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
 // saCalcD (sa.c:703-763): distance from each cell's vertex-average centre to
 // the nearest wall / wallT segment (point-to-segment, parameter clamped to
 // [0, 1]).  Segments (p0x, p0y, p1x, p1y) are SoA [4][nw] and are staged

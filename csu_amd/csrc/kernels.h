@@ -23,11 +23,15 @@ enum { F_RK = 1, F_CLAMP_UN = 2, F_LAST = 4, F_NEXT_LOCAL = 8, F_STORE_R = 16, F
 
 // Standalone primitive pass (first stage after the state was set from the
 // host).  Clamped values are written back to U, as the reference does.
+// Batch: the cells of a held solve keep their state (a batch's padding
+// cells hold their solve's free stream and are converted like the rest).
 template <int NV>
-__global__ __launch_bounds__(CSU_BLOCK) void k_prim(KP kp, double* __restrict__ U, double* __restrict__ P) {
+__global__ __launch_bounds__(CSU_BLOCK) void k_prim(KP kp, double* __restrict__ U, double* __restrict__ P,
+                                                    const SolveP* __restrict__ sol, const int* __restrict__ csolk) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = kp.nc;
   if (i >= nc) return;
+  if (sol && !sol[solve_of(csolk, blockIdx.x * blockDim.x)].active) return;
   double u[NV], u0[NV], p[NV + 1];
 #pragma unroll
   for (int k = 0; k < NV; k++) u0[k] = u[k] = U[k * nc + i];
@@ -75,9 +79,15 @@ template <int NV, bool VISC, bool LD, bool MIX = false, bool PU = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
   using TF = typename std::conditional<MIX, float, double>::type;
   constexpr int NS = NV + 1;
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int e0 = (kp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x;
+  const int i = e0 + threadIdx.x;
   const int nc = kp.nc, nb = kp.nb;
-  if (i >= nc) return;
+  if (g.sol) {
+    const int sid = solve_of(g.csolk, e0);
+    solve_kp(kp, g.sol[sid]);
+    s.pref2 += 6 * sid;
+  }
+  if (i >= kp.cend) return;
   const int np = g.cnp[i];
   const double x0 = g.xc[i], y0 = g.yc[i];
   double Pi[NS];
@@ -260,7 +270,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
         d2[j] = tgx * ex[j] + tgy * ey[j];
-        TF p0 = limiter_v2d<TF>(d1max, d1min, d2[j], (TF)pref2);
+        TF p0 = kp.limiter == 2 ? limiter_bj<TF>(d1max, d1min, d2[j]) : limiter_v2d<TF>(d1max, d1min, d2[j], (TF)pref2);
         ph = (j == 0) ? p0 : fmin(ph, p0);
       }
       if (store) s.phi[kk * nc + i] = ph;
@@ -335,13 +345,20 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
 }
 
 // Limiter type 1 global range (limiter.c:64-94): per-block min/max partials.
+// Lanes past the solve's last cell take its first cell's values.
 template <int NV>
-__global__ __launch_bounds__(CSU_BLOCK) void k_lim1_partial(KP kp, const double* __restrict__ P,
+__global__ __launch_bounds__(CSU_BLOCK) void k_lim1_partial(KP kp, Geo g, const double* __restrict__ P,
                                                             double* __restrict__ red) {
   __shared__ double smin[NV][CSU_BLOCK], smax[NV][CSU_BLOCK];
   int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc;
-  int ii = i < nc ? i : 0;
+  int c0 = 0;
+  if (g.sol) {
+    const SolveP& q = g.sol[solve_of(g.csolk, blockIdx.x * blockDim.x)];
+    c0 = q.c0;
+    kp.cend = q.c1;
+  }
+  int ii = i < kp.cend ? i : c0;
 #pragma unroll
   for (int jj = 0; jj < NV; jj++) {
     int mm = jj == 4 ? 5 : jj;  // limiter.c:350-357 (slot 5 also for jj == 5)
@@ -364,16 +381,24 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_partial(KP kp, const double*
     }
 }
 
+// One workgroup per solve: the partials of its blocks (red rows of length
+// stride), its reference values into pref2 [solve][6].
 template <int NV>
-__global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, const double* __restrict__ red,
-                                                          double* __restrict__ pref2) {
+__global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, int stride, const SolveP* __restrict__ sol,
+                                                          const double* __restrict__ red, double* __restrict__ pref2) {
   __shared__ double smin[CSU_BLOCK], smax[CSU_BLOCK];
   int t = threadIdx.x;
+  if (sol) {
+    const SolveP& q = sol[blockIdx.x];
+    red += q.c0 / CSU_BLOCK;
+    nblocks = (q.c1 - q.c0 + CSU_BLOCK - 1) / CSU_BLOCK;
+    pref2 += 6 * blockIdx.x;
+  }
   for (int jj = 0; jj < NV; jj++) {
-    double mn = red[(2 * jj) * nblocks], mx = red[(2 * jj + 1) * nblocks];
+    double mn = red[(2 * jj) * stride], mx = red[(2 * jj + 1) * stride];
     for (int b = t; b < nblocks; b += CSU_BLOCK) {
-      mn = fmin(mn, red[(2 * jj) * nblocks + b]);
-      mx = fmax(mx, red[(2 * jj + 1) * nblocks + b]);
+      mn = fmin(mn, red[(2 * jj) * stride + b]);
+      mx = fmax(mx, red[(2 * jj + 1) * stride + b]);
     }
     smin[t] = mn; smax[t] = mx;
     __syncthreads();
@@ -441,9 +466,11 @@ __device__ __forceinline__ void conv_flux(const KP& kp, const Geo& g, const St& 
 // MIX: the convective flux in FP32 from FP32 edge states, stored in FP32.
 template <int NV, bool VISC, bool MIX = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
-  const int f = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int e0f = (kp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x;
+  const int f = e0f + threadIdx.x;
   const int nc = kp.nc, nf = kp.nf;
-  if (f >= nf) return;
+  if (g.sol) solve_kp(kp, g.sol[solve_of(g.fsolk, e0f)]);
+  if (f >= kp.fend) return;
   const int e0 = g.fe0[f], e1 = g.fe1[f];
   {
     const int sl = g.fsl[f];
@@ -514,7 +541,8 @@ template <int NV, bool PU = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_bcflux(KP kp, Geo g, St s) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = kp.nc, nb = kp.nb;
-  if (q >= nb) return;
+  if (g.sol) solve_kp(kp, g.sol[solve_of(g.bsolk, blockIdx.x * blockDim.x)]);
+  if (q >= kp.bend) return;
   const int i = g.be0[q];
   double Pi[NV + 1], R[NV], Pg[NV + 1];
   if (PU) {  // primitives of the stage state (see k_grad)
@@ -548,10 +576,19 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
                                                       double* __restrict__ Uout, double* __restrict__ Rout,
                                                       double* __restrict__ red_res, double* __restrict__ red_dt) {
   __shared__ double sred[NV + 1][CSU_BLOCK];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = kp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int t = threadIdx.x, i = tile * blockDim.x + t;
   const int nc = kp.nc, nf = kp.nf, nb = kp.nb;
-  const bool active = i < nc;
+  bool held = false;  // batch: a solve that is not active keeps its state
+  if (g.sol) {
+    const int sid = solve_of(g.csolk, tile * blockDim.x);
+    const SolveP& q = g.sol[sid];
+    solve_kp(kp, q);
+    cfl = q.cfl;
+    dtg += sid;
+    held = !q.active;
+  }
+  const bool active = i < kp.cend;
   double R[NV];
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) R[kk] = 0.0;
@@ -679,7 +716,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
 #pragma unroll
       for (int kk = 0; kk < NV; kk++) un0[kk] = un[kk] = Un[kk * nc + i];
       // a time-accurate run that has taken its last step holds U bitwise
-      const bool hold = s.tube && s.tube[2] > 1.5;
+      const bool hold = (s.tube && s.tube[2] > 1.5) || held;
       if ((flags & F_CLAMP_UN) && !hold) {
         // first stage: the reference clamps U^n in place (solverCalcR(U))
         prim_point<NV>(kp, un, p);
@@ -745,8 +782,13 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_dt(KP kp, Geo g, int local, doubl
   __shared__ double smin[CSU_BLOCK];
   int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc;
+  if (g.sol) {
+    const SolveP& q = g.sol[solve_of(g.csolk, blockIdx.x * blockDim.x)];
+    kp.cend = q.c1;
+    cfl = q.cfl;
+  }
   double val = INFINITY;
-  if (i < nc) {
+  if (i < kp.cend) {
     double u[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) u[k] = U[k * nc + i];
@@ -763,10 +805,19 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_dt(KP kp, Geo g, int local, doubl
   if (t == 0) red[blockIdx.x] = smin[0];
 }
 
+// Global time step: CFL x the minimum of the block partials.  Batch (sol
+// non-null): one workgroup per solve, over its own blocks, with its CFL.
 __global__ __launch_bounds__(CSU_BLOCK) void k_dt_final(int nblocks, double cfl, const double* __restrict__ red,
-                                                        double* __restrict__ dtglob) {
+                                                        double* __restrict__ dtglob, const SolveP* __restrict__ sol) {
   __shared__ double smin[CSU_BLOCK];
   int t = threadIdx.x;
+  if (sol) {
+    const SolveP& q = sol[blockIdx.x];
+    red += q.c0 / CSU_BLOCK;
+    nblocks = (q.c1 - q.c0 + CSU_BLOCK - 1) / CSU_BLOCK;
+    cfl = q.cfl;
+    dtglob += blockIdx.x;
+  }
   double m = INFINITY;
   for (int b = t; b < nblocks; b += CSU_BLOCK) m = fmin(m, red[b]);
   smin[t] = m;
@@ -807,10 +858,19 @@ __global__ void k_time_clamp(double* __restrict__ ts, double* __restrict__ dtg, 
 template <int NV>
 // res_hist slot = ctr % cap: the iteration counter lives on the device so
 // that a captured iteration graph writes each replay's norms to its own slot.
-__global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, const double* __restrict__ red,
+// Partial rows have length stride.  Batch (sol non-null): one workgroup per
+// solve, over its own blocks; history slot rows hold gridDim.x solves.
+__global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, int stride, const SolveP* __restrict__ sol,
+                                                           const double* __restrict__ red,
                                                            double* __restrict__ res_hist, int cap,
                                                            const int* __restrict__ ctr) {
-  double* res = res_hist + (size_t)(ctr[0] % cap) * NV;
+  double* res = res_hist + ((size_t)(ctr[0] % cap) * gridDim.x + blockIdx.x) * NV;
+  if (sol) {
+    const SolveP& q = sol[blockIdx.x];
+    red += q.c0 / CSU_BLOCK;
+    nc = q.c1 - q.c0;
+    nblocks = (nc + CSU_BLOCK - 1) / CSU_BLOCK;
+  }
   __shared__ double s[NV][RES_THREADS];
   const int t = threadIdx.x;
   double a[NV];
@@ -818,7 +878,7 @@ __global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, 
   for (int kk = 0; kk < NV; kk++) a[kk] = 0.0;
   for (int b = t; b < nblocks; b += RES_THREADS) {
 #pragma unroll
-    for (int kk = 0; kk < NV; kk++) a[kk] += red[kk * nblocks + b];
+    for (int kk = 0; kk < NV; kk++) a[kk] += red[kk * stride + b];
   }
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) s[kk][t] = a[kk];
@@ -835,12 +895,22 @@ __global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, 
 
 // Wall force coefficients Cx_p, Cx_v, Cy_p, Cy_v, reference solverCalcCoeff3
 // (solver.c:1342-1401); friction per face comes from k_grad.
+// Batch (sol non-null): one workgroup per solve over its wall-list range,
+// its free stream and reference length; the slot counter is then advanced
+// by k_next_slot (a workgroup cannot advance it while others read it).
 __global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const int* __restrict__ wall,
                                                       const double* __restrict__ Pb, const double* __restrict__ wfric,
                                                       const double* __restrict__ bdsx, const double* __restrict__ bdsy,
                                                       double sref, int visc, double* __restrict__ coef_hist, int cap,
-                                                      int* __restrict__ ctr) {
-  double* coef = coef_hist + (size_t)(ctr[0] % cap) * 4;
+                                                      int* __restrict__ ctr, const SolveP* __restrict__ sol) {
+  double* coef = coef_hist + ((size_t)(ctr[0] % cap) * gridDim.x + blockIdx.x) * 4;
+  if (sol) {
+    const SolveP& q = sol[blockIdx.x];
+    solve_kp(kp, q);
+    wall += q.w0;
+    nwall = q.w1 - q.w0;
+    sref = q.sref;
+  }
   __shared__ double sh[4][CSU_BLOCK];
   int t = threadIdx.x;
   const double r = kp.pin[0], u = kp.pin[1], v = kp.pin[2], Pinf = kp.pin[3];
@@ -867,8 +937,13 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const in
     double c[4] = {sh[0][0] / sref, sh[1][0] / sref, sh[2][0] / sref, sh[3][0] / sref};
     if (kp.axi) { c[0] *= 2 * M_PI; c[1] *= 2 * M_PI; }
     for (int k = 0; k < 4; k++) coef[k] = c[k];
-    ctr[0] = ctr[0] + 1;  // last kernel of an iteration: advance the history slot
+    if (!sol) ctr[0] = ctr[0] + 1;  // last kernel of an iteration: advance the history slot
   }
+}
+
+// Batch: advance the history slot after every solve's forces.
+__global__ void k_next_slot(int* __restrict__ ctr) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] = ctr[0] + 1;
 }
 
 // Per-cell SST fields of the solution2.csv writer, reference sstInterMiT

@@ -166,6 +166,17 @@ int csu_case_load(const char* wd, csu_case** out)
         p->sst_l = opt_d(in, "Lsst", 1.0);
     }
     p->precision = opt_i(in, "precision", CSU_PREC_FP64);
+    /* not a key of the reference: "gradient, GG" selects Green-Gauss
+       gradients instead of its least squares */
+    {
+        const char* gs = csu_input_get(in, "gradient");
+        if (!gs || strcmp(gs, "LSQ") == 0) p->gradient = 0;
+        else if (strcmp(gs, "GG") == 0) p->gradient = 1;
+        else {
+            snprintf(g_missing, sizeof g_missing, "gradient=%s", gs);
+            csu_case_free(c); return CSU_ERR_INPUT;
+        }
+    }
 
     /* saCalcD (sa.c:703) is not run here: csu_dev_create computes the wall
        distance on the device when mesh.wall_dist is NULL; csu_wall_distance

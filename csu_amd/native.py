@@ -41,7 +41,7 @@ class Params(C.Structure):
     _fields_ = ([(n, C.c_int32) for n in _PARAM_INTS] + [(n, C.c_double) for n in _PARAM_DBLS] +
                 [("pin", C.c_double * 7), ("uin", C.c_double * 6),
                  ("sst_kfactor", C.c_double), ("sst_ofactor", C.c_double),
-                 ("sst_owallfactor", C.c_double), ("sst_l", C.c_double)])
+                 ("sst_owallfactor", C.c_double), ("sst_l", C.c_double), ("gradient", C.c_int32)])
 
     def as_dict(self) -> dict:
         d = {n: getattr(self, n) for n in _PARAM_INTS + _PARAM_DBLS}
@@ -67,8 +67,9 @@ _lib = None
 class JobResult(C.Structure):
     _fields_ = [("id", C.c_int32), ("status", C.c_int32), ("converged", C.c_int32), ("iters", C.c_int32),
                 ("iter_conv", C.c_int32), ("ncell", C.c_int32), ("t_submit", C.c_double), ("t_admit", C.c_double),
-                ("t_done", C.c_double), ("res0", C.c_double), ("res_last", C.c_double),
-                ("coef_last", C.c_double * 4), ("bytes", C.c_int64)]
+                ("t_done", C.c_double), ("res0", C.c_double), ("res_last", C.c_double), ("res_peak", C.c_double),
+                ("coef_last", C.c_double * 4), ("bytes", C.c_int64), ("iter_switch", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 def lib():
@@ -129,6 +130,19 @@ def lib():
     L.csu_dev_bytes.restype = C.c_int64
     L.csu_dev_footprint.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.c_int32]
     L.csu_dev_footprint.restype = C.c_int64
+    L.csu_dev_set_precision.argtypes = [C.c_void_p, C.c_int32]
+    L.csu_dev_autotune.argtypes = [C.c_void_p, C.c_int32, i32p]
+    L.csu_dev_launch_shape.argtypes = [C.c_void_p, i32p]
+    L.csu_batch_create.argtypes = [C.c_int32, C.POINTER(C.POINTER(MeshDesc)), C.POINTER(C.POINTER(Params)),
+                                   C.POINTER(C.c_void_p)]
+    L.csu_batch_size.argtypes = [C.c_void_p]
+    L.csu_batch_ncell.argtypes = [C.c_void_p, C.c_int32]
+    L.csu_batch_set_state.argtypes = [C.c_void_p, C.c_int32, f64p]
+    L.csu_batch_get_state.argtypes = [C.c_void_p, C.c_int32, f64p]
+    L.csu_batch_set_cfl.argtypes = [C.c_void_p, C.c_int32, C.c_double]
+    L.csu_batch_set_active.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.csu_batch_run.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_int32, f64p, C.c_double, C.c_int32,
+                                C.c_double, C.c_void_p]
     L.csu_server_create.argtypes = [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
     L.csu_server_destroy.argtypes = [C.c_void_p]
     L.csu_server_destroy.restype = None
@@ -136,6 +150,7 @@ def lib():
                                     C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     L.csu_server_step.argtypes = [C.c_void_p, C.c_double]
     L.csu_server_set_ramp.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
+    L.csu_server_set_options.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double]
     L.csu_server_results.argtypes = [C.c_void_p, C.POINTER(JobResult), C.c_int32]
     L.csu_server_pending.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.csu_server_state.argtypes = [C.c_void_p, C.c_int32, f64p]
@@ -362,6 +377,23 @@ class Device:
         check(lib().csu_dev_wall_tensor(self._h, _p(out, C.c_double)))
         return out[:self.case.nbface]
 
+    def set_precision(self, precision: int):
+        """Arithmetic of the following iterations (0 FP64, 1 mixed, 2 FP64
+        with long double least squares)."""
+        check(lib().csu_dev_set_precision(self._h, precision))
+
+    def autotune(self, iters: int = 4) -> dict:
+        """Time the candidate launch shapes and keep the fastest (before the
+        first iteration); returns the choice."""
+        ch = np.zeros(3, np.int32)
+        check(lib().csu_dev_autotune(self._h, iters, _p(ch, C.c_int32)))
+        return {"block": int(ch[0]), "lusgs_block": int(ch[1]), "xcd_map": int(ch[2])}
+
+    def launch_shape(self) -> dict:
+        ch = np.zeros(3, np.int32)
+        tuned = lib().csu_dev_launch_shape(self._h, _p(ch, C.c_int32))
+        return {"block": int(ch[0]), "lusgs_block": int(ch[1]), "xcd_map": int(ch[2]), "tuned": bool(tuned == 1)}
+
     def set_graphs(self, on: bool):
         check(lib().csu_dev_set_graphs(self._h, 1 if on else 0))
 
@@ -390,6 +422,99 @@ class Device:
             pass
 
 
+class BatchResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("converged", C.c_int32), ("iters", C.c_int32), ("iter_conv", C.c_int32),
+                ("iters_run", C.c_int32), ("iter_switch", C.c_int32), ("res0", C.c_double), ("res_last", C.c_double),
+                ("res_peak", C.c_double), ("coef_last", C.c_double * 4), ("cfl", C.c_double)]
+
+
+class Batch:
+    """Several independent solves in one device context (csu_batch_*): one
+    set of kernel launches per iteration covers every solve.  The cases
+    (and the params objects) are kept alive by the batch."""
+
+    def __init__(self, cases: list, params: list | None = None):
+        L = lib()
+        self.cases = list(cases)
+        self.prms = list(params) if params is not None else [c.params for c in self.cases]
+        n = len(self.cases)
+        self.n = n
+        self.nvar = self.prms[0].nvar
+        self.ncells = [c.ncell for c in self.cases]
+        meshes = (C.POINTER(MeshDesc) * n)(*[C.pointer(c.mesh) for c in self.cases])
+        prms = (C.POINTER(Params) * n)(*[C.pointer(p) for p in self.prms])
+        self._h = C.c_void_p()
+        check(L.csu_batch_create(n, meshes, prms, C.byref(self._h)))
+
+    def set_state(self, q: int, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nvar, self.ncells[q])
+        check(lib().csu_batch_set_state(self._h, q, _p(U, C.c_double)))
+
+    def get_state(self, q: int) -> np.ndarray:
+        U = np.empty((self.nvar, self.ncells[q]))
+        check(lib().csu_batch_get_state(self._h, q, _p(U, C.c_double)))
+        return U
+
+    def iterate(self, n: int, want=True):
+        """n iterations of every solve: res [n][solves][nvar], coef [n][solves][4]."""
+        res = np.empty((n, self.n, self.nvar)) if want else None
+        coef = np.empty((n, self.n, 4)) if want else None
+        check(lib().csu_dev_iterate(self._h, n, _p(res, C.c_double) if want else None,
+                                    _p(coef, C.c_double) if want else None))
+        return res, coef
+
+    def iterate_async(self, n: int):
+        """Enqueue n iterations of every solve and return (csu_dev_iterate_async)."""
+        check(lib().csu_dev_iterate_async(self._h, n))
+
+    def set_cfl(self, q: int, cfl: float):
+        check(lib().csu_batch_set_cfl(self._h, q, cfl))
+
+    def set_active(self, q: int, on: bool):
+        check(lib().csu_batch_set_active(self._h, q, 1 if on else 0))
+
+    def run(self, drop: float = 1e-8, max_iter: int = 5000, chunk: int = 50, cfl_max=None, growth: float = 1.0,
+            peak_ref: bool = False, switch_drop: float = 0.0):
+        out = (BatchResult * self.n)()
+        cm = None if cfl_max is None else np.ascontiguousarray(cfl_max, dtype=np.float64)
+        check(lib().csu_batch_run(self._h, drop, max_iter, chunk, _p(cm, C.c_double) if cm is not None else None,
+                                  growth, 1 if peak_ref else 0, switch_drop, out))
+        return [{k: (list(getattr(r, k)) if k == "coef_last" else getattr(r, k)) for k, _ in BatchResult._fields_}
+                for r in out]
+
+    def sync(self):
+        check(lib().csu_dev_sync(self._h))
+
+    def set_profiling(self, on: bool):
+        check(lib().csu_dev_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        return Device.kernel_times(self)
+
+    def autotune(self, iters: int = 4) -> dict:
+        return Device.autotune(self, iters)
+
+    def set_precision(self, precision: int):
+        return Device.set_precision(self, precision)
+
+    def launch_shape(self) -> dict:
+        return Device.launch_shape(self)
+
+    def bytes(self) -> int:
+        return lib().csu_dev_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            lib().csu_dev_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Continuous-batching solve server (include/csu_serve.h).  Cases passed
     to submit() are kept alive by the server object until it is closed."""
@@ -400,8 +525,10 @@ class Server:
         self._keep = {}
 
     def submit(self, case: "HostCase", U0=None, drop: float = 1e-8, max_iter: int = 5000,
-               keep_state: bool = False, cfl_ramp=None) -> int:
-        """cfl_ramp: (cfl_max, growth per chunk) or None."""
+               keep_state: bool = False, cfl_ramp=None, peak_ref: bool = False, switch_drop: float = 0.0) -> int:
+        """cfl_ramp: (cfl_max, growth per chunk) or None; peak_ref: drops from
+        the largest residual so far; switch_drop: mixed precision until that
+        drop, FP64 after."""
         jid = C.c_int32()
         u = None
         if U0 is not None:
@@ -412,6 +539,8 @@ class Server:
         self._keep[jid.value] = (case, u)
         if cfl_ramp is not None:
             check(lib().csu_server_set_ramp(self._h, jid.value, float(cfl_ramp[0]), float(cfl_ramp[1])))
+        if peak_ref or switch_drop:
+            check(lib().csu_server_set_options(self._h, jid.value, 1 if peak_ref else 0, float(switch_drop)))
         return jid.value
 
     def step(self, max_wait: float = 0.01) -> int:

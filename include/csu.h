@@ -82,7 +82,9 @@ typedef struct csu_params {
     int32_t dt_local;      /* 1: local time step                          */
     int32_t dt_local_n;    /* iteration after which dt_local switches off */
     int32_t laminar, sa, sst, sst_trans, tp;
-    int32_t limiter;       /* 0 or 1 (limiter.c:45-94)                    */
+    int32_t limiter;       /* 0 or 1 (limiter.c:45-94); 2: Barth-Jespersen
+                              (limiterBJ, solver.c:1132, which the
+                              reference never calls): an extension      */
     int32_t turb1order, visc_blazek, nlinear;
     int32_t precision;     /* CSU_PREC_*                                  */
     int32_t sa_cc;         /* SA: the saCC.c form (density-gradient terms) */
@@ -92,6 +94,9 @@ typedef struct csu_params {
     double pin[7];         /* inlet primitive state r u v p T [nu~ | k om] */
     double uin[6];         /* inlet conservative state                    */
     double sst_kfactor, sst_ofactor, sst_owallfactor, sst_l;
+    int32_t gradient;      /* 0: least squares (solver.c:1031-1074);
+                              1: Green-Gauss (an extension: planar runs,
+                              precision 0 or 1)                          */
 } csu_params;
 
 typedef struct csu_dev csu_dev;
@@ -131,6 +136,14 @@ int  csu_dev_iterate(csu_dev* dev, int32_t niter, double* res, double* coef);
  * number, as the reference does when a run is restarted from that state
  * with a new CFL in input.dat (its way of ramping the CFL). */
 int  csu_dev_set_cfl(csu_dev* dev, double cfl);
+
+/* Arithmetic of the following iterations (CSU_PREC_*), e.g. mixed
+ * precision until the residual has dropped far enough and FP64 from there
+ * on (mixed precision's FP32 fluxes level the residual off near a 1e-6
+ * drop; FP64 continues to the FP64 steady state).  The state carries over;
+ * works on a batch too (all its solves).  No counterpart in the
+ * reference, which runs in FP64 throughout. */
+int  csu_dev_set_precision(csu_dev* dev, int32_t precision);
 
 /* solverCalcPrimitive(solver, solver->U) (solver.c:1153-1218): the
  * positivity clamps applied to the device state in place, and the boundary
@@ -200,6 +213,77 @@ int  csu_dev_surf_faces(csu_dev* dev, double* out);
  * several host threads) race. */
 #define CSU_SURF_NOALLOC_MAX_NB 65536
 int  csu_dev_surf_faces_noalloc(csu_dev* dev, double* out);
+
+/* ---- Batches: many independent steady solves in one context ----------
+ * No counterpart in the reference, which runs one solve per process
+ * (solverSolve, solver.c:1836-1943); each solve of a batch follows that
+ * loop's semantics and gives, bit for bit, the results it gives alone.
+ * The meshes are concatenated into one set of arrays (each solve's cells,
+ * faces and boundary faces starting on a multiple of 256), so one set of
+ * kernel launches -- captured in one graph per iteration -- covers every
+ * solve.  Per solve: free stream, outlet pressure, wall temperature,
+ * AUSM+up reference Mach, limiter reference values, CFL number, reference
+ * length, and every reduction (residual norm, forces, global time step,
+ * limiter range, DPLUR stop test).  Options that select the kernels
+ * (model, flux, order, limiter, time scheme, stages, precision, gas, ...)
+ * must be the same for all solves (CSU_ERR_ARG otherwise): group unlike
+ * solves into separate batches.  csu_dev_iterate, csu_dev_iterate_async,
+ * csu_dev_async_result, csu_dev_sync, csu_dev_set_graphs and the profiling
+ * calls work on a batch; their histories then hold [niter][n][nvar] norms
+ * and [niter][n][4] forces.  The whole-mesh calls (set/get_state,
+ * residual, primitives, set_cfl, advance_time, wall tensor, SST fields,
+ * surface data) refuse a batch; a tube (time-accurate) run is not batched. */
+int  csu_batch_create(int32_t n, const csu_mesh_desc* const* meshes, const csu_params* const* prms,
+                      csu_dev** out);
+int  csu_batch_size(const csu_dev* dev);                    /* solves, or -1 */
+int  csu_batch_ncell(const csu_dev* dev, int32_t solve);    /* cells of a solve, or -1 */
+/* State of one solve, SoA [nvar][ncell of the solve] FP64. */
+int  csu_batch_set_state(csu_dev* dev, int32_t solve, const double* U);
+int  csu_batch_get_state(csu_dev* dev, int32_t solve, double* U);
+/* CFL number of one solve, as csu_dev_set_cfl for a single solve. */
+int  csu_batch_set_cfl(csu_dev* dev, int32_t solve, double cfl);
+/* Hold (on = 0) or resume a solve: a held solve's state stays as it is
+ * while the others iterate (its history rows are not meaningful). */
+int  csu_batch_set_active(csu_dev* dev, int32_t solve, int32_t on);
+
+typedef struct csu_batch_result {
+    int32_t status;        /* 0, CSU_ERR_DIVERGED, or -1: held by the caller */
+    int32_t converged;     /* reached the drop                            */
+    int32_t iters;         /* iterations up to the stopping one           */
+    int32_t iter_conv;     /* iteration that reached the drop, or -1      */
+    int32_t iters_run;     /* iterations run (whole chunks)               */
+    int32_t iter_switch;   /* mixed precision: iterations before FP64, or -1 */
+    double res0, res_last; /* density residual of the first / stopping iteration */
+    double res_peak;       /* largest density residual                    */
+    double coef_last[4];   /* Cx_p Cx_v Cy_p Cy_v of the stopping iteration */
+    double cfl;            /* CFL number at the end                       */
+} csu_batch_result;
+
+/* Run every active solve until its density residual (solverCalcRes,
+ * solver.c:867-892) has dropped by `drop` from its first iteration
+ * (peak_ref 0) or from its largest so far (peak_ref 1), is no longer
+ * finite, or max_iter iterations are done, `chunk` iterations at a time; a
+ * solve is held from the end of the chunk in which it stopped, so its
+ * state is that of its run alone after iters_run iterations.  With
+ * growth > 1 each running solve's CFL is multiplied by growth after every
+ * chunk, up to cfl_max[solve] (cfl_max NULL: no ramp).  A mixed-precision
+ * batch with switch_drop > 0 continues in FP64 once every running solve
+ * has reached that drop.  out: [n]. */
+int  csu_batch_run(csu_dev* dev, double drop, int32_t max_iter, int32_t chunk,
+                   const double* cfl_max, double growth, int32_t peak_ref, double switch_drop,
+                   csu_batch_result* out);
+
+/* Launch-shape autotuner (no counterpart in the reference).  Before the
+ * first iteration, time every candidate -- workgroup size of the gradient
+ * and face kernels (64, 128, 256), of the LU-SGS level launches (64, 128,
+ * 256), XCD-contiguous or plain tile order -- over `iters` iterations and
+ * keep the fastest; the context then continues from the state it had, and
+ * later contexts of the same kind of run and mesh-size bucket (log2 of the
+ * cell count) in the process take the same choice without timing.
+ * Results do not depend on these shapes.  choice: [3] or NULL.
+ * csu_dev_launch_shape reports the shapes in use (returns 1 if tuned). */
+int  csu_dev_autotune(csu_dev* dev, int32_t iters, int32_t* choice);
+int  csu_dev_launch_shape(const csu_dev* dev, int32_t* choice);
 
 /* Iterations done so far; local time step [ncell] of the last iteration. */
 int  csu_dev_iteration(const csu_dev* dev);

@@ -31,8 +31,11 @@ typedef struct {
     int32_t ncell;
     double t_submit, t_admit, t_done;   /* seconds on the server clock             */
     double res0, res_last; /* density residual of the first and last iteration     */
+    double res_peak;       /* largest density residual                             */
     double coef_last[4];   /* Cx_p Cx_v Cy_p Cy_v of the last iteration            */
     int64_t bytes;         /* device bytes the solve held                          */
+    int32_t iter_switch;   /* iterations run in mixed precision before FP64, or -1 */
+    int32_t pad_;
 } csu_job_result;
 
 /* max_slots: solves in flight at once; hbm_budget: device bytes the
@@ -58,6 +61,15 @@ int  csu_server_step(csu_server* s, double max_wait);
  * multiplied by growth (>= 1) up to cfl_max (csu_dev_set_cfl), the
  * device-side form of the reference's chain of restarts at rising CFL. */
 int  csu_server_set_ramp(csu_server* s, int32_t id, double cfl_max, double growth);
+
+/* Options of a job still in the queue.  peak_ref 1: residual drops are
+ * measured from the largest density residual so far instead of the first
+ * one (a solve started from a uniform free stream past a wall or an airfoil
+ * has a first density residual near rounding level, which makes drops from
+ * it meaningless).  switch_drop > 0 (a mixed-precision job): once the drop
+ * reaches it, the job continues in FP64 (csu_dev_set_precision), so it
+ * converges to the FP64 steady state. */
+int  csu_server_set_options(csu_server* s, int32_t id, int32_t peak_ref, double switch_drop);
 
 /* Move up to max finished results into out; returns the count. */
 int  csu_server_results(csu_server* s, csu_job_result* out, int32_t max);

@@ -544,13 +544,48 @@ static void grad_lsq(const csu_oracle* o, int i, int m, double* dUx, double* dUy
     *dUy = (-b * A + a * B) / (a * c - b * b);
 }
 
+/* Green-Gauss gradient (gradient 1; no counterpart in the reference, which
+   has least squares only): sum over the cell's faces of (u_f - u_i) S_f /
+   Omega, u_f the average of the two sides (ghost state on boundary faces),
+   S_f the face vector out of the cell; the face vectors of a closed cell
+   sum to zero, so this equals sum u_f S_f / Omega.  Planar runs only. */
+static void grad_gg(const csu_oracle* o, int i, int m, double* dUx, double* dUy)
+{
+    const int n = nvert(o, i), ns = o->nv + 1;
+    const double ui = o->P[(size_t)i * ns + m];
+    double ax = 0.0, ay = 0.0;
+    for (int j = 0; j < n; j++) {
+        const int nb = o->nbr[4 * i + j];
+        double sx, sy;
+        if (nb >= 0) {
+            const int f = o->cface[4 * i + j], fi = f > 0 ? f - 1 : -f - 1;
+            face_ds(o, o->face[4 * fi + 2], o->face[4 * fi + 3], &sx, &sy);
+            if (f < 0) { sx = -sx; sy = -sy; }
+        } else {
+            face_ds(o, o->bface[3 * (-1 - nb) + 1], o->bface[3 * (-1 - nb) + 2], &sx, &sy);
+        }
+        const double wx = 0.5 * sx / o->omega[i], wy = 0.5 * sy / o->omega[i];
+        const double du = nbr_P(o, i, j, m) - ui;
+        if (j == 0) { ax = wx * du; ay = wy * du; }
+        else { ax += wx * du; ay += wy * du; }
+    }
+    *dUx = ax;
+    *dUy = ay;
+}
+
+static void grad_any(const csu_oracle* o, int i, int m, double* dUx, double* dUy)
+{
+    if (o->prm.gradient == 1) grad_gg(o, i, m, dUx, dUy);
+    else grad_lsq(o, i, m, dUx, dUy);
+}
+
 static void update_grad(csu_oracle* o)
 {
 #pragma omp parallel for
     for (int i = 0; i < o->nc; i++)
         for (int k = 0; k < o->nv; k++) {
             double gx, gy;
-            grad_lsq(o, i, k > 3 ? k + 1 : k, &gx, &gy);
+            grad_any(o, i, k > 3 ? k + 1 : k, &gx, &gy);
             o->dPx[(size_t)k * o->nc + i] = gx;
             o->dPy[(size_t)k * o->nc + i] = gy;
         }
@@ -571,6 +606,15 @@ static double limiter_v2(double Ui, double Umin, double Umax, double d2, double 
     return ans / d2;
 }
 
+/* Barth-Jespersen, limiterBJ (solver.c:1132-1151): defined in the
+   reference but never called by its limiter loop; limiter 2 here. */
+static double limiter_bj(double Ui, double Umin, double Umax, double d2)
+{
+    if (d2 == 0) return 1;
+    if (d2 > 0) return fmin(1, (Umax - Ui) / d2);
+    return fmin(1, (Umin - Ui) / d2);
+}
+
 static void limiter_update(csu_oracle* o)
 {
     const int ns = o->nv + 1;
@@ -583,7 +627,7 @@ static void limiter_update(csu_oracle* o)
         o->Pref20[2] = o->Pref20[1];
         o->Pref20[3] = Pin[3] * Pin[3];
         for (int k = 4; k < o->nv; k++) o->Pref20[k] = Pin[k + 1] * Pin[k + 1];
-    } else {
+    } else if (o->prm.limiter == 1) {
         double mn[6], mx[6];
         for (int j = 0; j < o->nv; j++) {
             int m = j == 4 ? 5 : j;
@@ -632,7 +676,7 @@ static void inter(csu_oracle* o)
                     double xm = (o->xy[2 * p0] + o->xy[2 * p1]) * 0.5;
                     double ym = (o->xy[2 * p0 + 1] + o->xy[2 * p1 + 1]) * 0.5;
                     double d2 = o->dPx[(size_t)k * nc + i] * (xm - x0) + o->dPy[(size_t)k * nc + i] * (ym - y0);
-                    double p = limiter_v2(Pi, mn, mx, d2, pref2[k]);
+                    double p = o->prm.limiter == 2 ? limiter_bj(Pi, mn, mx, d2) : limiter_v2(Pi, mn, mx, d2, pref2[k]);
                     ph = j == 0 ? p : fmin(ph, p);
                 }
                 o->phi[(size_t)k * nc + i] = ph;
@@ -737,7 +781,7 @@ static void grad_T(csu_oracle* o)
 #pragma omp parallel for
     for (int i = 0; i < o->nc; i++) {
         double gx, gy;
-        grad_lsq(o, i, 4, &gx, &gy);
+        grad_any(o, i, 4, &gx, &gy);
         o->dPx[3 * (size_t)o->nc + i] = gx;
         o->dPy[3 * (size_t)o->nc + i] = gy;
     }
@@ -2073,6 +2117,8 @@ int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
        for 5 variables) */
     if (p->sst && p->order == 2 && p->limiter == 1)
         return CSU_ERR_UNSUPPORTED;
+    /* Green-Gauss: planar runs only */
+    if (p->gradient == 1 && m->axi == 1) return CSU_ERR_UNSUPPORTED;
     csu_oracle* o = calloc(1, sizeof(*o));
     o->nc = m->ncell; o->nf = m->nface; o->nb = m->nbface; o->nv = p->nvar; o->nn = m->nnode; o->axi = m->axi;
     o->prm = *p; o->dt_local = p->dt_local;

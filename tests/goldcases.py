@@ -21,7 +21,7 @@ SST = ["plate_sst_ausm", "plate_sst_ausmdv_wallT", "plate_sst_pup_axi", "plate_s
 PLATE = PLATE + SA + SST + NACA
 ALL = RAMP + PLATE
 # tube mode (time-accurate runs to tmax)
-TUBE = ["tube_roe_o1", "tube_ausmdv_o2", "tube_ausmdv_o2_barth", "tube_ausm_quad", "tube_pup_moving",
+TUBE = ["tube_roe_o1", "tube_ausmdv_o2", "tube_ausmdv_o2_lim1", "tube_ausm_quad", "tube_pup_moving",
         "tube_roe_tp"]
 
 

@@ -93,7 +93,7 @@ PLATE = [
 TUBE = [
     ("tube_roe_o1", dict(nx=100, ny=8), {}),
     ("tube_ausmdv_o2", dict(nx=250, ny=5), {"flux": "AUSMDV", "order": 2}),
-    ("tube_ausmdv_o2_barth", dict(nx=250, ny=5), {"flux": "AUSMDV", "order": 2, "limiter": 1}),
+    ("tube_ausmdv_o2_lim1", dict(nx=250, ny=5), {"flux": "AUSMDV", "order": 2, "limiter": 1}),
     ("tube_ausm_quad", dict(nx=80, ny=4, split=False), {"flux": "AUSM", "stages": 3}),
     ("tube_pup_moving", dict(nx=100, ny=6), {"flux": "AUSMpup", "stages": 4, "mach1": 0.3}),
     ("tube_roe_tp", dict(nx=100, ny=4), {"TP": 1, "tmax": 0.01}),

@@ -1,0 +1,48 @@
+"""Launch-shape autotuner (GPU): a tuned context continues from the state it
+had and gives, bit for bit, the results of an untuned one; every shape the
+tuner may pick gives the same results; later contexts of the bucket take
+the choice."""
+import numpy as np
+import pytest
+
+from csu_amd import cases, native, renumber
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(tmpcase, kind):
+    d = tmpcase("at")
+    if kind == "ramp":
+        cases.ramp_case(d, nx=60, ny=30, threads=1, nmax=1)
+    elif kind == "lusgs":
+        cases.write_case(d, cases.naca_mesh(ni=64, nwall=8, nfar=10, seed=1),
+                         cases.naca_input(threads=1, nmax=1, aoa=3.0, CFL="5"))
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+    else:
+        cases.write_case(d, cases.plate_mesh(nx=40, ny=16, y1=2e-3, growth=1.15, seed=2),
+                         cases.plate_input(threads=1, nmax=1, timeScheme="DPLUR", CFL="20"))
+    return native.HostCase(d, host_wall_distance=False)
+
+
+@pytest.mark.parametrize("kind", ["ramp", "lusgs", "dplur"])
+def test_tuned_equals_untuned(tmpcase, kind):
+    c = _case(tmpcase, kind)
+    ref = native.Device(c)
+    r0, k0 = ref.iterate(12)
+    U0 = ref.get_state()
+    ref.close()
+    dev = native.Device(c)
+    ch = dev.autotune(3)
+    print(kind, "choice", ch, flush=True)
+    assert ch["block"] in (64, 128, 256) and ch["lusgs_block"] in (64, 128, 256) and ch["xcd_map"] in (0, 1)
+    assert dev.launch_shape()["tuned"]
+    r1, k1 = dev.iterate(12)
+    assert np.array_equal(r0, r1) and np.array_equal(k0, k1)
+    assert np.array_equal(dev.get_state(), U0)
+    dev.close()
+    # a later context of the same bucket takes the choice without timing
+    dev2 = native.Device(c)
+    sh = dev2.launch_shape()
+    assert sh["tuned"] and (sh["block"], sh["lusgs_block"], sh["xcd_map"]) == \
+        (ch["block"], ch["lusgs_block"], ch["xcd_map"])
+    dev2.close()
