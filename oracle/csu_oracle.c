@@ -2285,6 +2285,45 @@ int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
     return CSU_OK;
 }
 
+/* Test hook, no reference counterpart: the DPLUR system of one implicit
+   step from the current state (solverCalcR, implicitCalcD, the blocks) and
+   the sweeps' result.  D [nc] diagonal with the time-step term; A
+   [nc][4][nv*nv] neighbour blocks (row-major, y_i += A x_j); nb [nc][4]
+   neighbour of each slot (-1: none); nrm [nc][4][3] its outward unit
+   normal and length; dq [6][nc] the SST source Jacobians dQk/d(r,rk,ro),
+   dQo/d(r,rk,ro) (zeros otherwise); R [nv][nc]; dW [nv][nc] after the
+   sweeps.  The state is not advanced (only clamped, as calc_R does). */
+int oracle_dplur_system(csu_oracle* o, double* D, double* A, int* nb, double* nrm, double* dq, double* R,
+                        double* dW)
+{
+    if (o->prm.time_scheme != CSU_TIME_DPLUR) return CSU_ERR_ARG;
+    const int nc = o->nc, nv = o->nv;
+    calc_R(o, o->U);
+    dplur(o);
+    memcpy(D, o->D, sizeof(double) * nc);
+    memcpy(A, o->A, sizeof(double) * (size_t)nc * 4 * nv * nv);
+    memcpy(R, o->R, sizeof(double) * (size_t)nv * nc);
+    memcpy(dW, o->dW1, sizeof(double) * (size_t)nv * nc);
+    for (int i = 0; i < nc; i++)
+        for (int s = 0; s < 4; s++) {
+            double dSx = 0, dSy = 0;
+            int j = face_from(o, i, s, &dSx, &dSy);
+            nb[4 * i + s] = j;
+            double dS = j >= 0 ? sqrt(dSx * dSx + dSy * dSy) : 0.0;
+            double* q = nrm + 3 * (4 * (size_t)i + s);
+            q[0] = j >= 0 ? dSx / dS : 0.0;
+            q[1] = j >= 0 ? dSy / dS : 0.0;
+            q[2] = dS;
+        }
+    for (int k = 0; k < 6 * nc; k++) dq[k] = 0.0;
+    if (nv == 6)
+        for (int i = 0; i < nc; i++) {
+            dq[i] = o->dQkdr[i]; dq[nc + i] = o->dQkdrk[i]; dq[2 * nc + i] = o->dQkdro[i];
+            dq[3 * nc + i] = o->dQodr[i]; dq[4 * nc + i] = o->dQodrk[i]; dq[5 * nc + i] = o->dQodro[i];
+        }
+    return CSU_OK;
+}
+
 /* CFL change between iterations: the next solverCalcDt / implicitCalcD uses it */
 int oracle_set_cfl(csu_oracle* o, double cfl)
 {

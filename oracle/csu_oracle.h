@@ -34,5 +34,9 @@ int  oracle_sst_fields(const csu_oracle* o, double* out);
 int  oracle_surf_faces(const csu_oracle* o, double* out);
 int  oracle_wall_tensor(const csu_oracle* o, double* out);   /* [nbface][4] txx txy tyy yp */
 int  oracle_set_threads(int n);
+/* DPLUR test hook: D [nc], A [nc][4][nv*nv], nb [nc][4], nrm [nc][4][3],
+   dq [6][nc], R [nv][nc], dW [nv][nc] of one implicit step */
+int  oracle_dplur_system(csu_oracle* o, double* D, double* A, int* nb, double* nrm, double* dq, double* R,
+                         double* dW);
 
 #endif

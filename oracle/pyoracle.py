@@ -95,6 +95,21 @@ class Oracle:
         out["Pb"] = out["Pb"][:nb]
         return out
 
+    def dplur_system(self) -> dict:
+        """Test hook: the DPLUR system of one implicit step from the current
+        state (D, neighbour blocks A, neighbours, normals, SST source
+        Jacobians, R) and the sweeps' result dW; the state is not advanced."""
+        nv, nc = self.nvar, self.ncell
+        out = {"D": np.empty(nc), "A": np.empty((nc, 4, nv, nv)), "nb": np.empty((nc, 4), np.int32),
+               "nrm": np.empty((nc, 4, 3)), "dq": np.empty((6, nc)), "R": np.empty((nv, nc)),
+               "dW": np.empty((nv, nc))}
+        rc = lib().oracle_dplur_system(self._h, _p(out["D"]), _p(out["A"]),
+                                       out["nb"].ctypes.data_as(C.POINTER(C.c_int32)), _p(out["nrm"]),
+                                       _p(out["dq"]), _p(out["R"]), _p(out["dW"]))
+        if rc != 0:
+            raise RuntimeError(f"oracle_dplur_system failed ({rc})")
+        return out
+
     def iterate(self, n: int):
         res = np.empty((n, self.nvar))
         coef = np.empty((n, 4))

@@ -230,6 +230,39 @@ def make_solution2():
         print("wrote", name, S.shape)
 
 
+# solution.csv of the reference executable (solverWriteSolution: cell to
+# node inverse-distance averages of U, velocity zeroed on viscous walls)
+# after n iterations: (name, kind, mesh kwargs, input overrides, n)
+SOLUTION = [
+    ("solution_ramp", "ramp", dict(nx=12, ny=6), {}, 30),
+    ("solution_plate_lam", "plate", dict(_PL, jitter=0.1), {}, 60),
+    ("solution_plate_sa_wallT", "plate", dict(_PL, jitter=0.1), {"laminar": "0", "sa": "1", "BC:plate": "wallT"}, 40),
+    ("solution_tube", "tube", dict(nx=100, ny=8), {}, 0),
+]
+
+
+def read_solution(path):
+    lines = [ln for ln in open(path).read().splitlines() if ln.strip()]
+    return lines[0], np.array([[float(v) for v in ln.strip().rstrip(",").split(",")] for ln in lines[1:]])
+
+
+def make_solution():
+    import subprocess
+    for name, kind, mk, inp, n in SOLUTION:
+        d = case_dir(name)
+        if kind == "ramp":
+            cases.write_case(d, cases.ramp_mesh(seed=0, **mk), cases.ramp_input(threads=1, nmax=n, **inp))
+        elif kind == "plate":
+            cases.write_case(d, cases.plate_mesh(**mk), cases.plate_input(threads=1, nmax=n, **inp))
+        else:
+            cases.write_case(d, cases.tube_mesh(**mk), cases.tube_input(threads=1, **inp))
+        subprocess.run([po.REF_EXE, d], capture_output=True, check=False, timeout=600)
+        head, S = read_solution(d + "solution.csv")
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kind=np.array(kind), mesh_kwargs=np.array(repr(mk)),
+                            inputs=np.array(repr(inp)), niter=n, header=np.array(head), solution=S)
+        print("wrote", name, S.shape)
+
+
 # surfData.csv of SA and SST runs (the reference executable's; writeSurf
 # names the marker).  The airfoil marker is closed, so its first node has
 # no faces in the reference's node numbering (its row is NaN).  A marker
@@ -327,6 +360,7 @@ if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref is not built (needs the reference sources): make -C oracle ref")
     steps = {"small": make_small, "config0": make_config0, "friction": make_friction, "tube": make_tube,
-             "solution2": make_solution2, "cfl_chain": make_cfl_chain, "surf": make_surf}
+             "solution2": make_solution2, "cfl_chain": make_cfl_chain, "surf": make_surf,
+             "solution": make_solution}
     for key in sys.argv[1:] or steps:   # default: every fixture set
         steps[key]()

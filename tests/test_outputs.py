@@ -183,3 +183,36 @@ def test_surf_marker_not_a_chain(tmpcase):
     mk = [k for k in range(c.nmark) if a["mark_type"][k] == 2]
     ne = int(np.sum(a["bface_mark"] == mk[0])) if mk else 0
     assert head.startswith("x,y,rho,u,v,p,T,n,") and S.shape == (ne + 1, 15)
+
+
+@pytest.mark.parametrize("name", ["solution_ramp", "solution_plate_lam", "solution_plate_sa_wallT", "solution_tube"])
+def test_solution_vs_reference(tmpcase, name):
+    """solution.csv (solverWriteSolution, solver.c:222-325: node values as
+    inverse-distance averages of the cell states, velocity zeroed on the
+    nodes of viscous walls) against the reference executable's file after
+    the same iterations: the host writer applied to the oracle's state (the
+    state the reference writes, unclamped), to the printed precision."""
+    g = goldcases.load(name)
+    d = write_case(g, tmpcase("s1"))
+    c = native.HostCase(d)
+    o = po.Oracle(c)
+    o.set_state(c.initial_state())
+    if c.tube:
+        o.advance_time(c.tmax)
+    else:
+        o.iterate(int(g["niter"]))
+    c.write_solution(o.get_state(), d + "solution.csv")
+    lines = [ln for ln in open(d + "solution.csv").read().splitlines() if ln.strip()]
+    assert lines[0] == str(g["header"])
+    S = np.array([[float(v) for v in ln.strip().rstrip(",").split(",")] for ln in lines[1:]])
+    ref = g["solution"]
+    assert S.shape == ref.shape
+    scale = np.maximum(np.abs(ref).max(axis=0), 1e-300)
+    print(name, "max rel", float(np.max(np.abs(S - ref) / scale)), flush=True)
+    assert np.max(np.abs(S - ref) / scale) <= 1e-10
+    # the velocity of every viscous-wall node is exactly zero
+    if c.params.laminar or c.params.sa:
+        a = c.arrays()
+        walls = [g2 for g2, m in enumerate(a["bface_mark"]) if a["mark_type"][m] in (3, 4)]
+        nodes = np.unique(a["bface"][walls][:, 1:])
+        assert np.all(S[nodes, 1:3] == 0.0)
