@@ -104,13 +104,44 @@ def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision:
     return info
 
 
+def small_batched(n: int, steps: int, warmup: int):
+    """The 200x100x2 (40k-cell) ramp of the reference's own configuration,
+    n solves at once in one batch context (Mach 2.5 + 0.01 k, so they are n
+    different solves): the form small meshes take to fill the GPU, where
+    one solve alone is bound by launch latency.  Same scheme as the
+    headline (Roe, LSQ, Venkatakrishnan K=5, RK5, CFL 0.5, FP64)."""
+    from csu_amd import cases, native
+    d0 = tempfile.mkdtemp(prefix="sb", dir="/tmp") + "/"
+    cases.ramp_case(d0, nx=200, ny=100, threads=1, nmax=steps)
+    hcs = []
+    for k in range(n):
+        d = tempfile.mkdtemp(prefix=f"sb{k}_", dir="/tmp") + "/"
+        os.symlink(d0 + "mesh.su2", d + "mesh.su2")
+        cases.write_input(d + "input.dat", cases.ramp_input(threads=1, nmax=steps, mach=f"{2.5 + 0.01 * k:.2f}"))
+        hcs.append(native.HostCase(d))
+    b = native.Batch(hcs)
+    b.iterate(warmup, want=False)
+    b.sync()
+    t0 = time.perf_counter()
+    b.iterate(steps, want=False)
+    b.sync()
+    dt = time.perf_counter() - t0
+    b.close()
+    nc = sum(c.ncell for c in hcs)
+    for c in hcs:
+        c.close()
+    return {"solves": n, "cells": nc, "steps": steps, "ms_per_step": 1e3 * dt / steps,
+            "cell_iter_per_s": nc * steps / dt}
+
+
 def serving_requests(n: int, seed: int = 4):
-    """Seeded requests for the serving measurement: supersonic Euler ramps
-    (15 deg, seeded node jitter), Mach U[2.4, 3.2], cell counts log-uniform
-    in [10k, 40k], LU-SGS at CFL 5 with Roe flux on colour/RCM-renumbered
-    meshes -- settings under which the reference's scheme reaches a 1e-8
-    residual drop (its laminar/SA plates level off near 1e-3, and LU-SGS at
-    higher CFL or Mach < 2.4 diverges on these meshes)."""
+    """Seeded requests for the converging serving measurement: supersonic
+    Euler ramps (15 deg, seeded node jitter), Mach U[2.4, 3.2], cell counts
+    log-uniform in [10k, 40k], LU-SGS at CFL 5 with Roe flux on
+    colour/RCM-renumbered meshes -- settings under which the reference's
+    scheme reaches a 1e-8 residual drop from its first iteration for most
+    requests (LU-SGS at higher CFL or Mach < 2.4 diverges on these
+    meshes)."""
     from csu_amd import cases, native, renumber
     rng = np.random.default_rng(seed)
     out = []
@@ -126,60 +157,124 @@ def serving_requests(n: int, seed: int = 4):
     return out
 
 
-def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 4000, chunk: int = 50):
+def serving_requests_mixed(n: int, seed: int = 6, lo: float = 2e4, hi: float = 1e6):
+    """The served stream as specified for the serving configuration: SA and
+    SST NACA 0012 airfoils and Euler ramps in turn, cell counts log-uniform
+    in [lo, hi], Mach U[0.2, 2.5], AoA U[-4, 10] deg, Re log-U[1e5, 1e7]
+    (through the pressure); airfoils AUSMDV, LU-SGS at CFL 20 ramped to 200;
+    ramps Roe, LU-SGS at CFL 5; all on colour/RCM-renumbered meshes."""
+    from csu_amd import cases, native, renumber
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        kind = ("naca_sa", "ramp", "naca_sst")[k % 3]
+        ncell = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        mach = float(rng.uniform(0.2, 2.5))
+        aoa = float(rng.uniform(-4.0, 10.0))
+        re = float(np.exp(rng.uniform(np.log(1e5), np.log(1e7))))
+        d = tempfile.mkdtemp(prefix=f"sm{k}_", dir="/tmp") + "/"
+        if kind == "ramp":
+            ny = max(8, int(round(np.sqrt(ncell / 4))))
+            cases.write_case(d, cases.ramp_mesh(nx=2 * ny, ny=ny, seed=k),
+                             cases.ramp_input(threads=1, nmax=1, mach=f"{mach:.4f}", timeScheme="LUSGS", CFL="5",
+                                              flux="ROE"))
+        else:
+            ni = max(64, int(round(ncell / 250)))
+            p = 183134.3 * (re / 6e6) * (0.15 / mach)
+            over = {"sa": "1"} if kind == "naca_sa" else {"sa": "0", "sst": "1"}
+            cases.write_case(d, cases.naca_mesh(ni=ni, nwall=40, nfar=105, seed=k),
+                             cases.naca_input(threads=1, nmax=1, aoa=aoa, mach=f"{mach:.4f}", pressure=f"{p:.6g}",
+                                              pout=f"{p:.6g}", **over))
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+        out.append((kind, native.HostCase(d, host_wall_distance=False)))
+    return out
+
+
+def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 4000, chunk: int = 50,
+              precision: int = 0, switch_drop: float = 0.0, peak_ref: bool = False, caps=None, ramps=None):
     """Feed the requests to a csu_server at the given arrival times (s);
-    returns the job results in submit order."""
+    returns the job results in submit order.  precision: the arithmetic of
+    every request (mixed with switch_drop > 0: FP64 from that drop on);
+    caps / ramps: per-kind iteration caps and CFL ramps."""
     from csu_amd import native
+    for _, c in reqs:
+        c.params.precision = precision
     srv = native.Server(max_slots=slots, hbm_budget=250e9, chunk=chunk)
     res, sub = {}, 0
     while len(res) < len(reqs):
         now = srv.clock()
         while sub < len(reqs) and arrivals[sub] <= now:
-            srv.submit(reqs[sub][1], drop=drop, max_iter=max_iter)
+            kind, c = reqs[sub]
+            srv.submit(c, drop=drop, max_iter=(caps or {}).get(kind, max_iter), peak_ref=peak_ref,
+                       switch_drop=switch_drop if precision == 1 else 0.0, cfl_ramp=(ramps or {}).get(kind))
             sub += 1
         srv.step(0.002)
         for r in srv.results():
             res[r["id"]] = r
     srv.close()
+    for _, c in reqs:
+        c.params.precision = 0
     return [res[i] for i in range(len(reqs))]
 
 
-def serving(n: int, rate: float, slots: int, log):
+def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=None, ramps=None):
     """Solves/hour and time-to-converge (1e-8 density-residual drop) of the
-    served form (`slots` solves in flight) against the plain one-at-a-time
-    path (one slot) on the same requests: a Poisson stream at `rate`
-    requests/s (p50/p99 time-to-converge), and all requests queued at once
-    (saturated throughput)."""
-    reqs = serving_requests(n)
+    served form -- `slots` solves in flight, mixed precision switched to
+    FP64 at a 1e-4 drop -- against the plain FP64 one-at-a-time path (one
+    slot) on the same requests: a Poisson stream at `rate` requests/s
+    (p50/p99 time-to-converge), and all requests queued at once (saturated
+    throughput); with the forces of the solves both forms converge, mixed
+    against FP64."""
+    n = len(reqs)
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     out = {"requests": n, "rate_per_s": rate, "slots": slots, "drop": 1e-8,
+           "drop_from": "largest residual so far" if peak_ref else "first iteration",
            "cells": [c.ncell for _, c in reqs], "kinds": [k for k, _ in reqs]}
-    for name, sl in (("served", slots), ("one_at_a_time", 1)):
-        sat = serve_run(reqs, sl, np.zeros(n))
+    forms = (("served", slots, 1), ("one_at_a_time", 1, 0))
+    sats = {}
+    for name, sl, prec in forms:
+        kw = dict(precision=prec, switch_drop=1e-4, peak_ref=peak_ref, caps=caps, ramps=ramps)
+        sat = serve_run(reqs, sl, np.zeros(n), **kw)
+        sats[name] = sat
         span = max(r["t_done"] for r in sat) - min(r["t_submit"] for r in sat)
         nconv = sum(r["converged"] for r in sat)
         log(f"serving {name}: saturated {n} solves ({nconv} converged) in {span:.2f} s")
-        st = serve_run(reqs, sl, arrivals)
+        st = serve_run(reqs, sl, arrivals, **kw)
         ttc = np.array([r["t_done"] - r["t_submit"] for r in st if r["converged"]] or [np.nan])
-        out[name] = {"solves_per_hr_saturated": 3600.0 * nconv / span,
+        by = {}
+        for (kind, _), r in zip(reqs, st):
+            b = by.setdefault(kind, {"requests": 0, "converged": 0, "diverged": 0})
+            b["requests"] += 1
+            b["converged"] += int(r["converged"])
+            b["diverged"] += int(r["status"] == 7)
+        out[name] = {"precision": "mixed, FP64 from a 1e-4 drop" if prec else "FP64",
+                     "solves_per_hr_saturated": 3600.0 * nconv / span,
                      "p50_ttc_s": float(np.percentile(ttc, 50)), "p99_ttc_s": float(np.percentile(ttc, 99)),
-                     "converged": int(sum(r["converged"] for r in st)),
+                     "converged": int(sum(r["converged"] for r in st)), "by_kind": by,
                      "iters_p50": float(np.median([r["iters"] for r in st])),
                      "cell_iters_per_s_saturated": sum(r["iters"] * r["ncell"] for r in sat) / span}
         log(f"serving {name}: stream p50 {out[name]['p50_ttc_s']:.3f} s p99 {out[name]['p99_ttc_s']:.3f} s")
-    out["speedup_solves_per_hr"] = (out["served"]["solves_per_hr_saturated"]
-                                    / out["one_at_a_time"]["solves_per_hr_saturated"])
+    one = out["one_at_a_time"]["solves_per_hr_saturated"]
+    out["speedup_solves_per_hr"] = out["served"]["solves_per_hr_saturated"] / one if one else None
+    both = [(a, b) for a, b in zip(sats["served"], sats["one_at_a_time"]) if a["converged"] and b["converged"]]
+    if both:
+        rel = [abs(a["coef_last"][k] + a["coef_last"][k + 1] - b["coef_last"][k] - b["coef_last"][k + 1])
+               / max(abs(b["coef_last"][k] + b["coef_last"][k + 1]), 1e-300) for a, b in both for k in (0, 2)]
+        out["forces_mixed_vs_fp64_rel_max"] = float(max(rel))
     return out
 
 
-def _config_solve(d: str, n: int, precision: int):
+def _config_solve(d: str, n: int, precision: int, graphs: bool = True):
     """n iterations of the case in d on the device at the given precision:
-    timing, histories, wall shear and final state."""
+    timing, histories, wall shear and final state.  graphs False: every
+    kernel launched on its own (for kernel traces)."""
     from csu_amd import native
     case = native.HostCase(d, host_wall_distance=False)
     case.params.precision = precision
     dev = native.Device(case)
+    if not graphs:
+        dev.set_graphs(False)
     dev.iterate(2, want=False)            # first residual, graph capture
     dev.sync()
     t0 = time.perf_counter()
@@ -201,7 +296,101 @@ def _config_solve(d: str, n: int, precision: int):
     return out
 
 
-def other_configs(log, n_plate: int = 500, n_naca: int = 500):
+def _converge(d: str, precision: int, drop: float, max_iter: int, chunk: int = 500, switch_drop: float = 0.0,
+              cfl_ramp=None):
+    """Run the case in d to a density-residual drop of `drop` from its
+    largest residual so far (a free-stream start past a wall has a first
+    density residual near rounding level), chunk iterations at a time; in
+    mixed precision with switch_drop > 0, FP64 from that drop on.  Returns
+    iterations, seconds, the forces, the wall shear and the state at the
+    stopping iteration's chunk end."""
+    from csu_amd import native
+    case = native.HostCase(d, host_wall_distance=False)
+    case.params.precision = precision
+    dev = native.Device(case)
+    peak, it, conv, it_switch, best, stall = 0.0, 0, None, None, np.inf, 0
+    cfl = float(case.params.cfl)
+    t0 = time.perf_counter()
+    while it < max_iter:
+        res, coef = dev.iterate(chunk)
+        r = res[:, 0]
+        if not np.isfinite(r).all():
+            break
+        for k in range(chunk):
+            peak = max(peak, r[k])
+            if conv is None and r[k] <= drop * peak:
+                conv = it + k + 1
+        it += chunk
+        if conv is not None:
+            break
+        if r[-1] < 0.9 * best:
+            best, stall = r[-1], 0
+        else:
+            stall += 1
+        if switch_drop and it_switch is None and (r[-1] <= switch_drop * peak or stall >= 4):
+            # the server's rule: FP64 from the switch drop, or once FP32
+            # rounding stalls the residual (no 10 % fall in four chunks)
+            dev.set_precision(0)
+            it_switch = it
+        if cfl_ramp and cfl < cfl_ramp[0]:
+            cfl = min(cfl * cfl_ramp[1], cfl_ramp[0])
+            dev.set_cfl(cfl)
+    dev.sync()
+    secs = time.perf_counter() - t0
+    out = dict(nc=case.ncell, iters=it, iter_conv=conv, iter_switch=it_switch, seconds=secs,
+               drop_reached=float(r[-1] / peak) if peak > 0 else None, coef=coef[-1].tolist(),
+               wt=dev.wall_tensor() if case.params.laminar or case.params.sa or case.params.sst else None,
+               U=dev.get_state())
+    dev.close()
+    case.close()
+    return out
+
+
+def converge_configs(log, plate_cfl: float = 200.0):
+    """Time to converge, FP64 against mixed precision switched to FP64 at a
+    1e-4 drop: the laminar plate (500x100 hybrid, AUSM, Venkatakrishnan K=5,
+    DPLUR) at CFL 200, where the reference's scheme reaches a 1e-8 drop from
+    the residual peak (at the configured CFL 50 it gets there too, about
+    four times slower); the NACA 0012 SA and SST airfoils (LU-SGS, CFL 20
+    ramped to 200), whose residual the reference's scheme does not bring
+    down (recorded: the drop reached)."""
+    from csu_amd import cases, renumber
+    out = {}
+    d = tempfile.mkdtemp(prefix="cv", dir="/tmp") + "/"
+    cases.write_case(d, cases.plate_mesh(nx=500, ny=100, growth=1.08, tri_frac=0.2, seed=1),
+                     cases.plate_input(threads=1, nmax=1, timeScheme="DPLUR", CFL=f"{plate_cfl:g}"))
+    r64 = _converge(d, 0, 1e-8, 40000)
+    rsw = _converge(d, 1, 1e-8, 40000, switch_drop=1e-4)
+    wt64, wtsw = r64.pop("wt")[:, 1], rsw.pop("wt")[:, 1]
+    m = np.abs(wt64) > 0
+    U64, Usw = r64.pop("U"), rsw.pop("U")
+    cd64, cdsw = r64["coef"][0] + r64["coef"][1], rsw["coef"][0] + rsw["coef"][1]
+    cl64, clsw = r64["coef"][2] + r64["coef"][3], rsw["coef"][2] + rsw["coef"][3]
+    out["laminar_plate_dplur"] = {
+        "workload": f"laminar flat plate M0.3 Re_L 1e5, 500x100 hybrid (20% tris), AUSM/Venkat K=5, DPLUR CFL "
+                    f"{plate_cfl:g}, 1e-8 drop from the density-residual peak",
+        "fp64": r64, "mixed_switch_1e-4": rsw,
+        "speedup_time_to_converge": r64["seconds"] / rsw["seconds"] if rsw["iter_conv"] else None,
+        "mixed_vs_fp64": {"CD_rel": abs(cdsw / cd64 - 1), "CL_rel": abs(clsw / cl64 - 1) if cl64 else None,
+                          "Cf_linf_rel": float(np.abs(wtsw[m] - wt64[m]).max() / np.abs(wt64[m]).max()),
+                          "field_linf_rel": (np.abs(Usw - U64).max(axis=1) / np.abs(U64).max(axis=1)).tolist()}}
+    log(f"plate convergence: fp64 {r64['iter_conv']} its {r64['seconds']:.1f} s, mixed+switch "
+        f"{rsw['iter_conv']} its {rsw['seconds']:.1f} s")
+    mesh = cases.naca_mesh(ni=1000, nwall=40, nfar=105, seed=2)
+    for model in ("sa", "sst"):
+        d = tempfile.mkdtemp(prefix="cv" + model, dir="/tmp") + "/"
+        over = {"sa": "1"} if model == "sa" else {"sa": "0", "sst": "1"}
+        cases.write_case(d, mesh, cases.naca_input(threads=1, nmax=1, aoa=2.616, **over))
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+        r = _converge(d, 0, 1e-8, 4000, chunk=200, cfl_ramp=(200.0, 1.1))
+        r.pop("U"), r.pop("wt")
+        out[f"naca_{model}_lusgs"] = {"workload": f"NACA 0012 {model.upper()} 250k, AUSMDV, LU-SGS CFL 20 -> 200 "
+                                                  "(x1.1 per 200 iterations), 4000 iterations", "fp64": r}
+        log(f"naca {model}: drop from peak after {r['iters']} iterations: {r['drop_reached']}")
+    return out
+
+
+def other_configs(log, n_plate: int = 500, n_naca: int = 500, graphs: bool = True):
     """The viscous configurations beside the headline one, each in plain
     FP64 and in mixed precision (FP32 gradients, limiter, edge states and
     convective flux; FP64 accumulation and update) on the same mesh and
@@ -237,8 +426,8 @@ def other_configs(log, n_plate: int = 500, n_naca: int = 500):
                       "AUSMDV, LU-SGS CFL 20 (colour/RCM order)"))
     out = {}
     for name, d, n, a_deg, label in specs:
-        r64 = _config_solve(d, n, 0)
-        r32 = _config_solve(d, n, 1)
+        r64 = _config_solve(d, n, 0, graphs)
+        r32 = _config_solve(d, n, 1, graphs)
         a = np.radians(a_deg)
 
         def lift_drag(c):
@@ -434,15 +623,34 @@ def main():
                     help="measure the many-solve mix in batched contexts only, not through the server")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the laminar-plate and NACA 0012 SA/SST measurements")
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the time-to-converge runs of the laminar plate and the airfoils")
     ap.add_argument("--serve-n", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-max-cells", type=float, default=1e6)
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 200x100x2 timing (profiling runs of the big mesh alone)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB ten times (the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--only-configs", type=int, default=0, metavar="N",
+                    help="only the laminar-plate and NACA 0012 SA/SST configurations, N iterations each, every "
+                         "kernel launched on its own (no graphs): the form kernel traces profile; prints their JSON")
+    ap.add_argument("--stream-calib", action="store_true",
+                    help="only run the library's FP64 streaming kernel over two 1 GiB arrays (11 launches): the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters and the practical HBM rate")
     a = ap.parse_args()
+    if a.stream_calib:
+        from csu_amd import native
+        print(json.dumps({"stream_kernel_bytes_read": 2 ** 30, "stream_kernel_bytes_written": 2 ** 30,
+                          "stream_kernel_gbs": native.stream_rate(2 ** 30, 10)}))
+        return
+    if a.only_configs:
+        lg = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+        print(json.dumps({"other_configs_eager": other_configs(lg, a.only_configs, a.only_configs, graphs=False)},
+                         default=lambda o: None))
+        return
     if a.copy_calib:
         import torch
         n = a.copy_calib * (1 << 20) // 8
@@ -484,6 +692,7 @@ def main():
         log(f"mixed done: {mx['seconds']:.3f} s")
     log("ramp 200x100x2")
     small = None if a.no_small else gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False)
+    small_b = None if a.no_small else small_batched(32, max(a.steps, 50), a.warmup)
     log("small done; cpu baseline" if not a.no_cpu else "small done")
     nc = big["nc"]
     ms = 1e3 * big["seconds"] / a.steps
@@ -496,6 +705,8 @@ def main():
     # the default workload runs the primitives-from-state stages unless the
     # stored form is selected through the library's environment switch
     pu = os.environ.get("CSU_STORED_PRIMITIVES", "0") != "1"
+    from csu_amd import native as _nat
+    stream = _nat.stream_rate(2 ** 30, 10)
     traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"], pu)
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -503,9 +714,13 @@ def main():
             "kernel": dom,
             "kernel_avg_ms": avg_ms,
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
+            "stream_kernel_gbs": stream,
+            "kernel_frac_of_stream": achieved / stream,
             "iteration_alg_bytes_per_cell": 2334.5,
             "iteration_frac": value * 2334.5 / (HBM_PEAK_GBS * 1e9)}
     others = None if a.no_configs else other_configs(log)
+    if others is not None and not a.no_converge:
+        others["time_to_converge"] = converge_configs(log)
     batch = None
     if a.batch:
         t0 = time.perf_counter()
@@ -518,7 +733,20 @@ def main():
             log(f"batch mix through the server: {batch['server_form']['cell_iter_per_s']:.3e} cell-iter/s")
         for _, c in reqs:
             c.close()
-    serve = None if a.no_serve else serving(a.serve_n, a.serve_rate, a.serve_slots, log)
+    serve = None
+    if not a.no_serve:
+        t0 = time.perf_counter()
+        reqs = serving_requests(a.serve_n)
+        serve = {"ramps": serving(reqs, a.serve_rate, a.serve_slots, log)}
+        for _, c in reqs:
+            c.close()
+        reqs = serving_requests_mixed(a.serve_n, hi=a.serve_max_cells)
+        log(f"serving stream: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f} s")
+        serve["airfoils_and_ramps"] = serving(reqs, a.serve_rate, a.serve_slots, log, peak_ref=True,
+                                              caps={"naca_sa": 1000, "naca_sst": 1000, "ramp": 3000},
+                                              ramps={"naca_sa": (200.0, 1.1), "naca_sst": (200.0, 1.1)})
+        for _, c in reqs:
+            c.close()
     cpu = None if a.no_cpu else cpu_baseline(200, 100, a.cpu_iters)
     line = {
         "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1,
@@ -528,7 +756,8 @@ def main():
                                f"{a.nx}x{a.ny}x2 tri mesh",
                    "cells": nc, "faces": big["nf"], "boundary_faces": big["nb"],
                    "config0_40k_cell_iter_per_s":
-                       None if small is None else small["nc"] * max(a.steps, 50) / small["seconds"]},
+                       None if small is None else small["nc"] * max(a.steps, 50) / small["seconds"],
+                   "config0_40k_x32_batched": small_b},
         "roofline": roof,
         "mixed_precision": mixed,
         "other_configs": others,

@@ -1655,7 +1655,8 @@ extern "C" int csu_batch_run(csu_dev* d, double drop, int32_t max_iter, int32_t 
     if (!d->sol_host[q].active) out[q].status = -1;  // held by the caller: not run
   }
   std::vector<double> res((size_t)chunk * ns * nv), coef((size_t)chunk * ns * 4);
-  std::vector<int> chunks(ns, 0);
+  std::vector<int> chunks(ns, 0), stall(ns, 0);
+  std::vector<double> best(ns, 0.0);
   int running = 0;
   for (int q = 0; q < ns; q++) running += d->sol_host[q].active;
   while (running > 0) {
@@ -1702,10 +1703,22 @@ extern "C" int csu_batch_run(csu_dev* d, double drop, int32_t max_iter, int32_t 
       }
     }
     // mixed precision: FP64 once every running solve has reached switch_drop
+    // or stalled (its residual has not fallen by 10 % in four chunks: FP32
+    // flux rounding dominates it)
+    for (int q = 0; q < ns; q++) {
+      if (!d->sol_host[q].active) continue;
+      if (chunks[q] == 1 || out[q].res_last < 0.9 * best[q]) {
+        best[q] = out[q].res_last;
+        stall[q] = 0;
+      } else {
+        stall[q]++;
+      }
+    }
     if (switch_drop > 0.0 && d->prm.precision == CSU_PREC_MIXED && running > 0) {
       bool all = true;
       for (int q = 0; q < ns; q++)
-        if (d->sol_host[q].active && !(out[q].res_last <= switch_drop * (peak_ref ? out[q].res_peak : out[q].res0)))
+        if (d->sol_host[q].active && !(out[q].res_last <= switch_drop * (peak_ref ? out[q].res_peak : out[q].res0)) &&
+            stall[q] < 4)
           all = false;
       if (all) {
         if ((rc = csu_dev_set_precision(d, CSU_PREC_FP64))) return rc;
@@ -1987,6 +2000,46 @@ extern "C" int csu_dev_surf_faces_noalloc(csu_dev* d, double* out) {
   return CSU_OK;
 }
 
+// HBM read + write rate of a plain FP64 streaming kernel over two arrays
+// of `bytes` each (the practical ceiling the stage kernels are compared
+// with), best of `reps` timed launches, on the current device.
+extern "C" int csu_stream_rate(int64_t bytes, int32_t reps, double* gbps) {
+  if (!gbps || bytes < 1024 || reps <= 0) return set_err(CSU_ERR_ARG, "bytes %lld reps %d", (long long)bytes, reps);
+  const size_t n2 = (size_t)bytes / 16;
+  double2 *x = nullptr, *y = nullptr;
+  if (hipMalloc(&x, n2 * 16) != hipSuccess) return set_err(CSU_ERR_NOMEM, "stream buffer");
+  if (hipMalloc(&y, n2 * 16) != hipSuccess) {
+    (void)hipFree(x);
+    return set_err(CSU_ERR_NOMEM, "stream buffer");
+  }
+  int rc = CSU_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipMemset(x, 0, n2 * 16) != hipSuccess)
+    rc = set_err(CSU_ERR_HIP, "stream set-up");
+  const unsigned grid = (unsigned)std::max(1, ncu) * 16;
+  float best = 1e30f;
+  for (int r = 0; r <= reps && !rc; r++) {   // launch 0 warms up
+    float ms = 0.f;
+    hipEventRecord(e0, nullptr);
+    hipLaunchKernelGGL(k_stream, dim3(grid), dim3(CSU_BLOCK), 0, nullptr, x, y, n2, 1.000001);
+    hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = set_err(CSU_ERR_HIP, "stream timing");
+    else if (r > 0 && ms < best)
+      best = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  if (rc) return rc;
+  *gbps = 2.0 * 16.0 * (double)n2 / (best * 1e-3) / 1e9;
+  return CSU_OK;
+}
+
 extern "C" int64_t csu_dev_bytes(const csu_dev* d) { return d ? (int64_t)d->bytes : -1; }
 
 // Device bytes csu_dev_create will allocate (plus a history ring of
@@ -1996,7 +2049,7 @@ extern "C" int64_t csu_dev_footprint(const csu_mesh_desc* m, const csu_params* p
   const double nc = m->ncell, nf = m->nface, nb = m->nbface, nv = p->nvar;
   const bool visc = p->laminar || p->sa || p->sst;
   double b = 0;
-  b += nc * (4 + 16 + 16 + 48 + 64 + 64 + 8) + nf * (8 + 1 + 56) + nb * (12 + 64 + 4);   // geometry
+  b += nc * (4 + 16 + 16 + 48 + 64 + 64 + 8) + nf * (8 + 1 + 24) + nb * (12 + 64 + 4);  // geometry
   b += nc * (24 * nv + 8) + 2 * nv * 8 * (nc / 256 + 1) + 128;                           // U, Uaux, R, dt, reductions
   b += (nv + 1) * 8 * (nc + nb) + 24 * nv * nc + 8 * nv * nf;                             // P, Pb, grad, phi, fflux
   if (p->order == 2) b += 32 * nv * nc;                                                   // edge states

@@ -44,6 +44,8 @@ struct Job {
   int chunk_n = 0;  // iterations of the chunk in flight
   bool res0_set = false;
   double peak = 0.0;  // largest density residual so far
+  double best = 0.0;  // smallest density residual so far (mixed precision: stall test)
+  int stalled = 0;    // chunks since the smallest residual last fell by 10 %
   double cfl = 0.0;  // current CFL of a ramped job
   int64_t est = 0;   // footprint estimate charged while the context is created
   int64_t charged = 0;  // device bytes of the context charged against the budget
@@ -280,8 +282,17 @@ static bool harvest(csu_server* s, size_t k) {
   }
   if (diverged) { retire(s, k, CSU_ERR_DIVERGED); return true; }
   if (j->res.converged || j->res.iters >= j->r.max_iter) { retire(s, k, CSU_OK); return true; }
+  // mixed precision levels off where its FP32 fluxes' rounding dominates
+  // the residual: FP64 from the switch drop, or once the residual has not
+  // fallen by 10 % in four chunks
+  if (j->res.iters <= j->chunk_n || j->res.res_last < 0.9 * j->best) {
+    j->best = j->res.res_last;
+    j->stalled = 0;
+  } else {
+    j->stalled++;
+  }
   if (j->r.switch_drop > 0.0 && j->res.iter_switch < 0 && j->r.prm->precision == CSU_PREC_MIXED &&
-      j->res.res_last <= j->r.switch_drop * (j->r.peak_ref ? j->peak : j->res.res0)) {
+      (j->res.res_last <= j->r.switch_drop * (j->r.peak_ref ? j->peak : j->res.res0) || j->stalled >= 4)) {
     // mixed precision has done its part: FP64 to the converged state
     if ((rc = csu_dev_set_precision(j->dev, CSU_PREC_FP64))) { retire(s, k, rc); return true; }
     j->res.iter_switch = j->res.iters;

@@ -4,6 +4,17 @@
 
 namespace csu {
 
+// HBM streaming reference: y = s x over n doubles, two per lane per step
+// (16-byte loads and stores), grid-stride.  Measures the read + write rate
+// the stage kernels' bytes are compared with.
+__global__ __launch_bounds__(CSU_BLOCK) void k_stream(const double2* __restrict__ x, double2* __restrict__ y,
+                                                      size_t n2, double s) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 v = x[i];
+    y[i] = make_double2(v.x * s, v.y * s);
+  }
+}
+
 // FP32 copy of an FP64 array (round to nearest, as a host conversion does).
 __global__ __launch_bounds__(CSU_BLOCK) void k_to_f32(const double* __restrict__ src, float* __restrict__ dst,
                                                       size_t n) {

@@ -270,8 +270,23 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
         d2[j] = tgx * ex[j] + tgy * ey[j];
-        TF p0 = kp.limiter == 2 ? limiter_bj<TF>(d1max, d1min, d2[j]) : limiter_v2d<TF>(d1max, d1min, d2[j], (TF)pref2);
-        ph = (j == 0) ? p0 : fmin(ph, p0);
+      }
+      // one limiter per launch (a uniform branch around the whole edge
+      // loop: a per-edge select would evaluate both functions' divisions)
+      if (kp.limiter == 2) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (j >= np) break;
+          const TF p0 = limiter_bj<TF>(d1max, d1min, d2[j]);
+          ph = (j == 0) ? p0 : fmin(ph, p0);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (j >= np) break;
+          const TF p0 = limiter_v2d<TF>(d1max, d1min, d2[j], (TF)pref2);
+          ph = (j == 0) ? p0 : fmin(ph, p0);
+        }
       }
       if (store) s.phi[kk * nc + i] = ph;
       // the state each face of this cell sees on this side (solver.c:598-603)

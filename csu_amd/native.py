@@ -131,6 +131,7 @@ def lib():
     L.csu_dev_footprint.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.c_int32]
     L.csu_dev_footprint.restype = C.c_int64
     L.csu_dev_set_precision.argtypes = [C.c_void_p, C.c_int32]
+    L.csu_stream_rate.argtypes = [C.c_int64, C.c_int32, f64p]
     L.csu_dev_autotune.argtypes = [C.c_void_p, C.c_int32, i32p]
     L.csu_dev_launch_shape.argtypes = [C.c_void_p, i32p]
     L.csu_batch_create.argtypes = [C.c_int32, C.POINTER(C.POINTER(MeshDesc)), C.POINTER(C.POINTER(Params)),
@@ -184,6 +185,13 @@ def read_input(path: str) -> list:
                 for i in range(L.csu_input_count(h))]
     finally:
         L.csu_input_free(h)
+
+
+def stream_rate(nbytes: float = 2 ** 30, reps: int = 10) -> float:
+    """GB/s (read + write) of the library's plain FP64 streaming kernel."""
+    out = C.c_double()
+    check(lib().csu_stream_rate(int(nbytes), reps, C.byref(out)))
+    return out.value
 
 
 class HostCase:

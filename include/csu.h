@@ -268,7 +268,8 @@ typedef struct csu_batch_result {
  * growth > 1 each running solve's CFL is multiplied by growth after every
  * chunk, up to cfl_max[solve] (cfl_max NULL: no ramp).  A mixed-precision
  * batch with switch_drop > 0 continues in FP64 once every running solve
- * has reached that drop.  out: [n]. */
+ * has reached that drop or stalled (no 10 % fall of its residual in four
+ * chunks).  out: [n]. */
 int  csu_batch_run(csu_dev* dev, double drop, int32_t max_iter, int32_t chunk,
                    const double* cfl_max, double growth, int32_t peak_ref, double switch_drop,
                    csu_batch_result* out);
@@ -284,6 +285,12 @@ int  csu_batch_run(csu_dev* dev, double drop, int32_t max_iter, int32_t chunk,
  * csu_dev_launch_shape reports the shapes in use (returns 1 if tuned). */
 int  csu_dev_autotune(csu_dev* dev, int32_t iters, int32_t* choice);
 int  csu_dev_launch_shape(const csu_dev* dev, int32_t* choice);
+
+/* Read + write rate (GB/s) of a plain FP64 streaming kernel over two
+ * arrays of `bytes` each on the current device, best of `reps` launches:
+ * the practical HBM ceiling the stage kernels are measured against (no
+ * counterpart in the reference). */
+int  csu_stream_rate(int64_t bytes, int32_t reps, double* gbps);
 
 /* Iterations done so far; local time step [ncell] of the last iteration. */
 int  csu_dev_iteration(const csu_dev* dev);

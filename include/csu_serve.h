@@ -67,8 +67,9 @@ int  csu_server_set_ramp(csu_server* s, int32_t id, double cfl_max, double growt
  * one (a solve started from a uniform free stream past a wall or an airfoil
  * has a first density residual near rounding level, which makes drops from
  * it meaningless).  switch_drop > 0 (a mixed-precision job): once the drop
- * reaches it, the job continues in FP64 (csu_dev_set_precision), so it
- * converges to the FP64 steady state. */
+ * reaches it, or the residual has not fallen by 10 % in four chunks (the
+ * level where FP32 flux rounding dominates it), the job continues in FP64
+ * (csu_dev_set_precision), so it converges to the FP64 steady state. */
 int  csu_server_set_options(csu_server* s, int32_t id, int32_t peak_ref, double switch_drop);
 
 /* Move up to max finished results into out; returns the count. */

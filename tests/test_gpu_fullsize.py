@@ -1,0 +1,141 @@
+"""Full-size configurations against the oracle (GPU): the 250k-cell NACA
+0012 O-mesh (40 quad layers from 1e-6 chord, growth 1.15, triangles to 50
+chords) with SA and SST, and the 500x100 laminar plate with DPLUR, over
+their first iterations; and the mixed-precision path on the viscous and
+turbulent configurations against FP64."""
+import numpy as np
+import pytest
+
+from csu_amd import cases, native, renumber
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _naca(tmpcase, model, ni=1000, nwall=40, nfar=105):
+    d = tmpcase("nf")
+    over = {"sa": "1"} if model == "sa" else {"sa": "0", "sst": "1"}
+    cases.write_case(d, cases.naca_mesh(ni=ni, nwall=nwall, nfar=nfar, seed=2),
+                     cases.naca_input(threads=1, nmax=1, aoa=2.616, **over))
+    renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+    return native.HostCase(d)
+
+
+def _plate(tmpcase, nx=500, ny=100):
+    d = tmpcase("pf")
+    cases.write_case(d, cases.plate_mesh(nx=nx, ny=ny, growth=1.08, tri_frac=0.2, seed=1),
+                     cases.plate_input(threads=1, nmax=1, timeScheme="DPLUR", CFL="50"))
+    return native.HostCase(d)
+
+
+@pytest.mark.parametrize("model", ["sa", "sst"])
+def test_naca_full_mesh_history(tmpcase, model):
+    """250,000 cells, LU-SGS at CFL 20 on the colour/RCM-renumbered mesh:
+    the first 12 iterations' residual norms and forces 1e-10 relative, the
+    state 1e-10."""
+    c = _naca(tmpcase, model)
+    assert c.ncell == 250000
+    print("naca full", model, c.ncell, flush=True)
+    dev = native.Device(c)
+    o = po.Oracle(c, threads=16)
+    rg, cg = dev.iterate(12)
+    ro, co = o.iterate(12)
+    rel = np.abs(rg / ro - 1)
+    print("max rel residual diff", rel.max(), flush=True)
+    assert rel.max() < 1e-10
+    assert np.abs(cg - co).max() <= 1e-10 * np.abs(co).max()
+    Ug, Uo = dev.get_state(), o.get_state()
+    scale = np.abs(Uo).max(axis=1, keepdims=True)
+    assert (np.abs(Ug - Uo) / scale).max() < 1e-10
+
+
+def test_plate_full_mesh_history(tmpcase):
+    """500x100 hybrid plate (56,030 cells), DPLUR at CFL 50 (Nlinear 20):
+    the first iteration 1e-10 (identical blocks); over 8 iterations no
+    further from the oracle than the oracle is from itself under a one-ulp
+    perturbation of the initial state (x10): the finite-difference blocks
+    (h = 1e-6) amplify last-bit differences."""
+    c = _plate(tmpcase)
+    print("plate full", c.ncell, flush=True)
+    dev = native.Device(c)
+    o = po.Oracle(c, threads=16)
+    rg, _ = dev.iterate(8)
+    ro, _ = o.iterate(8)
+    rel = np.abs(rg / ro - 1)
+    print("max rel", rel.max(), "first", rel[0].max(), flush=True)
+    assert rel[0].max() < 1e-10
+    o2 = po.Oracle(c, threads=16)
+    U0 = c.initial_state()
+    o2.set_state(U0 * (1 + 2.2e-16 * np.random.default_rng(7).standard_normal(U0.shape)))
+    r2, _ = o2.iterate(8)
+    spread = float(np.max(np.abs(r2 / ro - 1)))
+    print("oracle one-ulp spread", spread, flush=True)
+    assert rel.max() <= max(1e-10, 10 * spread)
+
+
+@pytest.mark.parametrize("kind", ["naca_sa", "naca_sst"])
+def test_mixed_precision_turbulent(tmpcase, kind):
+    """Mixed precision (FP32 gradients, limiter, edge states, convective
+    flux; FP64 viscous terms, sources, implicit operator, update) against
+    FP64 over 40 LU-SGS iterations on the SA / SST airfoils: the state
+    within 1e-4 of each field's range and the forces within 1e-3."""
+    c = _naca(tmpcase, kind[5:], ni=128, nwall=16, nfar=20)
+    out = {}
+    for prec in (0, 1):
+        p = native.Params.from_buffer_copy(c.params)
+        p.precision = prec
+        dev = native.Device(c, params=p)
+        r, k = dev.iterate(40)
+        out[prec] = (r, k, dev.get_state())
+        dev.close()
+    (r0, k0, U0), (r1, k1, U1) = out[0], out[1]
+    fu = float(np.max(np.abs(U1 - U0).max(axis=1) / np.abs(U0).max(axis=1)))
+    fc = float(np.max(np.abs(k1 - k0)) / np.abs(k0).max())
+    print(kind, "field rel", fu, "forces rel", fc, flush=True)
+    assert np.isfinite(r1).all()
+    assert fu < 1e-4 and fc < 1e-3
+
+
+def test_mixed_precision_plate_converged(tmpcase):
+    """The laminar plate with DPLUR is chaotic at the rounding level during
+    its transient (one ulp of the initial state moves the residual history
+    by O(1), as the full-mesh test shows), so mixed precision is compared
+    where it is meaningful: converged.  FP64 to a 1e-8 drop from the
+    residual peak, and mixed precision switched to FP64 at a 1e-4 drop, to
+    the same drop: the friction and pressure forces within 1e-6 of each
+    other and the fields within 1e-6 of each field's range."""
+    d = tmpcase("pm")
+    cases.write_case(d, cases.plate_mesh(nx=125, ny=25, growth=1.08 ** 4, tri_frac=0.2, seed=1),
+                     cases.plate_input(threads=1, nmax=1, timeScheme="DPLUR", CFL="500"))
+    c = native.HostCase(d)
+    out = {}
+    for prec in (0, 1):
+        p = native.Params.from_buffer_copy(c.params)
+        p.precision = prec
+        dev = native.Device(c, params=p)
+        peak, it, done, switched, best, stall = 0.0, 0, False, None, np.inf, 0
+        while it < 40000 and not done:
+            r, k = dev.iterate(500)
+            assert np.isfinite(r).all()
+            for x in r[:, 0]:
+                peak = max(peak, x)
+                done = done or x <= 1e-8 * peak
+            it += 500
+            # the server's rule: FP64 from a 1e-4 drop, or once the residual
+            # has not fallen by 10 % in four chunks (FP32 rounding level)
+            if r[-1, 0] < 0.9 * best:
+                best, stall = r[-1, 0], 0
+            else:
+                stall += 1
+            if prec == 1 and switched is None and (r[-1, 0] <= 1e-4 * peak or stall >= 4):
+                dev.set_precision(0)
+                switched = it
+        print("precision", prec, "iterations", it, "switched at", switched, flush=True)
+        assert done
+        out[prec] = (k[-1], dev.get_state())
+        dev.close()
+    (k0, U0), (k1, U1) = out[0], out[1]
+    fu = float(np.max(np.abs(U1 - U0).max(axis=1) / np.abs(U0).max(axis=1)))
+    fc = float(np.max(np.abs(k1 - k0)) / np.abs(k0).max())
+    print("converged plate: field rel", fu, "forces rel", fc, flush=True)
+    assert fu < 1e-6 and fc < 1e-6
