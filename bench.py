@@ -66,8 +66,8 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int, pu: bool = False)
 # for the primitives-from-state stages (True) and the stored form (False).
 # A counter pass cannot run inside this timed process, so the figure is
 # reported for that workload only.
-MEASURED_TRAFFIC = {("k_grad", 2000, 1000, True): 993.8e6, ("k_flux", 2000, 1000, True): 775.5e6,
-                    ("k_update", 2000, 1000, True): 626.6e6,
+MEASURED_TRAFFIC = {("k_grad", 2000, 1000, True): 993.9e6, ("k_flux", 2000, 1000, True): 775.6e6,
+                    ("k_update", 2000, 1000, True): 626.5e6,
                     ("k_grad", 2000, 1000, False): 1121.9e6, ("k_flux", 2000, 1000, False): 775.4e6,
                     ("k_update", 2000, 1000, False): 787.2e6}
 
@@ -191,15 +191,17 @@ def serving_requests_mixed(n: int, seed: int = 6, lo: float = 2e4, hi: float = 1
 
 
 def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 4000, chunk: int = 50,
-              precision: int = 0, switch_drop: float = 0.0, peak_ref: bool = False, caps=None, ramps=None):
+              precision: int = 0, switch_drop: float = 0.0, peak_ref: bool = False, caps=None, ramps=None,
+              batch: int = 1):
     """Feed the requests to a csu_server at the given arrival times (s);
     returns the job results in submit order.  precision: the arithmetic of
     every request (mixed with switch_drop > 0: FP64 from that drop on);
-    caps / ramps: per-kind iteration caps and CFL ramps."""
+    caps / ramps: per-kind iteration caps and CFL ramps; batch > 1: batched
+    units of up to that many solves per slot."""
     from csu_amd import native
     for _, c in reqs:
         c.params.precision = precision
-    srv = native.Server(max_slots=slots, hbm_budget=250e9, chunk=chunk)
+    srv = native.Server(max_slots=slots, hbm_budget=250e9, chunk=chunk, batch_solves=batch)
     res, sub = {}, 0
     while len(res) < len(reqs):
         now = srv.clock()
@@ -217,24 +219,27 @@ def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 40
     return [res[i] for i in range(len(reqs))]
 
 
-def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=None, ramps=None):
+def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=None, ramps=None, batch: int = 8,
+            batch_slots: int = 4):
     """Solves/hour and time-to-converge (1e-8 density-residual drop) of the
-    served form -- `slots` solves in flight, mixed precision switched to
-    FP64 at a 1e-4 drop -- against the plain FP64 one-at-a-time path (one
-    slot) on the same requests: a Poisson stream at `rate` requests/s
-    (p50/p99 time-to-converge), and all requests queued at once (saturated
-    throughput); with the forces of the solves both forms converge, mixed
-    against FP64."""
+    served forms -- batched units (`batch_slots` batch contexts of up to
+    `batch` solves in flight), and `slots` one-solve contexts in flight;
+    both mixed precision switched to FP64 at a 1e-4 drop -- against the
+    plain FP64 one-at-a-time path (one slot) on the same requests: a
+    Poisson stream at `rate` requests/s (p50/p99 time-to-converge), and all
+    requests queued at once (saturated throughput); with the forces of the
+    solves the served and one-at-a-time forms both converge, mixed against
+    FP64."""
     n = len(reqs)
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    out = {"requests": n, "rate_per_s": rate, "slots": slots, "drop": 1e-8,
-           "drop_from": "largest residual so far" if peak_ref else "first iteration",
+    out = {"requests": n, "rate_per_s": rate, "slots": slots, "batch_slots": batch_slots, "batch_solves": batch,
+           "drop": 1e-8, "drop_from": "largest residual so far" if peak_ref else "first iteration",
            "cells": [c.ncell for _, c in reqs], "kinds": [k for k, _ in reqs]}
-    forms = (("served", slots, 1), ("one_at_a_time", 1, 0))
+    forms = (("served", batch_slots, 1, batch), ("served_contexts", slots, 1, 1), ("one_at_a_time", 1, 0, 1))
     sats = {}
-    for name, sl, prec in forms:
-        kw = dict(precision=prec, switch_drop=1e-4, peak_ref=peak_ref, caps=caps, ramps=ramps)
+    for name, sl, prec, bt in forms:
+        kw = dict(precision=prec, switch_drop=1e-4, peak_ref=peak_ref, caps=caps, ramps=ramps, batch=bt)
         sat = serve_run(reqs, sl, np.zeros(n), **kw)
         sats[name] = sat
         span = max(r["t_done"] for r in sat) - min(r["t_submit"] for r in sat)
@@ -249,6 +254,8 @@ def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=Non
             b["converged"] += int(r["converged"])
             b["diverged"] += int(r["status"] == 7)
         out[name] = {"precision": "mixed, FP64 from a 1e-4 drop" if prec else "FP64",
+                     "form": (f"batched units: {sl} batch contexts of up to {bt} solves" if bt > 1 else
+                              f"{sl} one-solve context{'s' if sl > 1 else ''} in flight"),
                      "solves_per_hr_saturated": 3600.0 * nconv / span,
                      "p50_ttc_s": float(np.percentile(ttc, 50)), "p99_ttc_s": float(np.percentile(ttc, 99)),
                      "converged": int(sum(r["converged"] for r in st)), "by_kind": by,
@@ -257,6 +264,7 @@ def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=Non
         log(f"serving {name}: stream p50 {out[name]['p50_ttc_s']:.3f} s p99 {out[name]['p99_ttc_s']:.3f} s")
     one = out["one_at_a_time"]["solves_per_hr_saturated"]
     out["speedup_solves_per_hr"] = out["served"]["solves_per_hr_saturated"] / one if one else None
+    out["speedup_solves_per_hr_contexts"] = out["served_contexts"]["solves_per_hr_saturated"] / one if one else None
     both = [(a, b) for a, b in zip(sats["served"], sats["one_at_a_time"]) if a["converged"] and b["converged"]]
     if both:
         rel = [abs(a["coef_last"][k] + a["coef_last"][k + 1] - b["coef_last"][k] - b["coef_last"][k + 1])
@@ -580,6 +588,26 @@ def batch_mix_batched(reqs, iters: int, log, tune: bool = True):
     return out
 
 
+def serve_all(a, log):
+    """Both serving streams: the converging supersonic ramps, and the
+    specified airfoil-and-ramp stream."""
+    t0 = time.perf_counter()
+    reqs = serving_requests(a.serve_n)
+    serve = {"ramps": serving(reqs, a.serve_rate, a.serve_slots, log, batch=a.serve_batch,
+                              batch_slots=a.serve_batch_slots)}
+    for _, c in reqs:
+        c.close()
+    reqs = serving_requests_mixed(a.serve_n, hi=a.serve_max_cells)
+    log(f"serving stream: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f} s")
+    serve["airfoils_and_ramps"] = serving(reqs, a.serve_rate, a.serve_slots, log, peak_ref=True,
+                                          caps={"naca_sa": 1000, "naca_sst": 1000, "ramp": 3000},
+                                          ramps={"naca_sa": (200.0, 1.1), "naca_sst": (200.0, 1.1)},
+                                          batch=a.serve_batch, batch_slots=a.serve_batch_slots)
+    for _, c in reqs:
+        c.close()
+    return serve
+
+
 def cpu_baseline(nx: int, ny: int, iters: int):
     """Reference OpenMP build (oracle/_ref/csu_ref, compiled from the
     reference sources) on the host cores: Nc*N / (T(N) - T(0))."""
@@ -629,6 +657,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-max-cells", type=float, default=1e6)
+    ap.add_argument("--serve-batch", type=int, default=8, help="solves per batched unit of the served form")
+    ap.add_argument("--serve-batch-slots", type=int, default=4, help="batched units in flight")
+    ap.add_argument("--only-serve", action="store_true", help="only the serving measurements; prints their JSON")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 200x100x2 timing (profiling runs of the big mesh alone)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -665,6 +696,10 @@ def main():
         dt = (time.perf_counter() - t0) / 10
         print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8,
                           "copy_gb_per_s": 2 * n * 8 / dt / 1e9}))
+        return
+    if a.only_serve:
+        lg = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+        print(json.dumps({"serving": serve_all(a, lg)}, default=lambda o: None))
         return
     if a.gpus != 1:
         print(f"bench.py: the benchmark runs on one GPU only (asked for {a.gpus}); "
@@ -733,20 +768,7 @@ def main():
             log(f"batch mix through the server: {batch['server_form']['cell_iter_per_s']:.3e} cell-iter/s")
         for _, c in reqs:
             c.close()
-    serve = None
-    if not a.no_serve:
-        t0 = time.perf_counter()
-        reqs = serving_requests(a.serve_n)
-        serve = {"ramps": serving(reqs, a.serve_rate, a.serve_slots, log)}
-        for _, c in reqs:
-            c.close()
-        reqs = serving_requests_mixed(a.serve_n, hi=a.serve_max_cells)
-        log(f"serving stream: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f} s")
-        serve["airfoils_and_ramps"] = serving(reqs, a.serve_rate, a.serve_slots, log, peak_ref=True,
-                                              caps={"naca_sa": 1000, "naca_sst": 1000, "ramp": 3000},
-                                              ramps={"naca_sa": (200.0, 1.1), "naca_sst": (200.0, 1.1)})
-        for _, c in reqs:
-            c.close()
+    serve = None if a.no_serve else serve_all(a, log)
     cpu = None if a.no_cpu else cpu_baseline(200, 100, a.cpu_iters)
     line = {
         "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1,

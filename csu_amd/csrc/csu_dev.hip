@@ -893,6 +893,12 @@ static int batch_compatible(const csu_params* a, const csu_params* b, int axi_a,
          a->gas_r == b->gas_r && a->sst_owallfactor == b->sst_owallfactor && axi_a == axi_b;
 }
 
+extern "C" int csu_batch_compatible(const csu_mesh_desc* ma, const csu_params* pa, const csu_mesh_desc* mb,
+                                    const csu_params* pb) {
+  if (!ma || !pa || !mb || !pb) return -set_err(CSU_ERR_ARG, "null argument");
+  return batch_compatible(pa, pb, ma->axi, mb->axi) ? 1 : 0;
+}
+
 extern "C" int csu_batch_create(int32_t n, const csu_mesh_desc* const* meshes, const csu_params* const* prms,
                                 csu_dev** out) {
   if (!out) return set_err(CSU_ERR_ARG, "null out");

@@ -152,6 +152,9 @@ def lib():
     L.csu_server_step.argtypes = [C.c_void_p, C.c_double]
     L.csu_server_set_ramp.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
     L.csu_server_set_options.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double]
+    L.csu_server_set_batching.argtypes = [C.c_void_p, C.c_int32, C.c_int64]
+    L.csu_batch_compatible.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(MeshDesc),
+                                       C.POINTER(Params)]
     L.csu_server_results.argtypes = [C.c_void_p, C.POINTER(JobResult), C.c_int32]
     L.csu_server_pending.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.csu_server_state.argtypes = [C.c_void_p, C.c_int32, f64p]
@@ -192,6 +195,14 @@ def stream_rate(nbytes: float = 2 ** 30, reps: int = 10) -> float:
     out = C.c_double()
     check(lib().csu_stream_rate(int(nbytes), reps, C.byref(out)))
     return out.value
+
+
+def batch_compatible(a: "HostCase", b: "HostCase") -> bool:
+    """Whether two cases can share a batch context (csu_batch_compatible)."""
+    r = lib().csu_batch_compatible(C.byref(a.mesh), C.byref(a.params), C.byref(b.mesh), C.byref(b.params))
+    if r < 0:
+        raise CsuError(-r, lib().csu_last_error().decode(errors="replace"))
+    return r == 1
 
 
 class HostCase:
@@ -527,10 +538,15 @@ class Server:
     """Continuous-batching solve server (include/csu_serve.h).  Cases passed
     to submit() are kept alive by the server object until it is closed."""
 
-    def __init__(self, max_slots: int = 4, hbm_budget: float = 250e9, chunk: int = 25):
+    def __init__(self, max_slots: int = 4, hbm_budget: float = 250e9, chunk: int = 25, batch_solves: int = 1,
+                 batch_cells: int = 0):
+        """batch_solves > 1: each slot holds a batch of up to that many
+        solves (and batch_cells cells, 0: no limit) that share kernels."""
         self._h = C.c_void_p()
         check(lib().csu_server_create(max_slots, int(hbm_budget), chunk, C.byref(self._h)))
         self._keep = {}
+        if batch_solves > 1:
+            check(lib().csu_server_set_batching(self._h, batch_solves, int(batch_cells)))
 
     def submit(self, case: "HostCase", U0=None, drop: float = 1e-8, max_iter: int = 5000,
                keep_state: bool = False, cfl_ramp=None, peak_ref: bool = False, switch_drop: float = 0.0) -> int:

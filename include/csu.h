@@ -235,6 +235,10 @@ int  csu_dev_surf_faces_noalloc(csu_dev* dev, double* out);
  * surface data) refuse a batch; a tube (time-accurate) run is not batched. */
 int  csu_batch_create(int32_t n, const csu_mesh_desc* const* meshes, const csu_params* const* prms,
                       csu_dev** out);
+/* 1 if two solves can share a batch (the options that select the kernels
+ * match), 0 if not, < 0 on a null argument.  Host arithmetic only. */
+int  csu_batch_compatible(const csu_mesh_desc* mesh_a, const csu_params* prm_a, const csu_mesh_desc* mesh_b,
+                          const csu_params* prm_b);
 int  csu_batch_size(const csu_dev* dev);                    /* solves, or -1 */
 int  csu_batch_ncell(const csu_dev* dev, int32_t solve);    /* cells of a solve, or -1 */
 /* State of one solve, SoA [nvar][ncell of the solve] FP64. */

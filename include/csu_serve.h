@@ -2,8 +2,9 @@
  *
  * No counterpart in the reference, which runs one solve per process
  * (solver.c:1836-1943).  Each admitted solve is an independent device
- * context with its own stream; the server keeps several in flight so their
- * kernels overlap on the GPU, admits queued solves as slots and HBM budget
+ * context with its own stream (or, with batching on, a batch of solves in
+ * one context); the server keeps several in flight so their kernels
+ * overlap on the GPU, admits queued solves as slots and HBM budget
  * free up, and retires a solve when its density residual has dropped by the
  * requested factor from its first iteration (or its iteration cap is hit).
  * Because contexts share nothing and every reduction is a fixed tree, a
@@ -43,6 +44,23 @@ typedef struct {
  * solve between convergence checks. */
 int  csu_server_create(int32_t max_slots, int64_t hbm_budget, int32_t chunk, csu_server** out);
 void csu_server_destroy(csu_server* s);
+
+/* Batched units (no counterpart in the reference).  max_solves > 1: each
+ * slot holds a batch context (csu_batch_create) instead of one solve: at
+ * admission the oldest queued solve and every later queued solve that can
+ * share its kernels (csu_batch_compatible) are packed into one batch, up to
+ * max_solves solves and max_cells cells (0: no cell limit) within the HBM
+ * budget, and run with one set of launches per iteration.  A solve of a
+ * batch is held once it stops and its result is reported at once; the
+ * batch's memory is freed when its last solve stops.  Results are those of
+ * the solve alone, bit for bit, with two exceptions: a batch runs whole
+ * chunks, so a solve stopped by its iteration cap keeps the state of the
+ * end of that chunk; and a mixed-precision batch switches to FP64 as a
+ * whole, once every running solve has reached its switch drop or stalled
+ * (then the converged results agree to the converged tolerance, not bit
+ * for bit).  max_solves 1 (the default): one context per solve.  Set
+ * before the first csu_server_step. */
+int  csu_server_set_batching(csu_server* s, int32_t max_solves, int64_t max_cells);
 
 /* Queue a solve.  mesh and prm (and U0 if not NULL: initial state
  * [nvar][ncell], otherwise the free stream) must stay valid until the solve

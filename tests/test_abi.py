@@ -53,3 +53,18 @@ def test_footprint_estimate_scales(tmp_path):
     (b1, n1), (b2, n2), (b3, _) = out
     assert b2 > 3 * b1 / 2 and b3 > b2
     assert 400 < b2 / n2 < 4000
+
+
+def test_batch_compatible(tmp_path):
+    """csu_batch_compatible (host arithmetic only): flow conditions and CFL
+    may differ within a batch, the options that select kernels may not."""
+    cs = []
+    for k, inp in enumerate(({"mach": "2.0", "CFL": "0.5"}, {"mach": "3.0", "CFL": "0.8"}, {"flux": "AUSM"},
+                             {"timeScheme": "LUSGS"})):
+        d = str(tmp_path / f"b{k}") + "/"
+        cases.ramp_case(d, nx=8 + k, ny=4, threads=1, nmax=1, **inp)
+        cs.append(native.HostCase(d))
+    assert native.batch_compatible(cs[0], cs[1])
+    assert not native.batch_compatible(cs[0], cs[2])
+    assert not native.batch_compatible(cs[0], cs[3])
+    assert native.lib().csu_batch_compatible(None, None, None, None) < 0
