@@ -333,11 +333,12 @@ def _converge(d: str, precision: int, drop: float, max_iter: int, chunk: int = 5
             break
         if r[-1] < 0.9 * best:
             best, stall = r[-1], 0
-        else:
+        elif r[-1] < 0.1 * peak:
             stall += 1
         if switch_drop and it_switch is None and (r[-1] <= switch_drop * peak or stall >= 4):
             # the server's rule: FP64 from the switch drop, or once FP32
-            # rounding stalls the residual (no 10 % fall in four chunks)
+            # rounding stalls the residual (below a tenth of its peak, no
+            # 10 % fall in four chunks)
             dev.set_precision(0)
             it_switch = it
         if cfl_ramp and cfl < cfl_ramp[0]:

@@ -1708,25 +1708,27 @@ extern "C" int csu_batch_run(csu_dev* d, double drop, int32_t max_iter, int32_t 
         d->fresh = 1;
       }
     }
-    // mixed precision: FP64 once every running solve has reached switch_drop
-    // or stalled (its residual has not fallen by 10 % in four chunks: FP32
-    // flux rounding dominates it)
+    // mixed precision: FP64 as soon as one running solve has reached
+    // switch_drop or stalled (its residual, below a tenth of its peak, has
+    // not fallen by 10 % in four chunks: FP32 flux rounding dominates it);
+    // from there on mixed precision would hold that solve back, and FP64 is
+    // never wrong for the others
     for (int q = 0; q < ns; q++) {
       if (!d->sol_host[q].active) continue;
       if (chunks[q] == 1 || out[q].res_last < 0.9 * best[q]) {
         best[q] = out[q].res_last;
         stall[q] = 0;
-      } else {
+      } else if (out[q].res_last < 0.1 * out[q].res_peak) {
         stall[q]++;
       }
     }
     if (switch_drop > 0.0 && d->prm.precision == CSU_PREC_MIXED && running > 0) {
-      bool all = true;
+      bool any = false;
       for (int q = 0; q < ns; q++)
-        if (d->sol_host[q].active && !(out[q].res_last <= switch_drop * (peak_ref ? out[q].res_peak : out[q].res0)) &&
-            stall[q] < 4)
-          all = false;
-      if (all) {
+        if (d->sol_host[q].active && (out[q].res_last <= switch_drop * (peak_ref ? out[q].res_peak : out[q].res0) ||
+                                      stall[q] >= 4))
+          any = true;
+      if (any) {
         if ((rc = csu_dev_set_precision(d, CSU_PREC_FP64))) return rc;
         for (int q = 0; q < ns; q++)
           if (d->sol_host[q].active) out[q].iter_switch = out[q].iters;

@@ -13,10 +13,10 @@
 // freed when its last solve retires.  Units are independent, so the GPU
 // overlaps their kernels, and a solve's numbers do not depend on what else
 // runs (batched solves are bit-equal to their runs alone; only the
-// mixed-precision switch to FP64, taken by a batch as a whole, depends on
-// the batch).  Contexts are created on worker threads (host geometry,
-// uploads, wall distance), so admission overlaps with the iterations of the
-// solves already running.
+// mixed-precision switch to FP64, taken by a batch as a whole when its
+// first solve needs it, depends on the batch).  Contexts are created on
+// worker threads (host geometry, uploads, wall distance), so admission
+// overlaps with the iterations of the solves already running.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -381,11 +381,13 @@ static int scan_chunk(Job* j, int chunk_n, const double* res, const double* coef
     }
   }
   // mixed precision levels off where its FP32 fluxes' rounding dominates
-  // the residual: the stall count (no 10 % fall of the smallest residual)
+  // the residual: the stall count (chunks without a 10 % fall of the
+  // smallest residual), counted once the residual is below a tenth of its
+  // peak (before that a rising start-up transient is no stall)
   if (j->res.iters <= chunk_n || j->res.res_last < 0.9 * j->best) {
     j->best = j->res.res_last;
     j->stalled = 0;
-  } else {
+  } else if (j->res.res_last < 0.1 * j->peak) {
     j->stalled++;
   }
   return 0;
@@ -406,7 +408,7 @@ static int harvest(csu_server* s, size_t k) {
     return n;
   }
   int retired = 0;
-  bool any_mixed = false, all_switch = true;
+  bool to_fp64 = false;
   for (Job* j : u->jobs) {
     if (!j->running) continue;
     int status = CSU_OK;
@@ -416,11 +418,8 @@ static int harvest(csu_server* s, size_t k) {
       continue;
     }
     if (j->r.prm->precision == CSU_PREC_MIXED && j->r.switch_drop > 0.0 && j->res.iter_switch < 0) {
-      any_mixed = true;
       const double ref = j->r.peak_ref ? j->peak : j->res.res0;
-      if (!(j->res.res_last <= j->r.switch_drop * ref) && j->stalled < 4) all_switch = false;
-    } else {
-      all_switch = false;
+      if (j->res.res_last <= j->r.switch_drop * ref || j->stalled >= 4) to_fp64 = true;
     }
     if (j->r.cfl_growth > 1.0 && j->cfl < j->r.cfl_max) {
       j->cfl = std::min(j->cfl * j->r.cfl_growth, j->r.cfl_max);
@@ -428,10 +427,11 @@ static int harvest(csu_server* s, size_t k) {
       if (rc) break;
     }
   }
-  if (!rc && u->running > 0 && any_mixed && all_switch) {
+  if (!rc && u->running > 0 && to_fp64) {
     // mixed precision has done its part: FP64 to the converged state (a
-    // batch switches once every running solve has reached its switch drop
-    // or stalled)
+    // batch switches as soon as one running solve has reached its switch
+    // drop or stalled: from there on mixed precision would hold that solve
+    // back, and FP64 is never wrong for the others)
     if (!(rc = csu_dev_set_precision(u->dev, CSU_PREC_FP64)))
       for (Job* j : u->jobs)
         if (j->running) j->res.iter_switch = j->res.iters;

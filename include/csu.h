@@ -271,9 +271,9 @@ typedef struct csu_batch_result {
  * state is that of its run alone after iters_run iterations.  With
  * growth > 1 each running solve's CFL is multiplied by growth after every
  * chunk, up to cfl_max[solve] (cfl_max NULL: no ramp).  A mixed-precision
- * batch with switch_drop > 0 continues in FP64 once every running solve
+ * batch with switch_drop > 0 continues in FP64 as soon as one running solve
  * has reached that drop or stalled (no 10 % fall of its residual in four
- * chunks).  out: [n]. */
+ * chunks once below a tenth of its peak).  out: [n]. */
 int  csu_batch_run(csu_dev* dev, double drop, int32_t max_iter, int32_t chunk,
                    const double* cfl_max, double growth, int32_t peak_ref, double switch_drop,
                    csu_batch_result* out);

@@ -56,9 +56,9 @@ void csu_server_destroy(csu_server* s);
  * the solve alone, bit for bit, with two exceptions: a batch runs whole
  * chunks, so a solve stopped by its iteration cap keeps the state of the
  * end of that chunk; and a mixed-precision batch switches to FP64 as a
- * whole, once every running solve has reached its switch drop or stalled
- * (then the converged results agree to the converged tolerance, not bit
- * for bit).  max_solves 1 (the default): one context per solve.  Set
+ * whole, as soon as one running solve has reached its switch drop or
+ * stalled (then the converged results agree to the converged tolerance,
+ * not bit for bit).  max_solves 1 (the default): one context per solve.  Set
  * before the first csu_server_step. */
 int  csu_server_set_batching(csu_server* s, int32_t max_solves, int64_t max_cells);
 
@@ -85,9 +85,10 @@ int  csu_server_set_ramp(csu_server* s, int32_t id, double cfl_max, double growt
  * one (a solve started from a uniform free stream past a wall or an airfoil
  * has a first density residual near rounding level, which makes drops from
  * it meaningless).  switch_drop > 0 (a mixed-precision job): once the drop
- * reaches it, or the residual has not fallen by 10 % in four chunks (the
- * level where FP32 flux rounding dominates it), the job continues in FP64
- * (csu_dev_set_precision), so it converges to the FP64 steady state. */
+ * reaches it, or the residual has not fallen by 10 % in four chunks once
+ * below a tenth of its peak (the level where FP32 flux rounding dominates
+ * it), the job continues in FP64 (csu_dev_set_precision), so it converges
+ * to the FP64 steady state. */
 int  csu_server_set_options(csu_server* s, int32_t id, int32_t peak_ref, double switch_drop);
 
 /* Move up to max finished results into out; returns the count. */

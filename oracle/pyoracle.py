@@ -61,12 +61,16 @@ class Oracle:
     """CPU restatement over the same mesh description / params as the device."""
 
     def __init__(self, case, params=None, threads: int = 0):
+        """threads 0: one thread below 8,192 cells (the parallel loops' fork
+        and join cost more than they save there), otherwise every core up
+        to 16.  The oracle's loops have no reductions, so results do not
+        depend on the thread count."""
         L = lib()
         self.case = case
         self.prm = params if params is not None else case.params
         self.nvar, self.ncell, self.nbface = self.prm.nvar, case.ncell, case.nbface
-        if threads:
-            L.oracle_set_threads(threads)
+        self.threads = threads or (1 if case.ncell < 8192 else min(16, os.cpu_count() or 1))
+        L.oracle_set_threads(self.threads)
         self._h = C.c_void_p()
         rc = L.oracle_create(C.addressof(case.mesh), C.addressof(self.prm), C.byref(self._h))
         if rc != 0:
@@ -91,6 +95,7 @@ class Oracle:
         out = {k: np.empty((nv, nc)) for k in ("R", "dPx", "dPy", "phi")}
         out["P"] = np.empty((nc, nv + 1))
         out["Pb"] = np.empty((max(nb, 1), nv + 1))
+        lib().oracle_set_threads(self.threads)
         lib().oracle_residual(self._h, *[_p(out[k]) for k in ("R", "dPx", "dPy", "phi", "P", "Pb")])
         out["Pb"] = out["Pb"][:nb]
         return out
@@ -103,6 +108,7 @@ class Oracle:
         out = {"D": np.empty(nc), "A": np.empty((nc, 4, nv, nv)), "nb": np.empty((nc, 4), np.int32),
                "nrm": np.empty((nc, 4, 3)), "dq": np.empty((6, nc)), "R": np.empty((nv, nc)),
                "dW": np.empty((nv, nc))}
+        lib().oracle_set_threads(self.threads)
         rc = lib().oracle_dplur_system(self._h, _p(out["D"]), _p(out["A"]),
                                        out["nb"].ctypes.data_as(C.POINTER(C.c_int32)), _p(out["nrm"]),
                                        _p(out["dq"]), _p(out["R"]), _p(out["dW"]))
@@ -113,6 +119,7 @@ class Oracle:
     def iterate(self, n: int):
         res = np.empty((n, self.nvar))
         coef = np.empty((n, 4))
+        lib().oracle_set_threads(self.threads)
         lib().oracle_iterate(self._h, n, _p(res), _p(coef))
         return res, coef
 
@@ -124,6 +131,7 @@ class Oracle:
         """Time-accurate run to tmax (tube mode).  Returns (t, steps, finished)."""
         t = C.c_double(t0)
         n, fin = C.c_int(0), C.c_int(0)
+        lib().oracle_set_threads(self.threads)
         rc = lib().oracle_advance_time(self._h, tmax, max_iter, C.byref(t), C.byref(n), C.byref(fin))
         if rc != 0:
             raise RuntimeError(f"oracle_advance_time failed ({rc})")

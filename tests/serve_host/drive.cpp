@@ -4,6 +4,7 @@
 // stand-in's closed-form residual; batches must hold only solves that can
 // share kernels and respect their limits; every context must be freed.
 // Prints "ok" and exits 0 when every check holds.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -167,15 +168,43 @@ int main() {
     CHECK(mo.res.at(k).iter_switch == chunk * (int)std::ceil(tsw / chunk));
     last = std::max(last, mo.res.at(k).iter_switch);
   }
-  // in one batch: at the slowest one's switch, for the solves still running
+  // in one batch: at the first one's switch, for every solve still running
   // then (a solve that stopped before it ran mixed precision to the end)
+  int first = 1 << 30;
+  for (int k = 0; k < 4; k++) first = std::min(first, mo.res.at(k).iter_switch);
+  CHECK(first < last);
   int switched = 0;
   for (int k = 0; k < 4; k++) {
     const csu_job_result& r = mb.res.at(k);
-    if (r.iters > last) CHECK(r.iter_switch == last && ++switched);
+    if (r.iters > first) CHECK(r.iter_switch == first && ++switched);
     else CHECK(r.iter_switch == -1);
   }
-  CHECK(switched >= 1);
+  CHECK(switched >= 2);
+  // the stall rule: a residual that rises through a start-up transient is no
+  // stall; one at its floor, below a tenth of its peak, is after four chunks
+  // without a 10 % fall
+  std::vector<Spec> st(1);
+  st[0].ncell = 3000;
+  st[0].prec = CSU_PREC_MIXED;
+  st[0].cfl = 10.0;
+  st[0].mach = 4.0;
+  st[0].max_iter = 600;
+  st[0].switch_drop = 1e-6;
+  Run sr = serve(st, 1, 1, 0, big, chunk, drop);
+  int t_floor = 100 + (int)std::ceil(std::log(600.0) / 0.05);  // first iteration at the floor
+  int sw = -1, stall = 0;
+  double best = 0, peak = 0;
+  for (int c = 1; c * chunk <= 600 && sw < 0; c++) {
+    const int t = c * chunk;
+    for (int u = t - chunk + 1; u <= t; u++)
+      peak = std::fmax(peak, u < 100 ? 10.0 * (1.0 + 0.05 * u) : std::fmax(60.0 * std::exp(-0.05 * (u - 100)), 0.1));
+    const double r = t < 100 ? 10.0 * (1.0 + 0.05 * t) : std::fmax(60.0 * std::exp(-0.05 * (t - 100)), 0.1);
+    if (c == 1 || r < 0.9 * best) { best = r; stall = 0; }
+    else if (r < 0.1 * peak) stall++;
+    if (stall >= 4) sw = t;
+  }
+  CHECK(sw > t_floor && sw > 4 * chunk);
+  CHECK(sr.res.at(0).iter_switch == sw);
   std::printf("ok\n");
   return 0;
 }

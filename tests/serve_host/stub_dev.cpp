@@ -31,6 +31,8 @@ extern "C" int stub_max_batch(void) { return g_max_batch; }
 
 static double resid(const csu_dev* d, int q, int t) {
   if (d->mach[q] > 5.0 && t >= 25) return NAN;
+  if (d->mach[q] == 4.0)  // a start-up transient rising for 100 iterations, then a floor
+    return t < 100 ? 10.0 * (1.0 + 0.05 * t) : std::fmax(60.0 * std::exp(-0.05 * (t - 100)), 0.1);
   return 10.0 * std::exp(-0.01 * d->cfl[q] * t);
 }
 

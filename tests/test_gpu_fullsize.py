@@ -121,11 +121,12 @@ def test_mixed_precision_plate_converged(tmpcase):
                 peak = max(peak, x)
                 done = done or x <= 1e-8 * peak
             it += 500
-            # the server's rule: FP64 from a 1e-4 drop, or once the residual
-            # has not fallen by 10 % in four chunks (FP32 rounding level)
+            # the server's rule: FP64 from a 1e-4 drop, or once the residual,
+            # below a tenth of its peak, has not fallen by 10 % in four
+            # chunks (FP32 rounding level)
             if r[-1, 0] < 0.9 * best:
                 best, stall = r[-1, 0], 0
-            else:
+            elif r[-1, 0] < 0.1 * peak:
                 stall += 1
             if prec == 1 and switched is None and (r[-1, 0] <= 1e-4 * peak or stall >= 4):
                 dev.set_precision(0)
