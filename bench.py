@@ -223,20 +223,21 @@ def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=Non
             batch_slots: int = 4):
     """Solves/hour and time-to-converge (1e-8 density-residual drop) of the
     served forms -- batched units (`batch_slots` batch contexts of up to
-    `batch` solves in flight), and `slots` one-solve contexts in flight;
-    both mixed precision switched to FP64 at a 1e-4 drop -- against the
-    plain FP64 one-at-a-time path (one slot) on the same requests: a
-    Poisson stream at `rate` requests/s (p50/p99 time-to-converge), and all
-    requests queued at once (saturated throughput); with the forces of the
-    solves the served and one-at-a-time forms both converge, mixed against
-    FP64."""
+    `batch` solves in flight) in FP64 and in mixed precision switched to
+    FP64 at a 1e-4 drop, and `slots` one-solve contexts in flight in mixed
+    precision -- against the plain FP64 one-at-a-time path (one slot) on the
+    same requests: a Poisson stream at `rate` requests/s (p50/p99
+    time-to-converge), and all requests queued at once (saturated
+    throughput); with the forces of the solves the batched mixed and
+    one-at-a-time forms both converge, mixed against FP64."""
     n = len(reqs)
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     out = {"requests": n, "rate_per_s": rate, "slots": slots, "batch_slots": batch_slots, "batch_solves": batch,
            "drop": 1e-8, "drop_from": "largest residual so far" if peak_ref else "first iteration",
            "cells": [c.ncell for _, c in reqs], "kinds": [k for k, _ in reqs]}
-    forms = (("served", batch_slots, 1, batch), ("served_contexts", slots, 1, 1), ("one_at_a_time", 1, 0, 1))
+    forms = (("served_batched", batch_slots, 0, batch), ("served_batched_mixed", batch_slots, 1, batch),
+             ("served_contexts_mixed", slots, 1, 1), ("one_at_a_time", 1, 0, 1))
     sats = {}
     for name, sl, prec, bt in forms:
         kw = dict(precision=prec, switch_drop=1e-4, peak_ref=peak_ref, caps=caps, ramps=ramps, batch=bt)
@@ -263,9 +264,10 @@ def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=Non
                      "cell_iters_per_s_saturated": sum(r["iters"] * r["ncell"] for r in sat) / span}
         log(f"serving {name}: stream p50 {out[name]['p50_ttc_s']:.3f} s p99 {out[name]['p99_ttc_s']:.3f} s")
     one = out["one_at_a_time"]["solves_per_hr_saturated"]
-    out["speedup_solves_per_hr"] = out["served"]["solves_per_hr_saturated"] / one if one else None
-    out["speedup_solves_per_hr_contexts"] = out["served_contexts"]["solves_per_hr_saturated"] / one if one else None
-    both = [(a, b) for a, b in zip(sats["served"], sats["one_at_a_time"]) if a["converged"] and b["converged"]]
+    for name in ("served_batched", "served_batched_mixed", "served_contexts_mixed"):
+        out["speedup_solves_per_hr_" + name[7:]] = out[name]["solves_per_hr_saturated"] / one if one else None
+    both = [(a, b) for a, b in zip(sats["served_batched_mixed"], sats["one_at_a_time"])
+            if a["converged"] and b["converged"]]
     if both:
         rel = [abs(a["coef_last"][k] + a["coef_last"][k + 1] - b["coef_last"][k] - b["coef_last"][k + 1])
                / max(abs(b["coef_last"][k] + b["coef_last"][k + 1]), 1e-300) for a, b in both for k in (0, 2)]

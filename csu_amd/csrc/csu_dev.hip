@@ -2022,12 +2022,10 @@ extern "C" int csu_stream_rate(int64_t bytes, int32_t reps, double* gbps) {
   }
   int rc = CSU_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int dev = 0, ncu = 0;
-  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
-      hipMemset(x, 0, n2 * 16) != hipSuccess)
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || hipMemset(x, 0, n2 * 16) != hipSuccess)
     rc = set_err(CSU_ERR_HIP, "stream set-up");
-  const unsigned grid = (unsigned)std::max(1, ncu) * 16;
+  if (n2 > (size_t)CSU_BLOCK * 0x7fffffffu) rc = set_err(CSU_ERR_ARG, "stream of %lld bytes", (long long)bytes);
+  const unsigned grid = (unsigned)((n2 + CSU_BLOCK - 1) / CSU_BLOCK);
   float best = 1e30f;
   for (int r = 0; r <= reps && !rc; r++) {   // launch 0 warms up
     float ms = 0.f;

@@ -91,13 +91,14 @@ struct Geo {
 // Mutable per-stage arrays, passed by value.
 struct St {
   double *P, *Pb, *dPx, *dPy, *phi, *dTx, *dTy;
-  double *fflux, *vflux, *bvflux, *wfric, *miT;
+  double *fflux;        // convective face flux times dS, per face [nf][NV] (one gather per face in k_update)
+  double *vflux, *bvflux, *wfric, *miT;
   double *pref2;        // limiter reference values [solves][6]
   double *src;          // SA: Qt*Omega [nc]; SST: Qtk*Omega, Qto*Omega [2][nc]
   double *dq;           // SST source Jacobians dQk/d(r, rk, ro), dQo/d(r, rk, ro) [6][nc]
   double *rec;          // second order: reconstructed state at each cell edge [4][NV][nc]
   float *rec32;         // mixed precision: the same in FP32
-  float *fflux32;       // mixed precision: convective face fluxes in FP32 [NV][nf]
+  float *fflux32;       // mixed precision: convective face fluxes in FP32 [nf][NV]
   double *wten;         // viscous: wall shear tensor txx txy tyy yp per boundary face [4][nb]
   double *bflux;        // convective boundary-face flux times dS [NV][nb]
   const double *tube;   // time-accurate run: t, tmax, state (0 running, 1 last step, 2 past it), steps

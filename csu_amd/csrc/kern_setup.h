@@ -4,14 +4,20 @@
 
 namespace csu {
 
-// HBM streaming reference: y = s x over n doubles, two per lane per step
-// (16-byte loads and stores), grid-stride.  Measures the read + write rate
-// the stage kernels' bytes are compared with.
+// HBM streaming reference: y = s x over n doubles, two per lane (16-byte
+// accesses), one pair per lane over a grid that covers the arrays, with
+// non-temporal loads and stores.  Of the forms measured on MI355X (grid-
+// stride loops of 8-64 workgroups per CU: 4.4-4.9 TB/s read + write; one
+// pair per lane: 6.2 TB/s; four pairs per lane: 5.8 TB/s; a device-to-device
+// hipMemcpy: 5.3 TB/s) this one is the fastest, 6.7 TB/s: the practical
+// read + write ceiling the stage kernels' bytes are compared with.
 __global__ __launch_bounds__(CSU_BLOCK) void k_stream(const double2* __restrict__ x, double2* __restrict__ y,
                                                       size_t n2, double s) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
-    const double2 v = x[i];
-    y[i] = make_double2(v.x * s, v.y * s);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) {
+    const double a = __builtin_nontemporal_load(&x[i].x), b = __builtin_nontemporal_load(&x[i].y);
+    __builtin_nontemporal_store(a * s, &y[i].x);
+    __builtin_nontemporal_store(b * s, &y[i].y);
   }
 }
 

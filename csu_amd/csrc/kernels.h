@@ -176,8 +176,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       ex[j] = g.edx32[j * nc + i];
       ey[j] = g.edy32[j * nc + i];
     } else {
-      ex[j] = g.edx[j * nc + i];
-      ey[j] = g.edy[j * nc + i];
+      // read once per stage: non-temporal, so they do not displace the
+      // neighbour states in the caches
+      ex[j] = __builtin_nontemporal_load(&g.edx[j * nc + i]);
+      ey[j] = __builtin_nontemporal_load(&g.edy[j * nc + i]);
     }
   }
   // Least squares (solver.c:1031-1074).  Default: gx = sum_j wx_j du_j with
@@ -207,8 +209,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
         wx[j] = g.lwx32[j * nc + i];
         wy[j] = g.lwy32[j * nc + i];
       } else {
-        wx[j] = g.lwx[j * nc + i];
-        wy[j] = g.lwy[j * nc + i];
+        wx[j] = __builtin_nontemporal_load(&g.lwx[j * nc + i]);
+        wy[j] = __builtin_nontemporal_load(&g.lwy[j * nc + i]);
       }
     }
   }
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
       for (int j = 0; j < 4; j++) {
         if (j >= np) break;
         if (MIX) s.rec32[(j * NV + kk) * nc + i] = (float)(first ? u0 : u0 + (double)(d2[j] * ph));
-        else s.rec[(j * NV + kk) * nc + i] = first ? u0 : u0 + d2[j] * ph;
+        else __builtin_nontemporal_store(first ? u0 : u0 + d2[j] * ph, &s.rec[(j * NV + kk) * nc + i]);
       }
     }
   }
@@ -493,12 +495,12 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
       float fl[NV];
       conv_flux<NV, true>(kp, g, s, f, e0, e1, sl & 3, sl >> 2, fl);
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) s.fflux32[kk * nf + f] = fl[kk];
+      for (int kk = 0; kk < NV; kk++) s.fflux32[(size_t)f * NV + kk] = fl[kk];
     } else {
       double fl[NV];
       conv_flux<NV, false>(kp, g, s, f, e0, e1, sl & 3, sl >> 2, fl);
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) s.fflux[kk * nf + f] = fl[kk];
+      for (int kk = 0; kk < NV; kk++) __builtin_nontemporal_store(fl[kk], &s.fflux[(size_t)f * NV + kk]);
     }
   }
   const double dSx = g.fdsx[f], dSy = g.fdsy[f];
@@ -620,7 +622,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
     for (int j = 0; j < 4; j++) {
       const int f = cf[j], q = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) fv[j][kk] = MIX ? (double)s.fflux32[kk * nf + q] : s.fflux[kk * nf + q];
+      for (int kk = 0; kk < NV; kk++)
+        fv[j][kk] = MIX ? (double)s.fflux32[(size_t)q * NV + kk] : s.fflux[(size_t)q * NV + kk];
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
