@@ -75,7 +75,13 @@ MEASURED_TRAFFIC = {("k_grad", 2000, 1000, True): 993.9e6, ("k_flux", 2000, 1000
 _MESHES = {}
 
 
-def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision: int = 0, keep: bool = False):
+def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision: int = 0, keep: bool = False,
+            tune: bool = True, shape=None):
+    """The ramp on an nx x ny x 2 mesh: launch shapes autotuned (tune) or set
+    (shape: block, LU-SGS block, XCD map), warmup iterations, then `steps`
+    timed graph-replayed iterations; with profile, `steps` more with HIP
+    events around every kernel group (the per-kernel times; eager launches,
+    so that pass is slower and not the timed one)."""
     from csu_amd import cases, native
     d = tempfile.mkdtemp(prefix="bn", dir="/tmp") + "/"
     if (nx, ny) in _MESHES:       # same mesh, other options: link it, write input.dat
@@ -86,17 +92,29 @@ def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision:
         _MESHES[(nx, ny)] = d + "mesh.su2"
     case = native.HostCase(d)
     dev = native.Device(case)
+    if shape is not None:
+        dev.set_launch_shape(*shape)
+        shape = dev.launch_shape()
+    else:
+        shape = dev.autotune(4) if tune else dev.launch_shape()
     dev.iterate(warmup, want=False)
     dev.sync()
-    dev.set_profiling(profile)
     t0 = time.perf_counter()
     dev.iterate(steps, want=False)
     dev.sync()
     t1 = time.perf_counter()
-    kt = dev.kernel_times() if profile else {}
+    kt, tp = {}, None
+    if profile:
+        dev.set_profiling(True)
+        p0 = time.perf_counter()
+        dev.iterate(steps, want=False)
+        dev.sync()
+        tp = time.perf_counter() - p0
+        kt = dev.kernel_times()
+        dev.set_profiling(False)
     res, coef = dev.iterate(1)
-    info = dict(nc=case.ncell, nf=case.nface, nb=case.nbface, nv=case.nvar,
-                seconds=t1 - t0, kernels=kt, res=res[0].tolist(), coef=coef[0].tolist())
+    info = dict(nc=case.ncell, nf=case.nface, nb=case.nbface, nv=case.nvar, launch_shape=shape,
+                seconds=t1 - t0, profiled_seconds=tp, kernels=kt, res=res[0].tolist(), coef=coef[0].tolist())
     if keep:
         info["U"] = dev.get_state()
     dev.close()
@@ -663,6 +681,9 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=8, help="solves per batched unit of the served form")
     ap.add_argument("--serve-batch-slots", type=int, default=4, help="batched units in flight")
     ap.add_argument("--only-serve", action="store_true", help="only the serving measurements; prints their JSON")
+    ap.add_argument("--shape", default=None, metavar="B,L,X",
+                    help="launch shapes of the ramp runs instead of the autotuner's (e.g. 128,64,0: the choice a "
+                         "timed run reported), so that profiling runs launch the kernels the timed run did")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 200x100x2 timing (profiling runs of the big mesh alone)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -711,18 +732,21 @@ def main():
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     log(f"ramp {a.nx}x{a.ny}x2: generating, loading, {a.warmup}+{a.steps} iterations")
-    big = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=True, keep=not a.no_mixed)
+    shape = tuple(int(v) for v in a.shape.split(",")) if a.shape else None
+    big = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=True, keep=not a.no_mixed, shape=shape)
     log(f"big done: {big['seconds']:.3f} s")
     mixed = None
     if not a.no_mixed:
-        mx = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=False, precision=1, keep=True)
+        mx = gpu_run(a.nx, a.ny, a.steps, a.warmup, profile=True, precision=1, keep=True, shape=shape)
         U64, U32 = big.pop("U"), mx.pop("U")
         c64, c32 = big["coef"], mx["coef"]
         cx64, cy64, cx32, cy32 = c64[0] + c64[1], c64[2] + c64[3], c32[0] + c32[1], c32[2] + c32[3]
         mixed = {"value": mx["nc"] * a.steps / mx["seconds"], "unit": "cell-iter/s",
                  "ms_per_step": 1e3 * mx["seconds"] / a.steps,
                  "speedup_vs_fp64": big["seconds"] / mx["seconds"],
-                 "after_iterations": a.warmup + a.steps + 1,
+                 "after_iterations": a.warmup + 2 * a.steps + 1,
+                 "kernels_ms": {k: round(v[0], 5) for k, v in mx["kernels"].items()},
+                 "launch_shape": mx["launch_shape"],
                  "field_linf_rel": (np.abs(U32 - U64).max(axis=1) / np.abs(U64).max(axis=1)).tolist(),
                  "Cx_rel": abs(cx32 / cx64 - 1) if cx64 else None, "Cy_rel": abs(cy32 / cy64 - 1) if cy64 else None,
                  "residual_rel": abs(mx["res"][0] / big["res"][0] - 1)}
@@ -752,6 +776,10 @@ def main():
             "kernel": dom,
             "kernel_avg_ms": avg_ms,
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
+            "kernel_times_from": "a second pass of the same number of iterations, launched eagerly with HIP "
+                                 "events around every kernel group on the context's stream",
+            "profiled_pass_ms_per_step": 1e3 * big["profiled_seconds"] / a.steps,
+            "launch_shape": big["launch_shape"],
             "stream_kernel_gbs": stream,
             "kernel_frac_of_stream": achieved / stream,
             "iteration_alg_bytes_per_cell": 2334.5,

@@ -430,6 +430,29 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
   }
 }
 
+// Largest of v over a 64-lane wavefront (fmax: NaN is the identity), by
+// lane exchanges; every lane receives it.  The maximum does not depend on
+// the order in which it is taken.
+__device__ __forceinline__ double wave_fmax(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Largest of every thread's v in the workgroup (NT threads), in every
+// thread: wavefront exchanges, then one LDS round over the wavefronts.
+template <int NT = CSU_BLOCK>
+__device__ __forceinline__ double block_fmax(double v, double* sw) {
+  constexpr int NW = NT / 64;
+  v = wave_fmax(v);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = sw[0];
+#pragma unroll
+  for (int w = 1; w < NW; w++) r = fmax(r, sw[w]);
+  return r;
+}
+
 // One damped Jacobi sweep m (implicitMultA + update, implicit.c:938-1022):
 // y = D x + sum_j A_j x_j;  x_new = w (-R - y)/D + x, from buf[(state[1] +
 // m) & 1] into the other buffer; state[1] is the buffer holding the last
@@ -447,7 +470,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
                                                       const double* __restrict__ R, double* buf0, double* buf1,
                                                       const int* __restrict__ state, double* __restrict__ red_sw,
                                                       int m, const double* __restrict__ dq) {
-  __shared__ double smax[CSU_BLOCK];
+  __shared__ double sw0[CSU_BLOCK / 64], sw1[CSU_BLOCK / 64];
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc, stride = gridDim.x;
   // batch: the solve's own blocks and buffer state
@@ -459,35 +482,50 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     b0 = q.c0 / CSU_BLOCK;
     nblocks = (q.c1 - q.c0 + CSU_BLOCK - 1) / CSU_BLOCK;
   }
+  const int par = (state[2 * sid + 1] + m) & 1;
+  const double* x = par ? buf1 : buf0;
+  double* y = par ? buf0 : buf1;
+  const bool act = i < kp.cend;
+  // The sweep's loads are issued before the previous sweep's stop test, so
+  // their latency overlaps with its reduction (a stopped sweep discards
+  // them).  Neighbours of all slots first, then their x; slots without a
+  // face read the cell itself and are skipped.
+  const int ii = act ? i : 0;
+  double xi[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) xi[k] = x[k * nc + ii];
+  int cfj[4], nbj[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) cfj[j] = act ? g.cface[j * nc + ii] : 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int f = cfj[j], fi = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
+    const int e = (f > 0 ? g.fe1 : g.fe0)[fi];
+    nbj[j] = f == 0 ? ii : e;
+  }
+  double xj[4][NV];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int n = 0; n < NV; n++) xj[j][n] = x[n * nc + nbj[j]];
   if (m > 0) {
-    const double* prev = red_sw + (size_t)(m - 1) * stride + b0;
     // NaN is fmax's identity: the maximum is NaN only when every cell's
     // change is, which the reference's loop (implicit.c:1023-1031) does not
     // take as converged
+    const double* prev = red_sw + (size_t)(m - 1) * stride + b0;
     double pm = __builtin_nan("");
     for (int b = t; b < nblocks; b += CSU_BLOCK) pm = fmax(pm, prev[b]);
-    smax[t] = pm;
-    __syncthreads();
-    for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-      if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
-      __syncthreads();
-    }
-    const bool stopped = smax[0] < 1e-14;
-    __syncthreads();  // smax is reused below
-    if (stopped) {
+    if (block_fmax(pm, sw0) < 1e-14) {
       if (t == 0) red_sw[(size_t)m * stride + blockIdx.x] = -1.0;
       return;
     }
   }
-  const int par = (state[2 * sid + 1] + m) & 1;
-  const double* x = par ? buf1 : buf0;
-  double* y = par ? buf0 : buf1;
   double diff = __builtin_nan("");  // lanes past the last cell: fmax's identity
-  if (i < kp.cend) {
+  if (act) {
     const double D = diag(Dp, g.omega, dtg, i, sid);
-    double xi[NV], acc[NV];
+    double acc[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) { xi[k] = x[k * nc + i]; acc[k] = D * xi[k]; }
+    for (int k = 0; k < NV; k++) acc[k] = D * xi[k];
     double qa = D, qb = 0.0, qc = 0.0, qd = D, q0 = 0.0, q3 = 0.0, det = 0.0;
     bool blk = false;
     if (NV == 6) {
@@ -500,22 +538,6 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
         acc[5] = qc * xi[4] + qd * xi[5] - q3 * xi[0];
       }
     }
-    // neighbours of all slots first, then their x, then the products in
-    // slot order (slots without a face read the cell itself, skipped)
-    int cfj[4], nbj[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) cfj[j] = g.cface[j * nc + i];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int f = cfj[j], fi = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
-      const int e = (f > 0 ? g.fe1 : g.fe0)[fi];
-      nbj[j] = f == 0 ? i : e;
-    }
-    double xj[4][NV];
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-#pragma unroll
-      for (int n = 0; n < NV; n++) xj[j][n] = x[n * nc + nbj[j]];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       if (cfj[j] == 0) continue;
@@ -549,13 +571,8 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
       }
     }
   }
-  smax[t] = diff;
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st) smax[t] = fmax(smax[t], smax[t + st]);
-    __syncthreads();
-  }
-  if (t == 0) red_sw[(size_t)m * stride + blockIdx.x] = smax[0];
+  const double bm = block_fmax(diff, sw1);
+  if (t == 0) red_sw[(size_t)m * stride + blockIdx.x] = bm;
 }
 
 // Buffer of the DPLUR result of solve sid, whose first block is b0 (rows of

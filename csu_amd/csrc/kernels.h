@@ -500,7 +500,12 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_flux(KP kp, Geo g, St s) {
       double fl[NV];
       conv_flux<NV, false>(kp, g, s, f, e0, e1, sl & 3, sl >> 2, fl);
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++) __builtin_nontemporal_store(fl[kk], &s.fflux[(size_t)f * NV + kk]);
+      for (int kk = 0; kk < NV; kk++) {
+        // a face's 32 bytes (NV = 4) stream out whole; the 40- and 48-byte
+        // records of the turbulent forms straddle lines and are stored plainly
+        if (NV == 4) __builtin_nontemporal_store(fl[kk], &s.fflux[(size_t)f * NV + kk]);
+        else s.fflux[(size_t)f * NV + kk] = fl[kk];
+      }
     }
   }
   const double dSx = g.fdsx[f], dSy = g.fdsy[f];

@@ -153,6 +153,7 @@ def lib():
     L.csu_server_set_ramp.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
     L.csu_server_set_options.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double]
     L.csu_server_set_batching.argtypes = [C.c_void_p, C.c_int32, C.c_int64]
+    L.csu_dev_set_launch_shape.argtypes = [C.c_void_p, i32p]
     L.csu_batch_compatible.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(MeshDesc),
                                        C.POINTER(Params)]
     L.csu_server_results.argtypes = [C.c_void_p, C.POINTER(JobResult), C.c_int32]
@@ -412,6 +413,10 @@ class Device:
         ch = np.zeros(3, np.int32)
         tuned = lib().csu_dev_launch_shape(self._h, _p(ch, C.c_int32))
         return {"block": int(ch[0]), "lusgs_block": int(ch[1]), "xcd_map": int(ch[2]), "tuned": bool(tuned == 1)}
+
+    def set_launch_shape(self, block: int, lusgs_block: int, xcd_map: int):
+        ch = np.array([block, lusgs_block, xcd_map], np.int32)
+        check(lib().csu_dev_set_launch_shape(self._h, _p(ch, C.c_int32)))
 
     def set_graphs(self, on: bool):
         check(lib().csu_dev_set_graphs(self._h, 1 if on else 0))

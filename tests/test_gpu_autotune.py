@@ -46,3 +46,30 @@ def test_tuned_equals_untuned(tmpcase, kind):
     assert sh["tuned"] and (sh["block"], sh["lusgs_block"], sh["xcd_map"]) == \
         (ch["block"], ch["lusgs_block"], ch["xcd_map"])
     dev2.close()
+
+
+@pytest.mark.parametrize("kind", ["ramp", "lusgs", "dplur"])
+def test_every_set_shape_gives_the_same_bits(tmpcase, kind):
+    """csu_dev_set_launch_shape: each shape the tuner may pick, set by hand
+    (before the first iteration and between iterations), gives the default
+    shape's results bit for bit."""
+    c = _case(tmpcase, kind)
+    ref = native.Device(c)
+    r0, k0 = ref.iterate(8)
+    U0 = ref.get_state()
+    ref.close()
+    for shape in ((64, 64, 0), (128, 256, 1), (256, 128, 0)):
+        dev = native.Device(c)
+        dev.set_launch_shape(*shape)
+        sh = dev.launch_shape()
+        assert (sh["block"], sh["lusgs_block"], sh["xcd_map"]) == shape and not sh["tuned"]
+        r1, k1 = dev.iterate(4)
+        dev.set_launch_shape(64, 128, 1)
+        r2, k2 = dev.iterate(4)
+        assert np.array_equal(r0, np.concatenate([r1, r2])) and np.array_equal(k0, np.concatenate([k1, k2]))
+        assert np.array_equal(dev.get_state(), U0)
+        dev.close()
+    dev = native.Device(c)
+    with pytest.raises(native.CsuError):
+        dev.set_launch_shape(96, 64, 0)
+    dev.close()
