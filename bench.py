@@ -60,16 +60,18 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int, pu: bool = False)
     return 0.0
 
 
-# HBM bytes per launch on the default 2000x1000x2 workload, from rocprofv3
-# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) passes, both scaled by what the
-# counters read for a plain 1 GiB copy (profiles/euler_ramp_4M_rk5_fp64.md),
-# for the primitives-from-state stages (True) and the stored form (False).
-# A counter pass cannot run inside this timed process, so the figure is
-# reported for that workload only.
-MEASURED_TRAFFIC = {("k_grad", 2000, 1000, True): 993.9e6, ("k_flux", 2000, 1000, True): 775.6e6,
-                    ("k_update", 2000, 1000, True): 626.5e6,
-                    ("k_grad", 2000, 1000, False): 1121.9e6, ("k_flux", 2000, 1000, False): 775.4e6,
-                    ("k_update", 2000, 1000, False): 787.2e6}
+# HBM bytes per launch on the default 2000x1000x2 workload (primitives-from-
+# state stages), from rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0)
+# passes, both scaled by what the counters read for the library's streaming
+# kernel (1 GiB read, 1 GiB written), per launch shape (gradient/face
+# workgroup size, XCD tile map; the residual kernel always runs 256-thread
+# workgroups): the bytes depend on the shape through the caches.  A counter
+# pass cannot run inside this timed process, so the figure is reported for
+# that workload and the shapes measured only.
+MEASURED_TRAFFIC = {("k_grad", 2000, 1000, 128, 0): 1089.2e6, ("k_flux", 2000, 1000, 128, 0): 930.6e6,
+                    ("k_update", 2000, 1000, 128, 0): 664.3e6,
+                    ("k_grad", 2000, 1000, 256, 1): 993.9e6, ("k_flux", 2000, 1000, 256, 1): 775.6e6,
+                    ("k_update", 2000, 1000, 256, 1): 626.5e6}
 
 
 _MESHES = {}
@@ -681,6 +683,9 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=8, help="solves per batched unit of the served form")
     ap.add_argument("--serve-batch-slots", type=int, default=4, help="batched units in flight")
     ap.add_argument("--only-serve", action="store_true", help="only the serving measurements; prints their JSON")
+    ap.add_argument("--only-small", action="store_true",
+                    help="only the 200x100x2 (40k-cell) ramp: autotuned, --warmup then max(--steps, 50) timed "
+                         "graph-replayed iterations; prints its JSON (the form kernel traces of it profile)")
     ap.add_argument("--shape", default=None, metavar="B,L,X",
                     help="launch shapes of the ramp runs instead of the autotuner's (e.g. 128,64,0: the choice a "
                          "timed run reported), so that profiling runs launch the kernels the timed run did")
@@ -720,6 +725,13 @@ def main():
         dt = (time.perf_counter() - t0) / 10
         print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8,
                           "copy_gb_per_s": 2 * n * 8 / dt / 1e9}))
+        return
+    if a.only_small:
+        r = gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False)
+        n = max(a.steps, 50)
+        print(json.dumps({"config0_40k": {"cells": r["nc"], "steps": n, "ms_per_step": 1e3 * r["seconds"] / n,
+                                          "cell_iter_per_s": r["nc"] * n / r["seconds"],
+                                          "launch_shape": r["launch_shape"]}}))
         return
     if a.only_serve:
         lg = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
@@ -772,7 +784,9 @@ def main():
     traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"], pu)
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((dom, a.nx, a.ny, pu)),
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC.get((dom, a.nx, a.ny, big["launch_shape"]["block"],
+                                             big["launch_shape"]["xcd_map"])) if pu else None,
             "kernel": dom,
             "kernel_avg_ms": avg_ms,
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},

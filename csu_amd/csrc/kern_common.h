@@ -43,6 +43,50 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
+// Reductions: within a 64-lane wavefront by lane exchanges (a butterfly:
+// after each step the two partners hold the same value, since a + b and
+// b + a round alike, so every lane ends with the same result and the
+// combination order is fixed); across a workgroup's wavefronts by one LDS
+// round, combined in wavefront order.  Deterministic for sums; minima and
+// maxima are exact in any order.
+struct RedSum {
+  __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct RedMin {
+  __device__ double operator()(double a, double b) const { return fmin(a, b); }
+};
+struct RedMax {
+  __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+
+template <class Op>
+__device__ __forceinline__ double wave_reduce(double v, Op op) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// NR values per thread over a workgroup of NT threads (every thread must
+// call it); sw: LDS scratch of NR x NT/64 doubles.  Every thread receives
+// the NR results.
+template <int NT, int NR, class Op>
+__device__ __forceinline__ void block_reduce(double (&v)[NR], double (*sw)[NT / 64], Op op) {
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int r = 0; r < NR; r++) v[r] = wave_reduce(v[r], op);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int r = 0; r < NR; r++) sw[r][threadIdx.x >> 6] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    double a = sw[r][0];
+#pragma unroll
+    for (int w = 1; w < NW; w++) a = op(a, sw[r][w]);
+    v[r] = a;
+  }
+}
+
 // Batch: solve of the workgroup whose first element is e0 (uniform across
 // the workgroup), from a per-chunk table.
 __device__ __forceinline__ int solve_of(const int* solk, int e0) { return solk[e0 / CSU_CHUNK]; }

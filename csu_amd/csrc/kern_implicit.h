@@ -20,7 +20,7 @@ __device__ __forceinline__ double gamma_of(const GasD& g, double T) { return gas
 template <int NV, bool VISC>
 __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double wimp, double* __restrict__ Dp,
                                                     double* __restrict__ red_min) {
-  __shared__ double smin[CSU_BLOCK];
+  __shared__ double sw[1][CSU_BLOCK / 64];
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc;
   if (g.sol) kp.cend = g.sol[solve_of(g.csolk, blockIdx.x * blockDim.x)].c1;
@@ -98,13 +98,9 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double w
     Dp[i] = D;
     val = om / dtl;
   }
-  smin[t] = val;
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st) smin[t] = fmin(smin[t], smin[t + st]);
-    __syncthreads();
-  }
-  if (t == 0) red_min[blockIdx.x] = smin[0];
+  double m[1] = {val};
+  block_reduce<CSU_BLOCK, 1>(m, sw, RedMin());
+  if (t == 0) red_min[blockIdx.x] = m[0];
 }
 
 // diagonal with the global-time-step term (implicit.c:176-180); dtg holds
@@ -430,27 +426,12 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
   }
 }
 
-// Largest of v over a 64-lane wavefront (fmax: NaN is the identity), by
-// lane exchanges; every lane receives it.  The maximum does not depend on
-// the order in which it is taken.
-__device__ __forceinline__ double wave_fmax(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-// Largest of every thread's v in the workgroup (NT threads), in every
-// thread: wavefront exchanges, then one LDS round over the wavefronts.
-template <int NT = CSU_BLOCK>
-__device__ __forceinline__ double block_fmax(double v, double* sw) {
-  constexpr int NW = NT / 64;
-  v = wave_fmax(v);
-  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = sw[0];
-#pragma unroll
-  for (int w = 1; w < NW; w++) r = fmax(r, sw[w]);
-  return r;
+// Largest of every thread's v in the workgroup (fmax: NaN is the identity,
+// the maximum is exact in any order), in every thread.
+__device__ __forceinline__ double block_fmax(double v, double (*sw)[CSU_BLOCK / 64]) {
+  double r[1] = {v};
+  block_reduce<CSU_BLOCK, 1>(r, sw, RedMax());
+  return r[0];
 }
 
 // One damped Jacobi sweep m (implicitMultA + update, implicit.c:938-1022):
@@ -470,7 +451,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
                                                       const double* __restrict__ R, double* buf0, double* buf1,
                                                       const int* __restrict__ state, double* __restrict__ red_sw,
                                                       int m, const double* __restrict__ dq) {
-  __shared__ double sw0[CSU_BLOCK / 64], sw1[CSU_BLOCK / 64];
+  __shared__ double sw0[1][CSU_BLOCK / 64], sw1[1][CSU_BLOCK / 64];
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc, stride = gridDim.x;
   // batch: the solve's own blocks and buffer state

@@ -366,7 +366,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_grad(KP kp, Geo g, St s) {
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_partial(KP kp, Geo g, const double* __restrict__ P,
                                                             double* __restrict__ red) {
-  __shared__ double smin[NV][CSU_BLOCK], smax[NV][CSU_BLOCK];
+  __shared__ double swn[NV][CSU_BLOCK / 64], swx[NV][CSU_BLOCK / 64];
   int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc;
   int c0 = 0;
@@ -376,25 +376,18 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_partial(KP kp, Geo g, const 
     kp.cend = q.c1;
   }
   int ii = i < kp.cend ? i : c0;
+  double mn[NV], mx[NV];
 #pragma unroll
   for (int jj = 0; jj < NV; jj++) {
     int mm = jj == 4 ? 5 : jj;  // limiter.c:350-357 (slot 5 also for jj == 5)
-    double v = P[mm * nc + ii];
-    smin[jj][t] = v; smax[jj][t] = v;
+    mn[jj] = mx[jj] = P[mm * nc + ii];
   }
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st)
-      for (int jj = 0; jj < NV; jj++) {
-        smin[jj][t] = fmin(smin[jj][t], smin[jj][t + st]);
-        smax[jj][t] = fmax(smax[jj][t], smax[jj][t + st]);
-      }
-    __syncthreads();
-  }
+  block_reduce<CSU_BLOCK, NV>(mn, swn, RedMin());
+  block_reduce<CSU_BLOCK, NV>(mx, swx, RedMax());
   if (t == 0)
     for (int jj = 0; jj < NV; jj++) {
-      red[(2 * jj) * gridDim.x + blockIdx.x] = smin[jj][0];
-      red[(2 * jj + 1) * gridDim.x + blockIdx.x] = smax[jj][0];
+      red[(2 * jj) * gridDim.x + blockIdx.x] = mn[jj];
+      red[(2 * jj + 1) * gridDim.x + blockIdx.x] = mx[jj];
     }
 }
 
@@ -403,7 +396,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_partial(KP kp, Geo g, const 
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, int stride, const SolveP* __restrict__ sol,
                                                           const double* __restrict__ red, double* __restrict__ pref2) {
-  __shared__ double smin[CSU_BLOCK], smax[CSU_BLOCK];
+  __shared__ double swn[NV][CSU_BLOCK / 64], swx[NV][CSU_BLOCK / 64];
   int t = threadIdx.x;
   if (sol) {
     const SolveP& q = sol[blockIdx.x];
@@ -411,24 +404,23 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lim1_final(KP kp, int nblocks, in
     nblocks = (q.c1 - q.c0 + CSU_BLOCK - 1) / CSU_BLOCK;
     pref2 += 6 * blockIdx.x;
   }
+  double mn[NV], mx[NV];
+#pragma unroll
   for (int jj = 0; jj < NV; jj++) {
-    double mn = red[(2 * jj) * stride], mx = red[(2 * jj + 1) * stride];
+    mn[jj] = red[(2 * jj) * stride];
+    mx[jj] = red[(2 * jj + 1) * stride];
     for (int b = t; b < nblocks; b += CSU_BLOCK) {
-      mn = fmin(mn, red[(2 * jj) * stride + b]);
-      mx = fmax(mx, red[(2 * jj + 1) * stride + b]);
+      mn[jj] = fmin(mn[jj], red[(2 * jj) * stride + b]);
+      mx[jj] = fmax(mx[jj], red[(2 * jj + 1) * stride + b]);
     }
-    smin[t] = mn; smax[t] = mx;
-    __syncthreads();
-    for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-      if (t < st) { smin[t] = fmin(smin[t], smin[t + st]); smax[t] = fmax(smax[t], smax[t + st]); }
-      __syncthreads();
-    }
-    if (t == 0) {
-      double aux = (smax[0] - smin[0]) * kp.limK;
+  }
+  block_reduce<CSU_BLOCK, NV>(mn, swn, RedMin());
+  block_reduce<CSU_BLOCK, NV>(mx, swx, RedMax());
+  if (t == 0)
+    for (int jj = 0; jj < NV; jj++) {
+      double aux = (mx[jj] - mn[jj]) * kp.limK;
       pref2[jj] = aux * aux;
     }
-    __syncthreads();
-  }
 }
 
 // Convective flux of interior face f times dS, reference inter() face loop
@@ -597,7 +589,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
                                                       double cfl, double* dtl, const double* dtg, double* Un,
                                                       double* __restrict__ Uout, double* __restrict__ Rout,
                                                       double* __restrict__ red_res, double* __restrict__ red_dt) {
-  __shared__ double sred[NV + 1][CSU_BLOCK];
+  __shared__ double swr[NV][CSU_BLOCK / 64], swd[1][CSU_BLOCK / 64];
   const int tile = kp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int t = threadIdx.x, i = tile * blockDim.x + t;
   const int nc = kp.nc, nf = kp.nf, nb = kp.nb;
@@ -777,23 +769,17 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
     }
   }
   if (!(flags & (F_LAST | F_RES))) return;
-  // residual-norm partials (and global time-step minimum): fixed tree
+  // residual-norm partials (and global time-step minimum): fixed-order
+  // wavefront and workgroup reductions
+  double r2[NV], dmin[1] = {dt_next};
 #pragma unroll
-  for (int kk = 0; kk < NV; kk++) sred[kk][t] = R[kk] * R[kk];
-  sred[NV][t] = dt_next;
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st) {
-#pragma unroll
-      for (int kk = 0; kk < NV; kk++) sred[kk][t] += sred[kk][t + st];
-      sred[NV][t] = fmin(sred[NV][t], sred[NV][t + st]);
-    }
-    __syncthreads();
-  }
+  for (int kk = 0; kk < NV; kk++) r2[kk] = R[kk] * R[kk];
+  block_reduce<CSU_BLOCK, NV>(r2, swr, RedSum());
+  block_reduce<CSU_BLOCK, 1>(dmin, swd, RedMin());
   if (t == 0) {
 #pragma unroll
-    for (int kk = 0; kk < NV; kk++) red_res[kk * gridDim.x + tile] = sred[kk][0];
-    red_dt[tile] = sred[NV][0];
+    for (int kk = 0; kk < NV; kk++) red_res[kk * gridDim.x + tile] = r2[kk];
+    red_dt[tile] = dmin[0];
   }
 }
 
@@ -802,7 +788,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
 template <int NV>
 __global__ __launch_bounds__(CSU_BLOCK) void k_dt(KP kp, Geo g, int local, double cfl, const double* __restrict__ U,
                                                   double* __restrict__ dtl, double* __restrict__ red) {
-  __shared__ double smin[CSU_BLOCK];
+  __shared__ double sw[1][CSU_BLOCK / 64];
   int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc;
   if (g.sol) {
@@ -819,20 +805,16 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_dt(KP kp, Geo g, int local, doubl
     if (local) dtl[i] = cfl * val;
   }
   if (local) return;
-  smin[t] = val;
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st) smin[t] = fmin(smin[t], smin[t + st]);
-    __syncthreads();
-  }
-  if (t == 0) red[blockIdx.x] = smin[0];
+  double m[1] = {val};
+  block_reduce<CSU_BLOCK, 1>(m, sw, RedMin());
+  if (t == 0) red[blockIdx.x] = m[0];
 }
 
 // Global time step: CFL x the minimum of the block partials.  Batch (sol
 // non-null): one workgroup per solve, over its own blocks, with its CFL.
 __global__ __launch_bounds__(CSU_BLOCK) void k_dt_final(int nblocks, double cfl, const double* __restrict__ red,
                                                         double* __restrict__ dtglob, const SolveP* __restrict__ sol) {
-  __shared__ double smin[CSU_BLOCK];
+  __shared__ double sw[1][CSU_BLOCK / 64];
   int t = threadIdx.x;
   if (sol) {
     const SolveP& q = sol[blockIdx.x];
@@ -841,15 +823,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_dt_final(int nblocks, double cfl,
     cfl = q.cfl;
     dtglob += blockIdx.x;
   }
-  double m = INFINITY;
-  for (int b = t; b < nblocks; b += CSU_BLOCK) m = fmin(m, red[b]);
-  smin[t] = m;
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st) smin[t] = fmin(smin[t], smin[t + st]);
-    __syncthreads();
-  }
-  if (t == 0) dtglob[0] = smin[0] * cfl;
+  double m[1] = {INFINITY};
+  for (int b = t; b < nblocks; b += CSU_BLOCK) m[0] = fmin(m[0], red[b]);
+  block_reduce<CSU_BLOCK, 1>(m, sw, RedMin());
+  if (t == 0) dtglob[0] = m[0] * cfl;
 }
 
 // Time-accurate step control, the tube branch of solverSolve
@@ -894,7 +871,7 @@ __global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, 
     nc = q.c1 - q.c0;
     nblocks = (nc + CSU_BLOCK - 1) / CSU_BLOCK;
   }
-  __shared__ double s[NV][RES_THREADS];
+  __shared__ double sw[NV][RES_THREADS / 64];
   const int t = threadIdx.x;
   double a[NV];
 #pragma unroll
@@ -903,17 +880,10 @@ __global__ __launch_bounds__(RES_THREADS) void k_res_final(int nc, int nblocks, 
 #pragma unroll
     for (int kk = 0; kk < NV; kk++) a[kk] += red[kk * stride + b];
   }
+  block_reduce<RES_THREADS, NV>(a, sw, RedSum());
+  if (t == 0)
 #pragma unroll
-  for (int kk = 0; kk < NV; kk++) s[kk][t] = a[kk];
-  __syncthreads();
-  for (int st = RES_THREADS / 2; st > 0; st >>= 1) {
-    if (t < st) {
-#pragma unroll
-      for (int kk = 0; kk < NV; kk++) s[kk][t] += s[kk][t + st];
-    }
-    __syncthreads();
-  }
-  if (t < NV) res[t] = sqrt(s[t][0] / nc);
+    for (int kk = 0; kk < NV; kk++) res[kk] = sqrt(a[kk] / nc);
 }
 
 // Wall force coefficients Cx_p, Cx_v, Cy_p, Cy_v, reference solverCalcCoeff3
@@ -934,7 +904,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const in
     nwall = q.w1 - q.w0;
     sref = q.sref;
   }
-  __shared__ double sh[4][CSU_BLOCK];
+  __shared__ double sw[4][CSU_BLOCK / 64];
   int t = threadIdx.x;
   const double r = kp.pin[0], u = kp.pin[1], v = kp.pin[2], Pinf = kp.pin[3];
   const double q = 0.5 * r * (u * u + v * v);
@@ -949,15 +919,10 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_forces(KP kp, int nwall, const in
       vy -= wfric[kp.nb + gq] / q;
     }
   }
-  sh[0][t] = ax; sh[1][t] = vx; sh[2][t] = ay; sh[3][t] = vy;
-  __syncthreads();
-  for (int st = CSU_BLOCK / 2; st > 0; st >>= 1) {
-    if (t < st)
-      for (int k = 0; k < 4; k++) sh[k][t] += sh[k][t + st];
-    __syncthreads();
-  }
+  double f[4] = {ax, vx, ay, vy};
+  block_reduce<CSU_BLOCK, 4>(f, sw, RedSum());
   if (t == 0) {
-    double c[4] = {sh[0][0] / sref, sh[1][0] / sref, sh[2][0] / sref, sh[3][0] / sref};
+    double c[4] = {f[0] / sref, f[1] / sref, f[2] / sref, f[3] / sref};
     if (kp.axi) { c[0] *= 2 * M_PI; c[1] *= 2 * M_PI; }
     for (int k = 0; k < 4; k++) coef[k] = c[k];
     if (!sol) ctr[0] = ctr[0] + 1;  // last kernel of an iteration: advance the history slot
