@@ -124,3 +124,46 @@ def test_batched_mixed_switch_converges_to_fp64(tmpcase):
         fu = np.abs(smix[i] - sref[i]).max(axis=1) / np.abs(sref[i]).max(axis=1)
         assert fu.max() < 1e-6
     assert sum(ref[i]["converged"] for i in ref) >= 2
+
+
+def test_airfoil_requests_alone_and_in_the_mix(tmpcase):
+    """SA and SST NACA 0012 requests (LU-SGS, AUSMDV, drops from the peak,
+    CFL ramped 20 -> 200) served among ramps, as one-solve contexts and in
+    batched units, give their one-at-a-time results bit for bit (they stop
+    at their iteration cap: the reference's scheme does not converge these
+    O-meshes)."""
+    from csu_amd import renumber
+    hcs = []
+    for k, model in enumerate(("sa", "sst", "sa", "sst")):
+        d = tmpcase(f"af{k}")
+        over = {"sa": "1"} if model == "sa" else {"sa": "0", "sst": "1"}
+        cases.write_case(d, cases.naca_mesh(ni=64 + 16 * k, nwall=8, nfar=10, seed=k),
+                         cases.naca_input(threads=1, nmax=1, aoa=2.0 + k, **over))
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
+        hcs.append(native.HostCase(d, host_wall_distance=False))
+    hcs += [h for k, h in enumerate(_cases(tmpcase)) if k in (1, 4)]
+
+    def serve(slots, batch):
+        srv = native.Server(max_slots=slots, hbm_budget=8e9, chunk=20, batch_solves=batch)
+        ids = [srv.submit(c, drop=1e-8, max_iter=200, keep_state=True, peak_ref=True,
+                          cfl_ramp=(200.0, 1.1) if k < 4 else None) for k, c in enumerate(hcs)]
+        res = {}
+        while len(res) < len(ids):
+            srv.step(0.05)
+            for r in srv.results():
+                res[r["id"]] = r
+        st = {i: srv.state(i, hcs[i].nvar, hcs[i].ncell) for i in ids}
+        srv.close()
+        return res, st
+
+    one, sone = serve(1, 1)
+    for slots, batch in ((4, 1), (2, 4)):
+        mix, smix = serve(slots, batch)
+        for i in one:
+            a, b = mix[i], one[i]
+            print(slots, batch, i, a["status"], a["iters"], a["res_last"], flush=True)
+            assert (a["status"], a["iters"], a["converged"]) == (b["status"], b["iters"], b["converged"])
+            assert np.array_equal([a["res_peak"], a["res_last"]] + a["coef_last"],
+                                  [b["res_peak"], b["res_last"]] + b["coef_last"], equal_nan=True)
+            assert np.array_equal(smix[i], sone[i], equal_nan=True)
+    assert all(one[i]["iters"] > 0 for i in range(4))
