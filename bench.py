@@ -240,7 +240,7 @@ def serve_run(reqs, slots: int, arrivals, drop: float = 1e-8, max_iter: int = 40
 
 
 def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=None, ramps=None, batch: int = 8,
-            batch_slots: int = 4):
+            batch_slots: int = 4, chunk: int = 50):
     """Solves/hour and time-to-converge (1e-8 density-residual drop) of the
     served forms -- batched units (`batch_slots` batch contexts of up to
     `batch` solves in flight) in FP64 and in mixed precision switched to
@@ -254,13 +254,14 @@ def serving(reqs, rate: float, slots: int, log, peak_ref: bool = False, caps=Non
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     out = {"requests": n, "rate_per_s": rate, "slots": slots, "batch_slots": batch_slots, "batch_solves": batch,
+           "chunk": chunk,
            "drop": 1e-8, "drop_from": "largest residual so far" if peak_ref else "first iteration",
            "cells": [c.ncell for _, c in reqs], "kinds": [k for k, _ in reqs]}
     forms = (("served_batched", batch_slots, 0, batch), ("served_batched_mixed", batch_slots, 1, batch),
              ("served_contexts_mixed", slots, 1, 1), ("one_at_a_time", 1, 0, 1))
     sats = {}
     for name, sl, prec, bt in forms:
-        kw = dict(precision=prec, switch_drop=1e-4, peak_ref=peak_ref, caps=caps, ramps=ramps, batch=bt)
+        kw = dict(precision=prec, switch_drop=1e-4, peak_ref=peak_ref, caps=caps, ramps=ramps, batch=bt, chunk=chunk)
         sat = serve_run(reqs, sl, np.zeros(n), **kw)
         sats[name] = sat
         span = max(r["t_done"] for r in sat) - min(r["t_submit"] for r in sat)
@@ -616,8 +617,10 @@ def serve_all(a, log):
     specified airfoil-and-ramp stream."""
     t0 = time.perf_counter()
     reqs = serving_requests(a.serve_n)
+    # 25-iteration chunks: a solve is retired at the end of the chunk in
+    # which it converges, and these converge in ~1,000 iterations
     serve = {"ramps": serving(reqs, a.serve_rate, a.serve_slots, log, batch=a.serve_batch,
-                              batch_slots=a.serve_batch_slots)}
+                              batch_slots=a.serve_batch_slots, chunk=25)}
     for _, c in reqs:
         c.close()
     reqs = serving_requests_mixed(a.serve_n, hi=a.serve_max_cells)

@@ -199,8 +199,6 @@ static int validate(const csu_mesh_desc* m, const csu_params* p) {
   if ((p->sa && p->laminar) || (p->sst && (p->laminar || p->sa)))
     return set_err(CSU_ERR_UNSUPPORTED, "more than one of laminar / sa / sst set");
   if (p->nvar != (p->sa ? 5 : p->sst ? 6 : 4)) return set_err(CSU_ERR_ARG, "nvar %d does not match the model", p->nvar);
-  if ((p->sa || p->sst) && p->flux == CSU_FLUX_ROE)
-    return set_err(CSU_ERR_UNSUPPORTED, "Roe has no turbulence-scalar form in the reference (flux.c:1325-1366)");
   if (p->sst && p->order == 2 && p->limiter == 1)
     return set_err(CSU_ERR_UNSUPPORTED, "limiter 1 with SST: the reference's range arrays hold 5 variables (limiter.c:37-38)");
   if (p->time_scheme < 0 || p->time_scheme > 2) return set_err(CSU_ERR_ARG, "time scheme %d", p->time_scheme);

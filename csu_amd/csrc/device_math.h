@@ -394,14 +394,33 @@ __device__ __forceinline__ void flux_ausmpup(const GasD& g, T minf, int variant,
   }
 }
 
-// flux()/flux_sa() dispatch, flux.c:1300-1345 (Roe has no passive-scalar form)
+// Roe with turbulence scalars: an extension (the reference's flux_sa /
+// flux_sst have no Roe branch, flux.c:1325-1366).  The mean flow is Roe's
+// flux of the thermodynamic state; the scalars are carried by its mass flux
+// from the upwind side (f[0] > 0: left, as the AUSM family's Mm > 0 choice),
+// and for SST the turbulent energy k joins the total-energy flux the same
+// way, so that equal states give the physical flux rho u (e + p/rho +
+// |u|^2/2 + k).
+template <typename T, int NS>
+__device__ __forceinline__ void roe_scalars(const T* sL, const T* sR, T kL, T kR, T* f) {
+  const bool left = f[0] > (T)0;
+#pragma unroll
+  for (int q = 0; q < NS; q++) f[4 + q] = f[0] * (left ? sL[q] : sR[q]);
+  if (NS == 2) f[3] += f[0] * (left ? kL : kR);
+}
+
+// flux()/flux_sa()/flux_sst() dispatch, flux.c:1300-1366 (Roe with scalars:
+// roe_scalars above)
 template <typename T, int NS>
 __device__ __forceinline__ void flux_dispatch(const GasD& g, int kind, T efix, T minf,
                                               T rL, T uL, T vL, T pL, const T* sL,
                                               T rR, T uR, T vR, T pR, const T* sR,
                                               T kL, T kR, T* f) {
   switch (kind) {
-    case 0: flux_roe<T>(g, efix, rL, uL, vL, pL, rR, uR, vR, pR, f); break;
+    case 0:
+      flux_roe<T>(g, efix, rL, uL, vL, pL, rR, uR, vR, pR, f);
+      if (NS > 0) roe_scalars<T, NS>(sL, sR, kL, kR, f);
+      break;
     case 1: flux_ausm<T, NS>(g, rL, uL, vL, pL, sL, rR, uR, vR, pR, sR, kL, kR, f); break;
     case 2: flux_ausmdv<T, NS>(g, rL, uL, vL, pL, sL, rR, uR, vR, pR, sR, kL, kR, f); break;
     case 3: flux_ausmpup<T, NS>(g, minf, 0, rL, uL, vL, pL, sL, rR, uR, vR, pR, sR, kL, kR, f); break;

@@ -359,11 +359,22 @@ static void flux_ausmpup(const csu_oracle* o, const double* L, const double* Rt,
     for (int q = 0; q < ns; q++) f[4 + q] = mm * S[4 + q];
 }
 
+/* Roe with turbulence scalars (no reference counterpart: flux_sa / flux_sst
+   have no Roe branch, flux.c:1325-1366): Roe's mean-flow flux, the scalars
+   carried by its mass flux from the upwind side (f[0] > 0: left), and for
+   SST k added to the total-energy flux the same way. */
+static void roe_scalars(const double* L, const double* Rt, int ns, double* f)
+{
+    const double* S = f[0] > 0 ? L : Rt;
+    for (int q = 0; q < ns; q++) f[4 + q] = f[0] * S[4 + q];
+    if (ns == 2) f[3] += f[0] * S[4];
+}
+
 static void flux(const csu_oracle* o, const double* L, const double* Rt, double* f)  /* flux.c:1300-1366 */
 {
     int ns = o->nv - 4;
     switch (o->prm.flux) {
-        case 0: flux_roe(o, L, Rt, f); break;
+        case 0: flux_roe(o, L, Rt, f); if (ns > 0) roe_scalars(L, Rt, ns, f); break;
         case 1: flux_ausm(o, L, Rt, ns, f); break;
         case 2: flux_ausmdv(o, L, Rt, ns, f); break;
         case 3: flux_ausmpup(o, L, Rt, ns, 0, f); break;
@@ -2112,7 +2123,7 @@ static void* dup(const void* p, size_t n) { void* q = malloc(n ? n : 1); if (n) 
 
 int oracle_create(const csu_mesh_desc* m, const csu_params* p, csu_oracle** out)
 {
-    if ((p->sa || p->sst) && (!m->wall_dist || p->flux == CSU_FLUX_ROE)) return CSU_ERR_UNSUPPORTED;
+    if ((p->sa || p->sst) && !m->wall_dist) return CSU_ERR_UNSUPPORTED;
     /* SST: no global-range limiter (limiter.c:37-38 sizes its range arrays
        for 5 variables) */
     if (p->sst && p->order == 2 && p->limiter == 1)
@@ -2293,6 +2304,13 @@ int oracle_iterate(csu_oracle* o, int niter, double* res, double* coef)
    normal and length; dq [6][nc] the SST source Jacobians dQk/d(r,rk,ro),
    dQo/d(r,rk,ro) (zeros otherwise); R [nv][nc]; dW [nv][nc] after the
    sweeps.  The state is not advanced (only clamped, as calc_R does). */
+int oracle_face_flux(const csu_oracle* o, const double* L, const double* R, double* f)
+{
+    if (!o || !L || !R || !f) return CSU_ERR_ARG;
+    flux(o, L, R, f);
+    return CSU_OK;
+}
+
 int oracle_dplur_system(csu_oracle* o, double* D, double* A, int* nb, double* nrm, double* dq, double* R,
                         double* dW)
 {

@@ -38,5 +38,8 @@ int  oracle_set_threads(int n);
    dq [6][nc], R [nv][nc], dW [nv][nc] of one implicit step */
 int  oracle_dplur_system(csu_oracle* o, double* D, double* A, int* nb, double* nrm, double* dq, double* R,
                          double* dW);
+/* Face-flux test hook: the convective flux of the oracle's options between
+   face-frame states L and R ([nv]: r, u_n, u_t, p, scalars) into f [nv] */
+int  oracle_face_flux(const csu_oracle* o, const double* L, const double* R, double* f);
 
 #endif

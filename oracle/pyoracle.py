@@ -116,6 +116,17 @@ class Oracle:
             raise RuntimeError(f"oracle_dplur_system failed ({rc})")
         return out
 
+    def face_flux(self, L, R) -> np.ndarray:
+        """Test hook: the convective flux of the case's options between two
+        face-frame states [nv] (r, normal and tangential velocity, p,
+        scalars)."""
+        L = np.ascontiguousarray(L, dtype=np.float64)
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        f = np.empty(self.nvar)
+        if lib().oracle_face_flux(self._h, _p(L), _p(R), _p(f)) != 0:
+            raise RuntimeError("oracle_face_flux failed")
+        return f
+
     def iterate(self, n: int):
         res = np.empty((n, self.nvar))
         coef = np.empty((n, 4))
