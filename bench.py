@@ -412,11 +412,11 @@ def converge_configs(log, plate_cfl: float = 200.0):
     for model in ("sa", "sst"):
         d = tempfile.mkdtemp(prefix="cv" + model, dir="/tmp") + "/"
         over = {"sa": "1"} if model == "sa" else {"sa": "0", "sst": "1"}
-        cases.write_case(d, mesh, cases.naca_input(threads=1, nmax=1, aoa=2.616, **over))
+        cases.write_case(d, mesh, cases.naca_input(threads=1, nmax=1, aoa=2.616, flux="ROE", **over))
         renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
         r = _converge(d, 0, 1e-8, 4000, chunk=200, cfl_ramp=(200.0, 1.1))
         r.pop("U"), r.pop("wt")
-        out[f"naca_{model}_lusgs"] = {"workload": f"NACA 0012 {model.upper()} 250k, AUSMDV, LU-SGS CFL 20 -> 200 "
+        out[f"naca_{model}_lusgs"] = {"workload": f"NACA 0012 {model.upper()} 250k, Roe (scalar extension), LU-SGS CFL 20 -> 200 "
                                                   "(x1.1 per 200 iterations), 4000 iterations", "fp64": r}
         log(f"naca {model}: drop from peak after {r['iters']} iterations: {r['drop_reached']}")
     return out
@@ -436,8 +436,9 @@ def other_configs(log, n_plate: int = 500, n_naca: int = 500, graphs: bool = Tru
     NACA 0012: hybrid O-mesh, 40 quad layers from 1e-6 chord (growth 1.15)
       and triangles to 50 chords (1000 x (40 + 2x105) = 250,000 cells),
       Mach 0.15, Re 6e6, AoA drawn from seed 2 in [0, 10] deg, SA and SST,
-      AUSMDV (the reference's SA/SST fluxes have no Roe branch), LU-SGS at
-      CFL 20 on the colour/RCM-renumbered mesh."""
+      Roe with the turbulence scalars upwinded on its mass flux (an
+      extension: the reference's SA/SST fluxes have no Roe branch), LU-SGS
+      at CFL 20 on the colour/RCM-renumbered mesh."""
     from csu_amd import cases, renumber
     rng = np.random.default_rng(2)
     aoa = float(rng.uniform(0.0, 10.0))
@@ -451,11 +452,11 @@ def other_configs(log, n_plate: int = 500, n_naca: int = 500, graphs: bool = Tru
     for model in ("sa", "sst"):
         d = tempfile.mkdtemp(prefix="c2" + model, dir="/tmp") + "/"
         over = {"sa": "1"} if model == "sa" else {"sa": "0", "sst": "1"}
-        cases.write_case(d, mesh, cases.naca_input(threads=1, nmax=n_naca, aoa=aoa, **over))
+        cases.write_case(d, mesh, cases.naca_input(threads=1, nmax=n_naca, aoa=aoa, flux="ROE", **over))
         renumber.renumber_file(d + "mesh.su2", d + "mesh.su2")
         specs.append((f"naca_{model}_lusgs", d, n_naca, aoa,
                       f"NACA 0012 M0.15 Re 6e6 AoA {aoa:.3f} deg, {model.upper()}, 250k hybrid O-mesh, "
-                      "AUSMDV, LU-SGS CFL 20 (colour/RCM order)"))
+                      "Roe (scalar extension), LU-SGS CFL 20 (colour/RCM order)"))
     out = {}
     for name, d, n, a_deg, label in specs:
         r64 = _config_solve(d, n, 0, graphs)

@@ -1,5 +1,6 @@
 """Cell renumbering of an SU2 mesh: multicolour order with reverse
-Cuthill-McKee order inside each colour.
+Cuthill-McKee order inside each colour (for LU-SGS), or a Morton curve
+(for the explicit schemes' gathers).
 
 CSU's LU-SGS sweeps cells in index order, each cell using the final
 increments of its lower-index (forward) or higher-index (backward)
@@ -123,12 +124,53 @@ def lusgs_depth(cells, order=None) -> int:
     return int(lev.max()) + 1
 
 
-def renumber_file(src: str, dst: str) -> np.ndarray:
-    """Write dst: src with its elements in colour/RCM order.  Returns the
+def _read_nodes(lines) -> np.ndarray:
+    """Node coordinates [npoin, 2] of the NPOIN= block."""
+    for k, ln in enumerate(lines):
+        if ln.startswith("NPOIN="):
+            n = int(ln.split("=")[1].split()[0])
+            tok = np.array("".join(lines[k + 1:k + 1 + n]).split(), dtype=np.float64).reshape(n, -1)
+            return tok[:, :2]
+    raise ValueError("no NPOIN= line")
+
+
+def morton_order(nodes, cells) -> np.ndarray:
+    """New-to-old cell permutation along a Z-order (Morton) curve of the
+    cell centres: cells close in space get close indices, so a workgroup's
+    256 cells form a compact patch and their neighbours' data is mostly the
+    patch's own (for explicit runs; LU-SGS wants colour_order)."""
+    cells = np.asarray(cells)
+    nv = np.where(cells[:, 3] >= 0, 4, 3)
+    xy = nodes[np.where(cells >= 0, cells, cells[:, :1])]            # [n, 4, 2], the 4th slot repeats for triangles
+    c = xy[:, :3].sum(axis=1)
+    c[nv == 4] += xy[nv == 4, 3]
+    c /= nv[:, None]
+    lo, hi = c.min(axis=0), c.max(axis=0)
+    q = np.floor((c - lo) / np.maximum(hi - lo, 1e-300) * 65535.0).astype(np.uint64)
+    q = np.minimum(q, 65535)
+
+    def spread(v):
+        v = (v | (v << 8)) & np.uint64(0x00FF00FF)
+        v = (v | (v << 4)) & np.uint64(0x0F0F0F0F)
+        v = (v | (v << 2)) & np.uint64(0x33333333)
+        v = (v | (v << 1)) & np.uint64(0x55555555)
+        return v
+    key = spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1))
+    return np.argsort(key, kind="stable")
+
+
+def renumber_file(src: str, dst: str, order: str = "colour") -> np.ndarray:
+    """Write dst: src with its elements in colour/RCM order (order
+    "colour", for LU-SGS) or along a Morton curve ("morton").  Returns the
     new-to-old permutation."""
     lines = open(src).read().splitlines(keepends=True)
     k, n, cells = _read_elements(lines)
-    perm = colour_order(cells)
+    if order == "morton":
+        perm = morton_order(_read_nodes(lines), cells)
+    elif order == "colour":
+        perm = colour_order(cells)
+    else:
+        raise ValueError(f"order {order!r}")
     out = lines[:k + 1]
     for new, old in enumerate(perm):
         tok = lines[k + 1 + old].split()
