@@ -140,3 +140,52 @@ def test_mixed_precision_plate_converged(tmpcase):
     fc = float(np.max(np.abs(k1 - k0)) / np.abs(k0).max())
     print("converged plate: field rel", fu, "forces rel", fc, flush=True)
     assert fu < 1e-6 and fc < 1e-6
+
+
+def test_laminar_plate_dplur_converged_vs_oracle(tmpcase):
+    """The laminar-plate configuration's scheme (AUSM, Venkatakrishnan K=5,
+    DPLUR) on a 125x25 plate at CFL 500, run on the device to a 1e-8 drop
+    from the density-residual peak; that state is a converged solution of
+    the oracle's discrete equations: the oracle's residual there equals the
+    device's to 1e-9 and stays at the converged level over 200 oracle
+    iterations from it.  Those 200 iterations and 200 more on the device
+    end within 2e-9 of each field's range of each other (the fields move by
+    2e-9-2e-8 meanwhile) with the wall shear within 1e-9.  The residual
+    histories differ by up to a few per cent: near convergence the residual
+    is a difference of nearly equal fluxes, and the finite-difference DPLUR
+    blocks (h = 1e-6) turn rounding-level differences of the state into
+    such differences of it.  (The oracle needs minutes for the whole
+    convergence.)"""
+    d = tmpcase("pc")
+    cases.write_case(d, cases.plate_mesh(nx=125, ny=25, growth=1.08 ** 4, tri_frac=0.2, seed=1),
+                     cases.plate_input(threads=1, nmax=1, timeScheme="DPLUR", CFL="500"))
+    c = native.HostCase(d)
+    dev = native.Device(c)
+    peak, it, done = 0.0, 0, False
+    while it < 40000 and not done:
+        r, _ = dev.iterate(500)
+        assert np.isfinite(r).all()
+        peak = max(peak, float(r[:, 0].max()))
+        done = r[-1, 0] <= 1e-8 * peak
+        it += 500
+    print("device iterations", it, flush=True)
+    assert done
+    Ug = dev.get_state()
+    rg, _ = dev.iterate(200)
+    Ug2, wg = dev.get_state(), dev.wall_tensor()[:, 1]
+    o = po.Oracle(c)
+    o.set_state(Ug)
+    ro, _ = o.iterate(200)
+    Uo, wo = o.get_state(), o.wall_tensor()[:, 1]
+    scale = np.abs(Ug).max(axis=1)
+    m = np.abs(wo) > 0
+    assert m.sum() > 100
+    first = abs(ro[0, 0] / rg[0, 0] - 1)
+    hist = float(np.abs(ro[:, 0] / rg[:, 0] - 1).max())
+    rel = np.abs(Uo - Ug2).max(axis=1) / scale
+    cf = float(np.abs(wg[m] - wo[m]).max() / np.abs(wo[m]).max())
+    print("residual first", first, "history", hist, "oracle drop", ro[:, 0].max() / peak, "fields", rel,
+          "moved", np.abs(Ug2 - Ug).max(axis=1) / scale, "Cf", cf, flush=True)
+    assert first < 1e-9
+    assert ro[:, 0].max() <= 1.1e-8 * peak and hist < 0.1
+    assert rel.max() < 2e-9 and cf < 1e-9
