@@ -332,6 +332,7 @@ extern "C" int csu_wall_distance_dev(const csu_mesh_desc* m, double* wd) {
 struct HostMesh {
   int nc = 0, nf = 0, nb = 0;
   std::vector<int> cnp, cnbr, cface;                              // [nc], [4][nc], [4][nc]
+  std::vector<int> cfn;  // [4][nc] the other cell of face slot j, -1 where cface is 0
   std::vector<double> xc, yc, om, dsxm, dsym, dslat;              // [nc]
   std::vector<double> edx, edy, lwx, lwy;                         // [4][nc]
   std::vector<unsigned char> fsl, bmask;                          // [nf], [nc]
@@ -481,6 +482,13 @@ static int host_mesh(const csu_mesh_desc* m, const csu_params* p, HostMesh& h) {
     calc_ds(m, q[2], q[3], &dSx, &dSy);
     h.fdsx[f] = dSx; h.fdsy[f] = dSy; h.fds[f] = sqrt(dSx * dSx + dSy * dSy);
   }
+  // the implicit kernels' neighbour of each face slot, so that their
+  // neighbour gathers do not wait for a face-array lookup
+  h.cfn.assign(c4, -1);
+  for (size_t t = 0; t < c4; t++) {
+    const int f = h.cface[t];
+    if (f != 0) h.cfn[t] = f > 0 ? h.fe1[f - 1] : h.fe0[-f - 1];
+  }
   h.be0.assign(nb, 0); h.btype.assign(nb, 0); h.bdonor.assign(nb, -1);
   for (auto* v : {&h.bdsx, &h.bdsy, &h.bds, &h.bxm, &h.bym, &h.bnx, &h.bny, &h.bds2}) v->assign(nb, 0.0);
   h.wall.clear();
@@ -549,7 +557,7 @@ static void host_concat(const std::vector<HostMesh>& part, HostMesh& H, std::vec
   H.nc = (int)NC; H.nf = (int)NF; H.nb = (int)NB;
   const size_t C4 = 4 * NC;
   // padding: no kernel computes it; finite values keep any stray read harmless
-  H.cnp.assign(NC, 0); H.cnbr.assign(C4, INT32_MAX); H.cface.assign(C4, 0);
+  H.cnp.assign(NC, 0); H.cnbr.assign(C4, INT32_MAX); H.cface.assign(C4, 0); H.cfn.assign(C4, -1);
   H.xc.assign(NC, 0.0); H.yc.assign(NC, 0.0); H.om.assign(NC, 1.0);
   H.dsxm.assign(NC, 1.0); H.dsym.assign(NC, 1.0); H.dslat.assign(NC, 0.0);
   H.edx.assign(C4, 0.0); H.edy.assign(C4, 0.0); H.lwx.assign(C4, 0.0); H.lwy.assign(C4, 0.0);
@@ -575,6 +583,7 @@ static void host_concat(const std::vector<HostMesh>& part, HostMesh& H, std::vec
         const int nbr = h.cnbr[lj], f = h.cface[lj];
         H.cnbr[gj] = nbr == INT32_MAX ? nbr : nbr >= 0 ? nbr + c0 : -1 - ((-1 - nbr) + b0);
         H.cface[gj] = f > 0 ? f + f0 : f < 0 ? f - f0 : 0;
+        H.cfn[gj] = h.cfn[lj] >= 0 ? h.cfn[lj] + c0 : -1;
         H.edx[gj] = h.edx[lj]; H.edy[gj] = h.edy[lj]; H.lwx[gj] = h.lwx[lj]; H.lwy[gj] = h.lwy[lj];
       }
       if (lev) { H.levL[gi] = h.levL[i]; H.levU[gi] = h.levU[i]; }
@@ -794,7 +803,9 @@ static int dev_setup(csu_dev* d, const HostMesh& H, int ns, const csu_mesh_desc*
     HIPCHK(hipMemsetAsync(s.wfric, 0, sizeof(double) * 2 * (size_t)(nb ? nb : 1), d->stream));
   }
   if (p->time_scheme != CSU_TIME_RK) {
-    if ((rc = dalloc(d, &d->Dp, nc)) || (rc = dalloc(d, &d->dW0, nvc)) || (rc = dalloc(d, &d->dW1, nvc))) return rc;
+    if ((rc = dalloc(d, &d->Dp, nc)) || (rc = dalloc(d, &d->dW0, nvc)) || (rc = dalloc(d, &d->dW1, nvc)) ||
+        (rc = upload(d, &g.cfn, H.cfn)))
+      return rc;
     HIPCHK(hipMemsetAsync(d->dW0, 0, nvc * sizeof(double), d->stream));
     HIPCHK(hipMemsetAsync(d->dW1, 0, nvc * sizeof(double), d->stream));   // implicitInitDPLUR
   }
@@ -2082,7 +2093,7 @@ extern "C" int64_t csu_dev_footprint(const csu_mesh_desc* m, const csu_params* p
   b += (nv + 1) * 8 * (nc + nb) + 24 * nv * nc + 8 * nv * nf;                             // P, Pb, grad, phi, fflux
   if (p->order == 2) b += 32 * nv * nc;                                                   // edge states
   if (visc) b += nc * (16 + 16 + 48) + nf * (8 * (nv - 1) + 8) + nb * (8 * (nv - 1) + 16);
-  if (p->time_scheme != CSU_TIME_RK) b += nc * (8 + 16 * nv + 8);
+  if (p->time_scheme != CSU_TIME_RK) b += nc * (8 + 16 * nv + 8 + 16);
   if (p->time_scheme == CSU_TIME_DPLUR) b += 32 * nv * nv * nc + 8.0 * (p->nlinear > 0 ? p->nlinear : 1) * (nc / 256 + 1);
   b += (nv + 4) * 8 * (double)(hist > 0 ? hist : 1);
   // allocation granularity: every array rounds up (at most 2 MiB slack per large array)

@@ -118,6 +118,7 @@ struct Geo {
   const SolveP* sol;
   const int *csolk, *fsolk, *bsolk, *csol;
   const int *cnp, *cnbr, *cface;                        // [nc], [4][nc], [4][nc]
+  const int* cfn;  // implicit schemes: [4][nc] the other cell of face slot j, -1 where cface is 0
   const double *xc, *yc, *omega, *dsxm, *dsym, *dslat;  // [nc]
   const double *edx, *edy;                              // [4][nc] own-edge midpoint offsets
   const double *lwx, *lwy;                              // [4][nc] LSQ weights per neighbour slot

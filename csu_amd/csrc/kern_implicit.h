@@ -28,41 +28,41 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double w
   if (i < kp.cend) {
     const double* P = s.P;
     double Lc = 0, Lv = 0;
-    // all slots' face data, then all states, then the sums in slot order
-    int cfj[4], fij[4], e0j[4], e1j[4];
+    // all slots' face data and neighbours, then all states, then the sums
+    // in slot order.  The face's two states are the cell's and the
+    // neighbour's (cfn) in either order: the face means add them, and a sum
+    // of two doubles does not depend on their order.
+    int cfj[4], fij[4], nbj[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       cfj[j] = g.cface[j * nc + i];
+      nbj[j] = g.cfn[j * nc + i];
       fij[j] = cfj[j] > 0 ? cfj[j] - 1 : (cfj[j] < 0 ? -cfj[j] - 1 : 0);
     }
-    double sxj[4], syj[4], sj[4], Pa[4][4], Pb[4][4];
+    double sxj[4], syj[4], sj[4], Pa[4], Pb[4][4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      e0j[j] = g.fe0[fij[j]];
-      e1j[j] = g.fe1[fij[j]];
       sxj[j] = g.fdsx[fij[j]];
       syj[j] = g.fdsy[fij[j]];
       sj[j] = g.fds[fij[j]];
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int a = cfj[j] == 0 ? i : e0j[j], b = cfj[j] == 0 ? i : e1j[j];
+    for (int q = 0; q < 4; q++) Pa[q] = P[(q == 3 ? 4 : q) * nc + i];  // r u v T
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int m = q == 3 ? 4 : q;  // r u v T
-        Pa[j][q] = P[m * nc + a];
-        Pb[j][q] = P[m * nc + b];
-      }
+    for (int j = 0; j < 4; j++) {
+      const int b = cfj[j] == 0 ? i : nbj[j];
+#pragma unroll
+      for (int q = 0; q < 4; q++) Pb[j][q] = P[(q == 3 ? 4 : q) * nc + b];
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       if (cfj[j] == 0) continue;
       const int fi = fij[j];
       const double dSx = sxj[j], dSy = syj[j], dS = sj[j];
-      double r = (Pa[j][0] + Pb[j][0]) * 0.5;
-      double u = (Pa[j][1] + Pb[j][1]) * 0.5;
-      double v = (Pa[j][2] + Pb[j][2]) * 0.5;
-      double T = (Pa[j][3] + Pb[j][3]) * 0.5;
+      double r = (Pa[0] + Pb[j][0]) * 0.5;
+      double u = (Pa[1] + Pb[j][1]) * 0.5;
+      double v = (Pa[2] + Pb[j][2]) * 0.5;
+      double T = (Pa[3] + Pb[j][3]) * 0.5;
       double c = gas_T2c<double>(kp.gas, T);
       double nx = dSx / dS, ny = dSy / dS;
       Lc += (fabs(nx * u + ny * v) + c) * dS;
@@ -222,13 +222,13 @@ __device__ __forceinline__ void delta_flux(const KP& kp, const double* Pj, const
 // the flipped vector is the exact negation).
 __device__ __forceinline__ int face_from(const Geo& g, int nc, int i, int j, double& dSx, double& dSy, double& dS,
                                          int& fi) {
-  const int f = g.cface[j * nc + i];
+  const int f = g.cface[j * nc + i], nb = g.cfn[j * nc + i];
   if (f == 0) return -1;
   fi = f > 0 ? f - 1 : -f - 1;
   dS = g.fds[fi];
-  if (f > 0) { dSx = g.fdsx[fi]; dSy = g.fdsy[fi]; return g.fe1[fi]; }
+  if (f > 0) { dSx = g.fdsx[fi]; dSy = g.fdsy[fi]; return nb; }
   dSx = -g.fdsx[fi]; dSy = -g.fdsy[fi];
-  return g.fe0[fi];
+  return nb;
 }
 
 // One LU-SGS level.  DIR 0: forward sweep (implicitLUSGS_L, implicit.c:274-363),
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
     sj[j] = g.fds[fi];
     sxj[j] = g.fdsx[fi];
     syj[j] = g.fdsy[fi];
-    const int e = (f > 0 ? g.fe1 : g.fe0)[fi];
+    const int e = g.cfn[j * nc + i];  // loaded beside cface, not after it
     nbj[j] = f == 0 ? i : e;
   }
 #pragma unroll
@@ -467,51 +467,61 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
   const double* x = par ? buf1 : buf0;
   double* y = par ? buf0 : buf1;
   const bool act = i < kp.cend;
-  // The sweep's loads are issued before the previous sweep's stop test, so
-  // their latency overlaps with its reduction (a stopped sweep discards
-  // them).  Neighbours of all slots first, then their x; slots without a
-  // face read the cell itself and are skipped.
+  // Every load that does not depend on the stop test is issued before it:
+  // the cell's x, R, diagonal and neighbour ids (per face slot, cfn), then
+  // the neighbours' x.  Slots without a face read the cell itself and are
+  // skipped; lanes past the last cell read cell 0 and store nothing.
   const int ii = act ? i : 0;
-  double xi[NV];
+  double xi[NV], Ri[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) xi[k] = x[k * nc + ii];
-  int cfj[4], nbj[4];
-#pragma unroll
-  for (int j = 0; j < 4; j++) cfj[j] = act ? g.cface[j * nc + ii] : 0;
+  int nbj[4];
+  bool has[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int f = cfj[j], fi = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
-    const int e = (f > 0 ? g.fe1 : g.fe0)[fi];
-    nbj[j] = f == 0 ? ii : e;
+    const int c = act ? g.cfn[j * nc + ii] : -1;
+    has[j] = c >= 0;
+    nbj[j] = c >= 0 ? c : ii;
   }
+#pragma unroll
+  for (int k = 0; k < NV; k++) Ri[k] = R[k * nc + ii];
+  const double D = diag(Dp, g.omega, dtg, ii, sid);
+  double dqi[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) dqi[q] = NV == 6 ? dq[q * nc + ii] : 0.0;
   double xj[4][NV];
 #pragma unroll
   for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int n = 0; n < NV; n++) xj[j][n] = x[n * nc + nbj[j]];
   if (m > 0) {
-    // NaN is fmax's identity: the maximum is NaN only when every cell's
-    // change is, which the reference's loop (implicit.c:1023-1031) does not
-    // take as converged
+    // The maximum over the solve's blocks is at least this block's own
+    // (a number >= 1e-14 rules the stop out, so the reduction is skipped:
+    // the common case while the sweeps run).  Otherwise reduce every
+    // block's.  NaN is fmax's identity: the maximum is NaN only when every
+    // cell's change is, which the reference's loop (implicit.c:1023-1031)
+    // does not take as converged.  The branch is uniform in the block.
     const double* prev = red_sw + (size_t)(m - 1) * stride + b0;
-    double pm = __builtin_nan("");
-    for (int b = t; b < nblocks; b += CSU_BLOCK) pm = fmax(pm, prev[b]);
-    if (block_fmax(pm, sw0) < 1e-14) {
-      if (t == 0) red_sw[(size_t)m * stride + blockIdx.x] = -1.0;
-      return;
+    const double own = red_sw[(size_t)(m - 1) * stride + blockIdx.x];
+    if (!(own >= 1e-14)) {
+      double pm = __builtin_nan("");
+      for (int b = t; b < nblocks; b += CSU_BLOCK) pm = fmax(pm, prev[b]);
+      if (block_fmax(pm, sw0) < 1e-14) {
+        if (t == 0) red_sw[(size_t)m * stride + blockIdx.x] = -1.0;
+        return;
+      }
     }
   }
   double diff = __builtin_nan("");  // lanes past the last cell: fmax's identity
   if (act) {
-    const double D = diag(Dp, g.omega, dtg, i, sid);
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) acc[k] = D * xi[k];
     double qa = D, qb = 0.0, qc = 0.0, qd = D, q0 = 0.0, q3 = 0.0, det = 0.0;
     bool blk = false;
     if (NV == 6) {
-      q0 = dq[i]; q3 = dq[3 * nc + i];
-      qa = D - dq[nc + i]; qb = -dq[2 * nc + i]; qc = -dq[4 * nc + i]; qd = D - dq[5 * nc + i];
+      q0 = dqi[0]; q3 = dqi[3];
+      qa = D - dqi[1]; qb = -dqi[2]; qc = -dqi[4]; qd = D - dqi[5];
       det = qa * qd - qb * qc;
       blk = !(fabs(det) < 1e-14);
       if (blk) {
@@ -521,7 +531,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (cfj[j] == 0) continue;
+      if (!has[j]) continue;
       const double* Aj = A + (size_t)j * NV * NV * nc;
 #pragma unroll
       for (int k = 0; k < NV; k++)
@@ -532,9 +542,9 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     if (NV == 6 && blk) {
       double z[NV];
 #pragma unroll
-      for (int k = 0; k < 4; k++) z[k] = (-R[k * nc + i] - acc[k]) / D;
-      const double Ak = -R[4 * nc + i] - acc[4] + q0 * z[0];
-      const double Bo = -R[5 * nc + i] - acc[5] + q3 * z[0];
+      for (int k = 0; k < 4; k++) z[k] = (-Ri[k] - acc[k]) / D;
+      const double Ak = -Ri[4] - acc[4] + q0 * z[0];
+      const double Bo = -Ri[5] - acc[5] + q3 * z[0];
       z[4] = (Ak * qd - Bo * qb) / det;
       z[5] = (-Ak * qc + Bo * qa) / det;
 #pragma unroll
@@ -546,7 +556,7 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
     } else {
 #pragma unroll
       for (int k = 0; k < NV; k++) {
-        double yk = w * (-R[k * nc + i] - acc[k]) / D + xi[k];
+        double yk = w * (-Ri[k] - acc[k]) / D + xi[k];
         y[k * nc + i] = yk;
         if (k == 0) diff = fabs(yk - xi[0]);
       }
