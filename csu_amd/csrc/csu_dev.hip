@@ -1392,17 +1392,15 @@ static int implicit_iteration(csu_dev* d) {
     for (size_t l = 0; l + 1 < d->lvlL.size(); l++) {
       int off = d->lvlL[l], cnt = d->lvlL[l + 1] - off;
       hipLaunchKernelGGL((k_lusgs<NV, VISC, 0>), dim3((cnt + LB - 1) / LB), dim3(LB), 0, d->stream, kp, d->geo, d->st,
-                         d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0);
+                         d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0, (const double*)nullptr, (double*)nullptr);
     }
-    hipLaunchKernelGGL(k_lusgs_uinit<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->Dp, d->dtg, d->dW0,
-                       d->dW1, d->st.dq);
+    // the backward levels start each cell from D*dW0 and apply the final
+    // increment to the state themselves
     for (size_t l = 0; l + 1 < d->lvlU.size(); l++) {
       int off = d->lvlU[l], cnt = d->lvlU[l + 1] - off;
       hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3((cnt + LB - 1) / LB), dim3(LB), 0, d->stream, kp, d->geo, d->st,
-                         d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1);
+                         d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1, (const double*)d->dW0, d->U);
     }
-    hipLaunchKernelGGL(k_upd_imp<NV>, dim3(nb), dim3(CSU_BLOCK), 0, d->stream, nc, d->U, d->dW1, d->dW1,
-                       (const int*)nullptr, (const double*)nullptr, 0, 0, d->geo.sol, d->geo.csolk);
   } else {
     Prof p(d, "k_dplur");
     hipLaunchKernelGGL((k_blocks<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->U, d->A);

@@ -232,19 +232,39 @@ __device__ __forceinline__ int face_from(const Geo& g, int nc, int i, int j, dou
 }
 
 // One LU-SGS level.  DIR 0: forward sweep (implicitLUSGS_L, implicit.c:274-363),
-// dW = -R then lower neighbours; DIR 1: backward sweep (implicitLUSGS_U,
-// implicit.c:365-482), dW holds D*dW0 and upper neighbours are used.
+// dW = -R then lower neighbours.  DIR 1: backward sweep (implicitLUSGS_U,
+// implicit.c:365-482) from the forward result dW0: the cell's start value
+// D*dW0 (implicit.c:369-399), then upper neighbours; a cell's dW is final
+// once its level has run, so the level also applies it to the state
+// (solverUpdateUImplicit, solver.c:1945-1955; a held solve of a batch keeps
+// its state).
 template <int NV, bool VISC, int DIR>
 __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const int* __restrict__ cells, int ncell_lvl,
                                                      const double* __restrict__ Dp, const double* __restrict__ dtg,
-                                                     const double* __restrict__ R, double* dW) {
+                                                     const double* __restrict__ R, double* dW,
+                                                     const double* __restrict__ dW0, double* __restrict__ U) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncell_lvl) return;
   const int nc = kp.nc;
   const int i = cells[t];
+  const int sid = g.sol ? g.csol[i] : 0;
+  const double D = diag(Dp, g.omega, dtg, i, sid);
   double acc[NV];
+  if (DIR == 0) {
 #pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = DIR == 0 ? -R[k * nc + i] : dW[k * nc + i];
+    for (int k = 0; k < NV; k++) acc[k] = -R[k * nc + i];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = D * dW0[k * nc + i];
+    if (NV == 6) {  // implicit.c:372-392: the k-omega block times dW0
+      const double a = D - s.dq[nc + i], b = -s.dq[2 * nc + i], c = -s.dq[4 * nc + i], dd = D - s.dq[5 * nc + i];
+      const double det = a * dd - b * c;
+      if (!(fabs(det) < 1e-14)) {
+        acc[NV > 4 ? 4 : 0] = a * dW0[4 * nc + i] + b * dW0[5 * nc + i] - s.dq[i] * dW0[i];
+        acc[NV > 5 ? 5 : 0] = c * dW0[4 * nc + i] + dd * dW0[5 * nc + i] - s.dq[3 * nc + i] * dW0[i];
+      }
+    }
+  }
   // Face data of all slots is loaded before any slot is used (one round
   // trip per dependency level, not per slot); neighbour states are loaded
   // per slot, only for the neighbours this sweep uses (loading them all up
@@ -298,55 +318,30 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
 #pragma unroll
     for (int k = 0; k < NV; k++) acc[k] -= 0.5 * (dF[k] * dS - ra * d[k]);
   }
-  const double D = diag(Dp, g.omega, dtg, i, g.sol ? g.csol[i] : 0);
+  double out[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) out[k] = acc[k] / D;
   if (NV == 6) {
     // SST: k-omega block with the source Jacobians, Cramer's rule
     // (implicit.c:326-351, 447-472).  The reference divides by a long double
     // determinant that holds a double value; the quotient here is the
     // correctly rounded double one.
-#pragma unroll
-    for (int k = 0; k < 4; k++) dW[k * nc + i] = acc[k] / D;
+    constexpr int K = NV > 4 ? 4 : 0, O = NV > 5 ? 5 : 0;
     const double x0 = acc[0] / D;
-    const double A = acc[4] + s.dq[i] * x0;
-    const double B = acc[5] + s.dq[3 * nc + i] * x0;
+    const double A = acc[K] + s.dq[i] * x0;
+    const double B = acc[O] + s.dq[3 * nc + i] * x0;
     const double a = D - s.dq[nc + i], b = -s.dq[2 * nc + i], c = -s.dq[4 * nc + i], dd = D - s.dq[5 * nc + i];
     const double det = a * dd - b * c;
-    if (fabs(det) < 1e-14) {
-      dW[4 * nc + i] = acc[4] / D;
-      dW[5 * nc + i] = acc[5] / D;
-    } else {
-      dW[4 * nc + i] = (A * dd - B * b) / det;
-      dW[5 * nc + i] = (-A * c + B * a) / det;
-    }
-    return;
-  }
-#pragma unroll
-  for (int k = 0; k < NV; k++) dW[k * nc + i] = acc[k] / D;
-}
-
-// backward-sweep initial value dW1 = D dW0 (implicit.c:369-399)
-template <int NV>
-__global__ __launch_bounds__(CSU_BLOCK) void k_lusgs_uinit(KP kp, Geo g, const double* __restrict__ Dp,
-                                                           const double* __restrict__ dtg, const double* __restrict__ dW0,
-                                                           double* __restrict__ dW1, const double* __restrict__ dq) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nc = kp.nc;
-  int sid = 0;
-  if (g.sol) {
-    sid = solve_of(g.csolk, blockIdx.x * blockDim.x);
-    kp.cend = g.sol[sid].c1;
-  }
-  if (i >= kp.cend) return;
-  const double D = diag(Dp, g.omega, dtg, i, sid);
-#pragma unroll
-  for (int k = 0; k < NV; k++) dW1[k * nc + i] = D * dW0[k * nc + i];
-  if (NV == 6) {  // implicit.c:372-392: the k-omega block times dW0
-    const double a = D - dq[nc + i], b = -dq[2 * nc + i], c = -dq[4 * nc + i], dd = D - dq[5 * nc + i];
-    const double det = a * dd - b * c;
     if (!(fabs(det) < 1e-14)) {
-      dW1[4 * nc + i] = a * dW0[4 * nc + i] + b * dW0[5 * nc + i] - dq[i] * dW0[i];
-      dW1[5 * nc + i] = c * dW0[4 * nc + i] + dd * dW0[5 * nc + i] - dq[3 * nc + i] * dW0[i];
+      out[K] = (A * dd - B * b) / det;
+      out[O] = (-A * c + B * a) / det;
     }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) dW[k * nc + i] = out[k];
+  if (DIR == 1 && (!g.sol || g.sol[sid].active)) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) U[k * nc + i] += out[k];
   }
 }
 
@@ -571,8 +566,17 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_jacobi(KP kp, Geo g, const double
 // -1, or nlin), so the last one wrote buf[(state[2 sid + 1] + k) & 1].
 __device__ __forceinline__ int jac_result(const int* state, const double* red_sw, int nlin, int stride, int sid,
                                           int b0) {
-  int k = 0;
-  while (k < nlin && !(red_sw[(size_t)k * stride + b0] < 0.0)) k++;
+  // eight sweeps' marks per round trip (a loop that tests each mark before
+  // loading the next waits for one memory round trip per sweep)
+  int k = nlin;
+  for (int k0 = 0; k0 < nlin && k == nlin; k0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = k0 + q < nlin ? red_sw[(size_t)(k0 + q) * stride + b0] : 0.0;
+#pragma unroll
+    for (int q = 7; q >= 0; q--)
+      if (k0 + q < nlin && v[q] < 0.0) k = k0 + q;
+  }
   return (state[2 * sid + 1] + k) & 1;
 }
 
