@@ -82,6 +82,7 @@ struct csu_dev {
   int tuned;                   // launch shapes from the autotuner (1) or the defaults (0)
   // implicit schemes
   double *Dp, *dW0, *dW1, *A;  // diagonal (without the dt term), increments, DPLUR blocks
+  double* caux;                // LU-SGS: per cell c, max(4/(3r), gamma/r), mu/Pr of the iteration's state [3][nc]
   int* jstate;                 // DPLUR: [unused, buffer holding the last result]
   double* jred;                // DPLUR: per-sweep block maxima of |dx_rho| [nlinear][cell blocks]
   std::vector<int> lvlL, lvlU; // LU-SGS level offsets (forward / backward)
@@ -338,6 +339,7 @@ struct HostMesh {
   std::vector<unsigned char> fsl, bmask;                          // [nf], [nc]
   std::vector<int> fe0, fe1;                                      // [nf]
   std::vector<double> fdsx, fdsy, fds;                            // [nf]
+  std::vector<double> fnx, fny, fdd;  // [nf] unit normal, centre distance of the face's cells (LU-SGS)
   std::vector<int> be0, btype, bdonor;                            // [nb]
   std::vector<double> bdsx, bdsy, bds, bxm, bym, bnx, bny, bds2;  // [nb]
   std::vector<int> wall;                                          // boundary ids of wall / wallT faces
@@ -489,6 +491,17 @@ static int host_mesh(const csu_mesh_desc* m, const csu_params* p, HostMesh& h) {
     const int f = h.cface[t];
     if (f != 0) h.cfn[t] = f > 0 ? h.fe1[f - 1] : h.fe0[-f - 1];
   }
+  // LU-SGS face terms that depend on the geometry only, in the sweep's own
+  // expressions (implicit.c:288-313, 555-595): n = dS/|dS| (seen from e1 it
+  // is the exact negation) and the distance of the two cell centres (the
+  // same from either side: the differences only change sign)
+  h.fnx.assign(nf, 0.0); h.fny.assign(nf, 0.0); h.fdd.assign(nf, 0.0);
+  for (int f = 0; f < nf; f++) {
+    h.fnx[f] = h.fdsx[f] / h.fds[f];
+    h.fny[f] = h.fdsy[f] / h.fds[f];
+    const double ddx = h.xc[h.fe1[f]] - h.xc[h.fe0[f]], ddy = h.yc[h.fe1[f]] - h.yc[h.fe0[f]];
+    h.fdd[f] = sqrt(ddx * ddx + ddy * ddy);
+  }
   h.be0.assign(nb, 0); h.btype.assign(nb, 0); h.bdonor.assign(nb, -1);
   for (auto* v : {&h.bdsx, &h.bdsy, &h.bds, &h.bxm, &h.bym, &h.bnx, &h.bny, &h.bds2}) v->assign(nb, 0.0);
   h.wall.clear();
@@ -564,6 +577,7 @@ static void host_concat(const std::vector<HostMesh>& part, HostMesh& H, std::vec
   H.fsl.assign(NF ? NF : 1, 0); H.bmask.assign(NC, 0);
   H.fe0.assign(NF, 0); H.fe1.assign(NF, 0);
   H.fdsx.assign(NF, 0.0); H.fdsy.assign(NF, 0.0); H.fds.assign(NF, 1.0);
+  H.fnx.assign(NF, 0.0); H.fny.assign(NF, 0.0); H.fdd.assign(NF, 1.0);
   H.be0.assign(NB, 0); H.btype.assign(NB, 0); H.bdonor.assign(NB, -1);
   for (auto* v : {&H.bdsx, &H.bdsy, &H.bxm, &H.bym, &H.bnx, &H.bny}) v->assign(NB, 0.0);
   H.bds.assign(NB, 1.0); H.bds2.assign(NB, 1.0);
@@ -592,6 +606,7 @@ static void host_concat(const std::vector<HostMesh>& part, HostMesh& H, std::vec
       const size_t gf = (size_t)f0 + f;
       H.fe0[gf] = h.fe0[f] + c0; H.fe1[gf] = h.fe1[f] + c0; H.fsl[gf] = h.fsl[f];
       H.fdsx[gf] = h.fdsx[f]; H.fdsy[gf] = h.fdsy[f]; H.fds[gf] = h.fds[f];
+      H.fnx[gf] = h.fnx[f]; H.fny[gf] = h.fny[f]; H.fdd[gf] = h.fdd[f];
     }
     for (int g = 0; g < h.nb; g++) {
       const size_t gg = (size_t)b0 + g;
@@ -817,6 +832,9 @@ static int dev_setup(csu_dev* d, const HostMesh& H, int ns, const csu_mesh_desc*
     HIPCHK(hipMemsetAsync(d->jstate, 0, 2 * sizeof(int) * (size_t)ns, d->stream));
   }
   if (p->time_scheme == CSU_TIME_LUSGS) {
+    if ((rc = upload(d, &g.fnx, H.fnx)) || (rc = upload(d, &g.fny, H.fny)) || (rc = upload(d, &g.fdd, H.fdd)) ||
+        (rc = dalloc(d, &d->caux, 3 * (size_t)nc)))
+      return rc;
     // cells grouped by sweep level (the levels of a batch's solves merge:
     // cells of different solves never depend on each other)
     auto build = [&](const std::vector<int>& lev, std::vector<int>& off, int** dcells) -> int {
@@ -859,7 +877,7 @@ static csu_dev* dev_new() {
   csu_dev* d = new csu_dev();
   d->stream = nullptr; d->prof = 0; d->nsolve = 1;
   d->hist_cap = 0; d->res_hist = d->coef_hist = nullptr; d->fresh = 1;
-  d->Dp = d->dW0 = d->dW1 = d->A = nullptr; d->jstate = nullptr; d->jred = nullptr; d->cellsL = d->cellsU = nullptr;
+  d->Dp = d->dW0 = d->dW1 = d->A = d->caux = nullptr; d->jstate = nullptr; d->jred = nullptr; d->cellsL = d->cellsU = nullptr;
   d->hctr = nullptr; d->use_graphs = 1; d->bytes = 0; d->tstate = nullptr;
   d->hres = d->hcoef = nullptr; d->hcap = 0; d->async_start = 0; d->async_n = 0;
   for (int k = 0; k < 4; k++) { d->gexec[k] = nullptr; d->gU[k] = nullptr; }
@@ -1380,7 +1398,7 @@ static int implicit_iteration(csu_dev* d) {
   {
     Prof p(d, "k_impD");
     hipLaunchKernelGGL((k_impD<NV, VISC>), dim3(nb), dim3(CSU_BLOCK), 0, d->stream, kp, d->geo, d->st, d->prm.wimp,
-                       d->Dp, red_min);
+                       d->Dp, red_min, d->caux);
     hipLaunchKernelGGL(k_dt_final, dim3(d->nsolve), dim3(CSU_BLOCK), 0, d->stream, nb, d->prm.cfl, red_min, d->dtg,
                        d->geo.sol);
   }
@@ -1392,14 +1410,16 @@ static int implicit_iteration(csu_dev* d) {
     for (size_t l = 0; l + 1 < d->lvlL.size(); l++) {
       int off = d->lvlL[l], cnt = d->lvlL[l + 1] - off;
       hipLaunchKernelGGL((k_lusgs<NV, VISC, 0>), dim3((cnt + LB - 1) / LB), dim3(LB), 0, d->stream, kp, d->geo, d->st,
-                         d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0, (const double*)nullptr, (double*)nullptr);
+                         d->cellsL + off, cnt, d->Dp, d->dtg, d->R, d->dW0, (const double*)nullptr, (double*)nullptr,
+                         (const double*)d->caux);
     }
     // the backward levels start each cell from D*dW0 and apply the final
     // increment to the state themselves
     for (size_t l = 0; l + 1 < d->lvlU.size(); l++) {
       int off = d->lvlU[l], cnt = d->lvlU[l + 1] - off;
       hipLaunchKernelGGL((k_lusgs<NV, VISC, 1>), dim3((cnt + LB - 1) / LB), dim3(LB), 0, d->stream, kp, d->geo, d->st,
-                         d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1, (const double*)d->dW0, d->U);
+                         d->cellsU + off, cnt, d->Dp, d->dtg, d->R, d->dW1, (const double*)d->dW0, d->U,
+                         (const double*)d->caux);
     }
   } else {
     Prof p(d, "k_dplur");
@@ -2092,6 +2112,7 @@ extern "C" int64_t csu_dev_footprint(const csu_mesh_desc* m, const csu_params* p
   if (p->order == 2) b += 32 * nv * nc;                                                   // edge states
   if (visc) b += nc * (16 + 16 + 48) + nf * (8 * (nv - 1) + 8) + nb * (8 * (nv - 1) + 16);
   if (p->time_scheme != CSU_TIME_RK) b += nc * (8 + 16 * nv + 8 + 16);
+  if (p->time_scheme == CSU_TIME_LUSGS) b += nc * 24 + nf * 24;
   if (p->time_scheme == CSU_TIME_DPLUR) b += 32 * nv * nv * nc + 8.0 * (p->nlinear > 0 ? p->nlinear : 1) * (nc / 256 + 1);
   b += (nv + 4) * 8 * (double)(hist > 0 ? hist : 1);
   // allocation granularity: every array rounds up (at most 2 MiB slack per large array)

@@ -129,6 +129,7 @@ struct Geo {
   const double *wdist;                                  // [nc] wall distance (SA/SST)
   const int *fe0, *fe1;                                 // [nf]
   const double *fdsx, *fdsy, *fds;
+  const double *fnx, *fny, *fdd;  // LU-SGS: [nf] dS/|dS| and the distance of the face's cell centres
   const int *be0, *btype, *bdonor;                      // [nb]
   const double *bdsx, *bdsy, *bds, *bxm, *bym, *bnx, *bny, *bds2;
 };

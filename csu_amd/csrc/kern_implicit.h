@@ -19,7 +19,7 @@ __device__ __forceinline__ double gamma_of(const GasD& g, double T) { return gas
 // term: writes Dp = 0.5 w Lc (+ Lv/Omega) and per-block min of Omega/Lc.
 template <int NV, bool VISC>
 __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double wimp, double* __restrict__ Dp,
-                                                    double* __restrict__ red_min) {
+                                                    double* __restrict__ red_min, double* __restrict__ caux) {
   __shared__ double sw[1][CSU_BLOCK / 64];
   const int t = threadIdx.x, i = blockIdx.x * blockDim.x + t;
   const int nc = kp.nc;
@@ -48,6 +48,16 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_impD(KP kp, Geo g, St s, double w
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) Pa[q] = P[(q == 3 ? 4 : q) * nc + i];  // r u v T
+    if (caux) {
+      // LU-SGS: the cell-state factors of its neighbour terms, evaluated
+      // once per cell instead of once per face and sweep (same expressions
+      // of the same values: implicit.c:307-313, 555-595)
+      caux[i] = gas_T2c<double>(kp.gas, Pa[3]);
+      if (VISC) {
+        caux[nc + i] = fmax(4 / (3 * Pa[0]), gamma_of(kp.gas, Pa[3]) / Pa[0]);
+        caux[2 * nc + i] = sutherland(Pa[3]) / kp.pr;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int b = cfj[j] == 0 ? i : nbj[j];
@@ -242,7 +252,8 @@ template <int NV, bool VISC, int DIR>
 __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const int* __restrict__ cells, int ncell_lvl,
                                                      const double* __restrict__ Dp, const double* __restrict__ dtg,
                                                      const double* __restrict__ R, double* dW,
-                                                     const double* __restrict__ dW0, double* __restrict__ U) {
+                                                     const double* __restrict__ dW0, double* __restrict__ U,
+                                                     const double* __restrict__ caux) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncell_lvl) return;
   const int nc = kp.nc;
@@ -272,27 +283,29 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
   int cfj[4], nbj[4], fij[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) cfj[j] = g.cface[j * nc + i];
-  double sxj[4], syj[4], sj[4];
+  double nxj[4], nyj[4], sj[4], ddj[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const int f = cfj[j];
     const int fi = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
     fij[j] = fi;
     sj[j] = g.fds[fi];
-    sxj[j] = g.fdsx[fi];
-    syj[j] = g.fdsy[fi];
+    nxj[j] = g.fnx[fi];
+    nyj[j] = g.fny[fi];
+    ddj[j] = VISC ? g.fdd[fi] : 1.0;
     const int e = g.cfn[j * nc + i];  // loaded beside cface, not after it
     nbj[j] = f == 0 ? i : e;
   }
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    // the slot's neighbour and outward face vector (face_from: the vector
-    // is negated when i is the face's e1, implicit.c:288-305)
+    // the slot's neighbour and outward unit normal (face_from: negated when
+    // i is the face's e1, implicit.c:288-305; dS/|dS| with the sign of dS,
+    // stored per face)
     if (cfj[j] == 0) continue;
     const int nb = nbj[j], fi = fij[j];
-    const double dS = sj[j], dSx = cfj[j] > 0 ? sxj[j] : -sxj[j], dSy = cfj[j] > 0 ? syj[j] : -syj[j];
+    const double dS = sj[j];
     if (DIR == 0 ? !(nb < i) : !(nb > i)) continue;
-    const double nx = dSx / dS, ny = dSy / dS;
+    const double nx = cfj[j] > 0 ? nxj[j] : -nxj[j], ny = cfj[j] > 0 ? nyj[j] : -nyj[j];
     double Pj[NV + 1];
 #pragma unroll
     for (int m = 0; m < NV + 1; m++) Pj[m] = s.P[m * nc + nb];
@@ -301,19 +314,15 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_lusgs(KP kp, Geo g, St s, const i
     for (int k = 0; k < NV; k++) d[k] = dW[k * nc + nb];
     double dF[NV];
     delta_flux<NV>(kp, Pj, d, nx, ny, dF);
-    double c = gas_T2c<double>(kp.gas, Pj[4]);
+    // the neighbour's sound speed and viscous factors max(4/(3r), gamma/r)
+    // and mu/Pr (k_impD, per cell)
+    const double c = caux[nb];
     double ra = kp.wimp * (fabs(nx * Pj[1] + ny * Pj[2]) + c) * dS;
     if (NV >= 5) {
       // implicitFunc_sa / _sst (implicit.c:555-595, 1120-1161)
-      double r = Pj[0], T = Pj[4], mi = sutherland(T);
-      double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
-      double dd_ = sqrt(ddx * ddx + ddy * ddy);
-      ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr + s.miT[fi] / kp.pr_t) * dS / dd_;
+      ra += caux[nc + nb] * (caux[2 * nc + nb] + s.miT[fi] / kp.pr_t) * dS / ddj[j];
     } else if (VISC && kp.laminar) {
-      double r = Pj[0], T = Pj[4], mi = sutherland(T);
-      double ddx = g.xc[nb] - g.xc[i], ddy = g.yc[nb] - g.yc[i];
-      double dd_ = sqrt(ddx * ddx + ddy * ddy);
-      ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr) * dS / dd_;
+      ra += caux[nc + nb] * caux[2 * nc + nb] * dS / ddj[j];
     }
 #pragma unroll
     for (int k = 0; k < NV; k++) acc[k] -= 0.5 * (dF[k] * dS - ra * d[k]);
