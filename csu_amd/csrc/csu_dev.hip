@@ -2114,6 +2114,7 @@ extern "C" int64_t csu_dev_footprint(const csu_mesh_desc* m, const csu_params* p
   if (p->time_scheme != CSU_TIME_RK) b += nc * (8 + 16 * nv + 8 + 16);
   if (p->time_scheme == CSU_TIME_LUSGS) b += nc * 24 + nf * 24;
   if (p->time_scheme == CSU_TIME_DPLUR) b += 32 * nv * nv * nc + 8.0 * (p->nlinear > 0 ? p->nlinear : 1) * (nc / 256 + 1);
+  if (p->precision == 1) b += nc * 64 + nf * (12 + 4 * nv) + (p->order == 2 ? 16 * nv * nc : 0);  // FP32 copies
   b += (nv + 4) * 8 * (double)(hist > 0 ? hist : 1);
   // allocation granularity: every array rounds up (at most 2 MiB slack per large array)
   return (int64_t)(b * 1.02) + (int64_t)64 * 4096;

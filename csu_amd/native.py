@@ -327,6 +327,10 @@ class Device:
         check(lib().csu_dev_get_state(self._h, _p(U, C.c_double)))
         return U
 
+    def bytes(self) -> int:
+        """Device bytes the context holds (csu_dev_bytes)."""
+        return lib().csu_dev_bytes(self._h)
+
     def get_dt(self) -> np.ndarray:
         d = np.empty(self.ncell)
         check(lib().csu_dev_get_dt(self._h, _p(d, C.c_double)))
