@@ -731,7 +731,8 @@ def main():
                           "copy_gb_per_s": 2 * n * 8 / dt / 1e9}))
         return
     if a.only_small:
-        r = gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False)
+        r = gpu_run(200, 100, max(a.steps, 50), a.warmup, profile=False,
+                    shape=tuple(int(v) for v in a.shape.split(",")) if a.shape else None)
         n = max(a.steps, 50)
         print(json.dumps({"config0_40k": {"cells": r["nc"], "steps": n, "ms_per_step": 1e3 * r["seconds"] / n,
                                           "cell_iter_per_s": r["nc"] * n / r["seconds"],
