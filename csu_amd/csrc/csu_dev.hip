@@ -1969,7 +1969,8 @@ extern "C" int csu_dev_sst_fields(csu_dev* d, double* out) {
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(w.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost, d->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
-  hipFree(buf);
+  const hipError_t ef = hipFree(buf);
+  if (e == hipSuccess) e = ef;
   if (e != hipSuccess) return set_err(CSU_ERR_HIP, "sst fields: %s", hipGetErrorString(e));
   for (int i = 0; i < d->nc; i++)
     for (int k = 0; k < 5; k++) out[5 * (size_t)i + k] = w[(size_t)k * d->nc + i];
@@ -1996,7 +1997,8 @@ extern "C" int csu_dev_surf_faces(csu_dev* d, double* out) {
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(w.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost, d->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
-  hipFree(buf);
+  const hipError_t ef = hipFree(buf);
+  if (e == hipSuccess) e = ef;
   if (e != hipSuccess) return set_err(CSU_ERR_HIP, "surface faces: %s", hipGetErrorString(e));
   for (int q = 0; q < nb; q++)
     for (int k = 0; k < 11; k++) out[11 * (size_t)q + k] = w[(size_t)k * nb + q];
@@ -2080,10 +2082,10 @@ extern "C" int csu_stream_rate(int64_t bytes, int32_t reps, double* gbps) {
   float best = 1e30f;
   for (int r = 0; r <= reps && !rc; r++) {   // launch 0 warms up
     float ms = 0.f;
-    hipEventRecord(e0, nullptr);
+    if (hipEventRecord(e0, nullptr) != hipSuccess) { rc = set_err(CSU_ERR_HIP, "stream timing"); break; }
     hipLaunchKernelGGL(k_stream, dim3(grid), dim3(CSU_BLOCK), 0, nullptr, x, y, n2, 1.000001);
-    hipEventRecord(e1, nullptr);
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+    if (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
       rc = set_err(CSU_ERR_HIP, "stream timing");
     else if (r > 0 && ms < best)
       best = ms;

@@ -72,11 +72,12 @@ static char* slurp(const char* path, size_t* len)
 /* Stable LSD radix sort of idx[0..n) by key[idx[.]] (8-bit digits, only as
    many as the largest key needs): replaces the comparison sorts of the
    face enumeration, whose keys make every order total. */
-static void radix_sort(const uint64_t* key, uint32_t* idx, size_t n)
+static int radix_sort(const uint64_t* key, uint32_t* idx, size_t n)
 {
     uint64_t mx = 0;
     for (size_t i = 0; i < n; i++) if (key[i] > mx) mx = key[i];
     uint32_t* tmp = malloc(sizeof(uint32_t) * (n ? n : 1));
+    if (!tmp) return -1;
     for (int shift = 0; shift < 64 && (mx >> shift) != 0; shift += 8) {
         size_t cnt[257] = {0};
         for (size_t i = 0; i < n; i++) cnt[((key[idx[i]] >> shift) & 255) + 1]++;
@@ -85,6 +86,7 @@ static void radix_sort(const uint64_t* key, uint32_t* idx, size_t n)
         memcpy(idx, tmp, sizeof(uint32_t) * n);
     }
     free(tmp);
+    return 0;
 }
 
 static double tri_area(const double* xy, int p0, int p1, int p2)
@@ -127,6 +129,12 @@ static int err(const char* msg, const char* path)
     return CSU_ERR_IO;
 }
 
+static int nomem(const char* path)
+{
+    fprintf(stderr, "csu mesh: out of host memory (%s)\n", path);
+    return CSU_ERR_NOMEM;
+}
+
 int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
 {
     size_t flen = 0;
@@ -143,6 +151,7 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     if (nelem <= 0) { free(fbuf); return err("missing NELEM=", path); }
 
     int32_t* cv = malloc(sizeof(int32_t) * 4 * (size_t)nelem);
+    if (!cv) { free(fbuf); return nomem(path); }
     for (int i = 0; i < nelem; i++) {
         get_word(f, s, sizeof s);
         int type = atoi(s);
@@ -162,6 +171,7 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     if (strcmp(s, "NPOIN=") == 0) { get_word(f, s, sizeof s); npoin = atoi(s); }
     if (npoin <= 0) { free(cv); free(fbuf); return err("missing NPOIN=", path); }
     double* xy = malloc(sizeof(double) * 2 * (size_t)npoin);
+    if (!xy) { free(cv); free(fbuf); return nomem(path); }
     for (int i = 0; i < npoin; i++) {
         get_word(f, s, sizeof s); xy[2 * i] = strtod(s, NULL);
         get_word(f, s, sizeof s); xy[2 * i + 1] = strtod(s, NULL);
@@ -176,6 +186,10 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     int nb = 0, bcap = 1024;
     int32_t* bp = malloc(sizeof(int32_t) * 2 * (size_t)bcap);
     int32_t* bm = malloc(sizeof(int32_t) * (size_t)bcap);
+    if (!mname || !mcount || !bp || !bm) {
+        free(cv); free(xy); free(fbuf); free(mname); free(mcount); free(bp); free(bm);
+        return nomem(path);
+    }
     for (int m = 0; m < nmark; m++) {
         get_word(f, s, sizeof s);                       /* MARKER_TAG= */
         get_word(f, mname[m], sizeof mname[m]);
@@ -186,8 +200,14 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
         for (int j = 0; j < cnt; j++) {
             if (nb == bcap) {
                 bcap *= 2;
-                bp = realloc(bp, sizeof(int32_t) * 2 * (size_t)bcap);
-                bm = realloc(bm, sizeof(int32_t) * (size_t)bcap);
+                int32_t* nbp = realloc(bp, sizeof(int32_t) * 2 * (size_t)bcap);
+                if (nbp) bp = nbp;
+                int32_t* nbm = nbp ? realloc(bm, sizeof(int32_t) * (size_t)bcap) : NULL;
+                if (nbm) bm = nbm;
+                if (!nbp || !nbm) {
+                    free(cv); free(xy); free(fbuf); free(mname); free(mcount); free(bp); free(bm);
+                    return nomem(path);
+                }
             }
             get_word(f, s, sizeof s);                   /* line type (3) */
             get_word(f, s, sizeof s); bp[2 * nb] = atoi(s);
@@ -225,6 +245,7 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     /* directed edges in insertion order, paired on the undirected key */
     size_t nde = 0;
     dedge* de = malloc(sizeof(dedge) * 4 * (size_t)nelem);
+    if (!de) { free(cv); free(xy); free(bp); free(bm); free(mname); return nomem(path); }
     for (int i = 0; i < nelem; i++) {
         int np = cv[4 * i + 3] >= 0 ? 4 : 3;
         for (int k = 0; k < np; k++) {
@@ -240,14 +261,22 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
         /* sorted on (key, seq): de is built in seq order, the sort is stable */
         uint64_t* k = malloc(sizeof(uint64_t) * (nde ? nde : 1));
         uint32_t* ix = malloc(sizeof(uint32_t) * (nde ? nde : 1));
-        for (size_t i = 0; i < nde; i++) { k[i] = (uint64_t)de[i].key; ix[i] = (uint32_t)i; }
-        radix_sort(k, ix, nde);
         dedge* ds = malloc(sizeof(dedge) * (nde ? nde : 1));
+        if (!k || !ix || !ds) {
+            free(k); free(ix); free(ds); free(de); free(cv); free(xy); free(bp); free(bm); free(mname);
+            return nomem(path);
+        }
+        for (size_t i = 0; i < nde; i++) { k[i] = (uint64_t)de[i].key; ix[i] = (uint32_t)i; }
+        if (radix_sort(k, ix, nde)) {
+            free(k); free(ix); free(ds); free(de); free(cv); free(xy); free(bp); free(bm); free(mname);
+            return nomem(path);
+        }
         for (size_t i = 0; i < nde; i++) ds[i] = de[ix[i]];
         free(de); free(k); free(ix);
         de = ds;
     }
     rface* rf = malloc(sizeof(rface) * (nde / 2 + 1));
+    if (!rf) { free(de); free(cv); free(xy); free(bp); free(bm); free(mname); return nomem(path); }
     size_t nf = 0;
     int64_t hsize = (int64_t)nelem + npoin;
     for (size_t a = 0; a < nde;) {
@@ -274,12 +303,19 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
         /* sorted on (bucket, seq): both below 2^31, one 62-bit key */
         uint64_t* k = malloc(sizeof(uint64_t) * (nf ? nf : 1));
         uint32_t* ix = malloc(sizeof(uint32_t) * (nf ? nf : 1));
+        rface* rs = malloc(sizeof(rface) * (nf + 1));
+        if (!k || !ix || !rs) {
+            free(k); free(ix); free(rs); free(rf); free(cv); free(xy); free(bp); free(bm); free(mname);
+            return nomem(path);
+        }
         for (size_t i = 0; i < nf; i++) {
             k[i] = ((uint64_t)rf[i].bucket << 31) | (uint64_t)rf[i].seq;
             ix[i] = (uint32_t)i;
         }
-        radix_sort(k, ix, nf);
-        rface* rs = malloc(sizeof(rface) * (nf + 1));
+        if (radix_sort(k, ix, nf)) {
+            free(k); free(ix); free(rs); free(rf); free(cv); free(xy); free(bp); free(bm); free(mname);
+            return nomem(path);
+        }
         for (size_t i = 0; i < nf; i++) rs[i] = rf[ix[i]];
         free(rf); free(k); free(ix);
         rf = rs;
@@ -289,6 +325,10 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     int32_t* nbr = malloc(sizeof(int32_t) * 4 * (size_t)nelem);
     int32_t* cface = malloc(sizeof(int32_t) * 4 * (size_t)nelem);
     int* nn = calloc((size_t)nelem, sizeof(int));
+    if (!face || !nbr || !cface || !nn) {
+        free(face); free(nbr); free(cface); free(nn); free(rf); free(cv); free(xy); free(bp); free(bm); free(mname);
+        return nomem(path);
+    }
     for (int i = 0; i < 4 * nelem; i++) { nbr[i] = INT32_MAX; cface[i] = 0; }
     for (size_t j = 0; j < nf; j++) {
         face[4 * j + 0] = rf[j].e0; face[4 * j + 1] = rf[j].e1;
@@ -308,11 +348,20 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
 
     /* node -> cells incidence (ascending cell ids) */
     int* ninc = calloc((size_t)npoin + 1, sizeof(int));
+    if (!ninc) {
+        free(face); free(nbr); free(cface); free(nn); free(cv); free(xy); free(bp); free(bm); free(mname);
+        return nomem(path);
+    }
     for (int i = 0; i < nelem; i++)
         for (int k = 0; k < 4; k++) if (cv[4 * i + k] >= 0) ninc[cv[4 * i + k] + 1]++;
     for (int p = 0; p < npoin; p++) ninc[p + 1] += ninc[p];
     int* inc = malloc(sizeof(int) * (size_t)(ninc[npoin] > 0 ? ninc[npoin] : 1));
     int* fill = calloc((size_t)npoin, sizeof(int));
+    if (!inc || !fill) {
+        free(inc); free(fill); free(ninc); free(face); free(nbr); free(cface); free(nn); free(cv); free(xy);
+        free(bp); free(bm); free(mname);
+        return nomem(path);
+    }
     for (int i = 0; i < nelem; i++)
         for (int k = 0; k < 4; k++) {
             int p = cv[4 * i + k];
@@ -321,6 +370,11 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     free(fill);
 
     int32_t* bface = malloc(sizeof(int32_t) * 3 * (size_t)(nb ? nb : 1));
+    if (!bface) {
+        free(inc); free(ninc); free(face); free(nbr); free(cface); free(nn); free(cv); free(xy); free(bp); free(bm);
+        free(mname);
+        return nomem(path);
+    }
     for (int g = 0; g < nb; g++) {
         int p0 = bp[2 * g], p1 = bp[2 * g + 1], e = -1;
         for (int t = ninc[p0]; t < ninc[p0 + 1] && e < 0; t++) {
@@ -357,6 +411,7 @@ int csu_mesh_read_su2(const char* path, int32_t axi, csu_case* c)
     c->bface_mark = bm; c->cell_nbr = nbr; c->cell_face = cface;
     c->mark_name = mname;
     c->mark_type = calloc((size_t)(nmark > 0 ? nmark : 1), sizeof(int32_t));
+    if (!c->mark_type) return nomem(path);  /* the arrays above belong to c: csu_case_free releases them */
     c->mesh.node_xy = xy; c->mesh.cell_vert = cv; c->mesh.face = face;
     c->mesh.bface = bface; c->mesh.bface_mark = bm; c->mesh.mark_type = c->mark_type;
     c->mesh.cell_nbr = nbr; c->mesh.cell_face = cface; c->mesh.wall_dist = NULL;
