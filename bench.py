@@ -659,6 +659,21 @@ def cpu_baseline(nx: int, ny: int, iters: int):
                       f"{iters} iterations minus a 0-iteration run, threads={threads}"}
 
 
+def _finite(o):
+    """The result with every non-finite number (a percentile of no
+    converged solve, a ratio of nothing) as null, so that the line is strict
+    JSON."""
+    if isinstance(o, dict):
+        return {k: _finite(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [_finite(v) for v in o]
+    if isinstance(o, (float, np.floating)):
+        return float(o) if np.isfinite(o) else None
+    if isinstance(o, np.integer):
+        return int(o)
+    return o
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -740,7 +755,7 @@ def main():
         return
     if a.only_serve:
         lg = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
-        print(json.dumps({"serving": serve_all(a, lg)}, default=lambda o: None))
+        print(json.dumps(_finite({"serving": serve_all(a, lg)}), default=lambda o: None, allow_nan=False))
         return
     if a.gpus != 1:
         print(f"bench.py: the benchmark runs on one GPU only (asked for {a.gpus}); "
@@ -837,7 +852,7 @@ def main():
         "serving": serve,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(_finite(line), allow_nan=False))
 
 
 if __name__ == "__main__":
