@@ -80,7 +80,7 @@ _MESHES = {}
 def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision: int = 0, keep: bool = False,
             tune: bool = True, shape=None):
     """The ramp on an nx x ny x 2 mesh: launch shapes autotuned (tune) or set
-    (shape: block, LU-SGS block, XCD map), warmup iterations, then `steps`
+    (shape: block, LU-SGS block, XCD map[, tile form]), warmup iterations, then `steps`
     timed graph-replayed iterations; with profile, `steps` more with HIP
     events around every kernel group (the per-kernel times; eager launches,
     so that pass is slower and not the timed one)."""
@@ -705,9 +705,10 @@ def main():
     ap.add_argument("--only-small", action="store_true",
                     help="only the 200x100x2 (40k-cell) ramp: autotuned, --warmup then max(--steps, 50) timed "
                          "graph-replayed iterations; prints its JSON (the form kernel traces of it profile)")
-    ap.add_argument("--shape", default=None, metavar="B,L,X",
-                    help="launch shapes of the ramp runs instead of the autotuner's (e.g. 128,64,0: the choice a "
-                         "timed run reported), so that profiling runs launch the kernels the timed run did")
+    ap.add_argument("--shape", default=None, metavar="B,L,X[,T]",
+                    help="launch shapes of the ramp runs instead of the autotuner's (e.g. 128,64,0,0: the choice a "
+                         "timed run reported; T 1: the tile form of the face flux + update), so that profiling "
+                         "runs launch the kernels the timed run did")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 200x100x2 timing (profiling runs of the big mesh alone)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",

@@ -79,6 +79,8 @@ struct csu_dev {
   int pu;                      // RK stages recover the primitives from the state (k_grad's PU form)
   int sblk;                    // threads per workgroup of k_grad and k_flux
   int lblk;                    // threads per workgroup of an LU-SGS level launch
+  int tile_ok;                 // the tile form's face lists exist (primitives-from-state runs whose tiles fit)
+  int tile;                    // face fluxes evaluated per cell tile inside k_update, no k_flux (1)
   int tuned;                   // launch shapes from the autotuner (1) or the defaults (0)
   // implicit schemes
   double *Dp, *dW0, *dW1, *A;  // diagonal (without the dt term), increments, DPLUR blocks
@@ -645,7 +647,7 @@ struct TuneKey {
     return std::tie(nv, visc, scheme, prec, order, bucket) < std::tie(o.nv, o.visc, o.scheme, o.prec, o.order, o.bucket);
   }
 };
-struct TuneVal { int sblk, lblk, xcd; };
+struct TuneVal { int sblk, lblk, xcd, tile; };
 static std::mutex g_tune_mu;
 static std::map<TuneKey, TuneVal> g_tune;
 
@@ -661,8 +663,63 @@ static void tune_lookup(csu_dev* d) {
   if (it == g_tune.end()) return;
   d->sblk = it->second.sblk;
   d->lblk = it->second.lblk;
+  d->tile = it->second.tile;  // dropped again by dev_setup when the context has no tile lists
   d->kp.xcd = it->second.xcd;
   d->tuned = 1;
+}
+
+// Face lists of the tile form (k_update TILE): the interior faces of each
+// 256-cell tile (of a batch's solves: their real faces only), a face shared
+// by two tiles in both lists, and each cell slot's face as its place in its
+// own tile's list (signed like cface: + where the cell is the face's e0).
+// False when a tile has more faces than the workgroup's LDS list holds.
+static bool tile_lists(const HostMesh& H, const std::vector<SolveP>* sol, std::vector<int>& toff,
+                       std::vector<int>& tface, std::vector<short>& tsl) {
+  const int nc = H.nc, nt = (nc + CSU_BLOCK - 1) / CSU_BLOCK;
+  auto each_face = [&](auto fn) {
+    if (sol && !sol->empty()) {
+      for (const SolveP& q : *sol)
+        for (int f = q.f0; f < q.f1; f++) fn(f);
+    } else {
+      for (int f = 0; f < H.nf; f++) fn(f);
+    }
+  };
+  std::vector<int> cnt(nt, 0);
+  each_face([&](int f) {
+    const int a = H.fe0[f] / CSU_BLOCK, b = H.fe1[f] / CSU_BLOCK;
+    cnt[a]++;
+    if (b != a) cnt[b]++;
+  });
+  toff.assign(nt + 1, 0);
+  for (int t = 0; t < nt; t++) {
+    if (cnt[t] > CSU_TILE_FACES) return false;
+    toff[t + 1] = toff[t] + cnt[t];
+  }
+  tface.assign(toff[nt] > 0 ? toff[nt] : 1, 0);
+  std::vector<int> fill(toff.begin(), toff.end() - 1), loc0(H.nf > 0 ? H.nf : 1, -1), loc1(H.nf > 0 ? H.nf : 1, -1);
+  each_face([&](int f) {
+    const int a = H.fe0[f] / CSU_BLOCK, b = H.fe1[f] / CSU_BLOCK;
+    loc0[f] = fill[a] - toff[a];
+    tface[fill[a]++] = f;
+    if (b != a) {
+      loc1[f] = fill[b] - toff[b];
+      tface[fill[b]++] = f;
+    } else {
+      loc1[f] = loc0[f];
+    }
+  });
+  tsl.assign(4 * (size_t)nc, 0);
+  for (int j = 0; j < 4; j++)
+    for (int i = 0; i < nc; i++) {
+      const int cf = H.cface[(size_t)j * nc + i];
+      if (cf == 0) continue;
+      const int f = cf > 0 ? cf - 1 : -cf - 1;
+      const int l = cf > 0 ? loc0[f] : loc1[f];  // cf > 0: i is the face's e0
+      const int t = i / CSU_BLOCK;
+      if (l < 0 || l >= cnt[t] || (cf > 0 ? H.fe0[f] : H.fe1[f]) != i) return false;
+      tsl[(size_t)j * nc + i] = (short)(cf > 0 ? l + 1 : -(l + 1));
+    }
+  return true;
 }
 
 // Device state of a context from its host arrays: uploads, allocations, wall
@@ -693,6 +750,8 @@ static int dev_setup(csu_dev* d, const HostMesh& H, int ns, const csu_mesh_desc*
   // (no shared memory), so that a few hundred cell tiles reach every CU
   d->sblk = nc <= 262144 ? 64 : CSU_BLOCK;
   d->lblk = 64;
+  d->tile = 0;
+  d->tile_ok = 0;
   kp.xcd = 1;
   d->tuned = 0;
   tune_lookup(d);
@@ -781,6 +840,16 @@ static int dev_setup(csu_dev* d, const HostMesh& H, int ns, const csu_mesh_desc*
     d->sol_host = *sol;
   }
   if (p->order == 2 && !kp.mixed && (rc = dalloc(d, &s.rec, 4 * nvc))) return rc;
+  if (d->pu) {
+    std::vector<int> toff, tface;
+    std::vector<short> tsl;
+    if (tile_lists(H, sol, toff, tface, tsl)) {
+      if ((rc = upload(d, &g.toff, toff)) || (rc = upload(d, &g.tface, tface)) || (rc = upload(d, &g.tsl, tsl)))
+        return rc;
+      d->tile_ok = 1;
+    }
+  }
+  if (!d->tile_ok) d->tile = 0;
   if (kp.mixed) {
     auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
     if ((rc = upload(d, &g.lwx32, f32(H.lwx))) || (rc = upload(d, &g.lwy32, f32(H.lwy))) ||
@@ -1102,7 +1171,7 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
           hipLaunchKernelGGL((k_grad<4, false, false, false, true>), dim3((nc + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp,
                              d->geo, st);
       }
-      if (d->nf > 0) {
+      if (d->nf > 0 && !d->tile) {
         Prof p(d, "k_flux");
         if (kp.mixed)
           hipLaunchKernelGGL((k_flux<4, false, true>), dim3((d->nf + d->sblk - 1) / d->sblk), dim3(d->sblk), 0, d->stream, kp, d->geo, st);
@@ -1111,14 +1180,20 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
       }
       {
         Prof p(d, "k_update");
-        if (kp.mixed)
-          hipLaunchKernelGGL((k_update<4, false, true, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
-                             d->geo, st, flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
-                             d->red + (size_t)NV * d->nred);
+        const dim3 gu(nblk(nc)), bu(CSU_BLOCK);
+        double* const rdt = d->red + (size_t)NV * d->nred;
+        if (kp.mixed && d->tile)
+          hipLaunchKernelGGL((k_update<4, false, true, true, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
+                             rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
+        else if (kp.mixed)
+          hipLaunchKernelGGL((k_update<4, false, true, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
+                             rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
+        else if (d->tile)
+          hipLaunchKernelGGL((k_update<4, false, false, true, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
+                             rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
         else
-          hipLaunchKernelGGL((k_update<4, false, false, true>), dim3(nblk(nc)), dim3(CSU_BLOCK), 0, d->stream, kp,
-                             d->geo, st, flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red,
-                             d->red + (size_t)NV * d->nred);
+          hipLaunchKernelGGL((k_update<4, false, false, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
+                             rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
       }
       HIPCHK(hipGetLastError());
       return CSU_OK;
@@ -1547,11 +1622,13 @@ extern "C" int csu_dev_autotune(csu_dev* d, int32_t iters, int32_t* choice) {
   if (!rc) rc = enqueue_iterations(d, 2 * iters + 2);
   const int sbs[3] = {64, 128, 256};
   const bool lus = d->prm.time_scheme == CSU_TIME_LUSGS;
-  TuneVal best{d->sblk, d->lblk, d->kp.xcd};
+  TuneVal best{d->sblk, d->lblk, d->kp.xcd, d->tile};
   float best_ms = 1e30f;
+  for (int u = 0; u < (d->tile_ok ? 2 : 1) && !rc; u++)
   for (int a = 0; a < 3 && !rc; a++)
     for (int b = 0; b < (lus ? 3 : 1) && !rc; b++)
       for (int x = 1; x >= 0 && !rc; x--) {
+        d->tile = u;
         d->sblk = sbs[a];
         d->lblk = lus ? sbs[b] : d->lblk;
         d->kp.xcd = x;
@@ -1570,11 +1647,12 @@ extern "C" int csu_dev_autotune(csu_dev* d, int32_t iters, int32_t* choice) {
           ms = std::min(ms, t);
         }
         if (rc) break;
-        if (ms < best_ms) { best_ms = ms; best = TuneVal{d->sblk, d->lblk, d->kp.xcd}; }
+        if (ms < best_ms) { best_ms = ms; best = TuneVal{d->sblk, d->lblk, d->kp.xcd, d->tile}; }
       }
   d->sblk = best.sblk;
   d->lblk = best.lblk;
   d->kp.xcd = best.xcd;
+  d->tile = best.tile;
   const int rc2 = tune_reset(d, saved);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -1586,33 +1664,37 @@ extern "C" int csu_dev_autotune(csu_dev* d, int32_t iters, int32_t* choice) {
     g_tune[tune_key(d)] = best;
   }
   d->tuned = 1;
-  if (choice) { choice[0] = best.sblk; choice[1] = best.lblk; choice[2] = best.xcd; }
+  if (choice) { choice[0] = best.sblk; choice[1] = best.lblk; choice[2] = best.xcd; choice[3] = best.tile; }
   return CSU_OK;
 }
 
 // Launch shapes chosen by the caller (a shape the autotuner picked in an
 // earlier process, or a fixed one for profiling runs): gradient/face
 // workgroup and LU-SGS level workgroup (64, 128 or 256), XCD tile map (0
-// or 1).  Results do not depend on them.
+// or 1), tile form of the face flux + update (0 or 1; primitives-from-state
+// runs only).  Results do not depend on them.
 extern "C" int csu_dev_set_launch_shape(csu_dev* d, const int32_t* choice) {
   if (!d || !choice) return set_err(CSU_ERR_ARG, "null argument");
   auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
-  if (!ok(choice[0]) || !ok(choice[1]) || (choice[2] != 0 && choice[2] != 1))
-    return set_err(CSU_ERR_ARG, "launch shape %d %d %d", choice[0], choice[1], choice[2]);
+  if (!ok(choice[0]) || !ok(choice[1]) || (choice[2] != 0 && choice[2] != 1) || (choice[3] != 0 && choice[3] != 1))
+    return set_err(CSU_ERR_ARG, "launch shape %d %d %d %d", choice[0], choice[1], choice[2], choice[3]);
+  if (choice[3] && !d->tile_ok)
+    return set_err(CSU_ERR_ARG, "the tile form needs an inviscid second-order RK run whose tiles fit");
   HIPCHK(hipStreamSynchronize(d->stream));
   graphs_drop(d);  // the shapes are part of the captured launches
   d->sblk = choice[0];
   d->lblk = choice[1];
   d->kp.xcd = choice[2];
+  d->tile = choice[3];
   d->tuned = 0;
   return CSU_OK;
 }
 
 // Launch shapes in use: gradient/face workgroup, LU-SGS level workgroup,
-// XCD tile map; returns 1 when they came from the autotuner.
+// XCD tile map, tile form; returns 1 when they came from the autotuner.
 extern "C" int csu_dev_launch_shape(const csu_dev* d, int32_t* choice) {
   if (!d || !choice) return -set_err(CSU_ERR_ARG, "null argument");
-  choice[0] = d->sblk; choice[1] = d->lblk; choice[2] = d->kp.xcd;
+  choice[0] = d->sblk; choice[1] = d->lblk; choice[2] = d->kp.xcd; choice[3] = d->tile;
   return d->tuned;
 }
 
@@ -2117,6 +2199,7 @@ extern "C" int64_t csu_dev_footprint(const csu_mesh_desc* m, const csu_params* p
   if (p->time_scheme == CSU_TIME_LUSGS) b += nc * 24 + nf * 24;
   if (p->time_scheme == CSU_TIME_DPLUR) b += 32 * nv * nv * nc + 8.0 * (p->nlinear > 0 ? p->nlinear : 1) * (nc / 256 + 1);
   if (p->precision == 1) b += nc * 64 + nf * (12 + 4 * nv) + (p->order == 2 ? 16 * nv * nc : 0);  // FP32 copies
+  if (p->time_scheme == CSU_TIME_RK && !visc && p->order == 2) b += 8 * nc + 8 * nf + 4 * (nc / 256 + 2);  // tile lists
   b += (nv + 4) * 8 * (double)(hist > 0 ? hist : 1);
   // allocation granularity: every array rounds up (at most 2 MiB slack per large array)
   return (int64_t)(b * 1.02) + (int64_t)64 * 4096;

@@ -13,6 +13,9 @@ namespace csu {
 // solve, and a solve's per-block reduction partials are those of its own
 // run alone.
 #define CSU_CHUNK 256
+// tile form of the face flux + update: most interior faces a 256-cell tile
+// may have (its cells' faces, those shared with other tiles included)
+#define CSU_TILE_FACES 1024
 
 // Scalar options and free-stream data, passed by value.  nc, nf, nb are the
 // array strides (the whole batch); cend, fend, bend the end of the
@@ -119,6 +122,8 @@ struct Geo {
   const int *csolk, *fsolk, *bsolk, *csol;
   const int *cnp, *cnbr, *cface;                        // [nc], [4][nc], [4][nc]
   const int* cfn;  // implicit schemes: [4][nc] the other cell of face slot j, -1 where cface is 0
+  const int *toff, *tface;  // tile form: faces of tile t are tface[toff[t] .. toff[t+1])
+  const short* tsl;         // tile form: [4][nc] +-(1 + the slot's face's place in its tile's list), 0: none
   const double *xc, *yc, *omega, *dsxm, *dsym, *dslat;  // [nc]
   const double *edx, *edy;                              // [4][nc] own-edge midpoint offsets
   const double *lwx, *lwy;                              // [4][nc] LSQ weights per neighbour slot

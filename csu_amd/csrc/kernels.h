@@ -584,7 +584,13 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_bcflux(KP kp, Geo g, St s) {
 // fused with the next stage's solverCalcPrimitive, and on the last stage
 // the next time step (solver.c:895-950) and solverCalcRes partials
 // (solver.c:867-892).
-template <int NV, bool VISC, bool MIX = false, bool PU = false>
+// TILE (inviscid runs): no face-flux kernel.  The workgroup of a 256-cell
+// tile first evaluates the convective flux of every interior face of its
+// cells into LDS (conv_flux, as the face kernel does; a face between two
+// tiles is evaluated by both), then each cell sums its faces' fluxes from
+// LDS in its slot order: the same values in the same order, without the
+// face-flux array's write and gather.
+template <int NV, bool VISC, bool MIX = false, bool PU = false, bool TILE = false>
 __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int flags, double sigma, double rk_a,
                                                       double cfl, double* dtl, const double* dtg, double* Un,
                                                       double* __restrict__ Uout, double* __restrict__ Rout,
@@ -603,6 +609,22 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
     held = !q.active;
   }
   const bool active = i < kp.cend;
+  using TS = typename std::conditional<MIX, float, double>::type;
+  __shared__ TS sfl[TILE ? CSU_TILE_FACES * NV : 1];
+  if (TILE) {
+    static_assert(!TILE || !VISC, "tile form: inviscid runs only");
+    // the tile's faces (its cells' interior faces): g.tface[g.toff[tile] ..)
+    const int f0 = g.toff[tile], nft = g.toff[tile + 1] - f0;
+    for (int k = t; k < nft; k += blockDim.x) {
+      const int f = g.tface[f0 + k];
+      const int sl = g.fsl[f];
+      TS fl[NV];
+      conv_flux<NV, MIX>(kp, g, s, f, g.fe0[f], g.fe1[f], sl & 3, sl >> 2, fl);
+#pragma unroll
+      for (int kk = 0; kk < NV; kk++) sfl[k * NV + kk] = fl[kk];
+    }
+    __syncthreads();
+  }
   double R[NV];
 #pragma unroll
   for (int kk = 0; kk < NV; kk++) R[kk] = 0.0;
@@ -610,17 +632,20 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
   if (active) {
     // all face indices, then all face fluxes, then the sums in slot order:
     // two memory round trips instead of one pair per slot (slots without
-    // a face read face 0 and add nothing)
+    // a face read face 0 and add nothing).  Tile form: the slot's face as
+    // +-(its place in the tile's list + 1), with the face id's sign.
     int cf[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) cf[j] = g.cface[j * nc + i];
+    for (int j = 0; j < 4; j++) cf[j] = TILE ? (int)g.tsl[j * nc + i] : g.cface[j * nc + i];
     double fv[4][NV];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int f = cf[j], q = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
 #pragma unroll
-      for (int kk = 0; kk < NV; kk++)
-        fv[j][kk] = MIX ? (double)s.fflux32[(size_t)q * NV + kk] : s.fflux[(size_t)q * NV + kk];
+      for (int kk = 0; kk < NV; kk++) {
+        if (TILE) fv[j][kk] = (double)sfl[q * NV + kk];
+        else fv[j][kk] = MIX ? (double)s.fflux32[(size_t)q * NV + kk] : s.fflux[(size_t)q * NV + kk];
+      }
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {

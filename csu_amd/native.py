@@ -409,17 +409,20 @@ class Device:
     def autotune(self, iters: int = 4) -> dict:
         """Time the candidate launch shapes and keep the fastest (before the
         first iteration); returns the choice."""
-        ch = np.zeros(3, np.int32)
+        ch = np.zeros(4, np.int32)
         check(lib().csu_dev_autotune(self._h, iters, _p(ch, C.c_int32)))
-        return {"block": int(ch[0]), "lusgs_block": int(ch[1]), "xcd_map": int(ch[2])}
+        return {"block": int(ch[0]), "lusgs_block": int(ch[1]), "xcd_map": int(ch[2]), "tile": int(ch[3])}
 
     def launch_shape(self) -> dict:
-        ch = np.zeros(3, np.int32)
+        ch = np.zeros(4, np.int32)
         tuned = lib().csu_dev_launch_shape(self._h, _p(ch, C.c_int32))
-        return {"block": int(ch[0]), "lusgs_block": int(ch[1]), "xcd_map": int(ch[2]), "tuned": bool(tuned == 1)}
+        return {"block": int(ch[0]), "lusgs_block": int(ch[1]), "xcd_map": int(ch[2]), "tile": int(ch[3]),
+                "tuned": bool(tuned == 1)}
 
-    def set_launch_shape(self, block: int, lusgs_block: int, xcd_map: int):
-        ch = np.array([block, lusgs_block, xcd_map], np.int32)
+    def set_launch_shape(self, block: int, lusgs_block: int, xcd_map: int, tile: int = 0):
+        """tile 1 (inviscid second-order RK runs): the face fluxes evaluated
+        per cell tile inside the update kernel, no face-flux kernel."""
+        ch = np.array([block, lusgs_block, xcd_map, tile], np.int32)
         check(lib().csu_dev_set_launch_shape(self._h, _p(ch, C.c_int32)))
 
     def set_graphs(self, on: bool):
@@ -527,6 +530,9 @@ class Batch:
 
     def launch_shape(self) -> dict:
         return Device.launch_shape(self)
+
+    def set_launch_shape(self, block: int, lusgs_block: int, xcd_map: int, tile: int = 0):
+        return Device.set_launch_shape(self, block, lusgs_block, xcd_map, tile)
 
     def bytes(self) -> int:
         return lib().csu_dev_bytes(self._h)

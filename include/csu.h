@@ -281,17 +281,21 @@ int  csu_batch_run(csu_dev* dev, double drop, int32_t max_iter, int32_t chunk,
 /* Launch-shape autotuner (no counterpart in the reference).  Before the
  * first iteration, time every candidate -- workgroup size of the gradient
  * and face kernels (64, 128, 256), of the LU-SGS level launches (64, 128,
- * 256), XCD-contiguous or plain tile order -- over `iters` iterations (the
- * better of two runs) and keep the fastest; the context then continues from the state it had, and
- * later contexts of the same kind of run and mesh-size bucket (log2 of the
- * cell count) in the process take the same choice without timing.
- * Results do not depend on these shapes.  choice: [3] or NULL.
+ * 256), XCD-contiguous or plain tile order, and for inviscid second-order
+ * RK runs the tile form (the face fluxes of each 256-cell tile evaluated
+ * into LDS by the update kernel, no face-flux kernel or array) -- over
+ * `iters` iterations (the better of two runs) and keep the fastest; the
+ * context then continues from the state it had, and later contexts of the
+ * same kind of run and mesh-size bucket (log2 of the cell count) in the
+ * process take the same choice without timing.  Results do not depend on
+ * these choices.  choice: [4] (block, LU-SGS block, XCD map, tile) or NULL.
  * csu_dev_launch_shape reports the shapes in use (returns 1 if tuned). */
 int  csu_dev_autotune(csu_dev* dev, int32_t iters, int32_t* choice);
 int  csu_dev_launch_shape(const csu_dev* dev, int32_t* choice);
-/* Set the launch shapes explicitly (choice [3] as above: 64, 128 or 256;
- * 64, 128 or 256; 0 or 1), e.g. one the autotuner chose in an earlier
- * process, or a fixed one for profiling runs.  Results do not change. */
+/* Set the launch shapes explicitly (choice [4] as above: 64, 128 or 256;
+ * 64, 128 or 256; 0 or 1; 0 or 1, 1 only where the tile form exists), e.g.
+ * one the autotuner chose in an earlier process, or a fixed one for
+ * profiling runs.  Results do not change. */
 int  csu_dev_set_launch_shape(csu_dev* dev, const int32_t* choice);
 
 /* Read + write rate (GB/s) of a plain FP64 streaming kernel over two

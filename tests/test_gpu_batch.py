@@ -93,6 +93,23 @@ def test_ramp_batch_equals_alone(tmpcase, precision, inp):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_ramp_batch_tile_form_equals_alone(tmpcase, precision):
+    """The tile form of the face flux + update on a batch (tiles never span
+    two solves; a solve's faces are listed by its own tiles only): each
+    solve equals its run alone in the default form, bit for bit."""
+    cs = [_ramp(tmpcase, 24, 12, 1, 2.5), _ramp(tmpcase, 41, 19, 2, 3.0), _ramp(tmpcase, 9, 5, 3, 2.2)]
+    prms = _params(cs, precision)
+    b = native.Batch(cs, prms)
+    b.set_launch_shape(64, 64, 1, 1)
+    assert b.launch_shape()["tile"] == 1
+    n = 13
+    res, coef = b.iterate(n)
+    assert np.isfinite(res).all()
+    _check_equal(b, cs, prms, n, res, coef)
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("scheme", ["DPLUR", "LUSGS", "RK"])
 def test_plate_batch_equals_alone(tmpcase, precision, scheme):
     """Laminar plates at different Mach and Reynolds numbers (pressure)."""
