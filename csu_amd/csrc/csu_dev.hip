@@ -79,7 +79,7 @@ struct csu_dev {
   int pu;                      // RK stages recover the primitives from the state (k_grad's PU form)
   int sblk;                    // threads per workgroup of k_grad and k_flux
   int lblk;                    // threads per workgroup of an LU-SGS level launch
-  int tile_ok;                 // the tile form's face lists exist (primitives-from-state runs whose tiles fit)
+  int tile_ok;                 // the tile form's LDS list length (CSU_TILE_FACES_S or CSU_TILE_FACES), 0: no tile form
   int tile;                    // face fluxes evaluated per cell tile inside k_update, no k_flux (1)
   int tuned;                   // launch shapes from the autotuner (1) or the defaults (0)
   // implicit schemes
@@ -674,7 +674,7 @@ static void tune_lookup(csu_dev* d) {
 // own tile's list (signed like cface: + where the cell is the face's e0).
 // False when a tile has more faces than the workgroup's LDS list holds.
 static bool tile_lists(const HostMesh& H, const std::vector<SolveP>* sol, std::vector<int>& toff,
-                       std::vector<int>& tface, std::vector<short>& tsl) {
+                       std::vector<int>& tface, std::vector<short>& tsl, int* maxf) {
   const int nc = H.nc, nt = (nc + CSU_BLOCK - 1) / CSU_BLOCK;
   auto each_face = [&](auto fn) {
     if (sol && !sol->empty()) {
@@ -691,8 +691,10 @@ static bool tile_lists(const HostMesh& H, const std::vector<SolveP>* sol, std::v
     if (b != a) cnt[b]++;
   });
   toff.assign(nt + 1, 0);
+  *maxf = 0;
   for (int t = 0; t < nt; t++) {
     if (cnt[t] > CSU_TILE_FACES) return false;
+    *maxf = std::max(*maxf, cnt[t]);
     toff[t + 1] = toff[t] + cnt[t];
   }
   tface.assign(toff[nt] > 0 ? toff[nt] : 1, 0);
@@ -843,10 +845,11 @@ static int dev_setup(csu_dev* d, const HostMesh& H, int ns, const csu_mesh_desc*
   if (d->pu) {
     std::vector<int> toff, tface;
     std::vector<short> tsl;
-    if (tile_lists(H, sol, toff, tface, tsl)) {
+    int maxf = 0;
+    if (tile_lists(H, sol, toff, tface, tsl, &maxf)) {
       if ((rc = upload(d, &g.toff, toff)) || (rc = upload(d, &g.tface, tface)) || (rc = upload(d, &g.tsl, tsl)))
         return rc;
-      d->tile_ok = 1;
+      d->tile_ok = maxf <= CSU_TILE_FACES_S ? CSU_TILE_FACES_S : CSU_TILE_FACES;
     }
   }
   if (!d->tile_ok) d->tile = 0;
@@ -1182,15 +1185,22 @@ static int stage(csu_dev* d, int flags, double sigma, double rk_a, double* Un, d
         Prof p(d, "k_update");
         const dim3 gu(nblk(nc)), bu(CSU_BLOCK);
         double* const rdt = d->red + (size_t)NV * d->nred;
-        if (kp.mixed && d->tile)
-          hipLaunchKernelGGL((k_update<4, false, true, true, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
-                             rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
+        const bool ts = d->tile_ok == CSU_TILE_FACES_S;
+        if (kp.mixed && d->tile && ts)
+          hipLaunchKernelGGL((k_update<4, false, true, true, CSU_TILE_FACES_S>), gu, bu, 0, d->stream, kp, d->geo, st,
+                             flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
+        else if (kp.mixed && d->tile)
+          hipLaunchKernelGGL((k_update<4, false, true, true, CSU_TILE_FACES>), gu, bu, 0, d->stream, kp, d->geo, st,
+                             flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
         else if (kp.mixed)
           hipLaunchKernelGGL((k_update<4, false, true, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
                              rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
+        else if (d->tile && ts)
+          hipLaunchKernelGGL((k_update<4, false, false, true, CSU_TILE_FACES_S>), gu, bu, 0, d->stream, kp, d->geo, st,
+                             flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
         else if (d->tile)
-          hipLaunchKernelGGL((k_update<4, false, false, true, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
-                             rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
+          hipLaunchKernelGGL((k_update<4, false, false, true, CSU_TILE_FACES>), gu, bu, 0, d->stream, kp, d->geo, st,
+                             flags, sigma, rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);
         else
           hipLaunchKernelGGL((k_update<4, false, false, true>), gu, bu, 0, d->stream, kp, d->geo, st, flags, sigma,
                              rk_a, d->prm.cfl, d->dtl, d->dtg, Un, Uout, d->R, d->red, rdt);

@@ -13,8 +13,11 @@ namespace csu {
 // solve, and a solve's per-block reduction partials are those of its own
 // run alone.
 #define CSU_CHUNK 256
-// tile form of the face flux + update: most interior faces a 256-cell tile
-// may have (its cells' faces, those shared with other tiles included)
+// tile form of the face flux + update: the LDS face lists of a 256-cell
+// tile (its cells' interior faces, those shared with other tiles included):
+// the short one where every tile of the mesh fits (a triangle mesh in
+// column order has ~513), else the long one
+#define CSU_TILE_FACES_S 640
 #define CSU_TILE_FACES 1024
 
 // Scalar options and free-stream data, passed by value.  nc, nf, nb are the

@@ -584,13 +584,14 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_bcflux(KP kp, Geo g, St s) {
 // fused with the next stage's solverCalcPrimitive, and on the last stage
 // the next time step (solver.c:895-950) and solverCalcRes partials
 // (solver.c:867-892).
-// TILE (inviscid runs): no face-flux kernel.  The workgroup of a 256-cell
+// TILE > 0 (inviscid runs): no face-flux kernel.  The workgroup of a 256-cell
 // tile first evaluates the convective flux of every interior face of its
 // cells into LDS (conv_flux, as the face kernel does; a face between two
 // tiles is evaluated by both), then each cell sums its faces' fluxes from
 // LDS in its slot order: the same values in the same order, without the
-// face-flux array's write and gather.
-template <int NV, bool VISC, bool MIX = false, bool PU = false, bool TILE = false>
+// face-flux array's write and gather.  TILE is the LDS list's length (the
+// smaller one where every tile of the mesh fits: more workgroups per CU).
+template <int NV, bool VISC, bool MIX = false, bool PU = false, int TILE = 0>
 __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int flags, double sigma, double rk_a,
                                                       double cfl, double* dtl, const double* dtg, double* Un,
                                                       double* __restrict__ Uout, double* __restrict__ Rout,
@@ -610,9 +611,9 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
   }
   const bool active = i < kp.cend;
   using TS = typename std::conditional<MIX, float, double>::type;
-  __shared__ TS sfl[TILE ? CSU_TILE_FACES * NV : 1];
-  if (TILE) {
-    static_assert(!TILE || !VISC, "tile form: inviscid runs only");
+  __shared__ TS sfl[TILE > 0 ? TILE * NV : 1];
+  if (TILE > 0) {
+    static_assert(TILE == 0 || !VISC, "tile form: inviscid runs only");
     // the tile's faces (its cells' interior faces): g.tface[g.toff[tile] ..)
     const int f0 = g.toff[tile], nft = g.toff[tile + 1] - f0;
     for (int k = t; k < nft; k += blockDim.x) {
@@ -636,14 +637,14 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_update(KP kp, Geo g, St s, int fl
     // +-(its place in the tile's list + 1), with the face id's sign.
     int cf[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) cf[j] = TILE ? (int)g.tsl[j * nc + i] : g.cface[j * nc + i];
+    for (int j = 0; j < 4; j++) cf[j] = TILE > 0 ? (int)g.tsl[j * nc + i] : g.cface[j * nc + i];
     double fv[4][NV];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int f = cf[j], q = f > 0 ? f - 1 : (f < 0 ? -f - 1 : 0);
 #pragma unroll
       for (int kk = 0; kk < NV; kk++) {
-        if (TILE) fv[j][kk] = (double)sfl[q * NV + kk];
+        if (TILE > 0) fv[j][kk] = (double)sfl[q * NV + kk];
         else fv[j][kk] = MIX ? (double)s.fflux32[(size_t)q * NV + kk] : s.fflux[(size_t)q * NV + kk];
       }
     }

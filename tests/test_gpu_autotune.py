@@ -15,6 +15,8 @@ def _case(tmpcase, kind):
     d = tmpcase("at")
     if kind == "ramp":
         cases.ramp_case(d, nx=60, ny=30, threads=1, nmax=1)
+    elif kind == "ramp_quad":  # quads: the tile form's long LDS list
+        cases.ramp_case(d, nx=300, ny=300, threads=1, nmax=1, split=False)
     elif kind == "ramp_mixed":
         cases.ramp_case(d, nx=60, ny=30, threads=1, nmax=1, precision=1)
     elif kind == "ramp_lusgs":
@@ -30,8 +32,8 @@ def _case(tmpcase, kind):
     return native.HostCase(d, host_wall_distance=False)
 
 
-TILED = ("ramp", "ramp_mixed")  # runs that have the tile form
-KINDS = ["ramp", "ramp_mixed", "ramp_lusgs", "lusgs", "dplur"]
+TILED = ("ramp", "ramp_quad", "ramp_mixed")  # runs that have the tile form
+KINDS = ["ramp", "ramp_quad", "ramp_mixed", "ramp_lusgs", "lusgs", "dplur"]
 
 
 @pytest.mark.parametrize("kind", KINDS)
