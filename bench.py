@@ -36,12 +36,14 @@ METRIC = "cell-iters/s + solves/hr, p50/p99 time-to-converge vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int, pu: bool = False) -> float:
+def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int, pu: bool = False, tile_entries: int = 0) -> float:
     """Compulsory HBM bytes of one launch: every array the kernel must touch,
     each element counted once (neighbour re-reads are cache traffic).
     pu: the stages recover the primitives from the conservative state (the
     inviscid RK form), so k_grad reads U instead of P and k_update writes no
-    P."""
+    P.  tile_entries > 0: the tile form, whose k_update also evaluates the
+    face fluxes (the face kernel's bytes without the face-flux array, plus
+    the tiles' face lists of that many entries)."""
     ns = nv + 1
     if kernel == "k_prim":
         return nc * (8 * nv + 8 * ns)
@@ -53,6 +55,12 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int, pu: bool = False)
     if kernel == "k_flux":
         # e0, e1, edge slots, face vector; both edge states; the flux
         return nf * (8 + 1 + 24 + 2 * 8 * nv + 8 * nv)
+    if kernel == "k_update" and tile_entries:
+        # each face's e0, e1, edge slots, face vector and both edge states;
+        # the tiles' face lists; per cell the slots' list places, boundary
+        # mask, U^n, U out, dt; boundary fluxes
+        return (nf * (8 + 1 + 24 + 2 * 8 * nv) + tile_entries * 4 + (nc // 256 + 1) * 4
+                + nc * (8 + 1 + 8 * nv * 2 + 8 + 8) + nb * 8 * nv)
     if kernel == "k_update":
         # face list, boundary mask, U^n, U out, P out (not with pu), omega,
         # dt; each face flux once
@@ -64,17 +72,33 @@ def alg_bytes(kernel: str, nc: int, nf: int, nb: int, nv: int, pu: bool = False)
 # state stages), from rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0)
 # passes, both scaled by what the counters read for the library's streaming
 # kernel (1 GiB read, 1 GiB written), per launch shape (gradient/face
-# workgroup size, XCD tile map; the residual kernel always runs 256-thread
-# workgroups): the bytes depend on the shape through the caches.  A counter
+# workgroup size, XCD tile map, tile form; the residual kernel always runs
+# 256-thread workgroups) and cell order: the bytes depend on them through
+# the caches.  A counter
 # pass cannot run inside this timed process, so the figure is reported for
 # that workload and the shapes measured only.
-MEASURED_TRAFFIC = {("k_grad", 2000, 1000, 128, 0): 1089.2e6, ("k_flux", 2000, 1000, 128, 0): 930.6e6,
-                    ("k_update", 2000, 1000, 128, 0): 664.3e6,
-                    ("k_grad", 2000, 1000, 256, 1): 993.9e6, ("k_flux", 2000, 1000, 256, 1): 775.6e6,
-                    ("k_update", 2000, 1000, 256, 1): 626.5e6}
+MEASURED_TRAFFIC = {("k_grad", 2000, 1000, 128, 0, 0, "column"): 1089.2e6,
+                    ("k_flux", 2000, 1000, 128, 0, 0, "column"): 930.6e6,
+                    ("k_update", 2000, 1000, 128, 0, 0, "column"): 664.3e6,
+                    ("k_grad", 2000, 1000, 256, 1, 0, "column"): 993.9e6,
+                    ("k_flux", 2000, 1000, 256, 1, 0, "column"): 775.6e6,
+                    ("k_update", 2000, 1000, 256, 1, 0, "column"): 626.5e6,
+                    ("k_grad", 2000, 1000, 256, 0, 1, "morton"): 1190.8e6,
+                    ("k_update", 2000, 1000, 256, 0, 1, "morton"): 1555.0e6}
 
 
 _MESHES = {}
+# cell order of the ramp meshes (--order): "morton" renumbers the
+# generator's column-ordered mesh along a Morton curve of the cell centres
+# (csu_amd/renumber.py), so that a 256-cell tile's neighbours are mostly
+# inside it; "column" keeps the generator's order
+ORDER = "morton"
+
+
+def _ordered(d: str):
+    if ORDER == "morton":
+        from csu_amd import renumber
+        renumber.renumber_file(d + "mesh.su2", d + "mesh.su2", order="morton")
 
 
 def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision: int = 0, keep: bool = False,
@@ -91,6 +115,7 @@ def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision:
         cases.write_input(d + "input.dat", cases.ramp_input(threads=1, nmax=steps, precision=precision))
     else:
         cases.ramp_case(d, nx=nx, ny=ny, threads=1, nmax=steps, precision=precision)
+        _ordered(d)
         _MESHES[(nx, ny)] = d + "mesh.su2"
     case = native.HostCase(d)
     dev = native.Device(case)
@@ -117,6 +142,11 @@ def gpu_run(nx: int, ny: int, steps: int, warmup: int, profile: bool, precision:
     res, coef = dev.iterate(1)
     info = dict(nc=case.ncell, nf=case.nface, nb=case.nbface, nv=case.nvar, launch_shape=shape,
                 seconds=t1 - t0, profiled_seconds=tp, kernels=kt, res=res[0].tolist(), coef=coef[0].tolist())
+    if shape.get("tile"):
+        # entries of the tiles' face lists: every face once, twice when its
+        # cells lie in two 256-cell tiles
+        F = case.arrays()["face"]
+        info["tile_entries"] = int(len(F) + np.count_nonzero(F[:, 0] // 256 != F[:, 1] // 256))
     if keep:
         info["U"] = dev.get_state()
     dev.close()
@@ -133,6 +163,7 @@ def small_batched(n: int, steps: int, warmup: int):
     from csu_amd import cases, native
     d0 = tempfile.mkdtemp(prefix="sb", dir="/tmp") + "/"
     cases.ramp_case(d0, nx=200, ny=100, threads=1, nmax=steps)
+    _ordered(d0)
     hcs = []
     for k in range(n):
         d = tempfile.mkdtemp(prefix=f"sb{k}_", dir="/tmp") + "/"
@@ -709,6 +740,9 @@ def main():
                     help="launch shapes of the ramp runs instead of the autotuner's (e.g. 128,64,0,0: the choice a "
                          "timed run reported; T 1: the tile form of the face flux + update), so that profiling "
                          "runs launch the kernels the timed run did")
+    ap.add_argument("--order", choices=["morton", "column"], default="morton",
+                    help="cell order of the ramp meshes: renumbered along a Morton curve (default) or the "
+                         "generator's column order")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 200x100x2 timing (profiling runs of the big mesh alone)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -721,6 +755,8 @@ def main():
                     help="only run the library's FP64 streaming kernel over two 1 GiB arrays (11 launches): the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters and the practical HBM rate")
     a = ap.parse_args()
+    global ORDER
+    ORDER = a.order
     if a.stream_calib:
         from csu_amd import native
         print(json.dumps({"stream_kernel_bytes_read": 2 ** 30, "stream_kernel_bytes_written": 2 ** 30,
@@ -802,12 +838,13 @@ def main():
     pu = os.environ.get("CSU_STORED_PRIMITIVES", "0") != "1"
     from csu_amd import native as _nat
     stream = _nat.stream_rate(2 ** 30, 10)
-    traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"], pu)
+    traffic_alg = alg_bytes(dom, big["nc"], big["nf"], big["nb"], big["nv"], pu, big.get("tile_entries", 0))
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": MEASURED_TRAFFIC.get((dom, a.nx, a.ny, big["launch_shape"]["block"],
-                                             big["launch_shape"]["xcd_map"])) if pu else None,
+                                             big["launch_shape"]["xcd_map"], big["launch_shape"]["tile"],
+                                             ORDER)) if pu else None,
             "kernel": dom,
             "kernel_avg_ms": avg_ms,
             "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
@@ -841,7 +878,7 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"euler-ramp M2.5 Roe/LSQ/Venkat(K=5) RK5 CFL0.5 FP64, "
-                               f"{a.nx}x{a.ny}x2 tri mesh",
+                               f"{a.nx}x{a.ny}x2 tri mesh" + (", Morton cell order" if ORDER == "morton" else ""),
                    "cells": nc, "faces": big["nf"], "boundary_faces": big["nb"],
                    "config0_40k_cell_iter_per_s":
                        None if small is None else small["nc"] * max(a.steps, 50) / small["seconds"],
