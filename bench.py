@@ -84,7 +84,15 @@ MEASURED_TRAFFIC = {("k_grad", 2000, 1000, 128, 0, 0, "column"): 1089.2e6,
                     ("k_flux", 2000, 1000, 256, 1, 0, "column"): 775.6e6,
                     ("k_update", 2000, 1000, 256, 1, 0, "column"): 626.5e6,
                     ("k_grad", 2000, 1000, 256, 0, 1, "morton"): 1190.8e6,
-                    ("k_update", 2000, 1000, 256, 0, 1, "morton"): 1555.0e6}
+                    ("k_update", 2000, 1000, 256, 0, 1, "morton"): 1555.0e6,
+                    ("k_grad", 2000, 1000, 128, 0, 1, "morton"): 1264.0e6,
+                    ("k_update", 2000, 1000, 128, 0, 1, "morton"): 1554.9e6,
+                    ("k_grad", 2000, 1000, 64, 1, 1, "morton"): 999.7e6,
+                    ("k_update", 2000, 1000, 64, 1, 1, "morton"): 1165.1e6,
+                    ("k_grad", 2000, 1000, 128, 1, 1, "morton"): 999.7e6,
+                    ("k_update", 2000, 1000, 128, 1, 1, "morton"): 1164.9e6,
+                    ("k_grad", 2000, 1000, 256, 1, 1, "morton"): 999.7e6,
+                    ("k_update", 2000, 1000, 256, 1, 1, "morton"): 1165.1e6}
 
 
 _MESHES = {}
