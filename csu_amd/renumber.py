@@ -182,15 +182,17 @@ def renumber_file(src: str, dst: str, order: str = "colour") -> np.ndarray:
     return perm
 
 
-def renumber_case(src_dir: str, dst_dir: str) -> np.ndarray:
+def renumber_case(src_dir: str, dst_dir: str, order: str = "colour") -> np.ndarray:
     os.makedirs(dst_dir, exist_ok=True)
     for f in ("input.dat",):
         shutil.copy(os.path.join(src_dir, f), os.path.join(dst_dir, f))
-    return renumber_file(os.path.join(src_dir, "mesh.su2"), os.path.join(dst_dir, "mesh.su2"))
+    return renumber_file(os.path.join(src_dir, "mesh.su2"), os.path.join(dst_dir, "mesh.su2"), order)
 
 
 if __name__ == "__main__":
-    if len(sys.argv) != 3:
+    args = [a for a in sys.argv[1:] if not a.startswith("--order=")]
+    orders = [a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--order=")]
+    if len(args) != 2:
         sys.exit(__doc__)
-    p = renumber_case(sys.argv[1], sys.argv[2])
+    p = renumber_case(args[0], args[1], orders[-1] if orders else "colour")
     print(f"renumbered {len(p)} cells")
