@@ -16,7 +16,7 @@ oracle of the renumbered device run (the sweep result depends on the cell
 order, so it is the same sweep only on the same numbering).  Node lines and
 markers are copied verbatim; only the element lines are permuted.
 
-    python -m csu_amd.renumber <case>/ <new case>/     (copies input.dat)
+    python -m csu_amd.renumber <case>/ <new case>/ [--order=colour|morton]   (copies input.dat)
 """
 from __future__ import annotations
 
