@@ -369,17 +369,36 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
   if (g.sol) kp.cend = g.sol[solve_of(g.csolk, blockIdx.x * blockDim.x)].c1;
   if (i >= kp.cend) return;
   const double h = 1.0e-6;
-#pragma unroll 1
+  // Every slot's face data and neighbour state first (one memory round trip
+  // for all slots instead of one chain per slot); slots without a face
+  // read the cell itself and are skipped.
+  int nbs[4], fis[4];
+  double sxs[4], sys_[4], sds[4];
+#pragma unroll
   for (int j = 0; j < 4; j++) {
-    double dSx, dSy, dS;
-    int fi;
-    const int nb = face_from(g, nc, i, j, dSx, dSy, dS, fi);
+    fis[j] = 0;
+    sxs[j] = sys_[j] = 0.0;
+    sds[j] = 1.0;
+    nbs[j] = face_from(g, nc, i, j, sxs[j], sys_[j], sds[j], fis[j]);
+  }
+  double Ujs[4][NV], P1s[4], P2s[4], P3s[4], P4s[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int nb = nbs[j] >= 0 ? nbs[j] : i;
+#pragma unroll
+    for (int k = 0; k < NV; k++) Ujs[j][k] = U[k * nc + nb];
+    P1s[j] = s.P[nc + nb]; P2s[j] = s.P[2 * nc + nb]; P3s[j] = s.P[3 * nc + nb]; P4s[j] = s.P[4 * nc + nb];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int nb = nbs[j], fi = fis[j];
     if (nb < 0) continue;
+    const double dSx = sxs[j], dSy = sys_[j], dS = sds[j];
     const double nx = dSx / dS, ny = dSy / dS;
-    const double P3 = s.P[3 * nc + nb], P4 = s.P[4 * nc + nb];
+    const double P3 = P3s[j], P4 = P4s[j];
     double Uj[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) Uj[k] = U[k * nc + nb];
+    for (int k = 0; k < NV; k++) Uj[k] = Ujs[j][k];
     double rho = Uj[0], u = Uj[1] / rho, v = Uj[2] / rho, E = Uj[3] / rho;
     double V = (nx * Uj[1] + ny * Uj[2]) / Uj[0];
     double F0[NV];
@@ -388,26 +407,11 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
     if (NV == 6) F0[5] = Uj[5] * V;
     const double ee0 = NV == 6 ? E - 0.5 * (u * u + v * v) - Uj[4] / rho : E - 0.5 * (u * u + v * v);
     const double cv = gas_T2Cv<double>(kp.gas, P4);
-    double* Aj = A + (size_t)j * NV * NV * nc;
-#pragma unroll
-    for (int mm = 0; mm < NV; mm++) {
-      double Up[NV];
-#pragma unroll
-      for (int n = 0; n < NV; n++) Up[n] = mm == n ? Uj[n] + h : Uj[n];
-      rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
-      double ee1 = NV == 6 ? E - 0.5 * (u * u + v * v) - Up[4] / rho : E - 0.5 * (u * u + v * v);
-      double T = P4 + (ee1 - ee0) / cv;
-      double p = kp.gas.R * rho * T;
-      double Vp = (nx * Up[1] + ny * Up[2]) / Up[0];
-      double F[NV];
-      F[0] = Up[0] * Vp; F[1] = Up[1] * Vp + nx * p; F[2] = Up[2] * Vp + ny * p; F[3] = (Up[3] + p) * Vp;
-      if (NV >= 5) F[4] = Up[4] * Vp;
-      if (NV == 6) F[5] = Up[5] * Vp;
-#pragma unroll
-      for (int n = 0; n < NV; n++) Aj[(size_t)(n * NV + mm) * nc + i] = 0.5 * (F[n] - F0[n]) * dS / h;
-    }
+    // the diagonal's spectral-radius term, subtracted from each diagonal
+    // entry as it is stored (the reference subtracts it from the stored
+    // block, implicit.c:792-809: the same two roundings)
     double c = gas_T2c<double>(kp.gas, P4);
-    double ra = kp.wimp * (fabs(nx * s.P[nc + nb] + ny * s.P[2 * nc + nb]) + c) * dS;
+    double ra = kp.wimp * (fabs(nx * P1s[j] + ny * P2s[j]) + c) * dS;
     if (NV >= 5) {
       // implicitUpdateA_sa (implicit.c:815-936); SST the same with its
       // face eddy viscosity
@@ -425,8 +429,27 @@ __global__ __launch_bounds__(CSU_BLOCK) void k_blocks(KP kp, Geo g, St s, const 
       double dd_ = sqrt(ddx * ddx + ddy * ddy);
       ra += fmax(4 / (3 * r), gamma_of(kp.gas, T) / r) * (mi / kp.pr) * dS / dd_;
     }
+    double* Aj = A + (size_t)j * NV * NV * nc;
 #pragma unroll
-    for (int n = 0; n < NV; n++) Aj[(size_t)(n * NV + n) * nc + i] -= 0.5 * ra;
+    for (int mm = 0; mm < NV; mm++) {
+      double Up[NV];
+#pragma unroll
+      for (int n = 0; n < NV; n++) Up[n] = mm == n ? Uj[n] + h : Uj[n];
+      rho = Up[0]; u = Up[1] / rho; v = Up[2] / rho; E = Up[3] / rho;
+      double ee1 = NV == 6 ? E - 0.5 * (u * u + v * v) - Up[4] / rho : E - 0.5 * (u * u + v * v);
+      double T = P4 + (ee1 - ee0) / cv;
+      double p = kp.gas.R * rho * T;
+      double Vp = (nx * Up[1] + ny * Up[2]) / Up[0];
+      double F[NV];
+      F[0] = Up[0] * Vp; F[1] = Up[1] * Vp + nx * p; F[2] = Up[2] * Vp + ny * p; F[3] = (Up[3] + p) * Vp;
+      if (NV >= 5) F[4] = Up[4] * Vp;
+      if (NV == 6) F[5] = Up[5] * Vp;
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        const double a = 0.5 * (F[n] - F0[n]) * dS / h;
+        Aj[(size_t)(n * NV + mm) * nc + i] = n == mm ? a - 0.5 * ra : a;
+      }
+    }
   }
 }
 
