@@ -853,6 +853,9 @@ static int dev_setup(csu_dev* d, const HostMesh& H, int ns, const csu_mesh_desc*
     }
   }
   if (!d->tile_ok) d->tile = 0;
+  // untuned default: the tile form from 200k cells on, where it measured
+  // 4-6 % faster in every shape (below that, launch latency decides)
+  else if (!d->tuned && nc >= 200000) d->tile = 1;
   if (kp.mixed) {
     auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
     if ((rc = upload(d, &g.lwx32, f32(H.lwx))) || (rc = upload(d, &g.lwy32, f32(H.lwy))) ||

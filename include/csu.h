@@ -288,7 +288,8 @@ int  csu_batch_run(csu_dev* dev, double drop, int32_t max_iter, int32_t chunk,
  * context then continues from the state it had, and later contexts of the
  * same kind of run and mesh-size bucket (log2 of the cell count) in the
  * process take the same choice without timing.  Results do not depend on
- * these choices.  choice: [4] (block, LU-SGS block, XCD map, tile) or NULL.
+ * these choices (an untuned context takes the tile form from 200k cells
+ * on).  choice: [4] (block, LU-SGS block, XCD map, tile) or NULL.
  * csu_dev_launch_shape reports the shapes in use (returns 1 if tuned). */
 int  csu_dev_autotune(csu_dev* dev, int32_t iters, int32_t* choice);
 int  csu_dev_launch_shape(const csu_dev* dev, int32_t* choice);

@@ -91,3 +91,24 @@ def test_every_set_shape_gives_the_same_bits(tmpcase, kind):
         with pytest.raises(native.CsuError):
             dev.set_launch_shape(64, 64, 0, 1)
     dev.close()
+
+
+def test_large_inviscid_default_is_the_tile_form(tmpcase):
+    """An untuned inviscid second-order RK context of 200k cells or more
+    starts in the tile form, and gives the split form's results bit for
+    bit."""
+    d = tmpcase("lt")
+    cases.ramp_case(d, nx=320, ny=320, threads=1, nmax=1)
+    c = native.HostCase(d)
+    assert c.ncell >= 200000
+    dev = native.Device(c)
+    sh = dev.launch_shape()
+    assert sh["tuned"] or sh["tile"] == 1
+    r1, k1 = dev.iterate(3)
+    U1 = dev.get_state()
+    dev.close()
+    ref = native.Device(c)
+    ref.set_launch_shape(sh["block"], sh["lusgs_block"], sh["xcd_map"], 0)
+    r0, k0 = ref.iterate(3)
+    assert np.array_equal(r0, r1) and np.array_equal(k0, k1) and np.array_equal(ref.get_state(), U1)
+    ref.close()
